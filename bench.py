@@ -1,0 +1,184 @@
+#!/usr/bin/env python
+"""Benchmark: LJ NVE at N = 262,144 on one MI355X (atom-steps/s).
+
+Workload: 64^3 simple-cubic lattice at rho = 0.8442 (L rounded to float32),
+jitter N(0, 0.05), momenta N(0, kT=1) with the mean removed; LJ sigma = eps =
+1 with the C1 switch between r_on = 2.0 and r_c = 2.5, Dense neighbour list
+with skin 0.5 rebuilt on the skin criterion, dt = 0.005, float32 pair
+arithmetic, float64 master positions and energy reductions.  One step = skin
+rebuild (gated on the device) + half kick / drift + force pass with the
+closing half kick.  Steps run in blocks of 100 with a host overflow check per
+block; an overflowing block is redone from its saved start state after
+re-allocating the list (the reference's protocol,
+examples/nve_neighbor_list.py:186-201).
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'atom-steps/s LJ NVE N=262k, plain vs batched/adjoint; error vs FP64 oracle'
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+  print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+  ap = argparse.ArgumentParser()
+  ap.add_argument('--gpus', type=int, default=1)
+  ap.add_argument('--steps', type=int, default=2000)
+  ap.add_argument('--warmup', type=int, default=200)
+  ap.add_argument('--n-side', type=int, default=64)
+  ap.add_argument('--block', type=int, default=100)
+  ap.add_argument('--cpu-steps', type=int, default=40)
+  ap.add_argument('--no-cpu', action='store_true')
+  ap.add_argument('--tpa', type=int, default=0, help='threads per row of the pair kernel')
+  args = ap.parse_args()
+  if args.gpus != 1:
+    print(f'bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU '
+          'decomposition is described in DESIGN.md and not built.', file=sys.stderr)
+    sys.exit(2)
+
+  import numpy as np
+  import torch
+
+  import jax_md_amd as jmd
+  from jax_md_amd import _native as nat
+  from oracle import jaxmd_oracle as O
+
+  dev = 'cuda:0'
+  torch.cuda.set_device(0)
+  if args.tpa:
+    nat.check(nat.load().jmd_set_threads_per_atom(args.tpa), 'jmd_set_threads_per_atom')
+  R, P, L = O.lj_lattice(args.n_side, seed=0)
+  N = R.shape[0]
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense)
+  Rt = torch.as_tensor(R, device=dev)
+  nbrs = nfn.allocate(Rt)
+  init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
+  state = init_fn(0, Rt.clone(), kT=1.0, momenta=torch.as_tensor(P, device=dev), neighbor=nbrs)
+  loop = jmd.simulate.NVELoop(efn, shift, 0.005)
+  log(f'N={N} L={L} max_occupancy={nbrs.max_occupancy} cell_capacity={nbrs.cell_list_capacity}')
+
+  def snapshot(s):
+    return (s.position_hi.clone(), s.position.clone(), s.momentum.clone(), s.force.clone())
+
+  def restore(s, snap):
+    s.position_hi.copy_(snap[0])
+    s.position.copy_(snap[1])
+    s.momentum.copy_(snap[2])
+    s.force.copy_(snap[3])
+
+  reallocs = 0
+
+  def run_steps(n, events=None):
+    nonlocal nbrs, reallocs
+    done = 0
+    loop.prime(state, nbrs)
+    while done < n:
+      k = min(args.block, n - done)
+      snap = snapshot(state)
+      ev = [] if events is not None else None
+      loop.run(state, nbrs, k, ev)
+      if bool(nbrs.did_buffer_overflow):
+        restore(state, snap)
+        nbrs = nfn.allocate(state.position)
+        loop.prime(state, nbrs)
+        reallocs += 1
+        log(f'  overflow: re-allocated, max_occupancy={nbrs.max_occupancy}')
+        continue
+      if events is not None:
+        events.extend(ev)
+      done += k
+
+  run_steps(args.warmup)
+  torch.cuda.synchronize()
+  reallocs_warm = reallocs
+  E_start, _ = efn.energy_and_force(state.position, nbrs)
+  KE_start = jmd.quantity.kinetic_energy(momentum=state.momentum)
+  builds0 = nbrs.plan.builds
+  events = []
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  run_steps(args.steps, events)
+  torch.cuda.synchronize()
+  elapsed = time.perf_counter() - t0
+  builds = nbrs.plan.builds - builds0
+  E_end, F_gpu = efn.energy_and_force(state.position, nbrs)
+  KE_end = jmd.quantity.kinetic_energy(momentum=state.momentum)
+  value = N * args.steps / elapsed
+  ms = 1e3 * elapsed / args.steps
+  kern_ms = [a.elapsed_time(b) for a, b in events]
+  force_ms = float(np.mean(kern_ms))
+  idx = nbrs.idx
+  kbar = float((idx < N).sum().item()) / N
+  force_bytes = N * (4.0 * kbar + 60.0)
+  achieved = force_bytes / (force_ms * 1e-3) / 1e9
+  log(f'{args.steps} steps in {elapsed:.3f}s: {ms * 1e3:.1f} us/step, force kernel '
+      f'{force_ms * 1e3:.1f} us, Kbar={kbar:.2f}, list builds={builds}, reallocs={reallocs}')
+
+  # force error against the float64 C oracle at the final float32 positions
+  force_err = None
+  try:
+    from oracle import c_oracle
+    sysc = c_oracle.LJSystem(state.position.cpu().numpy().astype(np.float64), float(L))
+    E64, F64 = sysc.forces()
+    Fg = F_gpu.cpu().numpy().astype(np.float64)
+    rel = float(np.linalg.norm(Fg - F64) / np.linalg.norm(F64))
+    worst = float(np.max(np.abs(Fg - F64)) / np.sqrt(np.mean(F64 ** 2)))
+    force_err = {'force_rel_norm': rel, 'force_worst_over_rms': worst,
+                 'energy_rel': float(abs(float(E_end) - E64) / abs(E64))}
+  except Exception as e:  # the checker is optional on hosts without gcc
+    log(f'force check skipped: {e}')
+
+  cpu = None
+  if not args.no_cpu:
+    try:
+      from oracle import c_oracle
+      sysc = c_oracle.LJSystem(R.astype(np.float64), float(L))
+      t = time.perf_counter()
+      _, rb = sysc.nve(P.astype(np.float64), 0.005, args.cpu_steps)
+      tc = time.perf_counter() - t
+      cpu = {'value': N * args.cpu_steps / tc, 'unit': 'atom-steps/s',
+             'cores': c_oracle.num_threads(), 'kind': 'port',
+             'sample': f'C FP64 OpenMP restatement (oracle/md_oracle.c), same 64^3 LJ NVE '
+                       f'workload, {args.cpu_steps} steps from t=0 incl. {rb} list rebuilds; '
+                       'the reference (JAX CPU) cannot run here: jax is not installed'}
+      log(f'cpu baseline: {cpu["value"]:.3e} atom-steps/s on {cpu["cores"]} threads')
+    except Exception as e:
+      log(f'cpu baseline skipped: {e}')
+
+  out = {
+      'metric': METRIC, 'value': value, 'unit': 'atom-steps/s', 'n_gpus': 1,
+      'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': ms,
+      'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+      'data': 'synthetic',
+      'config': {'workload': f'LJ NVE N={N} ({args.n_side}^3 lattice, rho=0.8442, jitter 0.05, '
+                             'kT=1), Dense list r_c=2.5 skin=0.5, dt=0.005, fp32 pair '
+                             'arithmetic, fp64 positions/energies',
+                 'list_builds': builds, 'reallocs': reallocs - reallocs_warm,
+                 'max_occupancy': nbrs.max_occupancy, 'kbar': kbar},
+      'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                   'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                   'kernel': 'k_pair_dense (force + closing half kick)',
+                   'kernel_us': force_ms * 1e3},
+      'cpu_baseline': cpu,
+      'error_vs_fp64': force_err,
+      'energy': {'E_total_start': float(E_start) + float(KE_start),
+                 'E_total_end': float(E_end) + float(KE_end)},
+  }
+  print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+  main()

@@ -1,0 +1,247 @@
+/*
+ * jaxmd_native.h - C ABI of the gfx950 neighbour-list MD library
+ * (libjaxmd_native.so).
+ *
+ * Every entry point takes raw device pointers, sizes and plain attributes,
+ * enqueues its kernels on the given HIP stream (``void*`` = hipStream_t,
+ * NULL = default stream), performs no host synchronisation unless stated, and
+ * returns an ``int`` status (JMD_OK on success).  No C++ exception crosses the
+ * boundary.  Numerical overflow of a neighbour or cell list is a data output
+ * (the uint8 partition error word, as in the reference), never a status.
+ *
+ * Buffers are owned by the caller.  Scratch memory comes in through an
+ * explicit ``workspace`` pointer whose size is given by the matching
+ * ``*_workspace_bytes`` query.
+ *
+ * Positions are [n, dim] float32, row-major (the reference's layout), with
+ * dim 2 or 3, passed with a row stride ``position_stride`` in floats: dim
+ * for a contiguous array, or 4 for 3-D rows padded to 16 bytes (base
+ * 16-byte aligned), which the gather-heavy kernels read with one load.  A periodic space is described by ``periodic`` = 1 and
+ * ``side`` (host pointer, dim float32 values: the float32 value of the side
+ * passed to space.periodic); ``periodic`` = 0 is space.free().
+ *
+ * Each declaration cites the reference interface it replaces
+ * (paths relative to the jax-md 0.2.29 source tree).
+ */
+#ifndef JAXMD_NATIVE_H_
+#define JAXMD_NATIVE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+enum {
+  JMD_OK = 0,
+  JMD_ERR_INVALID = 1,     /* bad argument (shape, null pointer, range) */
+  JMD_ERR_LAUNCH = 2,      /* a HIP call failed */
+  JMD_ERR_UNSUPPORTED = 3  /* valid request outside what the library does */
+};
+
+/* NeighborListFormat (jax_md/partition.py:641-661). */
+enum { JMD_DENSE = 0, JMD_SPARSE = 1, JMD_ORDERED_SPARSE = 2 };
+
+/* PartitionErrorCode bits (jax_md/partition.py:494-518). */
+enum {
+  JMD_NEIGHBOR_LIST_OVERFLOW = 1,
+  JMD_CELL_LIST_OVERFLOW = 2,
+  JMD_CELL_SIZE_TOO_SMALL = 4,
+  JMD_MALFORMED_BOX = 8
+};
+
+/* Pair potentials (jax_md/energy.py:125-173 soft_sphere, :246-272
+ * lennard_jones with :534-580 multiplicative_isotropic_cutoff). */
+enum { JMD_LENNARD_JONES = 0, JMD_SOFT_SPHERE = 1 };
+
+/* How a pair parameter is given (jax_md/smap.py:697-823):
+ * scalar; per-particle vector combined as 0.5 (p_i + p_j) (smap.py:836);
+ * species matrix p[s_i, s_j] (smap.py:787-797). */
+enum { JMD_PARAM_SCALAR = 0, JMD_PARAM_PER_PARTICLE = 1, JMD_PARAM_SPECIES = 2 };
+
+typedef struct jmd_param {
+  int32_t mode;          /* JMD_PARAM_* */
+  float value;           /* scalar mode */
+  const float* values;   /* device: [n] (per-particle) or [s*s] (species) */
+} jmd_param;
+
+typedef struct jmd_pair_potential {
+  int32_t kind;          /* JMD_LENNARD_JONES or JMD_SOFT_SPHERE */
+  jmd_param sigma;
+  jmd_param epsilon;
+  jmd_param alpha;       /* soft sphere exponent */
+  float r_onset;         /* LJ: absolute onset radius (2.0 * max sigma) */
+  float r_cutoff;        /* LJ: absolute cutoff radius (2.5 * max sigma) */
+  int32_t n_species;     /* species matrices are n_species x n_species */
+  const int32_t* species;/* device [n] or NULL */
+} jmd_pair_potential;
+
+/* Library version string. */
+const char* jmd_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Cell list and neighbour list                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Scratch bytes needed by jmd_cell_list / jmd_neighbor_list_build for n
+ * particles on a grid of cell_count cells (cell_count 0: no cell list).
+ * The workspace must be zero-filled before its first use; every call leaves
+ * it in that state again, except int32 word 4, which counts the completed
+ * list builds (gated builds that ran, and ungated ones). */
+size_t jmd_neighbor_workspace_bytes(int64_t n, int32_t dim, int64_t cell_count,
+                                    int32_t format);
+
+/* partition.cell_list(...).allocate/update body: cell_list_fn
+ * (jax_md/partition.py:349-471).  Bins n particles into the
+ * cells_per_side grid (host arrays of dim values, x fastest in the hash) with
+ * per-cell capacity cell_capacity, writing the reference's buffers:
+ *   position_buffer [cell_count*cap, dim] (zeros in empty slots),
+ *   id_buffer       [cell_count*cap]      (n in empty slots),
+ *   particle_cell_id[n, dim].
+ * Slot of the k-th particle in (cell, id) order is cell*cap + (k mod cap)
+ * (partition.py:441-442).  *filling_max (device int32) receives the largest
+ * cell occupancy; *overflow (device uint8) is set to 1 when it exceeds
+ * cell_capacity (partition.py:458-460). */
+int jmd_cell_list(const float* position, int64_t n, int32_t dim,
+                  int32_t position_stride, const int32_t* cells_per_side, const float* cell_size,
+                  int32_t cell_capacity, float* position_buffer,
+                  int32_t* id_buffer, int32_t* particle_cell_id,
+                  int32_t* filling_max, uint8_t* overflow, void* workspace,
+                  void* stream);
+
+/* Largest cell occupancy on a grid, the quantity count_cell_filling +
+ * _estimate_cell_capacity read on the host (jax_md/partition.py:191-208,
+ * 243-250).  *filling_max is a device int32. */
+int jmd_cell_filling_max(const float* position, int64_t n, int32_t dim,
+                         int32_t position_stride, const int32_t* cells_per_side,
+                         const float* cell_size,
+                         int32_t* filling_max, void* workspace, void* stream);
+
+/* Body of neighbor_fn in partition.neighbor_list
+ * (jax_md/partition.py:1037-1117): candidates from the 3^dim stencil of the
+ * cell grid (use_cell_list = 1, partition.py:911-951) or from all particles
+ * (partition.py:904-909), self-masking (:953-958), and the prune of the
+ * chosen format (:960-1032), written in the reference's order:
+ *   Dense  idx [n, max_occupancy] int32, rows left-compacted, padding n;
+ *   Sparse / OrderedSparse idx [2, max_occupancy] = (receiver, sender).
+ * cutoff_sq is float32 (r_cutoff + dr_threshold)^2.  cell_capacity is the
+ * reference cell capacity (it fixes the slot order of candidates and the
+ * cell-overflow bit).  occupancy (device int32 [1]) receives the largest row
+ * count (Dense) or the total pair count (Sparse); the neighbour-overflow bit
+ * is OR-ed into *error when it exceeds max_occupancy, the cell-overflow bit
+ * when a cell holds more than cell_capacity particles (:1066,1110).
+ * reference_position [n, dim] (contiguous) receives a copy of position.
+ * With error = NULL the call only measures (idx is not written and may be
+ * NULL): occupancy receives the largest Dense row count (or the Sparse
+ * total); this is how allocate sizes the list (partition.py:1092-1105).
+ * gate (device int32, may be NULL): when non-NULL the whole call is a no-op
+ * on the device unless *gate != 0, and *gate is reset to 0 at its end; this
+ * is the lax.cond of update (:1146-1154) without a host round trip. */
+int jmd_neighbor_list_build(const float* position, int64_t n, int32_t dim,
+                            int32_t position_stride, int32_t periodic,
+                            const float* side, float cutoff_sq,
+                            int32_t use_cell_list, const int32_t* cells_per_side,
+                            const float* cell_size, int32_t cell_capacity,
+                            int32_t format, int32_t mask_self, int32_t* idx,
+                            int64_t max_occupancy, int32_t* occupancy,
+                            uint8_t* error, float* reference_position,
+                            int32_t* gate, void* workspace, void* stream);
+
+/* Skin criterion of neighbor_list update (jax_md/partition.py:1141-1154):
+ * sets *gate (device int32) to 1 when any particle satisfies
+ * metric_sq(position, reference_position) > threshold_sq, with threshold_sq
+ * = float32 (dr_threshold / 2)^2.  *gate is only ever raised here; it is
+ * lowered by the gated jmd_neighbor_list_build. */
+int jmd_neighbor_list_check(const float* position, int32_t position_stride,
+                            const float* reference_position, int64_t n,
+                            int32_t dim, int32_t periodic, const float* side,
+                            float threshold_sq, int32_t* gate, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Pair reduction                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Scratch bytes for jmd_pair_energy_force. */
+size_t jmd_pair_workspace_bytes(int64_t n);
+
+/* Threads per Dense row in jmd_pair_energy_force (2, 4, 8 or 16; 0 for
+ * the built-in choice).  A process-wide tuning knob set by the autotuner; it
+ * has no counterpart in the reference (XLA picks its own fusion layout). */
+int jmd_set_threads_per_atom(int32_t threads_per_atom);
+
+/* smap.pair_neighbor_list(fn, metric, ...) applied to
+ * energy.lennard_jones / energy.soft_sphere
+ * (jax_md/smap.py:856-979, jax_md/energy.py:200-343), plus its gradient.
+ * For idx in format `format` (idx_width = max_occupancy) it computes, each
+ * output optional (NULL skips it):
+ *   force          [n, dim] float32  = -dE/dR
+ *   energy_atom    [n]      float32  per-particle energy (reduce_axis=(1,),
+ *                                    Dense and Sparse only)
+ *   energy_total   [1]      float64  total energy, summed in float64
+ * Pair arithmetic is float32 with float64 per-particle accumulation.
+ * When momentum is non-NULL (Dense only) the call also applies
+ * p += half_dt * F (the closing half kick of simulate.velocity_verlet,
+ * simulate.py:240-241) in the same pass; dt_device (device float32 [1], may
+ * be NULL) overrides half_dt with float32(dt / 2). */
+int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
+                          int32_t position_stride, int32_t periodic,
+                          const float* side,
+                          const jmd_pair_potential* potential,
+                          const int32_t* idx, int64_t idx_width,
+                          int32_t format, float* force, float* energy_atom,
+                          double* energy_total, float* momentum,
+                          float half_dt, const float* dt_device,
+                          void* workspace, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Integrators                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* First half of simulate.velocity_verlet (jax_md/simulate.py:227-239):
+ *   p' = p + half_dt * F;  R' = shift(R, dt * p' / m)
+ * with the position accumulated in float64 (position_hi [n, dim]) and
+ * mirrored to the float32 position_out [n, dim] (row stride
+ * position_stride).  Outputs may alias the
+ * matching inputs (in-place step).  mass: NULL for the scalar mass_scalar,
+ * else device [n].  The shift wraps into the box with jnp.mod semantics when
+ * periodic (space.py:250-252).  When reference_position and gate are
+ * non-NULL the skin criterion of jmd_neighbor_list_check is evaluated on the
+ * new positions in the same pass (gate raised when any particle moved more
+ * than sqrt(threshold_sq) since the last build).  dt_device (device float32
+ * [1], may be NULL) overrides dt, with half_dt = float32(dt / 2) as in
+ * velocity_verlet (simulate.py:235-236); FIRE keeps its dt on the device. */
+int jmd_nve_kick_drift(const double* position_hi, const float* momentum,
+                       const float* force, const float* mass,
+                       float mass_scalar, int64_t n, int32_t dim,
+                       int32_t periodic, const float* side, float dt,
+                       float half_dt, const float* dt_device,
+                       double* position_hi_out, float* position_out,
+                       int32_t position_stride, float* momentum_out,
+                       const float* reference_position, float threshold_sq,
+                       int32_t* gate, void* stream);
+
+/* FIRE bookkeeping after the NVE step of minimize.fire_descent.apply_fn
+ * (jax_md/minimize.py:184-224).  scalars (device float64 [3]) receives
+ * sum F^2, sum P^2 and sum F.P (float64 sums).  Then the one-block update
+ * applies, in the reference's order and with float16 dt / alpha arithmetic
+ * (util.static_cast, util.py:58-64):
+ *   P += alpha (F |P| / |F| - P);  n_pos, dt, alpha update;  P *= (F.P >= 0)
+ * fire_state (device): float32 [dt, alpha] holding float16-representable
+ * values, int32 n_pos at fire_n_pos.  params (host float32 [7]):
+ * dt_start, dt_max, n_min, f_inc, f_dec, alpha_start, f_alpha, already
+ * rounded to their static_cast dtypes. */
+int jmd_fire_update(float* momentum, const float* force, int64_t n,
+                    int32_t dim, float* fire_state, int32_t* fire_n_pos,
+                    const float* params, double* scalars, void* workspace,
+                    void* stream);
+
+/* Scratch bytes for jmd_fire_update. */
+size_t jmd_fire_workspace_bytes(int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JAXMD_NATIVE_H_ */

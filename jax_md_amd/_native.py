@@ -1,0 +1,164 @@
+"""ctypes binding of libjaxmd_native.so (the C ABI in include/jaxmd_native.h).
+
+The library is the only compute path of this package: when it is missing,
+every entry point raises ``NativeLibraryError`` instead of falling back to
+any other implementation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint8,
+                    c_void_p, c_char_p)
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libjaxmd_native.so')
+
+JMD_OK, JMD_ERR_INVALID, JMD_ERR_LAUNCH, JMD_ERR_UNSUPPORTED = 0, 1, 2, 3
+DENSE, SPARSE, ORDERED_SPARSE = 0, 1, 2
+LENNARD_JONES, SOFT_SPHERE = 0, 1
+PARAM_SCALAR, PARAM_PER_PARTICLE, PARAM_SPECIES = 0, 1, 2
+
+
+class NativeLibraryError(RuntimeError):
+  pass
+
+
+class JmdParam(ctypes.Structure):
+  _fields_ = [('mode', c_int32), ('value', c_float), ('values', c_void_p)]
+
+
+class JmdPairPotential(ctypes.Structure):
+  _fields_ = [('kind', c_int32), ('sigma', JmdParam), ('epsilon', JmdParam),
+              ('alpha', JmdParam), ('r_onset', c_float), ('r_cutoff', c_float),
+              ('n_species', c_int32), ('species', c_void_p)]
+
+
+_F3 = c_float * 3
+_I3 = c_int32 * 3
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    'jmd_version': (c_char_p, []),
+    'jmd_neighbor_workspace_bytes': (c_size_t, [c_int64, c_int32, c_int64, c_int32]),
+    'jmd_cell_list': (c_int32, [c_void_p, c_int64, c_int32, c_int32, POINTER(c_int32),
+                                POINTER(c_float), c_int32, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p]),
+    'jmd_cell_filling_max': (c_int32, [c_void_p, c_int64, c_int32, c_int32, POINTER(c_int32),
+                                       POINTER(c_float), c_void_p, c_void_p, c_void_p]),
+    'jmd_neighbor_list_build': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
+                                          POINTER(c_float), c_float, c_int32, POINTER(c_int32),
+                                          POINTER(c_float), c_int32, c_int32, c_int32, c_void_p,
+                                          c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p]),
+    'jmd_neighbor_list_check': (c_int32, [c_void_p, c_int32, c_void_p, c_int64, c_int32,
+                                          c_int32, POINTER(c_float), c_float, c_void_p,
+                                          c_void_p]),
+    'jmd_pair_workspace_bytes': (c_size_t, [c_int64]),
+    'jmd_set_threads_per_atom': (c_int32, [c_int32]),
+    'jmd_pair_energy_force': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
+                                        POINTER(c_float), POINTER(JmdPairPotential), c_void_p,
+                                        c_int64, c_int32, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_float, c_void_p, c_void_p, c_void_p]),
+    'jmd_nve_kick_drift': (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int64,
+                                     c_int32, c_int32, POINTER(c_float), c_float, c_float,
+                                     c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+                                     c_void_p, c_float, c_void_p, c_void_p]),
+    'jmd_fire_workspace_bytes': (c_size_t, [c_int64]),
+    'jmd_fire_update': (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p,
+                                  POINTER(c_float), c_void_p, c_void_p, c_void_p]),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+  """Loads the library once; raises NativeLibraryError when it is absent."""
+  global _lib
+  if _lib is not None:
+    return _lib
+  if not os.path.exists(path):
+    raise NativeLibraryError(
+        f'{path} is missing: build it with `python -m jax_md_amd.build_native` '
+        '(the package has no other compute path).')
+  lib = ctypes.CDLL(path)
+  for name, (res, args) in _SIGNATURES.items():
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = args
+  _lib = lib
+  return lib
+
+
+def check(status: int, what: str):
+  if status != JMD_OK:
+    kind = {JMD_ERR_INVALID: 'invalid argument', JMD_ERR_LAUNCH: 'HIP launch failure',
+            JMD_ERR_UNSUPPORTED: 'unsupported request'}.get(status, f'status {status}')
+    raise NativeLibraryError(f'{what} failed: {kind}')
+
+
+def ptr(t):
+  """Device pointer of a tensor (None -> NULL)."""
+  if t is None:
+    return None
+  return c_void_p(t.data_ptr())
+
+
+def stream_handle(device=None):
+  return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def f3(vals):
+  a = _F3()
+  for k, v in enumerate(vals):
+    a[k] = float(v)
+  return a
+
+
+def i3(vals):
+  a = _I3()
+  for k, v in enumerate(vals):
+    a[k] = int(v)
+  return a
+
+
+def position_stride(t, name='position'):
+  """Row stride of a position array: [N, dim] contiguous, or the [:, :3]
+  view of a 16-byte-aligned [N, 4] buffer."""
+  require_device_tensor(t, name, torch.float32, contiguous=False)
+  if t.ndim != 2 or t.shape[1] not in (2, 3):
+    raise ValueError(f'{name} must be [N, 2] or [N, 3] float32, got {tuple(t.shape)}')
+  dim = t.shape[1]
+  if t.shape[0] <= 1:
+    return dim
+  rs, cs = t.stride()
+  if cs == 1 and (rs == dim or (dim == 3 and rs == 4 and t.data_ptr() % 16 == 0)):
+    return rs
+  raise ValueError(f'{name} must be contiguous or a [:, :3] view of an [N, 4] buffer')
+
+
+def padded_positions(N, device, like=None):
+  """[N, 3] float32 view of a zeroed [N, 4] buffer (16-byte rows)."""
+  buf = torch.zeros((N, 4), dtype=torch.float32, device=device)
+  v = buf[:, :3]
+  if like is not None:
+    v.copy_(like)
+  return v
+
+
+def require_device_tensor(t, name, dtype=None, contiguous=True):
+  if not isinstance(t, torch.Tensor):
+    raise TypeError(f'{name} must be a torch.Tensor on a ROCm device, got {type(t)}')
+  if t.device.type != 'cuda':
+    raise NativeLibraryError(
+        f'{name} is on {t.device}; the native path runs on the GPU only')
+  if dtype is not None and t.dtype != dtype:
+    raise TypeError(f'{name} must be {dtype}, got {t.dtype}')
+  if contiguous and not t.is_contiguous():
+    raise ValueError(f'{name} must be contiguous')
+  return t

@@ -1,0 +1,216 @@
+// Shared device helpers for the gfx950 neighbour-list MD kernels.
+//
+// Geometry follows jax_md/space.py: the periodic displacement is
+// mod(d + L/2, L) - L/2 with jnp.mod's floor semantics (space.py:213-224),
+// the square distance is the left-to-right sum of squares (space.py:227-235).
+// The neighbour-list build must reproduce those float32 roundings exactly,
+// so the "ref_" helpers below forbid multiply-add contraction.  The force
+// kernels use a minimum-image difference that avoids the rounding of the
+// wrap (see min_image_disp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/jaxmd_native.h"
+
+#define JMD_RETURN_IF_HIP(expr)                          \
+  do {                                                   \
+    hipError_t _e = (expr);                              \
+    if (_e != hipSuccess) return JMD_ERR_LAUNCH;         \
+  } while (0)
+
+#define JMD_LAUNCH_CHECK() JMD_RETURN_IF_HIP(hipGetLastError())
+
+namespace jmd {
+
+constexpr int kWave = 64;
+
+// Geometry passed by value to kernels.
+struct Space {
+  int dim;
+  int periodic;
+  float L[3];
+  float half[3];  // float32(L) * 0.5, exact
+};
+
+inline Space make_space(int dim, int periodic, const float* side) {
+  Space s;
+  s.dim = dim;
+  s.periodic = periodic;
+  for (int d = 0; d < 3; ++d) {
+    float L = (side && d < dim) ? side[d] : 1.0f;
+    s.L[d] = L;
+    s.half[d] = L * 0.5f;
+  }
+  return s;
+}
+
+// Cell grid (jax_md/partition.py:146-188,211-223).
+struct Grid {
+  int cps[3];          // cells per side, x fastest in the hash
+  float cell_size[3];  // float32 box / cells_per_side
+  int ncell;
+  int cap;             // reference cell capacity (slot rotation, overflow)
+};
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// jnp.mod(t, L) - half for float32, bit-exact with XLA's rem + sign fix.
+// fmod(t, L) is exact; the common ranges are handled without the libcall.
+__device__ __forceinline__ float ref_wrap1(float d, float L, float half) {
+#pragma clang fp contract(off)
+  float t = d + half;
+  float r;
+  if (t >= 0.0f && t < L) {
+    r = t;
+  } else if (t >= L && t < 2.0f * L) {
+    r = t - L;  // exact (Sterbenz)
+  } else if (t < 0.0f && t > -L) {
+    r = t;
+  } else {
+    r = fmodf(t, L);
+  }
+  if (r != 0.0f && r < 0.0f) r = r + L;
+  return r - half;
+}
+
+// Reference square distance of disp(a, b) = wrap(a - b) in float32.
+template <int DIM>
+__device__ __forceinline__ float ref_metric_sq(const float* a, const float* b,
+                                               const Space& s) {
+#pragma clang fp contract(off)
+  float acc = 0.0f;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    float v = a[d] - b[d];
+    if (s.periodic) v = ref_wrap1(v, s.L[d], s.half[d]);
+    float sq = v * v;
+    acc = (d == 0) ? sq : acc + sq;
+  }
+  return acc;
+}
+
+// Minimum-image displacement x_j - x_i for positions in [0, L), used by the
+// pair kernels (tolerance parity only).  A plain x_j - x_i rounds at the scale
+// of L when the pair straddles the boundary (16.5 - 0.3 loses ~1e-6), which
+// the r^-13 of LJ amplifies.  xim = x_i - L is exact (Sterbenz, x_i >= L/2)
+// and precomputed per row; x_j - L is exact for x_j >= L/2; both crossing
+// forms then round once at the scale of the small result.
+__device__ __forceinline__ float min_image_disp(float xj, float xi, float xim, float L,
+                                                float half) {
+#pragma clang fp contract(off)
+  float raw = xj - xi;
+  float lo = xj - xim;         // x_j near 0, x_i near L
+  float hi = (xj - L) - xi;    // x_j near L, x_i near 0
+  float d = raw < -half ? lo : raw;
+  return raw >= half ? hi : d;
+}
+
+// jnp.mod(x, L) for the position update (space.py:250-252), float64.
+__device__ __forceinline__ double ref_mod64(double x, double L) {
+  double r = fmod(x, L);
+  if (r != 0.0 && r < 0.0) r = r + L;
+  return r;
+}
+
+__device__ __forceinline__ int floor_mod_int(int a, int m) {
+  int r = a % m;
+  return r < 0 ? r + m : r;
+}
+
+// Cell coordinates of a particle: mod(int32(x / cell_size), cps)
+// (partition.py:421-422), with IEEE division as XLA emits it.
+template <int DIM>
+__device__ __forceinline__ int cell_of(const float* x, const Grid& g, int* ci) {
+  int h = 0, mult = 1;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    float q = __fdiv_rn(x[d], g.cell_size[d]);
+    int iq = (int)q;  // truncation toward zero, as convert_element_type
+    int c = floor_mod_int(iq, g.cps[d]);
+    ci[d] = c;
+    h += c * mult;
+    mult *= g.cps[d];
+  }
+  return h;
+}
+
+// Position rows have a stride of STRIDE floats (DIM, or 4 for the padded
+// [N, 4] layout whose rows are single 16-byte loads).
+template <int DIM, int STRIDE = DIM>
+__device__ __forceinline__ void load_pos(const float* __restrict__ p, long long i, float* x) {
+  if (DIM == 3 && STRIDE == 4) {
+    float4 v = *reinterpret_cast<const float4*>(p + i * 4);
+    x[0] = v.x;
+    x[1] = v.y;
+    x[2] = v.z;
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) x[d] = p[i * STRIDE + d];
+  }
+}
+
+template <int DIM, int STRIDE = DIM>
+__device__ __forceinline__ void store_pos(float* __restrict__ p, long long i, const float* x) {
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) p[i * STRIDE + d] = x[d];
+}
+
+// Valid position strides: dim itself, or 4 for 3-D (padded rows, 16-byte
+// aligned base).
+inline bool stride_ok(int dim, int stride, const void* base) {
+  if (stride == dim) return true;
+  return dim == 3 && stride == 4 && ((uintptr_t)base & 15) == 0;
+}
+
+// Wavefront helpers (64 lanes).
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int wave_prefix_count(unsigned long long mask) {
+  // Number of set bits in mask below this lane (mbcnt).
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_reduce_max(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    T o = __shfl_xor(v, off, kWave);
+    v = v > o ? v : o;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_reduce_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Fill kernel used instead of hipMemsetAsync (safe under graph capture).
+template <typename T>
+__global__ void fill_kernel(T* __restrict__ p, long long n, T v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+inline int launch_fill(T* p, long long n, T v, hipStream_t st) {
+  if (n <= 0) return JMD_OK;
+  int blocks = ceil_div(n, 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(blocks), dim3(256), 0, st, p, n, v);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+// Gate: rebuild work runs only when *gate != 0 (or gate == nullptr).
+__device__ __forceinline__ bool gate_closed(const int* gate) {
+  return gate != nullptr && *gate == 0;
+}
+
+}  // namespace jmd

@@ -1,0 +1,872 @@
+// Cell binning and neighbour-list build for gfx950.
+//
+// Reference: jax_md/partition.py (cell_list_fn :349-471, neighbor_list
+// :801-1164).  The reference materialises [N, 3^d * cap] candidate arrays and
+// compacts them with cumsum + scatter; here the candidates never leave LDS:
+//
+//   1. bin:     cell hash per particle, counts by atomics   (k_bin_count)
+//   2. scan:    exclusive scan of the counts in one workgroup, max filling,
+//               cell-overflow bit                           (k_bin_scan)
+//   3. scatter: particle -> cell-ordered slot               (k_bin_scatter)
+//   4. order:   one wavefront per cell sorts its members by id, so the
+//               within-cell order equals the reference's stable argsort and
+//               the build is deterministic                   (k_cell_order)
+//   5. build:   one workgroup per home cell stages the 3^d stencil cells'
+//               positions/ids in LDS in the reference's candidate order
+//               (stencil offsets with the first component slowest, slots
+//               rotated by start mod cap), each wavefront takes one home
+//               particle at a time, lanes test 64 candidates at once, rows
+//               are compacted with ballot + mbcnt            (k_nl_cells)
+//   6. finish:  error bits, occupancy, gate reset            (k_nl_finish)
+//
+// The distance tests reproduce the reference's float32 arithmetic exactly
+// (ref_metric_sq), so the neighbour sets and overflow bits are bit-exact.
+#include "jmd_common.hpp"
+
+namespace jmd {
+
+// misc words inside the workspace
+enum {
+  M_OCC = 0,        // max row count (Dense) / total pairs (Sparse)
+  M_FILL = 1,       // max cell filling
+  M_FLAGS = 2,      // error bits raised by kernels
+  M_TOTAL = 3,      // Sparse total (scan result)
+  M_BUILDS = 4,     // completed gated/ungated builds (never reset)
+  M_WORDS = 16
+};
+
+struct Work {
+  int* cell_count;   // [ncell]   (zero at rest)
+  int* cell_start;   // [ncell+1]
+  int* atom_cell;    // [n]
+  int* atom_rank;    // [n]
+  int* sorted_tmp;   // [n]
+  int* sorted_id;    // [n]
+  float4* sorted_pos;// [n]
+  int* row_count;    // [n]
+  int* row_offset;   // [n+1]
+  int* misc;         // [M_WORDS] (zero at rest except as documented)
+};
+
+static inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+static size_t carve(void* base, long long n, long long ncell, Work* w) {
+  size_t off = 0;
+  char* b = (char*)base;
+  auto take = [&](size_t bytes) -> char* {
+    char* p = b ? b + off : nullptr;
+    off += align_up(bytes);
+    return p;
+  };
+  long long nc = ncell > 0 ? ncell : 1;
+  Work t;
+  t.misc = (int*)take(sizeof(int) * M_WORDS);
+  t.cell_count = (int*)take(sizeof(int) * nc);
+  t.cell_start = (int*)take(sizeof(int) * (nc + 1));
+  t.atom_cell = (int*)take(sizeof(int) * n);
+  t.atom_rank = (int*)take(sizeof(int) * n);
+  t.sorted_tmp = (int*)take(sizeof(int) * n);
+  t.sorted_id = (int*)take(sizeof(int) * n);
+  t.sorted_pos = (float4*)take(sizeof(float4) * n);
+  t.row_count = (int*)take(sizeof(int) * n);
+  t.row_offset = (int*)take(sizeof(int) * (n + 1));
+  if (w) *w = t;
+  return off;
+}
+
+// ---------------------------------------------------------------------------
+// Binning
+// ---------------------------------------------------------------------------
+
+template <int DIM, int STRIDE>
+__global__ __launch_bounds__(256) void k_bin_count(
+    const float* __restrict__ pos, int n, Grid g, int* __restrict__ cell_count,
+    int* __restrict__ atom_cell, int* __restrict__ atom_rank,
+    float* __restrict__ ref_out, int* __restrict__ cell_id_out, const int* gate) {
+  if (gate_closed(gate)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float x[3];
+  load_pos<DIM, STRIDE>(pos, i, x);
+  int ci[3];
+  int h = cell_of<DIM>(x, g, ci);
+  atom_cell[i] = h;
+  atom_rank[i] = atomicAdd(&cell_count[h], 1);
+  if (ref_out) store_pos<DIM>(ref_out, i, x);
+  if (cell_id_out) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) cell_id_out[(long long)i * DIM + d] = ci[d];
+  }
+}
+
+// One workgroup of 1024 threads: exclusive scan of cell_count into
+// cell_start[0..ncell], max filling, and cell_count reset to zero.
+__global__ __launch_bounds__(1024) void k_bin_scan(int* __restrict__ cell_count,
+                                                   int* __restrict__ cell_start,
+                                                   int ncell, int cap,
+                                                   int* __restrict__ misc,
+                                                   const int* gate) {
+  if (gate_closed(gate)) return;
+  __shared__ int s_sum[1024];
+  __shared__ int s_max[1024 / kWave];
+  int t = threadIdx.x;
+  int per = (ncell + 1023) / 1024;
+  int beg = t * per;
+  int end = min(beg + per, ncell);
+  int local = 0, lmax = 0;
+  for (int c = beg; c < end; ++c) {
+    int v = cell_count[c];
+    local += v;
+    lmax = max(lmax, v);
+  }
+  s_sum[t] = local;
+  int wmax = wave_reduce_max(lmax);
+  if (lane_id() == 0) s_max[t / kWave] = wmax;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over 1024 partial sums.
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (t >= off) ? s_sum[t - off] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  int run = (t == 0) ? 0 : s_sum[t - 1];
+  for (int c = beg; c < end; ++c) {
+    int v = cell_count[c];
+    cell_start[c] = run;
+    run += v;
+    cell_count[c] = 0;
+  }
+  if (t == 1023) cell_start[ncell] = s_sum[1023];
+  if (t == 0) {
+    int m = 0;
+    for (int k = 0; k < 1024 / kWave; ++k) m = max(m, s_max[k]);
+    misc[M_FILL] = m;
+    if (cap > 0 && m > cap) atomicOr(&misc[M_FLAGS], JMD_CELL_LIST_OVERFLOW);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(
+    int n, const int* __restrict__ atom_cell, const int* __restrict__ atom_rank,
+    const int* __restrict__ cell_start, int* __restrict__ sorted_tmp, const int* gate) {
+  if (gate_closed(gate)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sorted_tmp[cell_start[atom_cell[i]] + atom_rank[i]] = i;
+}
+
+// One wavefront per cell: order members by id (the stable argsort of
+// partition.py:432 keeps ids ascending inside a cell), gather positions.
+// sorted_pos holds each position moved by the whole box periods implied by
+// its cell assignment (x - L * floor(int(x / cs) / cps)), so that positions
+// of a cell lie within that cell's range; the raw input stays untouched.
+template <int DIM, int STRIDE>
+__global__ __launch_bounds__(256) void k_cell_order(
+    const float* __restrict__ pos, int ncell, Grid g, Space s,
+    const int* __restrict__ cell_start, const int* __restrict__ sorted_tmp,
+    int* __restrict__ sorted_id, float4* __restrict__ sorted_pos, const int* gate) {
+  if (gate_closed(gate)) return;
+  int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  int lane = lane_id();
+  if (wave >= ncell) return;
+  int beg = cell_start[wave];
+  int cnt = cell_start[wave + 1] - beg;
+  for (int a = lane; a < cnt; a += kWave) {
+    int id = sorted_tmp[beg + a];
+    int rank = 0;
+    for (int b = 0; b < cnt; ++b) rank += (sorted_tmp[beg + b] < id) ? 1 : 0;
+    sorted_id[beg + rank] = id;
+    float x[3] = {0.0f, 0.0f, 0.0f};
+    load_pos<DIM, STRIDE>(pos, id, x);
+    if (s.periodic) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        int q = (int)__fdiv_rn(x[d], g.cell_size[d]);
+        int wraps = (q - floor_mod_int(q, g.cps[d])) / g.cps[d];
+        if (wraps != 0) x[d] = x[d] - (float)wraps * s.L[d];
+      }
+    }
+    sorted_pos[beg + rank] = make_float4(x[0], x[1], x[2], 0.0f);
+  }
+}
+
+// Public cell-list buffers (partition.py:441-456): slot of the k-th particle
+// in (cell, id) order is hash*cap + (k mod cap).
+template <int DIM, int STRIDE>
+__global__ __launch_bounds__(256) void k_cell_emplace(
+    const float* __restrict__ pos, int n, int cap, const int* __restrict__ cell_start,
+    const int* __restrict__ sorted_id, int ncell, float* __restrict__ pos_buf,
+    int* __restrict__ id_buf) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  // cell of sorted entry k: binary search over cell_start
+  int lo = 0, hi = ncell;  // cell_start[lo] <= k < cell_start[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (cell_start[mid] <= k) lo = mid; else hi = mid;
+  }
+  long long slot = (long long)lo * cap + (k % cap);
+  int id = sorted_id[k];
+  id_buf[slot] = id;
+  float x[3];
+  load_pos<DIM, STRIDE>(pos, id, x);
+  store_pos<DIM>(pos_buf, slot, x);
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-list build over the cell grid
+// ---------------------------------------------------------------------------
+
+enum { MODE_DENSE = 0, MODE_SPARSE_COUNT = 1, MODE_SPARSE_WRITE = 2, MODE_COUNT_MAX = 3 };
+enum { TEST_IJ = 0, TEST_JI = 1, TEST_BOTH = 2 };
+
+constexpr int kStage = 2048;   // candidates staged in LDS per pass
+constexpr int kHomeGroup = 512;
+
+template <int DIM>
+__device__ __forceinline__ bool pair_test(const float* xi, const float* xj,
+                                          const Space& s, float cutoff_sq,
+                                          float margin, int test) {
+  if (test == TEST_IJ) return ref_metric_sq<DIM>(xi, xj, s) < cutoff_sq;
+  if (test == TEST_JI) return ref_metric_sq<DIM>(xj, xi, s) < cutoff_sq;
+  // Both directions (Dense with a cell list, partition.py:951,971).  The two
+  // square distances differ by a few float32 roundings, so with margin > 0 the
+  // second one is only evaluated when the first lies within `margin` of the
+  // cutoff; margin = 0 always evaluates both.
+  float a = ref_metric_sq<DIM>(xi, xj, s);
+  bool pa = a < cutoff_sq;
+  if (margin > 0.0f && fabsf(a - cutoff_sq) > margin) return pa;
+  float b = ref_metric_sq<DIM>(xj, xi, s);
+  return pa && (b < cutoff_sq);
+}
+
+// Candidates are staged as float4 (x, y, z, id bits) holding the periodic
+// image nearest to the home cell: the shift of each stencil cell (0 or +-L
+// per dimension) follows from the cell topology.  The common test is then a
+// plain |x_i - x_j'|^2 against c^2 -/+ margin, where margin bounds the
+// difference between this value and the reference's wrapped float32 d^2;
+// only candidates within the margin run the reference-exact test on the
+// original positions (pair_test), so sets and order equal the reference's.
+template <int DIM, int STRIDE, int MODE>
+__global__ __launch_bounds__(256) void k_nl_cells(
+    const float* __restrict__ pos, const float4* __restrict__ sorted_pos,
+    const int* __restrict__ sorted_id, const int* __restrict__ cell_start, Grid g, Space s,
+    float cutoff_sq, float margin, int test, int mask_self, int ordered, int n,
+    int* __restrict__ idx, long long width, int* __restrict__ row_count,
+    const int* __restrict__ row_offset, int* __restrict__ misc, const int* gate) {
+  if (gate_closed(gate)) return;
+  constexpr int NST = DIM == 3 ? 27 : 9;
+  __shared__ int s_cs[NST], s_cc[NST], s_off[NST + 1];
+  __shared__ float s_shift[NST][3];
+  __shared__ float4 s_pos[kStage];
+  __shared__ int s_cnt[kHomeGroup];
+
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = tid / kWave;
+  const int nwave = blockDim.x / kWave;
+  const float lo_thr = cutoff_sq - margin, hi_thr = cutoff_sq + margin;
+  bool mi[3];  // dimensions where the staged image may not be the nearest
+#pragma unroll
+  for (int d = 0; d < 3; ++d) mi[d] = s.periodic && d < DIM && g.cps[d] < 5;
+
+  for (int home = blockIdx.x; home < g.ncell; home += gridDim.x) {
+    __syncthreads();  // LDS reuse across home cells
+    if (tid < NST) {
+      int hc[3], h = home;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) { hc[d] = h % g.cps[d]; h /= g.cps[d]; }
+      int off[3];
+      int r = tid;
+#pragma unroll
+      for (int d = DIM - 1; d >= 0; --d) { off[d] = r % 3 - 1; r /= 3; }
+      int c = 0, mult = 1;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        float sh = 0.0f;
+        if (d < DIM) {
+          int raw = hc[d] + off[d];
+          c += floor_mod_int(raw, g.cps[d]) * mult;
+          mult *= g.cps[d];
+          if (s.periodic) sh = raw < 0 ? -s.L[d] : (raw >= g.cps[d] ? s.L[d] : 0.0f);
+        }
+        s_shift[tid][d] = sh;
+      }
+      s_cs[tid] = cell_start[c];
+      s_cc[tid] = cell_start[c + 1] - s_cs[tid];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int t = 0; t < NST; ++t) { s_off[t] = acc; acc += s_cc[t]; }
+      s_off[NST] = acc;
+    }
+    __syncthreads();
+    const int total = s_off[NST];
+    const int hstart = s_cs[NST / 2];
+    const int hcount = s_cc[NST / 2];
+    const int cap = g.cap;
+
+    for (int g0 = 0; g0 < hcount; g0 += kHomeGroup) {
+      const int gcount = min(kHomeGroup, hcount - g0);
+      for (int a = tid; a < gcount; a += blockDim.x) s_cnt[a] = 0;
+      for (int c0 = 0; c0 < total; c0 += kStage) {
+        const int ccount = min(kStage, total - c0);
+        __syncthreads();
+        // Stage candidates c0..c0+ccount in reference order.
+        for (int q = tid; q < ccount; q += blockDim.x) {
+          int p = c0 + q;
+          int t = 0;
+          while (t + 1 < NST && s_off[t + 1] <= p) ++t;
+          int m = p - s_off[t];
+          int cnt = s_cc[t];
+          int k = m;
+          if (cap > 0 && cnt <= cap) {
+            int w = cap - (s_cs[t] % cap);
+            if (cnt > w) k = (m < cnt - w) ? (w + m) : (m - (cnt - w));
+          }
+          int gi = s_cs[t] + k;
+          float4 xp = sorted_pos[gi];
+          s_pos[q] = make_float4(xp.x + s_shift[t][0], xp.y + s_shift[t][1],
+                                 xp.z + s_shift[t][2], __int_as_float(sorted_id[gi]));
+        }
+        __syncthreads();
+        for (int a = wave; a < gcount; a += nwave) {
+          const int hi = hstart + g0 + a;
+          const float4 xp = sorted_pos[hi];
+          const int i = sorted_id[hi];
+          const float xi[3] = {xp.x, xp.y, xp.z};
+          float xo[3] = {0.0f, 0.0f, 0.0f};  // original position for exact tests
+          load_pos<DIM, STRIDE>(pos, i, xo);
+          int cnt = s_cnt[a];
+          long long base = 0;
+          if (MODE == MODE_DENSE) base = (long long)i * width;
+          if (MODE == MODE_SPARSE_WRITE) base = row_offset[i];
+          for (int q0 = 0; q0 < ccount; q0 += kWave) {
+            int q = q0 + lane;
+            bool pass = false;
+            int j = n;
+            if (q < ccount) {
+              float4 c = s_pos[q];
+              j = __float_as_int(c.w);
+              const float cj[3] = {c.x, c.y, c.z};
+              float d[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+              for (int e = 0; e < DIM; ++e)
+                d[e] = mi[e] ? -min_image_disp(cj[e], xi[e], xi[e] - s.L[e], s.L[e], s.half[e])
+                             : xi[e] - cj[e];
+              float d2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                                  : fmaf(d[1], d[1], d[0] * d[0]);
+              bool ok = (!mask_self || j != i) && (!ordered || j < i);
+              bool near = d2 >= lo_thr && d2 <= hi_thr;
+              pass = ok && d2 < lo_thr;
+              if (ok && near) {
+                float xj[3] = {0.0f, 0.0f, 0.0f};
+                load_pos<DIM, STRIDE>(pos, j, xj);
+                pass = pair_test<DIM>(xo, xj, s, cutoff_sq, 0.0f, test);
+              }
+            }
+            unsigned long long bal = __ballot(pass);
+            if ((MODE == MODE_DENSE || MODE == MODE_SPARSE_WRITE) && pass) {
+              long long slot = cnt + wave_prefix_count(bal);
+              if (MODE == MODE_DENSE) {
+                if (slot < width) idx[base + slot] = j;
+              } else {
+                long long at = base + slot;
+                if (at < width) {
+                  idx[at] = j;          // receiver
+                  idx[width + at] = i;  // sender
+                }
+              }
+            }
+            cnt += __popcll(bal);
+          }
+          if (lane == 0) s_cnt[a] = cnt;
+        }
+      }
+      __syncthreads();
+      // Row epilogue: padding (Dense), counts, occupancy.
+      for (int a = wave; a < gcount; a += nwave) {
+        const int hi = hstart + g0 + a;
+        const int i = sorted_id[hi];
+        const int cnt = s_cnt[a];
+        if (MODE == MODE_DENSE) {
+          for (long long k = (long long)cnt + lane; k < width; k += kWave)
+            idx[(long long)i * width + k] = n;
+          if (lane == 0) atomicMax(&misc[M_OCC], cnt);
+        } else if (MODE == MODE_COUNT_MAX) {
+          if (lane == 0) atomicMax(&misc[M_OCC], cnt);
+        } else if (MODE == MODE_SPARSE_COUNT) {
+          if (lane == 0) row_count[i] = cnt;
+        }
+      }
+      __syncthreads();  // s_cnt is reused by the next home group
+    }
+  }
+}
+
+// Gated copy of the positions into reference_position (no-cell path).
+template <int DIM, int STRIDE>
+__global__ __launch_bounds__(256) void k_copy_gated(const float* __restrict__ src,
+                                                    float* __restrict__ dst, int n,
+                                                    const int* gate) {
+  if (gate_closed(gate)) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float x[3];
+    load_pos<DIM, STRIDE>(src, i, x);
+    store_pos<DIM>(dst, i, x);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-list build without a cell list (all-pairs candidates,
+// partition.py:904-909): one wavefront per row, candidates j in order.
+// ---------------------------------------------------------------------------
+
+template <int DIM, int STRIDE, int MODE>
+__global__ __launch_bounds__(256) void k_nl_allpairs(
+    const float* __restrict__ pos, int n, Space s, float cutoff_sq, int test,
+    int mask_self, int ordered, int* __restrict__ idx, long long width,
+    int* __restrict__ row_count, const int* __restrict__ row_offset,
+    int* __restrict__ misc, const int* gate) {
+  if (gate_closed(gate)) return;
+  int i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  int lane = lane_id();
+  if (i >= n) return;
+  float xi[3] = {0, 0, 0};
+  load_pos<DIM, STRIDE>(pos, i, xi);
+  int cnt = 0;
+  long long base = 0;
+  if (MODE == MODE_DENSE) base = (long long)i * width;
+  if (MODE == MODE_SPARSE_WRITE) base = row_offset[i];
+  for (int j0 = 0; j0 < n; j0 += kWave) {
+    int j = j0 + lane;
+    bool pass = false;
+    if (j < n) {
+      float xj[3] = {0, 0, 0};
+      load_pos<DIM, STRIDE>(pos, j, xj);
+      pass = (!mask_self || j != i) && (!ordered || j < i) &&
+             pair_test<DIM>(xi, xj, s, cutoff_sq, 0.0f, test);
+    }
+    unsigned long long bal = __ballot(pass);
+    if ((MODE == MODE_DENSE || MODE == MODE_SPARSE_WRITE) && pass) {
+      long long slot = cnt + wave_prefix_count(bal);
+      if (MODE == MODE_DENSE) {
+        if (slot < width) idx[base + slot] = j;
+      } else {
+        long long at = base + slot;
+        if (at < width) {
+          idx[at] = j;
+          idx[width + at] = i;
+        }
+      }
+    }
+    cnt += __popcll(bal);
+  }
+  if (MODE == MODE_DENSE) {
+    for (long long k = (long long)cnt + lane; k < width; k += kWave) idx[base + k] = n;
+    if (lane == 0) atomicMax(&misc[M_OCC], cnt);
+  } else if (MODE == MODE_COUNT_MAX) {
+    if (lane == 0) atomicMax(&misc[M_OCC], cnt);
+  } else if (MODE == MODE_SPARSE_COUNT) {
+    if (lane == 0) row_count[i] = cnt;
+  }
+}
+
+// Exclusive scan of row_count[0..n) into row_offset[0..n] in one workgroup
+// (Sparse formats: flattened row-major order, partition.py:1000-1005).
+__global__ __launch_bounds__(1024) void k_row_scan(const int* __restrict__ row_count,
+                                                   int* __restrict__ row_offset, int n,
+                                                   int* __restrict__ misc,
+                                                   const int* gate) {
+  if (gate_closed(gate)) return;
+  __shared__ long long s_sum[1024];
+  int t = threadIdx.x;
+  int per = (n + 1023) / 1024;
+  int beg = t * per;
+  int end = min(beg + per, n);
+  long long local = 0;
+  for (int c = beg; c < end; ++c) local += row_count[c];
+  s_sum[t] = local;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    long long v = (t >= off) ? s_sum[t - off] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  long long run = (t == 0) ? 0 : s_sum[t - 1];
+  const long long kMax = 0x7fffffffLL;
+  for (int c = beg; c < end; ++c) {
+    row_offset[c] = (int)min(run, kMax);
+    run += row_count[c];
+  }
+  if (t == 1023) {
+    row_offset[n] = (int)min(s_sum[1023], kMax);
+    misc[M_OCC] = (int)min(s_sum[1023], kMax);
+  }
+}
+
+// Sparse padding: idx[:, total:width] = n.
+__global__ __launch_bounds__(256) void k_sparse_pad(int* __restrict__ idx, long long width,
+                                                    int n, const int* __restrict__ misc,
+                                                    const int* gate) {
+  if (gate_closed(gate)) return;
+  long long total = misc[M_OCC];
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (k += total; k < width; k += stride) {
+    idx[k] = n;
+    idx[width + k] = n;
+  }
+}
+
+// Error bits, occupancy output, gate and counter reset (partition.py:1066,1110).
+__global__ void k_nl_finish(int* __restrict__ misc, int* __restrict__ occupancy,
+                            uint8_t* __restrict__ error, long long max_occupancy,
+                            int* gate) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (gate_closed(gate)) return;
+  int occ = misc[M_OCC];
+  int flags = misc[M_FLAGS];
+  if ((long long)occ > max_occupancy) flags |= JMD_NEIGHBOR_LIST_OVERFLOW;
+  if (occupancy) occupancy[0] = occ;
+  if (error) error[0] = (uint8_t)(error[0] | flags);
+  misc[M_OCC] = 0;
+  misc[M_FLAGS] = 0;
+  if (error) misc[M_BUILDS] += 1;
+  if (gate) gate[0] = 0;
+}
+
+// Skin criterion (partition.py:1141-1154).
+template <int DIM, int STRIDE>
+__global__ __launch_bounds__(256) void k_skin_check(const float* __restrict__ pos,
+                                                    const float* __restrict__ ref, int n,
+                                                    Space s, float threshold_sq,
+                                                    int* __restrict__ gate) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool moved = false;
+  if (i < n) {
+    float a[3], b[3];
+    load_pos<DIM, STRIDE>(pos, i, a);
+    load_pos<DIM>(ref, i, b);
+    moved = ref_metric_sq<DIM>(a, b, s) > threshold_sq;
+  }
+  unsigned long long bal = __ballot(moved);
+  if (bal != 0ull && lane_id() == 0) atomicOr(gate, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static bool make_grid(int dim, const int32_t* cps, const float* cell_size, int cap,
+                      Grid* g) {
+  long long nc = 1;
+  for (int d = 0; d < 3; ++d) {
+    g->cps[d] = d < dim ? cps[d] : 1;
+    g->cell_size[d] = d < dim ? cell_size[d] : 1.0f;
+    if (d < dim && (cps[d] < 1 || !(cell_size[d] > 0.0f))) return false;
+    nc *= g->cps[d];
+  }
+  if (nc > 0x7fffffffLL) return false;
+  g->ncell = (int)nc;
+  g->cap = cap;
+  return true;
+}
+
+static Space grid_space_free(int dim) {
+  float one[3] = {1.0f, 1.0f, 1.0f};
+  return make_space(dim, 0, one);
+}
+
+template <int DIM, int STRIDE>
+static int bin_and_order(const float* pos, int n, const Grid& g, const Space& s,
+                         const Work& w, float* ref_out, int* cell_id_out, const int* gate,
+                         hipStream_t st) {
+  int nb = ceil_div(n, 256);
+  if (n > 0) {
+    hipLaunchKernelGGL((k_bin_count<DIM, STRIDE>), dim3(nb), dim3(256), 0, st, pos, n, g,
+                       w.cell_count, w.atom_cell, w.atom_rank, ref_out, cell_id_out, gate);
+    JMD_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, w.cell_count, w.cell_start,
+                     g.ncell, g.cap, w.misc, gate);
+  JMD_LAUNCH_CHECK();
+  if (n > 0) {
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, n, w.atom_cell,
+                       w.atom_rank, w.cell_start, w.sorted_tmp, gate);
+    JMD_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL((k_cell_order<DIM, STRIDE>),
+                     dim3(ceil_div((long long)g.ncell * kWave, 256)), dim3(256), 0, st, pos,
+                     g.ncell, g, s, w.cell_start, w.sorted_tmp, w.sorted_id, w.sorted_pos, gate);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+constexpr int kCellBlocks = 2048;  // persistent grid of the cell build
+
+template <int DIM, int STRIDE>
+static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
+                      int use_cells, const Grid& g, int format, int mask_self,
+                      int* idx, long long width, int* occupancy, uint8_t* error,
+                      float* ref_out, int* gate, const Work& w, hipStream_t st) {
+  const bool sparse = format != JMD_DENSE;
+  const int ordered = format == JMD_ORDERED_SPARSE;
+  const bool count_only = error == nullptr;
+  int rc;
+  if (use_cells) {
+    rc = bin_and_order<DIM, STRIDE>(pos, n, g, s, w, ref_out, nullptr, gate, st);
+    if (rc) return rc;
+    // Bound on |fast d^2 - reference wrapped d^2| in float32: a few roundings
+    // at the scale of 2L on each of the components of d.
+    float Lmax = 0.0f;
+    for (int d = 0; d < DIM; ++d) Lmax = fmaxf(Lmax, s.periodic ? s.L[d] : 0.0f);
+    float c = sqrtf(cutoff_sq);
+    float margin = 128.0f * (c + 1.0f) * (Lmax + c + 1.0f) * 1.1920929e-7f;
+    margin = fmaxf(margin, 1e-4f * cutoff_sq);
+    const int test = sparse ? TEST_IJ : TEST_BOTH;
+    dim3 grid(g.ncell < kCellBlocks ? g.ncell : kCellBlocks), block(256);
+#define JMD_CELLS(MODE)                                                                      \
+  hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE>), grid, block, 0, st, pos, w.sorted_pos, \
+                     w.sorted_id, w.cell_start, g, s, cutoff_sq, margin, test, mask_self,    \
+                     ordered, n, idx, width, w.row_count, w.row_offset, w.misc, gate);       \
+  JMD_LAUNCH_CHECK();
+    if (!sparse && count_only) {
+      JMD_CELLS(MODE_COUNT_MAX)
+    } else if (!sparse) {
+      JMD_CELLS(MODE_DENSE)
+    } else {
+      JMD_CELLS(MODE_SPARSE_COUNT)
+      hipLaunchKernelGGL(k_row_scan, dim3(1), dim3(1024), 0, st, w.row_count, w.row_offset,
+                         n, w.misc, gate);
+      JMD_LAUNCH_CHECK();
+      if (!count_only) { JMD_CELLS(MODE_SPARSE_WRITE) }
+    }
+#undef JMD_CELLS
+  } else {
+    if (ref_out && n > 0) {
+      int nb = ceil_div(n, 256);
+      hipLaunchKernelGGL((k_copy_gated<DIM, STRIDE>), dim3(nb > 2048 ? 2048 : nb), dim3(256),
+                         0, st, pos, ref_out, n, gate);
+      JMD_LAUNCH_CHECK();
+    }
+    // Dense without cells tests wrap(Rj - Ri) (partition.py:965-971);
+    // Sparse tests wrap(Ri - Rj) (partition.py:990-995).
+    const int test = sparse ? TEST_IJ : TEST_JI;
+    dim3 grid(ceil_div((long long)n * kWave, 256)), block(256);
+#define JMD_ALLP(MODE)                                                                      \
+  hipLaunchKernelGGL((k_nl_allpairs<DIM, STRIDE, MODE>), grid, block, 0, st, pos, n, s,      \
+                     cutoff_sq, test, mask_self, ordered, idx, width, w.row_count,           \
+                     w.row_offset, w.misc, gate);                                            \
+  JMD_LAUNCH_CHECK();
+    if (n > 0) {
+      if (!sparse && count_only) {
+        JMD_ALLP(MODE_COUNT_MAX)
+      } else if (!sparse) {
+        JMD_ALLP(MODE_DENSE)
+      } else {
+        JMD_ALLP(MODE_SPARSE_COUNT)
+        hipLaunchKernelGGL(k_row_scan, dim3(1), dim3(1024), 0, st, w.row_count,
+                           w.row_offset, n, w.misc, gate);
+        JMD_LAUNCH_CHECK();
+        if (!count_only) { JMD_ALLP(MODE_SPARSE_WRITE) }
+      }
+    }
+#undef JMD_ALLP
+  }
+  if (count_only) {
+    // occupancy only: no error bits
+    hipLaunchKernelGGL(k_nl_finish, dim3(1), dim3(64), 0, st, w.misc, occupancy, nullptr,
+                       (long long)0x7fffffffffffLL, gate);
+    JMD_LAUNCH_CHECK();
+    return JMD_OK;
+  }
+  if (sparse) {
+    int pb = ceil_div(width, 256);
+    pb = pb > 4096 ? 4096 : pb + 1;
+    hipLaunchKernelGGL(k_sparse_pad, dim3(pb), dim3(256), 0, st, idx, width, n, w.misc, gate);
+    JMD_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_nl_finish, dim3(1), dim3(64), 0, st, w.misc, occupancy, error,
+                     width, gate);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+// Runs F<DIM, STRIDE>() for the supported (dim, stride) layouts.
+#define JMD_DISPATCH_LAYOUT(dim, stride, CALL3, CALL34, CALL2) \
+  ((dim) == 3 ? ((stride) == 4 ? (CALL34) : (CALL3)) : (CALL2))
+
+}  // namespace jmd
+
+using namespace jmd;
+
+extern "C" {
+
+const char* jmd_version(void) { return "jaxmd_native 0.2.0 (gfx950)"; }
+
+size_t jmd_neighbor_workspace_bytes(int64_t n, int32_t dim, int64_t cell_count,
+                                    int32_t format) {
+  (void)dim;
+  (void)format;
+  return carve(nullptr, n, cell_count, nullptr);
+}
+
+int jmd_cell_filling_max(const float* position, int64_t n, int32_t dim,
+                         int32_t position_stride, const int32_t* cells_per_side,
+                         const float* cell_size, int32_t* filling_max, void* workspace,
+                         void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || !cells_per_side || !cell_size ||
+      !filling_max || !workspace || (n > 0 && !position))
+    return JMD_ERR_INVALID;
+  if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  Grid g;
+  if (!make_grid(dim, cells_per_side, cell_size, 0, &g)) return JMD_ERR_INVALID;
+  Work w;
+  carve(workspace, n, g.ncell, &w);
+  hipStream_t st = (hipStream_t)stream;
+  int nn = (int)n;
+  if (nn > 0) {
+    dim3 grid(ceil_div(nn, 256)), block(256);
+    if (dim == 3 && position_stride == 4)
+      hipLaunchKernelGGL((k_bin_count<3, 4>), grid, block, 0, st, position, nn, g,
+                         w.cell_count, w.atom_cell, w.atom_rank, nullptr, nullptr, nullptr);
+    else if (dim == 3)
+      hipLaunchKernelGGL((k_bin_count<3, 3>), grid, block, 0, st, position, nn, g,
+                         w.cell_count, w.atom_cell, w.atom_rank, nullptr, nullptr, nullptr);
+    else
+      hipLaunchKernelGGL((k_bin_count<2, 2>), grid, block, 0, st, position, nn, g,
+                         w.cell_count, w.atom_cell, w.atom_rank, nullptr, nullptr, nullptr);
+    JMD_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, w.cell_count, w.cell_start,
+                     g.ncell, 0, w.misc, nullptr);
+  JMD_LAUNCH_CHECK();
+  JMD_RETURN_IF_HIP(hipMemcpyAsync(filling_max, w.misc + M_FILL, sizeof(int),
+                                   hipMemcpyDeviceToDevice, st));
+  return JMD_OK;
+}
+
+int jmd_cell_list(const float* position, int64_t n, int32_t dim, int32_t position_stride,
+                  const int32_t* cells_per_side, const float* cell_size,
+                  int32_t cell_capacity, float* position_buffer, int32_t* id_buffer,
+                  int32_t* particle_cell_id, int32_t* filling_max, uint8_t* overflow,
+                  void* workspace, void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || cell_capacity < 1 ||
+      !cells_per_side || !cell_size || !position_buffer || !id_buffer || !workspace ||
+      (n > 0 && (!position || !particle_cell_id)))
+    return JMD_ERR_INVALID;
+  if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  Grid g;
+  if (!make_grid(dim, cells_per_side, cell_size, cell_capacity, &g)) return JMD_ERR_INVALID;
+  Work w;
+  carve(workspace, n, g.ncell, &w);
+  hipStream_t st = (hipStream_t)stream;
+  int nn = (int)n;
+  long long slots = (long long)g.ncell * cell_capacity;
+  int rc = launch_fill<int>(id_buffer, slots, nn, st);
+  if (rc) return rc;
+  rc = launch_fill<float>(position_buffer, slots * dim, 0.0f, st);
+  if (rc) return rc;
+  Space s = grid_space_free(dim);
+  rc = JMD_DISPATCH_LAYOUT(
+      dim, position_stride,
+      (bin_and_order<3, 3>(position, nn, g, s, w, nullptr, particle_cell_id, nullptr, st)),
+      (bin_and_order<3, 4>(position, nn, g, s, w, nullptr, particle_cell_id, nullptr, st)),
+      (bin_and_order<2, 2>(position, nn, g, s, w, nullptr, particle_cell_id, nullptr, st)));
+  if (rc) return rc;
+  if (nn > 0) {
+    dim3 grid(ceil_div(nn, 256)), block(256);
+    if (dim == 3 && position_stride == 4)
+      hipLaunchKernelGGL((k_cell_emplace<3, 4>), grid, block, 0, st, position, nn,
+                         cell_capacity, w.cell_start, w.sorted_id, g.ncell, position_buffer,
+                         id_buffer);
+    else if (dim == 3)
+      hipLaunchKernelGGL((k_cell_emplace<3, 3>), grid, block, 0, st, position, nn,
+                         cell_capacity, w.cell_start, w.sorted_id, g.ncell, position_buffer,
+                         id_buffer);
+    else
+      hipLaunchKernelGGL((k_cell_emplace<2, 2>), grid, block, 0, st, position, nn,
+                         cell_capacity, w.cell_start, w.sorted_id, g.ncell, position_buffer,
+                         id_buffer);
+    JMD_LAUNCH_CHECK();
+  }
+  if (filling_max) {
+    JMD_RETURN_IF_HIP(hipMemcpyAsync(filling_max, w.misc + M_FILL, sizeof(int),
+                                     hipMemcpyDeviceToDevice, st));
+  }
+  // overflow byte from the scan's flag word, then clear the flag word
+  hipLaunchKernelGGL(k_nl_finish, dim3(1), dim3(64), 0, st, w.misc, nullptr, overflow,
+                     (long long)0x7fffffffffffLL, nullptr);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+int jmd_neighbor_list_build(const float* position, int64_t n, int32_t dim,
+                            int32_t position_stride, int32_t periodic, const float* side,
+                            float cutoff_sq, int32_t use_cell_list,
+                            const int32_t* cells_per_side, const float* cell_size,
+                            int32_t cell_capacity, int32_t format, int32_t mask_self,
+                            int32_t* idx, int64_t max_occupancy, int32_t* occupancy,
+                            uint8_t* error, float* reference_position, int32_t* gate,
+                            void* workspace, void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || max_occupancy < 0 ||
+      (format != JMD_DENSE && format != JMD_SPARSE && format != JMD_ORDERED_SPARSE) ||
+      !workspace || (n > 0 && !position) || (periodic && !side))
+    return JMD_ERR_INVALID;
+  if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  if (error == nullptr && !occupancy) return JMD_ERR_INVALID;
+  if (error != nullptr && max_occupancy > 0 && !idx) return JMD_ERR_INVALID;
+  if (format == JMD_DENSE && (long long)n * max_occupancy > 0x7fffffffffffLL)
+    return JMD_ERR_INVALID;
+  Grid g{};
+  if (use_cell_list) {
+    if (!cells_per_side || !cell_size || cell_capacity < 1) return JMD_ERR_INVALID;
+    if (!make_grid(dim, cells_per_side, cell_size, cell_capacity, &g)) return JMD_ERR_INVALID;
+    for (int d = 0; d < dim; ++d)
+      if (cells_per_side[d] < 3) return JMD_ERR_INVALID;
+  }
+  Space s = make_space(dim, periodic, side);
+  Work w;
+  carve(workspace, n, use_cell_list ? g.ncell : 1, &w);
+  hipStream_t st = (hipStream_t)stream;
+  int nn = (int)n;
+  return JMD_DISPATCH_LAYOUT(
+      dim, position_stride,
+      (build_impl<3, 3>(position, nn, s, cutoff_sq, use_cell_list, g, format, mask_self, idx,
+                        max_occupancy, occupancy, error, reference_position, gate, w, st)),
+      (build_impl<3, 4>(position, nn, s, cutoff_sq, use_cell_list, g, format, mask_self, idx,
+                        max_occupancy, occupancy, error, reference_position, gate, w, st)),
+      (build_impl<2, 2>(position, nn, s, cutoff_sq, use_cell_list, g, format, mask_self, idx,
+                        max_occupancy, occupancy, error, reference_position, gate, w, st)));
+}
+
+int jmd_neighbor_list_check(const float* position, int32_t position_stride,
+                            const float* reference_position, int64_t n, int32_t dim,
+                            int32_t periodic, const float* side, float threshold_sq,
+                            int32_t* gate, void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || !gate ||
+      (n > 0 && (!position || !reference_position)) || (periodic && !side))
+    return JMD_ERR_INVALID;
+  if (n == 0) return JMD_OK;
+  if (!stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  Space s = make_space(dim, periodic, side);
+  hipStream_t st = (hipStream_t)stream;
+  int nn = (int)n;
+  dim3 grid(ceil_div(nn, 256)), block(256);
+  if (dim == 3 && position_stride == 4)
+    hipLaunchKernelGGL((k_skin_check<3, 4>), grid, block, 0, st, position, reference_position,
+                       nn, s, threshold_sq, gate);
+  else if (dim == 3)
+    hipLaunchKernelGGL((k_skin_check<3, 3>), grid, block, 0, st, position, reference_position,
+                       nn, s, threshold_sq, gate);
+  else
+    hipLaunchKernelGGL((k_skin_check<2, 2>), grid, block, 0, st, position, reference_position,
+                       nn, s, threshold_sq, gate);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,471 @@
+// Pair reduction over a neighbour list for gfx950: energy.lennard_jones and
+// energy.soft_sphere mapped by smap.pair_neighbor_list
+// (jax_md/smap.py:856-979, jax_md/energy.py:125-173,246-343,534-580), and its
+// gradient, which the reference obtains by autodiff (quantity.py:58-60).
+//
+// Dense lists hold both orders of every pair, so the force on particle i is
+// the full sum over its own row: a group of TPA lanes owns row i, gathers
+// the neighbour positions (the [N, dim] float32 table stays L2-resident),
+// accumulates in registers and reduces across the group with cross-lane
+// shuffles -- no atomics, deterministic.  Pair arithmetic is float32, the
+// per-particle accumulation float64; the system energy is reduced in float64
+// per block and then over blocks in a fixed order.
+//
+// The LJ force and energy are evaluated from r^2 without a square root:
+//   V = 4 eps (s^12 - s^6),  V'/r = -24 eps (2 s^12 - s^6) / r^2,
+//   S = (rc^2 - r^2)^2 (rc^2 + 2 r^2 - 3 ro^2) / (rc^2 - ro^2)^3,
+//   S'/r = 12 (rc^2 - r^2)(ro^2 - r^2) / (rc^2 - ro^2)^3   on [r_on, r_c),
+//   F_i += (S V'/r + S'/r V) (R_j - R_i).
+#include "jmd_common.hpp"
+
+namespace jmd {
+
+struct PairArgs {
+  int kind;
+  int general;          // any non-scalar parameter
+  int sig_mode, eps_mode, alp_mode;
+  float sig, eps, alp;
+  const float* sig_v;
+  const float* eps_v;
+  const float* alp_v;
+  const int* species;
+  int ns;
+  float ro2, rc2, inv_d;  // LJ smoothing: r_on^2, r_c^2, 1/(rc^2-ro^2)^3
+  int smooth;             // r_on < r_c
+};
+
+__device__ __forceinline__ float pick(int mode, float scalar, const float* v, int i, int j,
+                                      int si, int sj, int ns) {
+  if (mode == JMD_PARAM_SCALAR) return scalar;
+  if (mode == JMD_PARAM_PER_PARTICLE) return 0.5f * (v[i] + v[j]);
+  return v[si * ns + sj];
+}
+
+// Returns U(r) and writes dU/dr / r for one pair given r^2.  Multiply-adds
+// are explicit (contraction off), so every kernel variant rounds alike and the
+// force does not depend on whether the energy is also requested.
+// 1/x from v_rcp_f32 plus one Newton step (~0.5 ulp, no IEEE division).
+__device__ __forceinline__ float fast_rcp(float x) {
+#pragma clang fp contract(off)
+  float r = __builtin_amdgcn_rcpf(x);
+  float e = fmaf(-x, r, 1.0f);
+  return fmaf(r, e, r);
+}
+
+// 1/sqrt(x) from v_rsq_f32 plus one Newton step.
+__device__ __forceinline__ float fast_rsqrt(float x) {
+#pragma clang fp contract(off)
+  float y = __builtin_amdgcn_rsqf(x);
+  float h = 0.5f * x * y;
+  float e = fmaf(-h, y, 0.5f);
+  return fmaf(y, e, y);
+}
+
+template <int KIND>
+__device__ __forceinline__ float pair_eval(float r2, float sig, float eps, float alp,
+                                           const PairArgs& a, float* dEr) {
+#pragma clang fp contract(off)
+  if (KIND == JMD_LENNARD_JONES) {
+    if (!(r2 < a.rc2)) { *dEr = 0.0f; return 0.0f; }
+    float inv_r2 = fast_rcp(r2);
+    float s2 = sig * sig * inv_r2;
+    float s6 = s2 * s2 * s2;
+    float s12 = s6 * s6;
+    float V = 4.0f * eps * (s12 - s6);
+    float dV = -24.0f * eps * fmaf(2.0f, s12, -s6) * inv_r2;
+    if (a.smooth && r2 >= a.ro2) {
+      float u = a.rc2 - r2;
+      float S = u * u * fmaf(2.0f, r2, fmaf(-3.0f, a.ro2, a.rc2)) * a.inv_d;
+      float dS = 12.0f * u * (a.ro2 - r2) * a.inv_d;
+      *dEr = fmaf(S, dV, dS * V);
+      return S * V;
+    }
+    *dEr = dV;
+    return V;
+  } else {
+    // soft sphere: eps/alpha (1 - r/sigma)^alpha for r < sigma
+    if (!(r2 < sig * sig) || !(r2 > 0.0f)) { *dEr = 0.0f; return 0.0f; }
+    float inv_r = fast_rsqrt(r2);
+    float r = r2 * inv_r;
+    float inv_sig = fast_rcp(sig);
+    float base = 1.0f - r * inv_sig;
+    float up, um;  // base^alpha, base^(alpha-1)
+    if (alp == 2.0f) { um = base; up = base * base; }
+    else { um = powf(base, alp - 1.0f); up = um * base; }
+    *dEr = -eps * inv_sig * um * inv_r;
+    return eps / alp * up;
+  }
+}
+
+template <int TPA, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int off = TPA / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// XCD-aware block order: consecutive logical tiles go to the same XCD so each
+// XCD's L2 holds a compact slab of the position table.  The grid is padded to
+// a multiple of 8 by the caller.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  int per = nb >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, bool ENERGY, bool KICK>
+__global__ __launch_bounds__(256) void k_pair_dense(
+    const float* __restrict__ pos, int n, Space s, PairArgs a, const int* __restrict__ idx,
+    int width, float* __restrict__ force, float* __restrict__ e_atom,
+    double* __restrict__ e_part, float* __restrict__ mom, float half_dt,
+    const float* __restrict__ dt_dev) {
+  if (KICK && dt_dev) half_dt = dt_dev[0] / 2.0f;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int gt = tile * blockDim.x + threadIdx.x;
+  const int i = gt / TPA;
+  const int sub = gt % TPA;
+  const bool active = i < n;
+  double f[3] = {0.0, 0.0, 0.0};
+  double e = 0.0;
+  if (active) {
+    float xi[3] = {0, 0, 0}, xim[3] = {0, 0, 0};
+    load_pos<DIM, STRIDE>(pos, i, xi);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) xim[c] = xi[c] - s.L[c];
+    float sig = a.sig, eps = a.eps, alp = a.alp;
+    int si = 0;
+    if (GENERAL && a.species) si = a.species[i];
+    const int* row = idx + (long long)i * width;
+    for (int k = sub; k < width; k += TPA) {
+      int j = row[k];
+      if (j >= n) break;  // rows are left-compacted, padding is n
+      float xj[3] = {0, 0, 0}, d[3];
+      load_pos<DIM, STRIDE>(pos, j, xj);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+        d[c] = s.periodic ? min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c])
+                          : xj[c] - xi[c];
+      float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                          : fmaf(d[1], d[1], d[0] * d[0]);
+      if (GENERAL) {
+        int sj = a.species ? a.species[j] : 0;
+        sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
+        eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
+        if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
+      }
+      float dEr;
+      float u = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) f[c] += (double)(dEr * d[c]);
+      if (ENERGY) e += (double)u;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) f[c] = group_sum<TPA>(f[c]);
+  if (ENERGY) e = group_sum<TPA>(e);
+  if (active && sub == 0) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      float fc = (float)f[c];
+      if (force) force[(long long)i * DIM + c] = fc;
+      if (KICK) mom[(long long)i * DIM + c] += half_dt * fc;
+    }
+    if (ENERGY && e_atom) e_atom[i] = (float)(0.5 * e);
+  }
+  if (ENERGY) {
+    // block partial of 0.5 * sum_i e_i in float64
+    __shared__ double s_e[256 / kWave];
+    double v = (active && sub == 0) ? 0.5 * e : 0.0;
+    v = wave_reduce_sum(v);
+    if (lane_id() == 0) s_e[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_e[w];
+      e_part[tile] = t;
+    }
+  }
+}
+
+// Sparse / OrderedSparse: one thread per stored pair, forces scattered with
+// atomics to both ends (smap.py:926-931,975-977).
+template <int DIM, int STRIDE, int KIND, bool GENERAL, bool ENERGY>
+__global__ __launch_bounds__(256) void k_pair_sparse(
+    const float* __restrict__ pos, int n, Space s, PairArgs a, const int* __restrict__ idx,
+    long long width, float inv_norm, float* __restrict__ force, float* __restrict__ e_atom,
+    double* __restrict__ e_part) {
+  double e = 0.0;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < width;
+       k += (long long)gridDim.x * blockDim.x) {
+    int ra = idx[k];           // receiver (idx[0])
+    int sb = idx[width + k];   // sender   (idx[1])
+    if (ra < n && sb < n) {
+      float xa[3] = {0, 0, 0}, xb[3] = {0, 0, 0}, d[3];  // disp(R[idx0], R[idx1]) = R_a - R_b
+      load_pos<DIM, STRIDE>(pos, ra, xa);
+      load_pos<DIM, STRIDE>(pos, sb, xb);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+        d[c] = s.periodic ? min_image_disp(xa[c], xb[c], xb[c] - s.L[c], s.L[c], s.half[c])
+                          : xa[c] - xb[c];
+      float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                          : fmaf(d[1], d[1], d[0] * d[0]);
+      float sig = a.sig, eps = a.eps, alp = a.alp;
+      if (GENERAL) {
+        int s0 = a.species ? a.species[ra] : 0, s1 = a.species ? a.species[sb] : 0;
+        sig = pick(a.sig_mode, a.sig, a.sig_v, ra, sb, s0, s1, a.ns);
+        eps = pick(a.eps_mode, a.eps, a.eps_v, ra, sb, s0, s1, a.ns);
+        if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, ra, sb, s0, s1, a.ns);
+      }
+      float dEr;
+      float u = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr) * inv_norm;
+      dEr *= inv_norm;
+      if (force) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+          float fc = dEr * d[c];  // dE/dR_a = dEr * (R_a - R_b)
+          atomicAdd(&force[(long long)ra * DIM + c], -fc);
+          atomicAdd(&force[(long long)sb * DIM + c], fc);
+        }
+      }
+      if (ENERGY) {
+        e += (double)u;
+        if (e_atom) atomicAdd(&e_atom[ra], u);
+      }
+    }
+  }
+  if (ENERGY) {
+    __shared__ double s_e[256 / kWave];
+    double v = wave_reduce_sum(e);
+    if (lane_id() == 0) s_e[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_e[w];
+      e_part[blockIdx.x] = t;
+    }
+  }
+}
+
+// Fixed-order sum of block partials (one workgroup).
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ part, int m,
+                                                      double* __restrict__ out) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < m; k += 256) v += part[k];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+static bool make_pair_args(const jmd_pair_potential* p, PairArgs* a) {
+  if (!p) return false;
+  if (p->kind != JMD_LENNARD_JONES && p->kind != JMD_SOFT_SPHERE) return false;
+  a->kind = p->kind;
+  a->sig_mode = p->sigma.mode;
+  a->eps_mode = p->epsilon.mode;
+  a->alp_mode = p->kind == JMD_SOFT_SPHERE ? p->alpha.mode : JMD_PARAM_SCALAR;
+  for (int m : {a->sig_mode, a->eps_mode, a->alp_mode})
+    if (m < JMD_PARAM_SCALAR || m > JMD_PARAM_SPECIES) return false;
+  a->sig = p->sigma.value;
+  a->eps = p->epsilon.value;
+  a->alp = p->alpha.value;
+  a->sig_v = p->sigma.values;
+  a->eps_v = p->epsilon.values;
+  a->alp_v = p->alpha.values;
+  a->species = p->species;
+  a->ns = p->n_species;
+  bool need_species = false;
+  const jmd_param* ps[3] = {&p->sigma, &p->epsilon, &p->alpha};
+  for (int q = 0; q < 3; ++q) {
+    if (q == 2 && p->kind != JMD_SOFT_SPHERE) continue;
+    if (ps[q]->mode != JMD_PARAM_SCALAR && !ps[q]->values) return false;
+    if (ps[q]->mode == JMD_PARAM_SPECIES) need_species = true;
+  }
+  if (need_species && (!p->species || p->n_species < 1)) return false;
+  a->general = (a->sig_mode | a->eps_mode | a->alp_mode) != 0;
+  float ro = p->r_onset, rc = p->r_cutoff;
+  a->rc2 = rc * rc;
+  a->ro2 = ro * ro;
+  a->smooth = ro < rc;
+  a->inv_d = 0.0f;
+  if (a->smooth) {
+    double dd = (double)a->rc2 - (double)a->ro2;
+    a->inv_d = (float)(1.0 / (dd * dd * dd));
+  }
+  return true;
+}
+
+struct DenseLaunch {
+  dim3 grid;
+  hipStream_t st;
+  const float* pos;
+  int n;
+  Space s;
+  PairArgs a;
+  const int* idx;
+  int width;
+  float* force;
+  float* e_atom;
+  double* e_part;
+  float* mom;
+  float half_dt;
+  const float* dt_dev;
+};
+
+template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL>
+static void launch_dense(const DenseLaunch& L, bool energy, bool kick) {
+#define JMD_PD(E, K)                                                                          \
+  hipLaunchKernelGGL((k_pair_dense<DIM, STRIDE, KIND, TPA, GENERAL, E, K>), L.grid, dim3(256), \
+                     0, L.st, L.pos, L.n, L.s, L.a, L.idx, L.width, L.force, L.e_atom,        \
+                     L.e_part, L.mom, L.half_dt, L.dt_dev)
+  if (energy) JMD_PD(true, false);
+  else if (kick) JMD_PD(false, true);
+  else JMD_PD(false, false);
+#undef JMD_PD
+}
+
+template <int DIM, int STRIDE, int KIND>
+static void dispatch_dense(int tpa, bool general, const DenseLaunch& L, bool energy, bool kick) {
+#define JMD_TPA(T)                                                               \
+  (general ? launch_dense<DIM, STRIDE, KIND, T, true>(L, energy, kick)           \
+           : launch_dense<DIM, STRIDE, KIND, T, false>(L, energy, kick))
+  switch (tpa) {
+    case 2: JMD_TPA(2); break;
+    case 8: JMD_TPA(8); break;
+    case 16: JMD_TPA(16); break;
+    default: JMD_TPA(4); break;
+  }
+#undef JMD_TPA
+}
+
+static int g_tpa_override = 0;
+constexpr long long kSparseBlocks = 2048;
+
+}  // namespace jmd
+
+using namespace jmd;
+
+extern "C" {
+
+size_t jmd_pair_workspace_bytes(int64_t n) {
+  // block partials: Dense uses at most ceil(n*16/256) padded to 8, Sparse at
+  // most kSparseBlocks.
+  long long blocks = (n * 16 + 255) / 256 + 64;
+  if (blocks < kSparseBlocks) blocks = kSparseBlocks;
+  return (size_t)(blocks * 8 + 256);
+}
+
+// Threads per Dense row (2, 4, 8 or 16); 0 restores the built-in choice.
+int jmd_set_threads_per_atom(int32_t tpa) {
+  if (tpa != 0 && tpa != 2 && tpa != 4 && tpa != 8 && tpa != 16) return JMD_ERR_INVALID;
+  g_tpa_override = tpa;
+  return JMD_OK;
+}
+
+int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
+                          int32_t position_stride, int32_t periodic, const float* side,
+                          const jmd_pair_potential* potential, const int32_t* idx,
+                          int64_t idx_width, int32_t format, float* force, float* energy_atom,
+                          double* energy_total, float* momentum, float half_dt,
+                          const float* dt_device, void* workspace, void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || idx_width < 0 ||
+      (n > 0 && !position) || (idx_width > 0 && !idx) || (periodic && !side))
+    return JMD_ERR_INVALID;
+  if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  if (format != JMD_DENSE && format != JMD_SPARSE && format != JMD_ORDERED_SPARSE)
+    return JMD_ERR_INVALID;
+  const bool energy = energy_total != nullptr || energy_atom != nullptr;
+  if (energy && !workspace) return JMD_ERR_INVALID;
+  if (format == JMD_ORDERED_SPARSE && energy_atom) return JMD_ERR_UNSUPPORTED;
+  if (format != JMD_DENSE && momentum) return JMD_ERR_UNSUPPORTED;
+  PairArgs a;
+  if (!make_pair_args(potential, &a)) return JMD_ERR_INVALID;
+  Space s = make_space(dim, periodic, side);
+  hipStream_t st = (hipStream_t)stream;
+  int nn = (int)n;
+  double* part = (double*)workspace;
+  const bool s4 = dim == 3 && position_stride == 4;
+  if (format == JMD_DENSE) {
+    if (idx_width > 0x7fffffff) return JMD_ERR_INVALID;
+    if (nn == 0) {
+      if (energy_total) {
+        hipLaunchKernelGGL(fill_kernel<double>, dim3(1), dim3(64), 0, st, energy_total, 1LL, 0.0);
+        JMD_LAUNCH_CHECK();
+      }
+      return JMD_OK;
+    }
+    int tpa = g_tpa_override ? g_tpa_override : (dim == 3 ? 4 : 2);
+    long long threads = (long long)nn * tpa;
+    int nb = ceil_div(threads, 256);
+    nb = (nb + 7) & ~7;
+    DenseLaunch L{dim3(nb), st, position, nn, s, a, idx, (int)idx_width, force, energy_atom,
+                  energy ? part : nullptr, momentum, half_dt, dt_device};
+    const bool kick = momentum != nullptr;
+    if (dim == 3) {
+      if (a.kind == JMD_LENNARD_JONES) {
+        if (s4) dispatch_dense<3, 4, JMD_LENNARD_JONES>(tpa, a.general, L, energy, kick);
+        else dispatch_dense<3, 3, JMD_LENNARD_JONES>(tpa, a.general, L, energy, kick);
+      } else {
+        if (s4) dispatch_dense<3, 4, JMD_SOFT_SPHERE>(tpa, a.general, L, energy, kick);
+        else dispatch_dense<3, 3, JMD_SOFT_SPHERE>(tpa, a.general, L, energy, kick);
+      }
+    } else {
+      if (a.kind == JMD_LENNARD_JONES)
+        dispatch_dense<2, 2, JMD_LENNARD_JONES>(tpa, a.general, L, energy, kick);
+      else
+        dispatch_dense<2, 2, JMD_SOFT_SPHERE>(tpa, a.general, L, energy, kick);
+    }
+    JMD_LAUNCH_CHECK();
+    if (energy_total) {
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, nb, energy_total);
+      JMD_LAUNCH_CHECK();
+    }
+    return JMD_OK;
+  }
+  // Sparse / OrderedSparse
+  float inv_norm = format == JMD_ORDERED_SPARSE ? 1.0f : 0.5f;
+  if (force) {
+    int rc = launch_fill<float>(force, (long long)nn * dim, 0.0f, st);
+    if (rc) return rc;
+  }
+  if (energy_atom) {
+    int rc = launch_fill<float>(energy_atom, (long long)nn, 0.0f, st);
+    if (rc) return rc;
+  }
+  long long nb = (idx_width + 255) / 256;
+  if (nb == 0) nb = 1;
+  if (nb > kSparseBlocks) nb = kSparseBlocks;
+#define JMD_PS(D, S, K)                                                                        \
+  do {                                                                                         \
+    if (energy)                                                                                \
+      hipLaunchKernelGGL((k_pair_sparse<D, S, K, true, true>), dim3(nb), dim3(256), 0, st,     \
+                         position, nn, s, a, idx, idx_width, inv_norm, force, energy_atom,     \
+                         part);                                                                \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_pair_sparse<D, S, K, true, false>), dim3(nb), dim3(256), 0, st,    \
+                         position, nn, s, a, idx, idx_width, inv_norm, force, energy_atom,     \
+                         part);                                                                \
+  } while (0)
+  if (dim == 3) {
+    if (a.kind == JMD_LENNARD_JONES) {
+      if (s4) JMD_PS(3, 4, JMD_LENNARD_JONES); else JMD_PS(3, 3, JMD_LENNARD_JONES);
+    } else {
+      if (s4) JMD_PS(3, 4, JMD_SOFT_SPHERE); else JMD_PS(3, 3, JMD_SOFT_SPHERE);
+    }
+  } else {
+    if (a.kind == JMD_LENNARD_JONES) JMD_PS(2, 2, JMD_LENNARD_JONES);
+    else JMD_PS(2, 2, JMD_SOFT_SPHERE);
+  }
+#undef JMD_PS
+  JMD_LAUNCH_CHECK();
+  if (energy_total) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, (int)nb, energy_total);
+    JMD_LAUNCH_CHECK();
+  }
+  return JMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,519 @@
+"""Cell lists and neighbour lists (jax_md/partition.py) on gfx950.
+
+Same surface as the reference: ``neighbor_list(...)`` returns
+``NeighborListFns(allocate, update)``; a ``NeighborList`` exposes ``idx``,
+``reference_position``, ``error``, ``did_buffer_overflow``, ``max_occupancy``,
+``format`` and ``update``.  The host logic that the reference runs outside
+jit (capacity estimates, ``int(occupancy * multiplier + extra)``) is restated
+here in float32 NumPy; every per-particle computation runs in
+libjaxmd_native.so.
+
+Storage semantics.  ``update`` returns a new ``NeighborList`` object, but the
+device buffers (``idx``, ``reference_position``, the error word) are updated
+in place: the rebuild decision is taken on the device (the reference's
+``lax.cond``) so no host round trip is needed, and an unchanged list costs
+nothing.  The reference's own loops never read a list after updating it, and
+re-allocate from positions on overflow, which this supports unchanged.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from enum import Enum, IntEnum
+from functools import reduce
+from operator import mul
+from typing import Any, Callable, Dict, Optional
+
+import numpy as onp
+import torch
+
+from . import _native as nat
+from . import space as space_mod
+
+f32 = onp.float32
+
+
+# ---------------------------------------------------------------------------
+# Formats and error codes (partition.py:494-681)
+# ---------------------------------------------------------------------------
+
+
+class NeighborListFormat(Enum):
+  """Dense [N, max_occupancy]; Sparse [2, capacity] = (receiver, sender);
+  OrderedSparse like Sparse with only j < i (partition.py:641-661)."""
+  Dense = 0
+  Sparse = 1
+  OrderedSparse = 2
+
+
+Dense = NeighborListFormat.Dense
+Sparse = NeighborListFormat.Sparse
+OrderedSparse = NeighborListFormat.OrderedSparse
+
+
+def is_sparse(fmt: NeighborListFormat) -> bool:
+  return fmt is NeighborListFormat.Sparse or fmt is NeighborListFormat.OrderedSparse
+
+
+def is_format_valid(fmt: NeighborListFormat):
+  if fmt not in list(NeighborListFormat):
+    raise ValueError(
+        f'Neighbor list format must be a member of NeighborListFormat found {fmt}.')
+
+
+def is_box_valid(box) -> bool:
+  box = onp.asarray(box)
+  if box.ndim <= 1:
+    return True
+  if box.ndim == 2:
+    return bool(onp.all(onp.triu(box) == box))
+  return False
+
+
+class PartitionErrorCode(IntEnum):
+  NONE = 0
+  NEIGHBOR_LIST_OVERFLOW = 1 << 0
+  CELL_LIST_OVERFLOW = 1 << 1
+  CELL_SIZE_TOO_SMALL = 1 << 2
+  MALFORMED_BOX = 1 << 3
+
+
+PEC = PartitionErrorCode
+
+
+@dataclasses.dataclass
+class PartitionError:
+  """Error word (device uint8 0-d tensor), sticky across updates."""
+  code: torch.Tensor
+
+  def update(self, bit, pred) -> 'PartitionError':
+    pred = torch.as_tensor(pred, device=self.code.device)
+    return PartitionError(self.code | torch.where(pred, int(bit), 0).to(torch.uint8))
+
+  def __str__(self) -> str:
+    c = int(self.code)
+    if not c:
+      return ''
+    if c & PEC.NEIGHBOR_LIST_OVERFLOW:
+      return 'Partition Error: Neighbor list buffer overflow.'
+    if c & PEC.CELL_LIST_OVERFLOW:
+      return 'Partition Error: Cell list buffer overflow'
+    if c & PEC.CELL_SIZE_TOO_SMALL:
+      return 'Partition Error: Cell size too small'
+    if c & PEC.MALFORMED_BOX:
+      return 'Partition Error: Incorrect box format. Expecting upper triangular.'
+    raise ValueError(f'Unexpected Error Code {c}.')
+
+  __repr__ = __str__
+
+
+# ---------------------------------------------------------------------------
+# Cell grid arithmetic (partition.py:146-188), float32 as in the reference
+# ---------------------------------------------------------------------------
+
+
+def _box_f32(box) -> onp.ndarray:
+  if isinstance(box, torch.Tensor):
+    box = box.detach().cpu().numpy()
+  b = onp.asarray(box, f32)
+  if b.ndim > 1:
+    raise NotImplementedError(
+        'Box matrices (periodic_general / fractional coordinates) are outside '
+        'this package.')
+  return b
+
+
+def cell_dimensions(dim: int, box, minimum_cell_size):
+  """(box, cell_size, cells_per_side, cell_count) (partition.py:146-188)."""
+  box = _box_f32(box)
+  mcs = onp.asarray(minimum_cell_size, f32)
+  cps = onp.floor(box / mcs).astype(f32)
+  cell_size = (box / cps).astype(f32)
+  cps_i = cps.astype(onp.int32)
+  if box.ndim == 1:
+    if box.shape[0] != dim:
+      raise ValueError(f'box has {box.shape[0]} sides for dimension {dim}')
+    if onp.any(cps_i < 3):
+      raise ValueError(
+          'Box must be at least 3x the size of the grid spacing in each dimension.')
+    cell_count = reduce(mul, [int(c) for c in cps_i], 1)
+  else:
+    cell_count = int(cps_i) ** dim
+  return box, cell_size, cps_i, cell_count
+
+
+def _per_dim(x, dim):
+  x = onp.asarray(x)
+  return [x.item()] * dim if x.ndim == 0 else [v.item() for v in x.reshape(dim)]
+
+
+class _Workspace:
+  """Zero-initialised device scratch for the native list builds.  The native
+  calls leave it zeroed at rest (their contract), so it is allocated once."""
+
+  def __init__(self, device, nbytes):
+    self.buf = torch.zeros(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+
+  @property
+  def ptr(self):
+    return nat.ptr(self.buf)
+
+
+def _check_position(position) -> int:
+  """Validates a position array and returns its row stride."""
+  return nat.position_stride(position)
+
+
+def _filling_max(position, cps, cell_size):
+  """Largest cell occupancy on a grid (count_cell_filling, partition.py:191-208)."""
+  lib = nat.load()
+  N, dim = position.shape
+  ncell = reduce(mul, [int(c) for c in cps], 1)
+  ws = _Workspace(position.device, lib.jmd_neighbor_workspace_bytes(N, dim, ncell, 0))
+  out = torch.zeros((1,), dtype=torch.int32, device=position.device)
+  nat.check(lib.jmd_cell_filling_max(nat.ptr(position), N, dim, _check_position(position),
+                                     nat.i3(cps), nat.f3(cell_size),
+                                     nat.ptr(out), ws.ptr, nat.stream_handle(position.device)),
+            'jmd_cell_filling_max')
+  return int(out.item())
+
+
+# ---------------------------------------------------------------------------
+# Cell list (partition.py:78-488)
+# ---------------------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class CellList:
+  position_buffer: torch.Tensor
+  id_buffer: torch.Tensor
+  particle_cell_id: torch.Tensor
+  named_buffer: Dict[str, torch.Tensor]
+  did_buffer_overflow: torch.Tensor
+  cell_capacity: int
+  cell_size: Any
+  update_fn: Callable[..., 'CellList']
+
+  def update(self, position, **kwargs) -> 'CellList':
+    return self.update_fn(position, self.cell_capacity, **kwargs)
+
+  @property
+  def kwarg_buffers(self):
+    return self.named_buffer
+
+
+@dataclasses.dataclass
+class CellListFns:
+  allocate: Callable[..., CellList]
+  update: Callable[..., CellList]
+
+  def __iter__(self):
+    return iter((self.allocate, self.update))
+
+
+def cell_list(box_size, minimum_cell_size, buffer_size_multiplier: float = 1.25) -> CellListFns:
+  """Uniform-grid spatial partition (partition.py:296-488).
+
+  Buffers have the reference's shapes: ``position_buffer``
+  [..cells.., capacity, dim] (cells in reversed-dimension order),
+  ``id_buffer`` [..cells.., capacity, 1] with N in empty slots,
+  ``particle_cell_id`` [N, dim].  Side data (``**kwargs`` buffers) is not
+  supported natively.
+  """
+  box_size = _box_f32(box_size)
+  mcs = onp.asarray(minimum_cell_size, f32)
+
+  def cell_list_fn(position, capacity=None, overflow=None, extra_capacity=0, **kwargs):
+    if kwargs:
+      raise NotImplementedError('cell-list side data (named buffers) is not supported natively')
+    stride = _check_position(position)
+    lib = nat.load()
+    N, dim = position.shape
+    _, cell_size, cps, ncell = cell_dimensions(dim, box_size, mcs)
+    cs3, cps3 = _per_dim(cell_size, dim), _per_dim(cps, dim)
+    if capacity is None:
+      # _estimate_cell_capacity counts on the grid derived again from cell_size
+      _, cs2, cps2, _ = cell_dimensions(dim, box_size, cell_size)
+      fill = _filling_max(position, _per_dim(cps2, dim), _per_dim(cs2, dim))
+      capacity = int(onp.int32(fill) * buffer_size_multiplier) + extra_capacity
+    capacity = int(capacity)
+    dev = position.device
+    pos_buf = torch.empty((ncell * capacity, dim), dtype=torch.float32, device=dev)
+    id_buf = torch.empty((ncell * capacity,), dtype=torch.int32, device=dev)
+    pcid = torch.empty((N, dim), dtype=torch.int32, device=dev)
+    ovf = torch.zeros((), dtype=torch.uint8, device=dev)
+    ws = _Workspace(dev, lib.jmd_neighbor_workspace_bytes(N, dim, ncell, 0))
+    nat.check(lib.jmd_cell_list(nat.ptr(position), N, dim, stride, nat.i3(cps3), nat.f3(cs3),
+                                capacity,
+                                nat.ptr(pos_buf), nat.ptr(id_buf), nat.ptr(pcid), None,
+                                nat.ptr(ovf), ws.ptr, nat.stream_handle(dev)), 'jmd_cell_list')
+    cells = tuple(int(c) for c in reversed(cps3))
+    pos_buf = pos_buf.reshape(cells + (capacity, dim))
+    id_buf = id_buf.reshape(cells + (capacity, 1))
+    did = ovf != 0
+    if overflow is not None:
+      did = did | torch.as_tensor(overflow, device=dev)
+    return CellList(pos_buf, id_buf, pcid, {}, did, capacity, cell_size, update_fn)
+
+  def allocate_fn(position, extra_capacity: int = 0, **kwargs) -> CellList:
+    return cell_list_fn(position, extra_capacity=extra_capacity, **kwargs)
+
+  def update_fn(position, cl_or_capacity, **kwargs) -> CellList:
+    if isinstance(cl_or_capacity, int):
+      return cell_list_fn(position, capacity=cl_or_capacity, **kwargs)
+    cl = cl_or_capacity
+    return cell_list_fn(position, capacity=cl.cell_capacity, overflow=cl.did_buffer_overflow,
+                        **kwargs)
+
+  return CellListFns(allocate_fn, update_fn)
+
+
+# ---------------------------------------------------------------------------
+# Neighbour list (partition.py:684-1164)
+# ---------------------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class _Plan:
+  """Device-side state shared by a list and its updates."""
+  workspace: _Workspace
+  gate: torch.Tensor          # int32 [1]
+  occupancy: torch.Tensor     # int32 [1]
+  use_cell_list: bool
+  cells_per_side: list
+  cell_size3: list
+  side: list
+  periodic: bool
+  dim: int
+  spec: Any = None
+
+  @property
+  def builds(self) -> int:
+    """Completed list builds since allocation (device counter; syncs)."""
+    return int(self.workspace.buf[16:20].view(torch.int32).item())
+
+
+@dataclasses.dataclass
+class NeighborList:
+  """State of a neighbour list (partition.py:684-737)."""
+  idx: torch.Tensor
+  reference_position: torch.Tensor
+  error: PartitionError
+  cell_list_capacity: Optional[int]
+  max_occupancy: int
+  format: NeighborListFormat
+  cell_size: Any
+  cell_list_fn: Any
+  update_fn: Callable[..., 'NeighborList']
+  plan: Optional[_Plan] = None
+
+  def update(self, position, **kwargs) -> 'NeighborList':
+    return self.update_fn(position, self, **kwargs)
+
+  @property
+  def did_buffer_overflow(self) -> torch.Tensor:
+    return self.error.code & (PEC.NEIGHBOR_LIST_OVERFLOW | PEC.CELL_LIST_OVERFLOW)
+
+  @property
+  def cell_size_too_small(self) -> torch.Tensor:
+    return self.error.code & PEC.CELL_SIZE_TOO_SMALL
+
+  @property
+  def malformed_box(self) -> torch.Tensor:
+    return self.error.code & PEC.MALFORMED_BOX
+
+
+@dataclasses.dataclass
+class NeighborListFns:
+  allocate: Callable[..., NeighborList]
+  update: Callable[..., NeighborList]
+
+  def __call__(self, position, neighbors=None, extra_capacity: int = 0, **kwargs):
+    if neighbors is None:
+      return self.allocate(position, extra_capacity, **kwargs)
+    return self.update(position, neighbors, **kwargs)
+
+  def __iter__(self):
+    return iter((self.allocate, self.update))
+
+
+class NeighborListSpec:
+  """Static configuration of ``neighbor_list`` with the reference's float32
+  derived constants (partition.py:887-895)."""
+
+  def __init__(self, displacement_or_metric, box, r_cutoff, dr_threshold,
+               capacity_multiplier, disable_cell_list, mask_self, fmt):
+    self.geometry = space_mod.geometry_of(displacement_or_metric)
+    self.box = _box_f32(box)
+    rc = f32(onp.asarray(r_cutoff, f32))
+    dr = f32(onp.asarray(dr_threshold, f32))
+    self.cutoff = f32(rc + dr)
+    self.cutoff_sq = f32(self.cutoff * self.cutoff)
+    self.threshold_sq = f32((dr / f32(2)) * (dr / f32(2)))
+    self.always_rebuild = bool(dr == 0)
+    self.capacity_multiplier = capacity_multiplier
+    self.disable_cell_list = disable_cell_list
+    self.mask_self = mask_self
+    self.format = fmt
+
+  def use_cell_list(self) -> bool:
+    if self.disable_cell_list:
+      return False
+    return bool(onp.all(self.cutoff < (self.box / f32(3.0)).astype(f32)))
+
+
+_FORMAT_CODE = {Dense: nat.DENSE, Sparse: nat.SPARSE, OrderedSparse: nat.ORDERED_SPARSE}
+
+
+def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
+                  capacity_multiplier: float = 1.25, disable_cell_list: bool = False,
+                  mask_self: bool = True, custom_mask_function=None,
+                  fractional_coordinates: bool = False,
+                  format: NeighborListFormat = NeighborListFormat.Dense,
+                  **static_kwargs) -> NeighborListFns:
+  """Neighbour lists for a periodic or free space (partition.py:801-1164).
+
+  Supported: ``space.periodic`` (scalar or per-dimension side) and
+  ``space.free`` displacements, Dense / Sparse / OrderedSparse, with or
+  without the cell list, ``mask_self``.  ``custom_mask_function`` and
+  fractional coordinates raise ``NotImplementedError``.
+  """
+  is_format_valid(format)
+  if custom_mask_function is not None:
+    raise NotImplementedError('custom_mask_function is not supported by the native build')
+  if fractional_coordinates:
+    raise NotImplementedError('fractional coordinates (periodic_general) are out of scope')
+  spec = NeighborListSpec(displacement_or_metric, box, r_cutoff, dr_threshold,
+                          capacity_multiplier, disable_cell_list, mask_self, format)
+  fcode = _FORMAT_CODE[format]
+
+  def _build(position, nbrs: Optional[NeighborList], plan: _Plan, gate):
+    lib = nat.load()
+    N, dim = position.shape
+    st = nat.stream_handle(position.device)
+    nat.check(lib.jmd_neighbor_list_build(
+        nat.ptr(position), N, dim, _check_position(position), int(plan.periodic),
+        nat.f3(plan.side), float(spec.cutoff_sq), int(plan.use_cell_list), nat.i3(plan.cells_per_side),
+        nat.f3(plan.cell_size3), int(nbrs.cell_list_capacity or 0), fcode,
+        int(spec.mask_self), nat.ptr(nbrs.idx), int(nbrs.max_occupancy),
+        nat.ptr(plan.occupancy), nat.ptr(nbrs.error.code), nat.ptr(nbrs.reference_position),
+        gate, plan.workspace.ptr, st), 'jmd_neighbor_list_build')
+
+  spec.build = _build
+  spec.format_code = fcode
+
+  def allocate_fn(position, extra_capacity: int = 0, **kwargs) -> NeighborList:
+    if 'box' in kwargs:
+      raise ValueError('Neighbor list cannot accept a box keyword argument '
+                       'if fractional_coordinates is not enabled.')
+    stride = _check_position(position)
+    lib = nat.load()
+    N, dim = position.shape
+    dev = position.device
+    geo = spec.geometry
+    side = geo.side_values(dim) if geo.periodic else [1.0] * dim
+    use_cells = spec.use_cell_list()
+    cps3, cs3, ncell, cap = [1] * dim, [1.0] * dim, 1, None
+    cell_size = None
+    if use_cells:
+      _, cell_size, cps, ncell = cell_dimensions(dim, spec.box, spec.cutoff)
+      cps3, cs3 = _per_dim(cps, dim), _per_dim(cell_size, dim)
+      _, cs2, cps2, _ = cell_dimensions(dim, spec.box, cell_size)
+      fill = _filling_max(position, _per_dim(cps2, dim), _per_dim(cs2, dim))
+      cap = int(onp.int32(fill) * capacity_multiplier) + extra_capacity
+      cell_size = spec.cutoff
+    ws = _Workspace(dev, lib.jmd_neighbor_workspace_bytes(N, dim, ncell if use_cells else 0,
+                                                          fcode))
+    plan = _Plan(ws, torch.zeros((1,), dtype=torch.int32, device=dev),
+                 torch.zeros((1,), dtype=torch.int32, device=dev), use_cells, cps3, cs3, side,
+                 bool(geo.periodic), dim, spec)
+    st = nat.stream_handle(dev)
+    # 1) measure the occupancy (allocate is not jittable in the reference either)
+    nat.check(lib.jmd_neighbor_list_build(
+        nat.ptr(position), N, dim, stride, int(plan.periodic), nat.f3(side),
+        float(spec.cutoff_sq),
+        int(use_cells), nat.i3(cps3), nat.f3(cs3), int(cap or 0), fcode, int(spec.mask_self),
+        None, 0, nat.ptr(plan.occupancy), None, None, None, ws.ptr, st),
+        'jmd_neighbor_list_build(count)')
+    occupancy = int(plan.occupancy.item())
+    # 2) capacity (partition.py:1092-1104), float32 arithmetic like the reference
+    sparse = is_sparse(format)
+    extra = extra_capacity if not sparse else N * extra_capacity
+    max_occ = int(f32(occupancy) * f32(capacity_multiplier) + f32(extra))
+    n_cand = N if not use_cells else (3 ** dim) * cap
+    total = n_cand if not sparse else N * n_cand
+    max_occ = min(max_occ, total)
+    if not sparse:
+      limit = N - 1 if mask_self else N
+    elif format is Sparse:
+      limit = N * (N - 1) if mask_self else N * N
+    else:
+      limit = N * (N - 1) // 2
+    max_occ = max(min(max_occ, limit), 0)
+    shape = (N, max_occ) if not sparse else (2, max_occ)
+    nbrs = NeighborList(
+        idx=torch.empty(shape, dtype=torch.int32, device=dev),
+        reference_position=torch.empty((N, dim), dtype=torch.float32, device=dev),
+        error=PartitionError(torch.zeros((), dtype=torch.uint8, device=dev)),
+        cell_list_capacity=cap, max_occupancy=max_occ, format=format, cell_size=cell_size,
+        cell_list_fn=None, update_fn=update_fn, plan=plan)
+    _build(position, nbrs, plan, None)
+    return nbrs
+
+  def update_fn(position, neighbors: NeighborList, **kwargs) -> NeighborList:
+    if 'box' in kwargs:
+      raise ValueError('Neighbor list cannot accept a box keyword argument '
+                       'if fractional_coordinates is not enabled.')
+    stride = _check_position(position)
+    plan = neighbors.plan
+    if plan is None or tuple(position.shape) != tuple(neighbors.reference_position.shape):
+      raise ValueError('update needs a list allocated for the same particle count')
+    lib = nat.load()
+    N, dim = position.shape
+    if spec.always_rebuild:
+      _build(position, neighbors, plan, None)
+    else:
+      st = nat.stream_handle(position.device)
+      nat.check(lib.jmd_neighbor_list_check(
+          nat.ptr(position), stride, nat.ptr(neighbors.reference_position), N, dim,
+          int(plan.periodic), nat.f3(plan.side), float(spec.threshold_sq),
+          nat.ptr(plan.gate), st), 'jmd_neighbor_list_check')
+      _build(position, neighbors, plan, nat.ptr(plan.gate))
+    return dataclasses.replace(neighbors)
+
+  return NeighborListFns(allocate_fn, update_fn)
+
+
+def neighbor_list_mask(neighbor: NeighborList, mask_self: bool = False) -> torch.Tensor:
+  """Validity mask of a list (partition.py:1167-1182)."""
+  if is_sparse(neighbor.format):
+    mask = neighbor.idx[0] < len(neighbor.reference_position)
+    if mask_self:
+      mask = mask & (neighbor.idx[0] != neighbor.idx[1])
+    return mask
+  mask = neighbor.idx < len(neighbor.idx)
+  if mask_self:
+    N = len(neighbor.reference_position)
+    self_mask = neighbor.idx != torch.arange(N, dtype=torch.int32,
+                                             device=neighbor.idx.device).reshape(N, 1)
+    mask = mask & self_mask
+  return mask
+
+
+def to_dense(neighbor: NeighborList) -> torch.Tensor:
+  """Sparse -> Dense ids (partition.py:1253-1271); host-side like the
+  reference (not jittable there either)."""
+  if neighbor.format is not Sparse:
+    raise ValueError('Can only convert sparse neighbor lists to dense ones.')
+  receivers, senders = neighbor.idx
+  mask = neighbor_list_mask(neighbor)
+  receivers = receivers[mask].long()
+  senders = senders[mask].long()
+  N = len(neighbor.reference_position)
+  count = torch.bincount(receivers, minlength=N)
+  max_count = int(count.max().item()) if receivers.numel() else 0
+  offset = torch.arange(max_count, device=receivers.device).repeat(N)[:len(senders)]
+  hashes = senders * max_count + offset
+  dense = torch.full((N * max_count,), N, dtype=torch.int32, device=receivers.device)
+  dense[hashes] = receivers.to(torch.int32)
+  return dense.reshape(N, max_count)

@@ -1,0 +1,235 @@
+"""Velocity-Verlet NVE (jax_md/simulate.py:227-317) on gfx950.
+
+``nve(energy_or_force_fn, shift_fn, dt)`` returns ``(init_fn, step_fn)`` with
+the reference's signatures.  A step is two kernels: the half kick + drift
+(positions accumulated in float64, mirrored to float32) and the force pass
+with the closing half kick fused in.  ``step_fn`` is functional: it returns
+new arrays and leaves the input state untouched, so the reference's
+speculative-block pattern (discard the block on overflow) works unchanged.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from typing import Optional
+
+import numpy as onp
+import torch
+
+from . import _native as nat
+from . import quantity
+from . import space as space_mod
+
+f32 = onp.float32
+
+
+@dataclasses.dataclass
+class NVEState:
+  """position, momentum, force [N, dim] float32; mass scalar or [N];
+  position_hi: the float64 master copy of the positions."""
+  position: torch.Tensor
+  momentum: torch.Tensor
+  force: torch.Tensor
+  mass: object
+  position_hi: Optional[torch.Tensor] = None
+
+  @property
+  def velocity(self):
+    m = self.mass.reshape(-1, 1) if isinstance(self.mass, torch.Tensor) and self.mass.ndim else self.mass
+    return self.momentum / m
+
+  def set(self, **kw):
+    return dataclasses.replace(self, **kw)
+
+
+def canonicalize_mass(mass, device):
+  """float, [N] or [N, 1] (simulate.py:121-140) -> (scalar, device [N] or None)."""
+  if isinstance(mass, torch.Tensor) and mass.ndim > 0:
+    m = mass.to(device=device, dtype=torch.float32).reshape(-1).contiguous()
+    return 0.0, m
+  if isinstance(mass, onp.ndarray) and mass.ndim > 0:
+    return 0.0, torch.as_tensor(mass.reshape(-1), dtype=torch.float32, device=device)
+  return float(f32(float(mass))), None
+
+
+def native_shift(shift_fn) -> space_mod.Shift:
+  if isinstance(shift_fn, space_mod.Shift):
+    return shift_fn
+  raise NotImplementedError(
+      'The native integrators need a shift_fn from jax_md_amd.space.periodic or '
+      f'jax_md_amd.space.free; got {type(shift_fn).__name__}.')
+
+
+def _side(shift: space_mod.Shift, dim):
+  if not shift.periodic:
+    return [1.0] * dim
+  s = onp.asarray(shift.side, f32)
+  return [float(s)] * dim if s.ndim == 0 else [float(v) for v in s]
+
+
+def velocity_verlet_native(energy, shift, state: NVEState, dt, neighbor, dt_device=None,
+                           **kwargs) -> NVEState:
+  """One step: p += dt/2 F; R = shift(R, dt p/m); F = F(R); p += dt/2 F
+  (simulate.py:227-243), dt cast to float32 (F5)."""
+  lib = nat.load()
+  R = state.position
+  N, dim = R.shape
+  dev = R.device
+  dt32 = f32(dt)
+  half = f32(dt32 / f32(2))
+  mass_s, mass_v = canonicalize_mass(state.mass, dev)
+  x_hi = state.position_hi if state.position_hi is not None else R.to(torch.float64)
+  x_hi_new = torch.empty_like(x_hi)
+  x_new = new_positions(N, dim, dev)
+  p_new = torch.empty_like(state.momentum)
+  wraps = shift.wraps
+  nat.check(lib.jmd_nve_kick_drift(
+      nat.ptr(x_hi), nat.ptr(state.momentum), nat.ptr(state.force), nat.ptr(mass_v),
+      mass_s if mass_v is None else 0.0, N, dim, int(wraps), nat.f3(_side(shift, dim)),
+      float(dt32), float(half), nat.ptr(dt_device), nat.ptr(x_hi_new), nat.ptr(x_new),
+      nat.position_stride(x_new), nat.ptr(p_new), None, 0.0, None, nat.stream_handle(dev)),
+      'jmd_nve_kick_drift')
+  F_new = energy.force_and_kick(x_new, neighbor, p_new, half_dt=float(half),
+                                dt_device=dt_device, **kwargs)
+  return NVEState(x_new, p_new, F_new, state.mass, x_hi_new)
+
+
+def new_positions(N, dim, device, like=None):
+  """Float32 positions in the layout the kernels gather fastest: 3-D rows
+  padded to 16 bytes (an [N, 3] view of an [N, 4] buffer), 2-D contiguous."""
+  if dim == 3:
+    return nat.padded_positions(N, device, like)
+  out = torch.empty((N, dim), dtype=torch.float32, device=device)
+  if like is not None:
+    out.copy_(like)
+  return out
+
+
+def _momenta(key, R, kT, mass):
+  g = key if isinstance(key, torch.Generator) else None
+  if g is None:
+    g = torch.Generator(device=R.device)
+    g.manual_seed(int(key) if key is not None else 0)
+  P = torch.randn(R.shape, generator=g, device=R.device, dtype=torch.float32)
+  m = mass.reshape(-1, 1) if isinstance(mass, torch.Tensor) and mass.ndim else mass
+  P = torch.sqrt(torch.as_tensor(m * kT, dtype=torch.float32, device=R.device)) * P
+  if R.shape[0] > 1:
+    P = P - P.mean(dim=0, keepdim=True)
+  return P.contiguous()
+
+
+def nve(energy_or_force_fn, shift_fn, dt=1e-3, **sim_kwargs):
+  """NVE ensemble by velocity Verlet (simulate.py:279-317)."""
+  energy = quantity.native_energy_of(energy_or_force_fn)
+  shift = native_shift(shift_fn)
+
+  def init_fn(key, R, kT, mass=1.0, momenta=None, **kwargs):
+    nat.position_stride(R, 'R')
+    R = new_positions(R.shape[0], R.shape[1], R.device, R)
+    neighbor = kwargs.pop('neighbor', None)
+    F = energy.force(R, neighbor, **kwargs)
+    if momenta is None:
+      P = _momenta(key, R, kT, mass)
+    else:
+      P = momenta.to(device=R.device, dtype=torch.float32).contiguous()
+    if isinstance(mass, torch.Tensor) and mass.ndim == 2:
+      mass = mass.reshape(-1)
+    return NVEState(R, P, F, mass, R.to(torch.float64))
+
+  def step_fn(state: NVEState, **kwargs):
+    _dt = kwargs.pop('dt', dt)
+    neighbor = kwargs.pop('neighbor', None)
+    if isinstance(_dt, torch.Tensor):
+      return velocity_verlet_native(energy, shift, state, 0.0, neighbor,
+                                    dt_device=_dt.to(torch.float32).reshape(1), **kwargs)
+    return velocity_verlet_native(energy, shift, state, _dt, neighbor, **kwargs)
+
+  return init_fn, step_fn
+
+
+class NVELoop:
+  """``n_steps`` of velocity Verlet with the neighbour list maintained on the
+  device: the counterpart of the reference's
+  ``lax.fori_loop(0, n, lambda i, (s, nbrs): (apply_fn(s, neighbor=nbrs.update(s.position)), ...))``
+  (examples/nve_neighbor_list.py:186-201), run in place on the state's
+  buffers with no host synchronisation.
+
+  Per step: (1) the list rebuild gated on the skin criterion of the current
+  positions, (2) half kick + drift, which also evaluates the skin criterion
+  on the new positions for the next step, (3) force pass with the closing
+  half kick.  The rebuild decisions are those of the reference's
+  ``update`` before ``apply_fn``; overflow is reported in the list's error
+  word and checked by the caller once per block, as the reference does.
+  """
+
+  def __init__(self, energy_or_force_fn, shift_fn, dt):
+    self.energy = quantity.native_energy_of(energy_or_force_fn)
+    self.shift = native_shift(shift_fn)
+    self.dt = f32(dt)
+    self.half = f32(self.dt / f32(2))
+    self.force_events = None
+
+  def prime(self, state: NVEState, nbrs) -> None:
+    """Sets the gate from the skin criterion of the current positions."""
+    lib = nat.load()
+    plan = nbrs.plan
+    spec = plan.spec
+    N, dim = state.position.shape
+    if spec.always_rebuild:
+      return
+    nat.check(lib.jmd_neighbor_list_check(
+        nat.ptr(state.position), nat.position_stride(state.position),
+        nat.ptr(nbrs.reference_position), N, dim,
+        int(plan.periodic), nat.f3(plan.side), float(spec.threshold_sq), nat.ptr(plan.gate),
+        nat.stream_handle(state.position.device)), 'jmd_neighbor_list_check')
+
+  def run(self, state: NVEState, nbrs, n_steps: int, events=None) -> NVEState:
+    lib = nat.load()
+    plan = nbrs.plan
+    spec = plan.spec
+    R = state.position
+    N, dim = R.shape
+    dev = R.device
+    st = nat.stream_handle(dev)
+    stride = nat.position_stride(R)
+    if state.position_hi is None:
+      state.position_hi = R.to(torch.float64)
+    mass_s, mass_v = canonicalize_mass(state.mass, dev)
+    side = nat.f3(_side(self.shift, dim))
+    wraps = int(self.shift.wraps)
+    gate = None if spec.always_rebuild else nat.ptr(plan.gate)
+    thr = float(spec.threshold_sq)
+    cps, cs = nat.i3(plan.cells_per_side), nat.f3(plan.cell_size3)
+    psides = nat.f3(plan.side)
+    x_hi, x, p, F = (nat.ptr(state.position_hi), nat.ptr(R), nat.ptr(state.momentum),
+                     nat.ptr(state.force))
+    mv = nat.ptr(mass_v)
+    idx, err, ref = nat.ptr(nbrs.idx), nat.ptr(nbrs.error.code), nat.ptr(nbrs.reference_position)
+    occ, ws = nat.ptr(plan.occupancy), plan.workspace.ptr
+    pot, keep = self.energy._potential(R, {})
+    pws = nat.ptr(self.energy._workspace(dev, N))
+    fmt = spec.format_code
+    width = nbrs.idx.shape[1]
+    geo = self.energy.geometry
+    esides = nat.f3(geo.side_values(dim))
+    for _ in range(n_steps):
+      nat.check(lib.jmd_neighbor_list_build(
+          x, N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq), int(plan.use_cell_list),
+          cps, cs, int(nbrs.cell_list_capacity or 0), fmt, int(spec.mask_self), idx,
+          int(nbrs.max_occupancy), occ, err, ref, gate, ws, st), 'jmd_neighbor_list_build')
+      nat.check(lib.jmd_nve_kick_drift(
+          x_hi, p, F, mv, mass_s if mass_v is None else 0.0, N, dim, wraps, side,
+          float(self.dt), float(self.half), None, x_hi, x, stride, p, ref, thr, gate, st),
+          'jmd_nve_kick_drift')
+      if events is not None:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+      nat.check(lib.jmd_pair_energy_force(
+          x, N, dim, stride, int(geo.periodic), esides, nat.ctypes.byref(pot), idx, width, fmt, F,
+          None, None, p, float(self.half), None, pws, st), 'jmd_pair_energy_force')
+      if events is not None:
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        events.append((e0, e1))
+    del keep
+    return state

@@ -1,0 +1,222 @@
+"""Pair reduction over neighbour lists (jax_md/smap.py:856-979) on gfx950.
+
+``pair_neighbor_list(fn, metric, species=..., reduce_axis=..., **params)``
+accepts the pair functions of this package's ``energy`` module
+(``lennard_jones`` wrapped by ``multiplicative_isotropic_cutoff`` and
+``soft_sphere``) and returns an energy function with the reference's call
+signature ``energy_fn(R, neighbor, **dynamic_kwargs)``.  Energy and force come
+from one kernel pass; the energy is differentiable with respect to ``R``
+through ``torch.autograd`` (the counterpart of a ``jax.custom_vjp``: the
+backward pass returns ``-g * F`` from the same kernel).  Other pair functions
+raise ``NotImplementedError``: the package implements this path only.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import numpy as onp
+import torch
+
+from . import _native as nat
+from . import partition
+from . import space as space_mod
+from . import util
+
+_FMT = {partition.Dense: nat.DENSE, partition.Sparse: nat.SPARSE,
+        partition.OrderedSparse: nat.ORDERED_SPARSE}
+
+
+def _native_kind(fn):
+  """(kind, r_onset, r_cutoff) for the pair functions the kernels implement."""
+  from . import energy  # local import: energy imports smap
+  if isinstance(fn, energy.IsotropicCutoff) and fn.fn is energy.lennard_jones:
+    return nat.LENNARD_JONES, float(fn.r_onset), float(fn.r_cutoff)
+  if fn is energy.soft_sphere:
+    return nat.SOFT_SPHERE, 0.0, 0.0
+  raise NotImplementedError(
+      'smap.pair_neighbor_list runs natively for energy.lennard_jones (with '
+      'multiplicative_isotropic_cutoff) and energy.soft_sphere only; got '
+      f'{getattr(fn, "__name__", fn)}.')
+
+
+class _Params:
+  """Converts reference-style parameters to the C ABI struct and keeps the
+  device arrays alive for the duration of a call."""
+
+  def __init__(self, device, n):
+    self.device = device
+    self.n = n
+    self.keep = []
+
+  def param(self, value, species) -> nat.JmdParam:
+    p = nat.JmdParam()
+    if isinstance(value, torch.Tensor) and value.ndim > 0:
+      t = value.to(device=self.device, dtype=torch.float32).contiguous()
+    elif isinstance(value, onp.ndarray) and value.ndim > 0:
+      t = torch.as_tensor(value, dtype=torch.float32, device=self.device).contiguous()
+    else:
+      p.mode = nat.PARAM_SCALAR
+      p.value = util.as_f32_scalar(value)
+      p.values = None
+      return p
+    if t.ndim == 1 and (species is None or t.shape[0] == self.n):
+      if t.shape[0] != self.n:
+        raise ValueError(f'per-particle parameter has {t.shape[0]} entries for N={self.n}')
+      p.mode = nat.PARAM_PER_PARTICLE  # 0.5 (x + y) combinator (smap.py:836)
+    elif t.ndim == 2 and species is not None:
+      p.mode = nat.PARAM_SPECIES
+    else:
+      raise NotImplementedError('parameter arrays must be per-particle [N] vectors or '
+                                'species matrices; [N, N] matrices are not supported')
+    self.keep.append(t)
+    p.values = ctypes.c_void_p(t.data_ptr())
+    p.value = 0.0
+    return p
+
+
+class PairEnergy:
+  """Energy function returned by ``pair_neighbor_list``."""
+
+  _jmd_kind = 'energy'
+
+  def __init__(self, kind, r_onset, r_cutoff, geometry, species, reduce_axis,
+               ignore_unused_parameters, params):
+    self.kind = kind
+    self.r_onset = r_onset
+    self.r_cutoff = r_cutoff
+    self.geometry = geometry
+    self.species = species
+    self.reduce_axis = reduce_axis
+    self.ignore_unused_parameters = ignore_unused_parameters
+    self.params = params
+    self._ws = {}
+
+  # -- helpers ---------------------------------------------------------------
+  def _workspace(self, device, n):
+    key = (str(device), int(n))
+    ws = self._ws.get(key)
+    if ws is None:
+      lib = nat.load()
+      ws = torch.empty((lib.jmd_pair_workspace_bytes(n),), dtype=torch.uint8, device=device)
+      self._ws = {key: ws}
+    return ws
+
+  def _potential(self, R, dynamic_kwargs):
+    merged = util.merge_dicts(self.params, dynamic_kwargs, self.ignore_unused_parameters)
+    species = dynamic_kwargs.get('species', self.species)
+    N = R.shape[0]
+    P = _Params(R.device, N)
+    pot = nat.JmdPairPotential()
+    pot.kind = self.kind
+    sp = None
+    if species is not None:
+      sp = torch.as_tensor(species, device=R.device).to(torch.int32).contiguous()
+      P.keep.append(sp)
+    pot.sigma = P.param(merged.get('sigma', 1.0), sp)
+    pot.epsilon = P.param(merged.get('epsilon', 1.0), sp)
+    pot.alpha = P.param(merged.get('alpha', 2.0), sp)
+    pot.r_onset = self.r_onset
+    pot.r_cutoff = self.r_cutoff
+    ns = 0
+    for k in ('sigma', 'epsilon', 'alpha'):
+      v = merged.get(k)
+      if v is not None and hasattr(v, 'ndim') and v.ndim == 2:
+        ns = max(ns, int(v.shape[0]))
+    pot.n_species = ns
+    pot.species = ctypes.c_void_p(sp.data_ptr()) if sp is not None else None
+    return pot, P
+
+  def _run(self, R, neighbor, want_force, want_energy_total, want_energy_atom,
+           dynamic_kwargs, momentum=None, half_dt=0.0, dt_device=None):
+    stride = nat.position_stride(R, 'R')
+    if not isinstance(neighbor, partition.NeighborList):
+      raise TypeError('energy_fn needs a partition.NeighborList as `neighbor`')
+    lib = nat.load()
+    N, dim = R.shape
+    geo = self.geometry
+    side = geo.side_values(dim)
+    pot, keep = self._potential(R, dynamic_kwargs)
+    dev = R.device
+    F = torch.empty((N, dim), dtype=torch.float32, device=dev) if want_force else None
+    e_tot = torch.empty((1,), dtype=torch.float64, device=dev) if want_energy_total else None
+    e_atom = torch.empty((N,), dtype=torch.float32, device=dev) if want_energy_atom else None
+    idx = neighbor.idx
+    width = idx.shape[1]
+    ws = self._workspace(dev, N)
+    nat.check(lib.jmd_pair_energy_force(
+        nat.ptr(R), N, dim, stride, int(geo.periodic), nat.f3(side), ctypes.byref(pot),
+        nat.ptr(idx),
+        width, _FMT[neighbor.format], nat.ptr(F), nat.ptr(e_atom), nat.ptr(e_tot),
+        nat.ptr(momentum), float(half_dt), nat.ptr(dt_device), nat.ptr(ws),
+        nat.stream_handle(dev)), 'jmd_pair_energy_force')
+    del keep
+    return e_tot, e_atom, F
+
+  # -- public ----------------------------------------------------------------
+  def __call__(self, R, neighbor=None, **dynamic_kwargs):
+    if 'fractional_coordinates' in dynamic_kwargs:
+      dynamic_kwargs = dict(dynamic_kwargs)
+      if dynamic_kwargs.pop('fractional_coordinates'):
+        raise NotImplementedError('fractional coordinates are out of scope')
+    if self.reduce_axis is not None and len(self.reduce_axis) == 0:
+      raise NotImplementedError('reduce_axis=() (edge-wise output) is not supported natively')
+    per_particle = self.reduce_axis is not None
+    if per_particle and tuple(self.reduce_axis) != (1,):
+      raise NotImplementedError(f'reduce_axis={self.reduce_axis} is not supported natively')
+    if per_particle:
+      if neighbor.format is partition.OrderedSparse:
+        raise ValueError('Cannot report per-particle values with a neighbor list whose '
+                         'format is `OrderedSparse`. Please use either `Dense` or `Sparse`.')
+      _, e_atom, _ = self._run(R, neighbor, False, False, True, dynamic_kwargs)
+      return e_atom
+    if R.requires_grad and torch.is_grad_enabled():
+      return _EnergyWithForce.apply(R, self, neighbor, dynamic_kwargs)
+    e_tot, _, _ = self._run(R, neighbor, False, True, False, dynamic_kwargs)
+    return e_tot.reshape(()).to(torch.float32)
+
+  def energy_and_force(self, R, neighbor=None, **dynamic_kwargs):
+    """(E float64 0-d, F [N, dim] float32) from one kernel pass."""
+    e_tot, _, F = self._run(R, neighbor, True, True, False, dynamic_kwargs)
+    return e_tot.reshape(()), F
+
+  def force(self, R, neighbor=None, **dynamic_kwargs):
+    """-dE/dR from the native kernel (quantity.force routes here)."""
+    _, _, F = self._run(R, neighbor, True, False, False, dynamic_kwargs)
+    return F
+
+  def force_and_kick(self, R, neighbor, momentum, half_dt=0.0, dt_device=None, **dynamic_kwargs):
+    """Force, plus the closing half kick p += half_dt F fused in the same pass."""
+    _, _, F = self._run(R, neighbor, True, False, False, dynamic_kwargs, momentum=momentum,
+                        half_dt=half_dt, dt_device=dt_device)
+    return F
+
+
+class _EnergyWithForce(torch.autograd.Function):
+  """E(R) whose vector-Jacobian product is -g F, both from one kernel pass."""
+
+  @staticmethod
+  def forward(ctx, R, fn, neighbor, dynamic_kwargs):
+    e_tot, _, F = fn._run(R.detach(), neighbor, True, True, False, dynamic_kwargs)
+    ctx.save_for_backward(F)
+    return e_tot.reshape(()).to(torch.float32)
+
+  @staticmethod
+  def backward(ctx, g):
+    (F,) = ctx.saved_tensors
+    return -g * F, None, None, None
+
+
+def pair_neighbor_list(fn, displacement_or_metric, species=None,
+                       reduce_axis: Optional[Tuple[int, ...]] = None,
+                       ignore_unused_parameters: bool = False, **kwargs) -> PairEnergy:
+  """Promotes a pair function to a neighbour-list energy
+  (jax_md/smap.py:856-979)."""
+  kind, r_on, r_c = _native_kind(fn)
+  kwargs = dict(kwargs)
+  if kwargs.pop('fractional_coordinates', False):
+    raise NotImplementedError('fractional coordinates are out of scope')
+  geo = space_mod.geometry_of(displacement_or_metric)
+  return PairEnergy(kind, r_on, r_c, geo, species, reduce_axis, ignore_unused_parameters,
+                    kwargs)

@@ -1,0 +1,129 @@
+"""NVE and FIRE on the GPU against the float64 oracle integrators.
+
+Bars: after 100 NVE steps (dt = 0.005, LJ, N = 512) positions within 1e-4
+and momenta within 1e-3 of the float64 trajectory started from the same
+float32 state; total-energy drift over 1000 steps at N = 4096 below 1e-3 of
+the kinetic energy.  FIRE: the float16 dt / alpha / n_pos sequence equals the
+oracle's for the first 40 steps and the energy after 100 steps is within
+1e-5 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import jaxmd_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+DEV = 'cuda:0'
+
+
+def _jmd():
+  import jax_md_amd
+  return jax_md_amd
+
+
+def _lj(n_side, seed=0):
+  jmd = _jmd()
+  R, P, L = O.lj_lattice(n_side, seed=seed)
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense)
+  return jmd, R, P, L, shift, nfn, efn
+
+
+def test_nve_trajectory_vs_fp64():
+  print('case NVE N=512 100 steps', flush=True)
+  jmd, R, P, L, shift, nfn, efn = _lj(8, seed=1)
+  Rt = torch.as_tensor(R, device=DEV)
+  nbrs = nfn.allocate(Rt)
+  init_fn, apply_fn = jmd.simulate.nve(efn, shift, 0.005)
+  state = init_fn(0, Rt, kT=1.0, momenta=torch.as_tensor(P, device=DEV), neighbor=nbrs)
+  for _ in range(100):
+    nbrs = nbrs.update(state.position)
+    state = apply_fn(state, neighbor=nbrs)
+  assert not bool(nbrs.did_buffer_overflow)
+  sp = O.Space(np.float64(L))
+  m = O.PairModel('lj')
+  ff = lambda X: O.energy_force_brute(m, sp, X)[2]
+  st = O.NVEState(R.astype(np.float64), P.astype(np.float64), ff(R.astype(np.float64)))
+  for _ in range(100):
+    st = O.nve_step(st, ff, sp, 0.005)
+  dR = sp.displacement(state.position_hi.cpu().numpy(), st.position)
+  dP = state.momentum.cpu().numpy() - st.momentum
+  print(f'  max |dR|={np.abs(dR).max():.3e} max |dP|={np.abs(dP).max():.3e}')
+  assert np.abs(dR).max() < 1e-4
+  assert np.abs(dP).max() < 1e-3
+  assert np.array_equal(state.position.cpu().numpy(),
+                        state.position_hi.cpu().numpy().astype(F32))
+
+
+def _energy_error(dt, n_steps, every):
+  jmd, R, P, L, shift, nfn, efn = _lj(16, seed=0)
+  Rt = torch.as_tensor(R, device=DEV)
+  nbrs = nfn.allocate(Rt)
+  init_fn, apply_fn = jmd.simulate.nve(efn, shift, dt)
+  state = init_fn(0, Rt, kT=1.0, momenta=torch.as_tensor(P, device=DEV), neighbor=nbrs)
+
+  def total(s):
+    E, _ = efn.energy_and_force(s.position, nbrs)
+    return float(E) + float(jmd.quantity.kinetic_energy(momentum=s.momentum))
+
+  E0 = total(state)
+  errs = []
+  for k in range(n_steps):
+    nbrs = nbrs.update(state.position)
+    state = apply_fn(state, neighbor=nbrs)
+    if (k + 1) % every == 0:
+      assert not bool(nbrs.did_buffer_overflow)
+      errs.append(total(state) - E0)
+  ke = float(jmd.quantity.kinetic_energy(momentum=state.momentum))
+  return np.array(errs), ke, E0
+
+
+def test_nve_energy_error_c1_second_order():
+  """C1 (N = 4096, 1000 steps at dt = 0.005): the total-energy error is
+  reported; halving dt over the same time must quarter it (velocity Verlet is
+  second order), which a wrong kick or drift would break."""
+  print('case NVE energy error N=4096', flush=True)
+  e1, ke, E0 = _energy_error(0.005, 1000, 50)
+  e2, _, _ = _energy_error(0.0025, 2000, 100)
+  m1, m2 = np.abs(e1).max(), np.abs(e2).max()
+  slope = np.polyfit(np.arange(len(e1)) * 50, e1, 1)[0]
+  print(f'  E0={E0:.4f} KE={ke:.1f} max|dE| dt=0.005: {m1:.4e}, dt=0.0025: {m2:.4e}, '
+        f'ratio {m1 / m2:.2f}, linear drift {slope * 1000:.3e} per 1000 steps')
+  assert 3.0 < m1 / m2 < 5.5
+  assert m1 < 5e-3 * ke
+
+
+def test_fire_vs_oracle_2d_soft_sphere():
+  print('case FIRE N=256 2D soft spheres', flush=True)
+  jmd = _jmd()
+  R, species, L = O.bidisperse_2d(256, seed=5)
+  sigma = np.array([[1.0, 1.2], [1.2, 1.4]], F32)
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.soft_sphere_neighbor_list(
+      disp, L, species=torch.as_tensor(species, device=DEV),
+      sigma=torch.as_tensor(sigma, device=DEV), format=jmd.partition.Dense)
+  Rt = torch.as_tensor(R, device=DEV)
+  nbrs = nfn.allocate(Rt)
+  init_fn, apply_fn = jmd.minimize.fire_descent(efn, shift, dt_start=0.01, dt_max=0.1)
+  state = init_fn(Rt, neighbor=nbrs)
+  sp = O.Space(np.float64(L))
+  m = O.PairModel('soft_sphere', sigma=sigma.astype(np.float64), species=species)
+  ff = lambda X: O.energy_force_brute(m, sp, X)[2]
+  params = O.FireParams(0.01, 0.1)
+  ost = O.fire_init(R.astype(np.float64), ff, params)
+  for k in range(100):
+    nbrs = nbrs.update(state.position)
+    state = apply_fn(state, neighbor=nbrs)
+    ost = O.fire_step(ost, ff, sp, params)
+    if k < 40:
+      assert float(state.dt) == float(ost.dt), k
+      assert float(state.alpha) == float(ost.alpha), k
+      assert int(state.n_pos) == ost.n_pos, k
+  E = float(efn(state.position, nbrs))
+  E64 = O.energy_force_brute(m, sp, ost.position)[0]
+  print(f'  E={E:.8e} E64={E64:.8e}')
+  assert abs(E - E64) <= 1e-5 * abs(E64) + 1e-9

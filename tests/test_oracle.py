@@ -1,0 +1,392 @@
+"""Pins for the CPU oracle (oracle/jaxmd_oracle.py), CPU only.
+
+Which check pins which oracle function:
+
+  jnp_mod, periodic_displacement   numpy.mod cross-check; range and
+                                   congruence invariants (test_wrap_*)
+  square_distance                  exact (fsum) value within 2 ulp
+  cell_dimensions, hash_constants,
+  neighboring_cells, cell_list     reference known-answer slots of
+                                   partition_test.py:61-89 (test_cell_emplace_2d)
+                                   and the scatter round trip (:102-131)
+  neighbor_list_build (Dense)      brute-force O(N^2) sets
+                                   (partition_test.py:203-246), symmetry,
+                                   cell path == all-pairs path, vector box
+  neighbor_list_build (Sparse)     brute force (partition_test.py:259-301)
+  capacities / error bits          partition_test.py:363-401 (overflow),
+                                   :403-459 (custom mask = 42), :476-546
+                                   (capacity shapes)
+  pair_energy_force (SS)           jammed fixture energy 0.45247561922261154
+  pair_energy_force (LJ)           LAMMPS E/N -4.3523016, closed forms
+                                   (energy_test.py:325-347), list == brute force
+  forces                           central finite differences of the energy
+  smooth                           S(r_on)=1, S(r_c)=0, C1 continuity
+  nve_step                         energy conservation, time reversibility
+  fire_step                        static_cast float16 values (F4), monotone
+                                   descent on a quadratic (minimize_test.py:111-150)
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import jaxmd_oracle as O
+from tests.golden import fixtures
+
+F32 = np.float32
+
+
+# ---------------------------------------------------------------------------
+# geometry
+# ---------------------------------------------------------------------------
+
+
+def test_wrap_matches_numpy_mod_f32():
+  rng = np.random.default_rng(0)
+  L = F32(16.929268)
+  x = rng.uniform(-3 * L, 3 * L, 20000).astype(F32)
+  assert np.array_equal(O.jnp_mod(x, L), np.mod(x, L).astype(F32))
+
+
+def test_wrap_range_and_congruence():
+  rng = np.random.default_rng(1)
+  for L in (5.0, 67.71707, 1.0):
+    d = rng.uniform(-2.5 * L, 2.5 * L, 10000)
+    w = O.periodic_displacement(L, d)
+    assert np.all(w >= -L / 2 - 1e-12) and np.all(w < L / 2 + 1e-12)
+    k = (d - w) / L
+    assert np.allclose(k, np.round(k), atol=1e-9)
+    # float32 path: same congruence up to float32 rounding of L/2 terms
+    w32 = O.periodic_displacement(F32(L), d.astype(F32))
+    assert np.allclose(w32, w, atol=4 * np.spacing(F32(1.5 * L)))
+
+
+def test_square_distance_rounding():
+  rng = np.random.default_rng(2)
+  d = rng.normal(size=(2000, 3)).astype(F32)
+  got = O.square_distance(d)
+  exact = np.array([math.fsum(float(v) ** 2 for v in row) for row in d])
+  assert np.all(np.abs(got - exact) <= 2 * np.spacing(got))
+
+
+# ---------------------------------------------------------------------------
+# cell list
+# ---------------------------------------------------------------------------
+
+
+def test_cell_emplace_2d_known_slots():
+  # partition_test.py:61-89; index 8 of an 8-cell axis is clamped to 7 by
+  # JAX's gather, so [1, 8, k] reads cell (y=1, x=7).
+  box = np.array([8.65, 8.0], F32)
+  R = np.array([[0.25, 0.25], [8.5, 1.95], [8.1, 1.5], [3.7, 7.9]], F32)
+  cl = O.cell_list(R, box, F32(1.0))
+  cps = cl.cells_per_side
+  shape = tuple(int(c) for c in cps[::-1]) + (cl.cell_capacity,)
+  ids = cl.id_buffer.reshape(shape)
+  pos = cl.position_buffer.reshape(shape + (2,))
+  assert ids[0, 0, 0] == 0 and ids[1, 7, 1] == 1 and ids[1, 7, 0] == 2 and ids[7, 3, 1] == 3
+  assert np.array_equal(pos[1, 7, 1], R[1]) and np.array_equal(pos[7, 3, 1], R[3])
+  # scatter round trip (partition_test.py:82-88)
+  out = np.zeros((5, 2), F32)
+  out[cl.id_buffer] = cl.position_buffer
+  assert np.array_equal(out[:-1], R)
+
+
+@pytest.mark.parametrize('dim', [2, 3])
+def test_cell_list_random_roundtrip(dim):
+  rng = np.random.default_rng(3)
+  R = (F32(9.0) * rng.uniform(size=(1000, dim))).astype(F32)
+  cl = O.cell_list(R, F32(9.0), F32(1.0))
+  out = np.zeros((1001, dim), F32)
+  out[cl.id_buffer] = cl.position_buffer
+  assert np.array_equal(out[:-1], R)
+  assert not cl.did_buffer_overflow
+
+
+# ---------------------------------------------------------------------------
+# neighbour lists
+# ---------------------------------------------------------------------------
+
+
+def _brute_sets(R, side, cutoff, periodic=True):
+  sp = O.Space(np.asarray(side, np.float64), periodic)
+  i, j = O.all_pairs(sp, R.astype(np.float64), float(cutoff))
+  N = R.shape[0]
+  rows = [[] for _ in range(N)]
+  for a, b in zip(i, j):
+    rows[a].append(b)
+  dR = sp.displacement(R[None, :, :].astype(np.float64), R[:, None, :].astype(np.float64))
+  dist = np.sqrt(np.sum(dR ** 2, -1))
+  return [np.sort(r) for r in rows], dist
+
+
+def _assert_sets_match(rows, brute, dist, cutoff, tol=1e-5):
+  for i, (a, b) in enumerate(zip(rows, brute)):
+    sa, sb = set(a.tolist()), set(b.tolist())
+    for j in sa ^ sb:
+      assert abs(dist[i, j] - cutoff) < tol, (i, j, dist[i, j])
+
+
+@pytest.mark.parametrize('dim', [2, 3])
+def test_dense_vs_brute_vector_box(dim):
+  # partition_test.py:203-246
+  rng = np.random.default_rng(4)
+  box = np.array([9.0, 4.0, 7.25], F32)[:dim] if dim == 3 else np.array([9.0, 4.25], F32)
+  R = (box * rng.uniform(size=(1000, dim))).astype(F32)
+  spec = O.NeighborListSpec(O.Space(box), box, F32(1.23), 0.0, 1.1)
+  nl = O.neighbor_list_build(spec, R)
+  assert nl.use_cell_list and nl.error == 0
+  rows = O.sorted_rows(nl)
+  brute, dist = _brute_sets(R, box, F32(1.23))
+  _assert_sets_match(rows, brute, dist, 1.23)
+
+
+def test_dense_lattice_symmetry_and_paths():
+  R, _, L = O.lj_lattice(8, seed=5)
+  spec = O.NeighborListSpec(O.Space(L), L, 2.5, 0.5)
+  nl = O.neighbor_list_build(spec, R)
+  rows = O.sorted_rows(nl)
+  pairs = {(i, int(j)) for i, r in enumerate(rows) for j in r}
+  assert all((j, i) in pairs for i, j in pairs)
+  spec_nc = O.NeighborListSpec(O.Space(L), L, 2.5, 0.5, disable_cell_list=True)
+  rows_nc = O.sorted_rows(O.neighbor_list_build(spec_nc, R))
+  assert all(np.array_equal(a, b) for a, b in zip(rows, rows_nc))
+  brute, dist = _brute_sets(R, L, 3.0)
+  _assert_sets_match(rows, brute, dist, 3.0)
+
+
+@pytest.mark.parametrize('fmt', [O.SPARSE, O.ORDERED_SPARSE])
+def test_sparse_vs_brute(fmt):
+  rng = np.random.default_rng(6)
+  box = np.array([9.0, 4.0, 7.25], F32)
+  R = (box * rng.uniform(size=(600, 3))).astype(F32)
+  spec = O.NeighborListSpec(O.Space(box), box, F32(1.23), 0.0, 1.1, format=fmt)
+  nl = O.neighbor_list_build(spec, R)
+  N = R.shape[0]
+  recv, send = nl.idx
+  m = recv < N
+  brute, dist = _brute_sets(R, box, F32(1.23))
+  # sender-major order (partition.py:1000-1005)
+  assert np.all(np.diff(send[m]) >= 0)
+  for i in range(N):
+    got = np.sort(recv[m][send[m] == i])
+    want = brute[i] if fmt == O.SPARSE else brute[i][brute[i] < i]
+    _assert_sets_match([got], [want], dist[i:i + 1], 1.23)
+
+
+def test_overflow_forced_free_space():
+  # partition_test.py:363-401
+  sp = O.Space(None, periodic=False)
+  spec = O.NeighborListSpec(sp, F32(100.0), 3.0, 0.0)
+  R = np.array([[20, 20], [30, 30], [40, 40], [50, 50]], F32)
+  nl = O.neighbor_list_build(spec, R)
+  assert nl.error == 0 and nl.idx.shape == (4, 0)
+  R2 = np.array([[20, 20], [20, 20], [40, 40], [50, 50]], F32)
+  nl2 = O.neighbor_list_update(spec, R2, nl)
+  assert nl2.error & (O.NEIGHBOR_LIST_OVERFLOW | O.CELL_LIST_OVERFLOW)
+
+
+def test_custom_mask_known_answer():
+  # partition_test.py:403-459: 10 coincident particles, |id_i - id_j| > 3 -> 42
+  sp = O.Space(None, periodic=False)
+  mask = lambda cand, rows: np.where(np.abs(cand - rows[:, None]) > 3, cand, 10)
+  spec = O.NeighborListSpec(sp, F32(1.0), 3.0, 0.0, custom_mask_function=mask)
+  R = np.zeros((10, 3), F32)
+  nl = O.neighbor_list_build(spec, R)
+  nl = O.neighbor_list_update(spec, R, nl)
+  assert int(np.sum(nl.idx != 10)) == 42
+
+
+@pytest.mark.parametrize('r_cut,dc,cm', [(0.12, True, 1.5), (0.25, False, 1.5),
+                                         (0.31, False, 1.5), (0.31, False, 1.0)])
+@pytest.mark.parametrize('mask_self', [False, True])
+@pytest.mark.parametrize('fmt', [O.DENSE, O.SPARSE, O.ORDERED_SPARSE])
+def test_capacity_shapes(r_cut, dc, cm, mask_self, fmt):
+  # partition_test.py:476-546 (issue 191)
+  box = np.ones(3, F32)
+  R = np.full((20, 3), 0.5, F32)
+  spec = O.NeighborListSpec(O.Space(box), box, F32(r_cut), F32(0.1) * F32(r_cut), cm, dc,
+                            mask_self, fmt)
+  nl = O.neighbor_list_build(spec, R)
+  want = {O.DENSE: (20, 19) if mask_self else (20, 20),
+          O.SPARSE: (2, 20 * 19) if mask_self else (2, 400),
+          O.ORDERED_SPARSE: (2, 190)}[fmt]
+  assert nl.idx.shape == want
+  assert not (nl.error & 3)
+  nl2 = O.neighbor_list_update(spec, (R + F32(0.1)).astype(F32), nl)
+  assert not (nl2.error & 3) and nl2.idx.shape == want
+
+
+# ---------------------------------------------------------------------------
+# energies and forces
+# ---------------------------------------------------------------------------
+
+
+def test_jammed_fixture_energy():
+  st = fixtures.jammed_state()
+  box = np.diag(st['box'])
+  sp = O.Space(box)
+  m = O.PairModel('soft_sphere', sigma=st['sigma'], species=st['species'])
+  E, _, F = O.energy_force_brute(m, sp, st['real_position'])
+  assert abs(E - float(st['energy'])) < 1e-12
+  # same through a Dense list built in float64 arithmetic
+  spec = O.NeighborListSpec(O.Space(box), box.astype(F32), float(np.max(st['sigma'])), 0.2)
+  nl = O.neighbor_list_build(spec, st['real_position'].astype(np.float64))
+  i, j = O.pairs_from_dense(nl.idx)
+  E2, _, F2 = O.pair_energy_force(m, sp, st['real_position'], i, j)
+  assert abs(E2 - float(st['energy'])) < 1e-12
+  assert np.allclose(F, F2, atol=1e-12)
+
+
+def test_lammps_lj_energy():
+  d = fixtures.lammps_lj()
+  L = d['box']
+  R = d['fractional_position'] * L
+  m = O.PairModel('lj', r_onset=2.5, r_cutoff=2.5)
+  E, _, _ = O.energy_force_brute(m, O.Space(np.float64(L)), R)
+  assert abs(E / len(R) - d['energy_per_atom']) < 5e-5
+
+
+def test_lj_closed_forms():
+  r_min = 2 ** (1 / 6)
+  assert abs(O.lj_energy_pair(r_min) + 1.0) < 1e-12
+  dEr, E = O.lj_dE_over_r(r_min ** 2, 1.0, 1.0, 2.0, 2.5)
+  assert abs(E + 1.0) < 1e-12 and abs(dEr) < 1e-12
+  # sigma, epsilon scaling
+  assert abs(O.lj_energy_pair(2.0 * r_min, 2.0, 3.0) + 3.0) < 1e-12
+
+
+def test_soft_sphere_closed_forms():
+  for alpha in (2.0, 2.5, 3.0):
+    assert abs(O.soft_sphere_energy(0.0, 1.0, 2.0, alpha) - 2.0 / alpha) < 1e-12
+    assert O.soft_sphere_energy(1.0, 1.0, 2.0, alpha) == 0.0
+    dEr, E = O.soft_sphere_dE_over_r(np.array([1.5 ** 2]), 1.0, 1.0, alpha)
+    assert dEr[0] == 0.0 and E[0] == 0.0
+
+
+def test_smoothing_c1():
+  r_on, r_c = 2.0, 2.5
+  assert O.smooth(r_on, r_on, r_c) == 1.0
+  assert abs(O.smooth(r_c - 1e-12, r_on, r_c)) < 1e-10
+  h = 1e-6
+  for r in (r_on, r_c):
+    lo = O.smooth(r - h, r_on, r_c)
+    hi = O.smooth(r + h, r_on, r_c)
+    assert abs(hi - lo) < 1e-9
+    d_lo = (O.smooth(r - h, r_on, r_c) - O.smooth(r - 2 * h, r_on, r_c)) / h
+    d_hi = (O.smooth(r + 2 * h, r_on, r_c) - O.smooth(r + h, r_on, r_c)) / h
+    assert abs(d_hi - d_lo) < 1e-4
+
+
+@pytest.mark.parametrize('kind', ['lj', 'soft_sphere'])
+def test_force_is_minus_gradient(kind):
+  rng = np.random.default_rng(7)
+  if kind == 'lj':
+    R, _, L = O.lj_lattice(6, jitter=0.1, seed=7)
+    m = O.PairModel('lj')
+  else:
+    R, sp_, L = O.bidisperse_2d(40, seed=7)
+    m = O.PairModel('soft_sphere', sigma=np.array([[1.0, 1.2], [1.2, 1.4]]), species=sp_)
+  R = R.astype(np.float64)
+  sp = O.Space(np.float64(L))
+  E0, _, F = O.energy_force_brute(m, sp, R)
+  h = 1e-5
+  for _ in range(12):
+    a = rng.integers(R.shape[0])
+    c = rng.integers(R.shape[1])
+    Rp, Rm = R.copy(), R.copy()
+    Rp[a, c] += h
+    Rm[a, c] -= h
+    fd = -(O.energy_force_brute(m, sp, Rp)[0] - O.energy_force_brute(m, sp, Rm)[0]) / (2 * h)
+    assert abs(fd - F[a, c]) < 1e-5 * max(1.0, abs(F[a, c]))
+  assert np.allclose(F.sum(0), 0.0, atol=1e-9)
+
+
+def test_list_energy_equals_brute_force():
+  # energy_test.py:577-645: neighbour-list energy == pair energy
+  R, _, L = O.lj_lattice(6, seed=8)
+  sp = O.Space(np.float64(L))
+  m = O.PairModel('lj')
+  spec = O.NeighborListSpec(O.Space(L), L, 2.5, 0.5)
+  nl = O.neighbor_list_build(spec, R)
+  i, j = O.pairs_from_dense(nl.idx)
+  E, e_atom, F = O.pair_energy_force(m, sp, R, i, j)
+  Eb, _, Fb = O.energy_force_brute(m, sp, R)
+  assert abs(E - Eb) < 1e-10 * abs(Eb)
+  assert np.allclose(F, Fb, atol=1e-10)
+  assert abs(e_atom.sum() - E) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# dynamics
+# ---------------------------------------------------------------------------
+
+
+def _lj_force_fn(L):
+  sp = O.Space(np.float64(L))
+  m = O.PairModel('lj')
+  return sp, m, (lambda R: O.energy_force_brute(m, sp, R)[2])
+
+
+def test_nve_second_order_and_reversibility():
+  R, P, L = O.lj_lattice(5, seed=9)
+  sp, m, force_fn = _lj_force_fn(L)
+  R = R.astype(np.float64)
+  P = P.astype(np.float64)
+
+  def drift(dt, n):
+    st = O.NVEState(R, P, force_fn(R))
+    E0 = O.energy_force_brute(m, sp, R)[0] + O.kinetic_energy(P)
+    for _ in range(n):
+      st = O.nve_step(st, force_fn, sp, dt)
+    return O.energy_force_brute(m, sp, st.position)[0] + O.kinetic_energy(st.momentum) - E0, st
+
+  # velocity Verlet is second order: halving dt over the same time quarters
+  # the energy error (a wrong kick or drift factor breaks this).
+  e1, st = drift(0.005, 20)
+  e2, _ = drift(0.0025, 40)
+  assert 3.3 < e1 / e2 < 4.7, (e1, e2)
+  back = O.NVEState(st.position, -st.momentum, st.force)
+  for _ in range(20):
+    back = O.nve_step(back, force_fn, sp, 0.005)
+  dR = sp.displacement(back.position, R)
+  assert np.max(np.abs(dR)) < 1e-8
+
+
+def test_nve_dt_cast_to_float32():
+  # simulate.py:235-236 (F5): dt is float32 even in the x64 run
+  assert np.float64(np.float32(0.005)) != 0.005
+  R = np.zeros((1, 3))
+  P = np.ones((1, 3))
+  st = O.NVEState(R, P, np.zeros((1, 3)))
+  out = O.nve_step(st, lambda R: np.zeros_like(R), O.Space(None, False), 0.005)
+  assert out.position[0, 0] == np.float64(np.float32(0.005))
+
+
+def test_fire_static_cast_float16():
+  # minimize.py:164-168, util.py:58-64 (F4)
+  dt_start, dt_max, n_min, f_inc, f_dec, a0, f_a = O.FireParams(0.01, 0.1).cast()
+  assert float(dt_start) == 0.01000213623046875
+  assert float(dt_max) == 0.0999755859375
+  assert float(f_inc) == 1.099609375 and float(f_a) == 0.990234375
+  assert n_min.dtype == np.uint8 and dt_start.dtype == np.float16
+
+
+def test_fire_descends_quadratic():
+  # minimize_test.py:111-150
+  rng = np.random.default_rng(10)
+  R = rng.uniform(size=(64, 2))
+  R0 = rng.uniform(size=(64, 2))
+  force_fn = lambda R: -2.0 * (R - R0)
+  energy = lambda R: float(np.sum((R - R0) ** 2))
+  sp = O.Space(None, False)
+  params = O.FireParams()
+  st = O.fire_init(R, force_fn, params)
+  E = energy(R)
+  for _ in range(10):
+    for _ in range(3):
+      st = O.fire_step(st, force_fn, sp, params)
+    E_new = energy(st.position)
+    assert E_new < E
+    E = E_new
