@@ -7,10 +7,16 @@ jitter N(0, 0.05), momenta N(0, kT=1) with the mean removed; LJ sigma = eps =
 with skin 0.5 rebuilt on the skin criterion, dt = 0.005, float32 pair
 arithmetic, float64 master positions and energy reductions.  One step = skin
 rebuild (gated on the device) + half kick / drift + force pass with the
-closing half kick.  Steps run in blocks of 100 with a host overflow check per
-block; an overflowing block is redone from its saved start state after
-re-allocating the list (the reference's protocol,
-examples/nve_neighbor_list.py:186-201).
+closing half kick.  Steps run in blocks of 100, each block one HIP graph
+replay, with a host overflow check per block; an overflowing block is redone
+from its saved start state after re-allocating the list (the reference's
+protocol, examples/nve_neighbor_list.py:186-201).
+
+The force kernel's duration is measured with HIP events around a graph of
+20 back-to-back force launches on the loop's own cell-ordered buffers (the
+state the timed region ended in), on the stream they run on.  HBM traffic per
+launch comes from the committed counter summary of the same kernel and
+workload (profiles/k_pair_dense_lj_n262144_pmc.json), when present.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -32,6 +38,47 @@ def log(*a):
   print(*a, file=sys.stderr, flush=True)
 
 
+def force_kernel_us(jmd, loop, state, nbrs, efn, reps=20):
+  """Average duration of the force + half-kick kernel, from HIP events around
+  a graph of `reps` launches on the loop's cell-ordered buffers."""
+  import torch
+  b = loop._bufs
+  if b is not None:
+    import dataclasses
+    x, idx = b['x'], b['idx']
+    nb = dataclasses.replace(nbrs, idx=idx)
+  else:
+    x, nb = state.position, nbrs
+  p = state.momentum.clone() if b is None else b['p'].clone()
+  half = float(loop.half)
+  efn.force_and_kick(x, nb, p, half_dt=half)  # warm
+  g = torch.cuda.CUDAGraph()
+  with torch.cuda.graph(g):
+    for _ in range(reps):
+      efn.force_and_kick(x, nb, p, half_dt=half)
+  s = torch.cuda.current_stream()
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  g.replay()
+  e0.record(s)
+  for _ in range(5):
+    g.replay()
+  e1.record(s)
+  torch.cuda.synchronize()
+  return e0.elapsed_time(e1) * 1e3 / (5 * reps)
+
+
+def counter_traffic(N):
+  """HBM bytes per force launch from the committed counter summary, or None."""
+  path = os.path.join(ROOT, 'profiles', f'k_pair_dense_lj_n{N}_pmc.json')
+  try:
+    with open(path) as f:
+      d = json.load(f)
+    return float(d['traffic_bytes_per_launch'])
+  except (OSError, KeyError, ValueError):
+    return None
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument('--gpus', type=int, default=1)
@@ -39,9 +86,10 @@ def main():
   ap.add_argument('--warmup', type=int, default=200)
   ap.add_argument('--n-side', type=int, default=64)
   ap.add_argument('--block', type=int, default=100)
-  ap.add_argument('--cpu-steps', type=int, default=40)
+  ap.add_argument('--cpu-steps', type=int, default=250)
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--tpa', type=int, default=0, help='threads per row of the pair kernel')
+  ap.add_argument('--eager', action='store_true', help='no HIP graphs (host-launched steps)')
   args = ap.parse_args()
   if args.gpus != 1:
     print(f'bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU '
@@ -67,7 +115,7 @@ def main():
   nbrs = nfn.allocate(Rt)
   init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
   state = init_fn(0, Rt.clone(), kT=1.0, momenta=torch.as_tensor(P, device=dev), neighbor=nbrs)
-  loop = jmd.simulate.NVELoop(efn, shift, 0.005)
+  loop = jmd.simulate.NVELoop(efn, shift, 0.005, graph=not args.eager)
   log(f'N={N} L={L} max_occupancy={nbrs.max_occupancy} cell_capacity={nbrs.cell_list_capacity}')
 
   def snapshot(s):
@@ -81,15 +129,14 @@ def main():
 
   reallocs = 0
 
-  def run_steps(n, events=None):
+  def run_steps(n):
     nonlocal nbrs, reallocs
     done = 0
     loop.prime(state, nbrs)
     while done < n:
       k = min(args.block, n - done)
       snap = snapshot(state)
-      ev = [] if events is not None else None
-      loop.run(state, nbrs, k, ev)
+      loop.run(state, nbrs, k)
       if bool(nbrs.did_buffer_overflow):
         restore(state, snap)
         nbrs = nfn.allocate(state.position)
@@ -97,9 +144,8 @@ def main():
         reallocs += 1
         log(f'  overflow: re-allocated, max_occupancy={nbrs.max_occupancy}')
         continue
-      if events is not None:
-        events.extend(ev)
       done += k
+      log(f'  {done}/{n} steps')
 
   run_steps(args.warmup)
   torch.cuda.synchronize()
@@ -107,10 +153,9 @@ def main():
   E_start, _ = efn.energy_and_force(state.position, nbrs)
   KE_start = jmd.quantity.kinetic_energy(momentum=state.momentum)
   builds0 = nbrs.plan.builds
-  events = []
   torch.cuda.synchronize()
   t0 = time.perf_counter()
-  run_steps(args.steps, events)
+  run_steps(args.steps)
   torch.cuda.synchronize()
   elapsed = time.perf_counter() - t0
   builds = nbrs.plan.builds - builds0
@@ -118,14 +163,17 @@ def main():
   KE_end = jmd.quantity.kinetic_energy(momentum=state.momentum)
   value = N * args.steps / elapsed
   ms = 1e3 * elapsed / args.steps
-  kern_ms = [a.elapsed_time(b) for a, b in events]
-  force_ms = float(np.mean(kern_ms))
   idx = nbrs.idx
   kbar = float((idx < N).sum().item()) / N
+  force_us = force_kernel_us(jmd, loop, state, nbrs, efn)
+  # algorithmic bytes of one force launch: idx row (4 B per valid entry) plus
+  # per-atom state: x read 16 (padded row), F write 12, p read+write 24,
+  # row-end padding entry 4, energy/params 4 -> 4 Kbar + 60 B per atom
   force_bytes = N * (4.0 * kbar + 60.0)
-  achieved = force_bytes / (force_ms * 1e-3) / 1e9
+  achieved = force_bytes / (force_us * 1e-6) / 1e9
   log(f'{args.steps} steps in {elapsed:.3f}s: {ms * 1e3:.1f} us/step, force kernel '
-      f'{force_ms * 1e3:.1f} us, Kbar={kbar:.2f}, list builds={builds}, reallocs={reallocs}')
+      f'{force_us:.1f} us, Kbar={kbar:.2f}, list builds={builds}, reallocs={reallocs}')
+  traffic = counter_traffic(N)
 
   # force error against the float64 C oracle at the final float32 positions
   force_err = None
@@ -169,9 +217,9 @@ def main():
                  'list_builds': builds, 'reallocs': reallocs - reallocs_warm,
                  'max_occupancy': nbrs.max_occupancy, 'kbar': kbar},
       'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                   'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                   'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
                    'kernel': 'k_pair_dense (force + closing half kick)',
-                   'kernel_us': force_ms * 1e3},
+                   'kernel_us': force_us, 'algorithmic_bytes': force_bytes},
       'cpu_baseline': cpu,
       'error_vs_fp64': force_err,
       'energy': {'E_total_start': float(E_start) + float(KE_start),

@@ -73,7 +73,9 @@ typedef struct jmd_pair_potential {
   jmd_param epsilon;
   jmd_param alpha;       /* soft sphere exponent */
   float r_onset;         /* LJ: absolute onset radius (2.0 * max sigma) */
-  float r_cutoff;        /* LJ: absolute cutoff radius (2.5 * max sigma) */
+  float r_cutoff;        /* LJ: absolute cutoff radius (2.5 * max sigma);
+                            soft sphere: an upper bound of sigma over all
+                            pairs, or 0 when unknown (a hint only) */
   int32_t n_species;     /* species matrices are n_species x n_species */
   const int32_t* species;/* device [n] or NULL */
 } jmd_pair_potential;
@@ -158,6 +160,30 @@ int jmd_neighbor_list_check(const float* position, int32_t position_stride,
                             const float* reference_position, int64_t n,
                             int32_t dim, int32_t periodic, const float* side,
                             float threshold_sq, int32_t* gate, void* stream);
+
+/* Cell order of the particles: order[s] is the id of the s-th particle when
+ * sorted by cell hash (cells_per_side grid, x fastest) and by id inside a
+ * cell, the order of the reference's cell list (partition.py:432);
+ * inverse[order[s]] = s.  Either output may be NULL.  No counterpart call in
+ * the reference: it is the spatial layout the native block loop runs in. */
+int jmd_cell_order(const float* position, int64_t n, int32_t dim,
+                   int32_t position_stride, const int32_t* cells_per_side,
+                   const float* cell_size, int32_t* order, int32_t* inverse,
+                   void* workspace, void* stream);
+
+/* Row permutation: out[r] = in[perm[r]] (inverse = 0) or out[perm[r]] =
+ * in[r] (inverse = 1) for n rows of row_bytes bytes with the given row
+ * strides (all multiples of 4 bytes); in and out must not alias. */
+int jmd_permute_rows(const void* in, void* out, const int32_t* perm, int64_t n,
+                     int32_t row_bytes, int32_t in_stride_bytes,
+                     int32_t out_stride_bytes, int32_t inverse, void* stream);
+
+/* Relabels a neighbour list under a particle permutation: Dense
+ * out[perm[r], k] = perm[in[r, k]]; Sparse / OrderedSparse entries keep their
+ * place and each id j becomes perm[j].  Padding n stays n. */
+int jmd_remap_neighbor_ids(const int32_t* idx_in, int32_t* idx_out,
+                           const int32_t* perm, int64_t n, int32_t format,
+                           int64_t width, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Pair reduction                                                            */

@@ -15,7 +15,7 @@ from ctypes import (POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_ui
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, 'libjaxmd_native.so')
+LIB_PATH = os.environ.get('JAX_MD_AMD_LIB', os.path.join(_HERE, 'libjaxmd_native.so'))
 
 JMD_OK, JMD_ERR_INVALID, JMD_ERR_LAUNCH, JMD_ERR_UNSUPPORTED = 0, 1, 2, 3
 DENSE, SPARSE, ORDERED_SPARSE = 0, 1, 2
@@ -67,6 +67,12 @@ _SIGNATURES = {
                                      c_int32, c_int32, POINTER(c_float), c_float, c_float,
                                      c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
                                      c_void_p, c_float, c_void_p, c_void_p]),
+    'jmd_cell_order': (c_int32, [c_void_p, c_int64, c_int32, c_int32, POINTER(c_int32),
+                                 POINTER(c_float), c_void_p, c_void_p, c_void_p, c_void_p]),
+    'jmd_permute_rows': (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_int32,
+                                   c_int32, c_int32, c_void_p]),
+    'jmd_remap_neighbor_ids': (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int32,
+                                         c_int64, c_void_p]),
     'jmd_fire_workspace_bytes': (c_size_t, [c_int64]),
     'jmd_fire_update': (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p,
                                   POINTER(c_float), c_void_p, c_void_p, c_void_p]),

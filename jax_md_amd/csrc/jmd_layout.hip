@@ -1,0 +1,133 @@
+// Particle reordering for the native block loop on gfx950.
+//
+// The pair kernel's cost is set by how many distinct cache lines a
+// wavefront's neighbour gather touches.  Inside a block of steps the loop
+// keeps particles in cell order (the order of the cell list: cells in hash
+// order, ids ascending inside a cell), so that the particles of a wavefront
+// share a cell and their neighbours lie in a few contiguous runs.  The
+// reference's user order is restored at the end of the block, and the
+// neighbour list is translated back to user ids.
+#include "jmd_common.hpp"
+
+namespace jmd {
+
+// out[r] = in[perm[r]] for rows of `row_bytes` bytes (multiple of 4).
+__global__ __launch_bounds__(256) void k_gather_rows(const uint32_t* __restrict__ in,
+                                                     uint32_t* __restrict__ out,
+                                                     const int* __restrict__ perm, int n,
+                                                     int row_words, int in_stride_words,
+                                                     int out_stride_words) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)n * row_words;
+  long long step = (long long)gridDim.x * blockDim.x;
+  for (; t < total; t += step) {
+    int r = (int)(t / row_words);
+    int w = (int)(t - (long long)r * row_words);
+    out[(long long)r * out_stride_words + w] = in[(long long)perm[r] * in_stride_words + w];
+  }
+}
+
+// out[perm[r]] = in[r] (the inverse of k_gather_rows).
+__global__ __launch_bounds__(256) void k_scatter_rows(const uint32_t* __restrict__ in,
+                                                      uint32_t* __restrict__ out,
+                                                      const int* __restrict__ perm, int n,
+                                                      int row_words, int in_stride_words,
+                                                      int out_stride_words) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)n * row_words;
+  long long step = (long long)gridDim.x * blockDim.x;
+  for (; t < total; t += step) {
+    int r = (int)(t / row_words);
+    int w = (int)(t - (long long)r * row_words);
+    out[(long long)perm[r] * out_stride_words + w] = in[(long long)r * in_stride_words + w];
+  }
+}
+
+// Dense list in permuted ids -> user ids: out[perm[r], k] = perm[in[r, k]]
+// (padding n stays n).
+__global__ __launch_bounds__(256) void k_remap_dense(const int* __restrict__ in,
+                                                     int* __restrict__ out,
+                                                     const int* __restrict__ perm, int n,
+                                                     int width) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)n * width;
+  long long step = (long long)gridDim.x * blockDim.x;
+  for (; t < total; t += step) {
+    int r = (int)(t / width);
+    int k = (int)(t - (long long)r * width);
+    int j = in[t];
+    out[(long long)perm[r] * width + k] = j < n ? perm[j] : n;
+  }
+}
+
+// Sparse list in permuted ids -> user ids (entries keep their order).
+__global__ __launch_bounds__(256) void k_remap_ids(const int* __restrict__ in,
+                                                   int* __restrict__ out,
+                                                   const int* __restrict__ perm, int n,
+                                                   long long count) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long step = (long long)gridDim.x * blockDim.x;
+  for (; t < count; t += step) {
+    int j = in[t];
+    out[t] = j < n ? perm[j] : n;
+  }
+}
+
+static int grid_for(long long total) {
+  long long b = (total + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (int)b;
+}
+
+}  // namespace jmd
+
+using namespace jmd;
+
+extern "C" {
+
+int jmd_permute_rows(const void* in, void* out, const int32_t* perm, int64_t n,
+                     int32_t row_bytes, int32_t in_stride_bytes, int32_t out_stride_bytes,
+                     int32_t inverse, void* stream) {
+  if (n < 0 || n > 0x7fffffff || row_bytes <= 0 || (row_bytes & 3) || (in_stride_bytes & 3) ||
+      (out_stride_bytes & 3) || in_stride_bytes < row_bytes || out_stride_bytes < row_bytes ||
+      (n > 0 && (!in || !out || !perm)) || in == out)
+    return JMD_ERR_INVALID;
+  if (n == 0) return JMD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int rw = row_bytes / 4;
+  long long total = n * (long long)rw;
+  if (inverse)
+    hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(total)), dim3(256), 0, st,
+                       (const uint32_t*)in, (uint32_t*)out, perm, (int)n, rw,
+                       in_stride_bytes / 4, out_stride_bytes / 4);
+  else
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(total)), dim3(256), 0, st,
+                       (const uint32_t*)in, (uint32_t*)out, perm, (int)n, rw,
+                       in_stride_bytes / 4, out_stride_bytes / 4);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+int jmd_remap_neighbor_ids(const int32_t* idx_in, int32_t* idx_out, const int32_t* perm,
+                           int64_t n, int32_t format, int64_t width, void* stream) {
+  if (n < 0 || n > 0x7fffffff || width < 0 || (n > 0 && !perm) || idx_in == idx_out ||
+      (width > 0 && (!idx_in || !idx_out)))
+    return JMD_ERR_INVALID;
+  if (format != JMD_DENSE && format != JMD_SPARSE && format != JMD_ORDERED_SPARSE)
+    return JMD_ERR_INVALID;
+  if (width == 0 || n == 0) return JMD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (format == JMD_DENSE) {
+    if (width > 0x7fffffff) return JMD_ERR_INVALID;
+    hipLaunchKernelGGL(k_remap_dense, dim3(grid_for(n * width)), dim3(256), 0, st, idx_in,
+                       idx_out, perm, (int)n, (int)width);
+  } else {
+    hipLaunchKernelGGL(k_remap_ids, dim3(grid_for(2 * width)), dim3(256), 0, st, idx_in,
+                       idx_out, perm, (int)n, 2 * width);
+  }
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+}  // extern "C"

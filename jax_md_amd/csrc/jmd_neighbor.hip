@@ -32,6 +32,7 @@ enum {
   M_FLAGS = 2,      // error bits raised by kernels
   M_TOTAL = 3,      // Sparse total (scan result)
   M_BUILDS = 4,     // completed gated/ungated builds (never reset)
+  M_FAR = 5,        // some position lies >= 2 periods outside the box
   M_WORDS = 16
 };
 
@@ -75,42 +76,41 @@ static size_t carve(void* base, long long n, long long ncell, Work* w) {
 }
 
 // ---------------------------------------------------------------------------
-// Binning
+// Phases, written as device functions over a (block, grid) decomposition.
 // ---------------------------------------------------------------------------
 
 template <int DIM, int STRIDE>
-__global__ __launch_bounds__(256) void k_bin_count(
-    const float* __restrict__ pos, int n, Grid g, int* __restrict__ cell_count,
-    int* __restrict__ atom_cell, int* __restrict__ atom_rank,
-    float* __restrict__ ref_out, int* __restrict__ cell_id_out, const int* gate) {
-  if (gate_closed(gate)) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float x[3];
-  load_pos<DIM, STRIDE>(pos, i, x);
-  int ci[3];
-  int h = cell_of<DIM>(x, g, ci);
-  atom_cell[i] = h;
-  atom_rank[i] = atomicAdd(&cell_count[h], 1);
-  if (ref_out) store_pos<DIM>(ref_out, i, x);
-  if (cell_id_out) {
+__device__ __forceinline__ void dev_bin_count(const float* __restrict__ pos, int n,
+                                              const Grid& g, int* __restrict__ cell_count,
+                                              int* __restrict__ atom_cell,
+                                              int* __restrict__ atom_rank,
+                                              float* __restrict__ ref_out,
+                                              int* __restrict__ cell_id_out, int t0,
+                                              int tstride) {
+  for (int i = t0; i < n; i += tstride) {
+    float x[3];
+    load_pos<DIM, STRIDE>(pos, i, x);
+    int ci[3];
+    int h = cell_of<DIM>(x, g, ci);
+    atom_cell[i] = h;
+    atom_rank[i] = atomicAdd(&cell_count[h], 1);
+    if (ref_out) store_pos<DIM>(ref_out, i, x);
+    if (cell_id_out) {
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) cell_id_out[(long long)i * DIM + d] = ci[d];
+      for (int d = 0; d < DIM; ++d) cell_id_out[(long long)i * DIM + d] = ci[d];
+    }
   }
 }
 
-// One workgroup of 1024 threads: exclusive scan of cell_count into
-// cell_start[0..ncell], max filling, and cell_count reset to zero.
-__global__ __launch_bounds__(1024) void k_bin_scan(int* __restrict__ cell_count,
-                                                   int* __restrict__ cell_start,
-                                                   int ncell, int cap,
-                                                   int* __restrict__ misc,
-                                                   const int* gate) {
-  if (gate_closed(gate)) return;
+// One workgroup: exclusive scan of cell_count into cell_start[0..ncell], max
+// filling, cell-overflow flag, and cell_count reset to zero.
+__device__ __forceinline__ void dev_bin_scan(int* __restrict__ cell_count,
+                                             int* __restrict__ cell_start, int ncell, int cap,
+                                             int* __restrict__ misc) {
   __shared__ int s_sum[1024];
   __shared__ int s_max[1024 / kWave];
-  int t = threadIdx.x;
-  int per = (ncell + 1023) / 1024;
+  const int t = threadIdx.x, nt = blockDim.x;
+  int per = (ncell + nt - 1) / nt;
   int beg = t * per;
   int end = min(beg + per, ncell);
   int local = 0, lmax = 0;
@@ -123,8 +123,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(int* __restrict__ cell_count,
   int wmax = wave_reduce_max(lmax);
   if (lane_id() == 0) s_max[t / kWave] = wmax;
   __syncthreads();
-  // Hillis-Steele inclusive scan over 1024 partial sums.
-  for (int off = 1; off < 1024; off <<= 1) {
+  for (int off = 1; off < nt; off <<= 1) {  // Hillis-Steele inclusive scan
     int v = (t >= off) ? s_sum[t - off] : 0;
     __syncthreads();
     s_sum[t] += v;
@@ -137,22 +136,22 @@ __global__ __launch_bounds__(1024) void k_bin_scan(int* __restrict__ cell_count,
     run += v;
     cell_count[c] = 0;
   }
-  if (t == 1023) cell_start[ncell] = s_sum[1023];
+  if (t == nt - 1) cell_start[ncell] = s_sum[nt - 1];
   if (t == 0) {
     int m = 0;
-    for (int k = 0; k < 1024 / kWave; ++k) m = max(m, s_max[k]);
+    for (int k = 0; k < nt / kWave; ++k) m = max(m, s_max[k]);
     misc[M_FILL] = m;
     if (cap > 0 && m > cap) atomicOr(&misc[M_FLAGS], JMD_CELL_LIST_OVERFLOW);
   }
+  __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_bin_scatter(
-    int n, const int* __restrict__ atom_cell, const int* __restrict__ atom_rank,
-    const int* __restrict__ cell_start, int* __restrict__ sorted_tmp, const int* gate) {
-  if (gate_closed(gate)) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  sorted_tmp[cell_start[atom_cell[i]] + atom_rank[i]] = i;
+__device__ __forceinline__ void dev_bin_scatter(int n, const int* __restrict__ atom_cell,
+                                                const int* __restrict__ atom_rank,
+                                                const int* __restrict__ cell_start,
+                                                int* __restrict__ sorted_tmp, int t0,
+                                                int tstride) {
+  for (int i = t0; i < n; i += tstride) sorted_tmp[cell_start[atom_cell[i]] + atom_rank[i]] = i;
 }
 
 // One wavefront per cell: order members by id (the stable argsort of
@@ -161,33 +160,79 @@ __global__ __launch_bounds__(256) void k_bin_scatter(
 // its cell assignment (x - L * floor(int(x / cs) / cps)), so that positions
 // of a cell lie within that cell's range; the raw input stays untouched.
 template <int DIM, int STRIDE>
+__device__ __forceinline__ void dev_cell_order(const float* __restrict__ pos, const Grid& g,
+                                               const Space& s,
+                                               const int* __restrict__ cell_start,
+                                               const int* __restrict__ sorted_tmp,
+                                               int* __restrict__ sorted_id,
+                                               float4* __restrict__ sorted_pos,
+                                               int* __restrict__ misc, int w0, int wstride) {
+  const int lane = lane_id();
+  bool far = false;
+  for (int cell = w0; cell < g.ncell; cell += wstride) {
+    int beg = cell_start[cell];
+    int cnt = cell_start[cell + 1] - beg;
+    for (int a = lane; a < cnt; a += kWave) {
+      int id = sorted_tmp[beg + a];
+      int rank = 0;
+      for (int b = 0; b < cnt; ++b) rank += (sorted_tmp[beg + b] < id) ? 1 : 0;
+      sorted_id[beg + rank] = id;
+      float x[3] = {0.0f, 0.0f, 0.0f};
+      load_pos<DIM, STRIDE>(pos, id, x);
+      if (s.periodic) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          int q = (int)__fdiv_rn(x[d], g.cell_size[d]);
+          int wraps = (q - floor_mod_int(q, g.cps[d])) / g.cps[d];
+          if (wraps != 0) x[d] = x[d] - (float)wraps * s.L[d];
+          // x - wraps * L rounds at the scale of |x| once |wraps| >= 2; the
+          // build then skips its fast test (see dev_nl_cells).
+          if (wraps > 1 || wraps < -1) far = true;
+        }
+      }
+      sorted_pos[beg + rank] = make_float4(x[0], x[1], x[2], 0.0f);
+    }
+  }
+  if (__ballot(far) != 0ull && lane == 0) atomicOr(&misc[M_FAR], 1);
+}
+
+template <int DIM, int STRIDE>
+__global__ __launch_bounds__(256) void k_bin_count(
+    const float* __restrict__ pos, int n, Grid g, int* __restrict__ cell_count,
+    int* __restrict__ atom_cell, int* __restrict__ atom_rank,
+    float* __restrict__ ref_out, int* __restrict__ cell_id_out, const int* gate) {
+  if (gate_closed(gate)) return;
+  dev_bin_count<DIM, STRIDE>(pos, n, g, cell_count, atom_cell, atom_rank, ref_out, cell_id_out,
+                             blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ __launch_bounds__(1024) void k_bin_scan(int* __restrict__ cell_count,
+                                                   int* __restrict__ cell_start,
+                                                   int ncell, int cap,
+                                                   int* __restrict__ misc,
+                                                   const int* gate) {
+  if (gate_closed(gate)) return;
+  dev_bin_scan(cell_count, cell_start, ncell, cap, misc);
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(
+    int n, const int* __restrict__ atom_cell, const int* __restrict__ atom_rank,
+    const int* __restrict__ cell_start, int* __restrict__ sorted_tmp, const int* gate) {
+  if (gate_closed(gate)) return;
+  dev_bin_scatter(n, atom_cell, atom_rank, cell_start, sorted_tmp,
+                  blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+template <int DIM, int STRIDE>
 __global__ __launch_bounds__(256) void k_cell_order(
     const float* __restrict__ pos, int ncell, Grid g, Space s,
     const int* __restrict__ cell_start, const int* __restrict__ sorted_tmp,
-    int* __restrict__ sorted_id, float4* __restrict__ sorted_pos, const int* gate) {
+    int* __restrict__ sorted_id, float4* __restrict__ sorted_pos, int* __restrict__ misc,
+    const int* gate) {
   if (gate_closed(gate)) return;
-  int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  int lane = lane_id();
-  if (wave >= ncell) return;
-  int beg = cell_start[wave];
-  int cnt = cell_start[wave + 1] - beg;
-  for (int a = lane; a < cnt; a += kWave) {
-    int id = sorted_tmp[beg + a];
-    int rank = 0;
-    for (int b = 0; b < cnt; ++b) rank += (sorted_tmp[beg + b] < id) ? 1 : 0;
-    sorted_id[beg + rank] = id;
-    float x[3] = {0.0f, 0.0f, 0.0f};
-    load_pos<DIM, STRIDE>(pos, id, x);
-    if (s.periodic) {
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        int q = (int)__fdiv_rn(x[d], g.cell_size[d]);
-        int wraps = (q - floor_mod_int(q, g.cps[d])) / g.cps[d];
-        if (wraps != 0) x[d] = x[d] - (float)wraps * s.L[d];
-      }
-    }
-    sorted_pos[beg + rank] = make_float4(x[0], x[1], x[2], 0.0f);
-  }
+  dev_cell_order<DIM, STRIDE>(pos, g, s, cell_start, sorted_tmp, sorted_id, sorted_pos, misc,
+                              (blockIdx.x * blockDim.x + threadIdx.x) / kWave,
+                              gridDim.x * blockDim.x / kWave);
 }
 
 // Public cell-list buffers (partition.py:441-456): slot of the k-th particle
@@ -240,6 +285,26 @@ __device__ __forceinline__ bool pair_test(const float* xi, const float* xj,
   return pa && (b < cutoff_sq);
 }
 
+// Arguments of the cell-grid build.
+struct BuildArgs {
+  const float* pos;
+  int n;
+  Grid g;
+  Space s;
+  float cutoff_sq;
+  float margin;
+  int test;
+  int mask_self;
+  int ordered;
+  int* idx;
+  long long width;
+  float* ref_out;
+  int* occupancy;
+  uint8_t* error;
+  int* gate;
+  Work w;
+};
+
 // Candidates are staged as float4 (x, y, z, id bits) holding the periodic
 // image nearest to the home cell: the shift of each stencil cell (0 or +-L
 // per dimension) follows from the cell topology.  The common test is then a
@@ -247,30 +312,33 @@ __device__ __forceinline__ bool pair_test(const float* xi, const float* xj,
 // difference between this value and the reference's wrapped float32 d^2;
 // only candidates within the margin run the reference-exact test on the
 // original positions (pair_test), so sets and order equal the reference's.
-template <int DIM, int STRIDE, int MODE>
-__global__ __launch_bounds__(256) void k_nl_cells(
-    const float* __restrict__ pos, const float4* __restrict__ sorted_pos,
-    const int* __restrict__ sorted_id, const int* __restrict__ cell_start, Grid g, Space s,
-    float cutoff_sq, float margin, int test, int mask_self, int ordered, int n,
-    int* __restrict__ idx, long long width, int* __restrict__ row_count,
-    const int* __restrict__ row_offset, int* __restrict__ misc, const int* gate) {
-  if (gate_closed(gate)) return;
+// MINIMG: some dimension has fewer than 5 cells, where the staged image is
+// not always the nearest one and a minimum-image fold is applied.
+template <int DIM, int STRIDE, int MODE, bool MINIMG>
+__device__ void dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
   constexpr int NST = DIM == 3 ? 27 : 9;
   __shared__ int s_cs[NST], s_cc[NST], s_off[NST + 1];
   __shared__ float s_shift[NST][3];
   __shared__ float4 s_pos[kStage];
   __shared__ int s_cnt[kHomeGroup];
+  __shared__ int s_occ;
 
+  const Grid& g = A.g;
+  const Space& s = A.s;
+  const int n = A.n;
+  const long long width = A.width;
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wave = tid / kWave;
   const int nwave = blockDim.x / kWave;
-  const float lo_thr = cutoff_sq - margin, hi_thr = cutoff_sq + margin;
-  bool mi[3];  // dimensions where the staged image may not be the nearest
-#pragma unroll
-  for (int d = 0; d < 3; ++d) mi[d] = s.periodic && d < DIM && g.cps[d] < 5;
+  // Positions canonicalised with a rounding beyond the margin's bound: every
+  // candidate takes the reference-exact test.
+  const float margin = A.w.misc[M_FAR] ? __builtin_inff() : A.margin;
+  const float lo_thr = A.cutoff_sq - margin, hi_thr = A.cutoff_sq + margin;
+  const float far = 3.0e18f;  // staged padding: never within the cutoff
+  int occ_local = 0;          // largest row count seen by this wavefront
 
-  for (int home = blockIdx.x; home < g.ncell; home += gridDim.x) {
+  for (int home = home0; home < g.ncell; home += home_stride) {
     __syncthreads();  // LDS reuse across home cells
     if (tid < NST) {
       int hc[3], h = home;
@@ -292,8 +360,8 @@ __global__ __launch_bounds__(256) void k_nl_cells(
         }
         s_shift[tid][d] = sh;
       }
-      s_cs[tid] = cell_start[c];
-      s_cc[tid] = cell_start[c + 1] - s_cs[tid];
+      s_cs[tid] = A.w.cell_start[c];
+      s_cc[tid] = A.w.cell_start[c + 1] - s_cs[tid];
     }
     __syncthreads();
     if (tid == 0) {
@@ -312,9 +380,15 @@ __global__ __launch_bounds__(256) void k_nl_cells(
       for (int a = tid; a < gcount; a += blockDim.x) s_cnt[a] = 0;
       for (int c0 = 0; c0 < total; c0 += kStage) {
         const int ccount = min(kStage, total - c0);
+        const int cpad = (ccount + kWave - 1) & ~(kWave - 1);
         __syncthreads();
-        // Stage candidates c0..c0+ccount in reference order.
-        for (int q = tid; q < ccount; q += blockDim.x) {
+        // Stage candidates c0..c0+ccount in reference order, padded to a
+        // multiple of 64 with far-away entries.
+        for (int q = tid; q < cpad; q += blockDim.x) {
+          if (q >= ccount) {
+            s_pos[q] = make_float4(far, far, far, __int_as_float(n));
+            continue;
+          }
           int p = c0 + q;
           int t = 0;
           while (t + 1 < NST && s_off[t + 1] <= p) ++t;
@@ -326,56 +400,53 @@ __global__ __launch_bounds__(256) void k_nl_cells(
             if (cnt > w) k = (m < cnt - w) ? (w + m) : (m - (cnt - w));
           }
           int gi = s_cs[t] + k;
-          float4 xp = sorted_pos[gi];
+          float4 xp = A.w.sorted_pos[gi];
           s_pos[q] = make_float4(xp.x + s_shift[t][0], xp.y + s_shift[t][1],
-                                 xp.z + s_shift[t][2], __int_as_float(sorted_id[gi]));
+                                 xp.z + s_shift[t][2], __int_as_float(A.w.sorted_id[gi]));
         }
         __syncthreads();
         for (int a = wave; a < gcount; a += nwave) {
           const int hi = hstart + g0 + a;
-          const float4 xp = sorted_pos[hi];
-          const int i = sorted_id[hi];
+          const float4 xp = A.w.sorted_pos[hi];
+          const int i = A.w.sorted_id[hi];
           const float xi[3] = {xp.x, xp.y, xp.z};
-          float xo[3] = {0.0f, 0.0f, 0.0f};  // original position for exact tests
-          load_pos<DIM, STRIDE>(pos, i, xo);
           int cnt = s_cnt[a];
           long long base = 0;
           if (MODE == MODE_DENSE) base = (long long)i * width;
-          if (MODE == MODE_SPARSE_WRITE) base = row_offset[i];
-          for (int q0 = 0; q0 < ccount; q0 += kWave) {
-            int q = q0 + lane;
-            bool pass = false;
-            int j = n;
-            if (q < ccount) {
-              float4 c = s_pos[q];
-              j = __float_as_int(c.w);
-              const float cj[3] = {c.x, c.y, c.z};
-              float d[3] = {0.0f, 0.0f, 0.0f};
+          if (MODE == MODE_SPARSE_WRITE) base = A.w.row_offset[i];
+          for (int q0 = 0; q0 < cpad; q0 += kWave) {
+            const int q = q0 + lane;
+            const float4 c = s_pos[q];
+            const int j = __float_as_int(c.w);
+            const float cj[3] = {c.x, c.y, c.z};
+            float d[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-              for (int e = 0; e < DIM; ++e)
-                d[e] = mi[e] ? -min_image_disp(cj[e], xi[e], xi[e] - s.L[e], s.L[e], s.half[e])
-                             : xi[e] - cj[e];
-              float d2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
-                                  : fmaf(d[1], d[1], d[0] * d[0]);
-              bool ok = (!mask_self || j != i) && (!ordered || j < i);
-              bool near = d2 >= lo_thr && d2 <= hi_thr;
-              pass = ok && d2 < lo_thr;
-              if (ok && near) {
-                float xj[3] = {0.0f, 0.0f, 0.0f};
-                load_pos<DIM, STRIDE>(pos, j, xj);
-                pass = pair_test<DIM>(xo, xj, s, cutoff_sq, 0.0f, test);
+            for (int e = 0; e < DIM; ++e)
+              d[e] = MINIMG ? -min_image_disp(cj[e], xi[e], xi[e] - s.L[e], s.L[e], s.half[e])
+                            : xi[e] - cj[e];
+            const float d2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                                      : fmaf(d[1], d[1], d[0] * d[0]);
+            const bool ok = (!A.mask_self || j != i) && (!A.ordered || j < i) && j < n;
+            bool pass = ok && d2 < lo_thr;
+            const bool near = ok && d2 >= lo_thr && d2 <= hi_thr;
+            if (__any(near)) {  // rare: reference-exact test on original positions
+              if (near) {
+                float xo[3] = {0.0f, 0.0f, 0.0f}, xj[3] = {0.0f, 0.0f, 0.0f};
+                load_pos<DIM, STRIDE>(A.pos, i, xo);
+                load_pos<DIM, STRIDE>(A.pos, j, xj);
+                pass = pair_test<DIM>(xo, xj, s, A.cutoff_sq, 0.0f, A.test);
               }
             }
-            unsigned long long bal = __ballot(pass);
+            const unsigned long long bal = __ballot(pass);
             if ((MODE == MODE_DENSE || MODE == MODE_SPARSE_WRITE) && pass) {
-              long long slot = cnt + wave_prefix_count(bal);
+              const long long slot = cnt + wave_prefix_count(bal);
               if (MODE == MODE_DENSE) {
-                if (slot < width) idx[base + slot] = j;
+                if (slot < width) A.idx[base + slot] = j;
               } else {
-                long long at = base + slot;
+                const long long at = base + slot;
                 if (at < width) {
-                  idx[at] = j;          // receiver
-                  idx[width + at] = i;  // sender
+                  A.idx[at] = j;          // receiver
+                  A.idx[width + at] = i;  // sender
                 }
               }
             }
@@ -388,21 +459,50 @@ __global__ __launch_bounds__(256) void k_nl_cells(
       // Row epilogue: padding (Dense), counts, occupancy.
       for (int a = wave; a < gcount; a += nwave) {
         const int hi = hstart + g0 + a;
-        const int i = sorted_id[hi];
+        const int i = A.w.sorted_id[hi];
         const int cnt = s_cnt[a];
         if (MODE == MODE_DENSE) {
           for (long long k = (long long)cnt + lane; k < width; k += kWave)
-            idx[(long long)i * width + k] = n;
-          if (lane == 0) atomicMax(&misc[M_OCC], cnt);
-        } else if (MODE == MODE_COUNT_MAX) {
-          if (lane == 0) atomicMax(&misc[M_OCC], cnt);
+            A.idx[(long long)i * width + k] = n;
         } else if (MODE == MODE_SPARSE_COUNT) {
-          if (lane == 0) row_count[i] = cnt;
+          if (lane == 0) A.w.row_count[i] = cnt;
         }
+        occ_local = max(occ_local, cnt);
       }
       __syncthreads();  // s_cnt is reused by the next home group
     }
   }
+  // One atomic per workgroup: same-address atomics from every row would
+  // serialise in the L2.
+  if (MODE == MODE_DENSE || MODE == MODE_COUNT_MAX) {
+    if (tid == 0) s_occ = 0;
+    __syncthreads();
+    if (lane == 0) atomicMax(&s_occ, occ_local);
+    __syncthreads();
+    if (tid == 0 && s_occ > 0) atomicMax(&A.w.misc[M_OCC], s_occ);
+  }
+}
+
+template <int DIM, int STRIDE, int MODE, bool MINIMG>
+__global__ __launch_bounds__(256) void k_nl_cells(BuildArgs A) {
+  if (gate_closed(A.gate)) return;
+  dev_nl_cells<DIM, STRIDE, MODE, MINIMG>(A, blockIdx.x, gridDim.x);
+}
+
+// Error bits, occupancy output, gate and counter reset (partition.py:1066,1110).
+__device__ __forceinline__ void dev_finish(int* __restrict__ misc, int* __restrict__ occupancy,
+                                           uint8_t* __restrict__ error,
+                                           long long max_occupancy, int* gate) {
+  int occ = misc[M_OCC];
+  int flags = misc[M_FLAGS];
+  if ((long long)occ > max_occupancy) flags |= JMD_NEIGHBOR_LIST_OVERFLOW;
+  if (occupancy) occupancy[0] = occ;
+  if (error) error[0] = (uint8_t)(error[0] | flags);
+  misc[M_OCC] = 0;
+  misc[M_FLAGS] = 0;
+  misc[M_FAR] = 0;
+  if (error) misc[M_BUILDS] += 1;
+  if (gate) gate[0] = 0;
 }
 
 // Gated copy of the positions into reference_position (no-cell path).
@@ -430,12 +530,15 @@ __global__ __launch_bounds__(256) void k_nl_allpairs(
     int* __restrict__ row_count, const int* __restrict__ row_offset,
     int* __restrict__ misc, const int* gate) {
   if (gate_closed(gate)) return;
+  __shared__ int s_occ;
+  if (threadIdx.x == 0) s_occ = 0;
+  __syncthreads();
   int i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   int lane = lane_id();
-  if (i >= n) return;
+  int cnt = 0;
+  if (i < n) {
   float xi[3] = {0, 0, 0};
   load_pos<DIM, STRIDE>(pos, i, xi);
-  int cnt = 0;
   long long base = 0;
   if (MODE == MODE_DENSE) base = (long long)i * width;
   if (MODE == MODE_SPARSE_WRITE) base = row_offset[i];
@@ -465,11 +568,14 @@ __global__ __launch_bounds__(256) void k_nl_allpairs(
   }
   if (MODE == MODE_DENSE) {
     for (long long k = (long long)cnt + lane; k < width; k += kWave) idx[base + k] = n;
-    if (lane == 0) atomicMax(&misc[M_OCC], cnt);
-  } else if (MODE == MODE_COUNT_MAX) {
-    if (lane == 0) atomicMax(&misc[M_OCC], cnt);
   } else if (MODE == MODE_SPARSE_COUNT) {
     if (lane == 0) row_count[i] = cnt;
+  }
+  }
+  if (MODE == MODE_DENSE || MODE == MODE_COUNT_MAX) {
+    if (lane == 0 && cnt > 0) atomicMax(&s_occ, cnt);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_occ > 0) atomicMax(&misc[M_OCC], s_occ);
   }
 }
 
@@ -521,21 +627,12 @@ __global__ __launch_bounds__(256) void k_sparse_pad(int* __restrict__ idx, long 
   }
 }
 
-// Error bits, occupancy output, gate and counter reset (partition.py:1066,1110).
 __global__ void k_nl_finish(int* __restrict__ misc, int* __restrict__ occupancy,
                             uint8_t* __restrict__ error, long long max_occupancy,
                             int* gate) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (gate_closed(gate)) return;
-  int occ = misc[M_OCC];
-  int flags = misc[M_FLAGS];
-  if ((long long)occ > max_occupancy) flags |= JMD_NEIGHBOR_LIST_OVERFLOW;
-  if (occupancy) occupancy[0] = occ;
-  if (error) error[0] = (uint8_t)(error[0] | flags);
-  misc[M_OCC] = 0;
-  misc[M_FLAGS] = 0;
-  if (error) misc[M_BUILDS] += 1;
-  if (gate) gate[0] = 0;
+  dev_finish(misc, occupancy, error, max_occupancy, gate);
 }
 
 // Skin criterion (partition.py:1141-1154).
@@ -600,12 +697,48 @@ static int bin_and_order(const float* pos, int n, const Grid& g, const Space& s,
   }
   hipLaunchKernelGGL((k_cell_order<DIM, STRIDE>),
                      dim3(ceil_div((long long)g.ncell * kWave, 256)), dim3(256), 0, st, pos,
-                     g.ncell, g, s, w.cell_start, w.sorted_tmp, w.sorted_id, w.sorted_pos, gate);
+                     g.ncell, g, s, w.cell_start, w.sorted_tmp, w.sorted_id, w.sorted_pos, w.misc,
+                     gate);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }
 
 constexpr int kCellBlocks = 2048;  // persistent grid of the cell build
+
+template <int DIM, int STRIDE, bool MINIMG>
+static int launch_cells(const BuildArgs& A, int mode, hipStream_t st) {
+  dim3 grid(A.g.ncell < kCellBlocks ? A.g.ncell : kCellBlocks), block(256);
+  switch (mode) {
+    case MODE_DENSE:
+      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_DENSE, MINIMG>), grid, block, 0, st, A);
+      break;
+    case MODE_SPARSE_COUNT:
+      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_SPARSE_COUNT, MINIMG>), grid, block, 0,
+                         st, A);
+      break;
+    case MODE_SPARSE_WRITE:
+      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_SPARSE_WRITE, MINIMG>), grid, block, 0,
+                         st, A);
+      break;
+    default:
+      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_COUNT_MAX, MINIMG>), grid, block, 0, st,
+                         A);
+      break;
+  }
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+template <int DIM, int STRIDE>
+static int cells_dispatch(const BuildArgs& A, int mode, hipStream_t st) {
+  // The staged image of a stencil cell is the nearest one when every
+  // periodic dimension has at least 5 cells (|d| < 2 cell sizes <= 2L/5).
+  bool minimg = false;
+  if (A.s.periodic)
+    for (int d = 0; d < DIM; ++d) minimg = minimg || A.g.cps[d] < 5;
+  return minimg ? launch_cells<DIM, STRIDE, true>(A, mode, st)
+                : launch_cells<DIM, STRIDE, false>(A, mode, st);
+}
 
 template <int DIM, int STRIDE>
 static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
@@ -617,34 +750,44 @@ static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
   const bool count_only = error == nullptr;
   int rc;
   if (use_cells) {
-    rc = bin_and_order<DIM, STRIDE>(pos, n, g, s, w, ref_out, nullptr, gate, st);
-    if (rc) return rc;
+    BuildArgs A;
+    A.pos = pos;
+    A.n = n;
+    A.g = g;
+    A.s = s;
+    A.cutoff_sq = cutoff_sq;
     // Bound on |fast d^2 - reference wrapped d^2| in float32: a few roundings
     // at the scale of 2L on each of the components of d.
     float Lmax = 0.0f;
     for (int d = 0; d < DIM; ++d) Lmax = fmaxf(Lmax, s.periodic ? s.L[d] : 0.0f);
     float c = sqrtf(cutoff_sq);
     float margin = 128.0f * (c + 1.0f) * (Lmax + c + 1.0f) * 1.1920929e-7f;
-    margin = fmaxf(margin, 1e-4f * cutoff_sq);
-    const int test = sparse ? TEST_IJ : TEST_BOTH;
-    dim3 grid(g.ncell < kCellBlocks ? g.ncell : kCellBlocks), block(256);
-#define JMD_CELLS(MODE)                                                                      \
-  hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE>), grid, block, 0, st, pos, w.sorted_pos, \
-                     w.sorted_id, w.cell_start, g, s, cutoff_sq, margin, test, mask_self,    \
-                     ordered, n, idx, width, w.row_count, w.row_offset, w.misc, gate);       \
-  JMD_LAUNCH_CHECK();
+    A.margin = fmaxf(margin, 1e-4f * cutoff_sq);
+    A.test = sparse ? TEST_IJ : TEST_BOTH;
+    A.mask_self = mask_self;
+    A.ordered = ordered;
+    A.idx = idx;
+    A.width = width;
+    A.ref_out = ref_out;
+    A.occupancy = occupancy;
+    A.error = error;
+    A.gate = gate;
+    A.w = w;
+    rc = bin_and_order<DIM, STRIDE>(pos, n, g, s, w, ref_out, nullptr, gate, st);
+    if (rc) return rc;
     if (!sparse && count_only) {
-      JMD_CELLS(MODE_COUNT_MAX)
+      rc = cells_dispatch<DIM, STRIDE>(A, MODE_COUNT_MAX, st);
     } else if (!sparse) {
-      JMD_CELLS(MODE_DENSE)
+      rc = cells_dispatch<DIM, STRIDE>(A, MODE_DENSE, st);
     } else {
-      JMD_CELLS(MODE_SPARSE_COUNT)
+      rc = cells_dispatch<DIM, STRIDE>(A, MODE_SPARSE_COUNT, st);
+      if (rc) return rc;
       hipLaunchKernelGGL(k_row_scan, dim3(1), dim3(1024), 0, st, w.row_count, w.row_offset,
                          n, w.misc, gate);
       JMD_LAUNCH_CHECK();
-      if (!count_only) { JMD_CELLS(MODE_SPARSE_WRITE) }
+      if (!count_only) rc = cells_dispatch<DIM, STRIDE>(A, MODE_SPARSE_WRITE, st);
     }
-#undef JMD_CELLS
+    if (rc) return rc;
   } else {
     if (ref_out && n > 0) {
       int nb = ceil_div(n, 256);
@@ -695,6 +838,16 @@ static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
   return JMD_OK;
 }
 
+__global__ __launch_bounds__(256) void k_order_out(const int* __restrict__ sorted_id, int n,
+                                                   int* __restrict__ order,
+                                                   int* __restrict__ inverse) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int id = sorted_id[s];
+  if (order) order[s] = id;
+  if (inverse) inverse[id] = s;
+}
+
 // Runs F<DIM, STRIDE>() for the supported (dim, stride) layouts.
 #define JMD_DISPATCH_LAYOUT(dim, stride, CALL3, CALL34, CALL2) \
   ((dim) == 3 ? ((stride) == 4 ? (CALL34) : (CALL3)) : (CALL2))
@@ -705,7 +858,7 @@ using namespace jmd;
 
 extern "C" {
 
-const char* jmd_version(void) { return "jaxmd_native 0.2.0 (gfx950)"; }
+const char* jmd_version(void) { return "jaxmd_native 0.3.0 (gfx950)"; }
 
 size_t jmd_neighbor_workspace_bytes(int64_t n, int32_t dim, int64_t cell_count,
                                     int32_t format) {
@@ -841,6 +994,37 @@ int jmd_neighbor_list_build(const float* position, int64_t n, int32_t dim,
                         max_occupancy, occupancy, error, reference_position, gate, w, st)),
       (build_impl<2, 2>(position, nn, s, cutoff_sq, use_cell_list, g, format, mask_self, idx,
                         max_occupancy, occupancy, error, reference_position, gate, w, st)));
+}
+
+int jmd_cell_order(const float* position, int64_t n, int32_t dim, int32_t position_stride,
+                   const int32_t* cells_per_side, const float* cell_size, int32_t* order,
+                   int32_t* inverse, void* workspace, void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || !cells_per_side || !cell_size ||
+      !workspace || (n > 0 && (!position || (!order && !inverse))))
+    return JMD_ERR_INVALID;
+  if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  Grid g;
+  if (!make_grid(dim, cells_per_side, cell_size, 0, &g)) return JMD_ERR_INVALID;
+  if (n == 0) return JMD_OK;
+  Work w;
+  carve(workspace, n, g.ncell, &w);
+  hipStream_t st = (hipStream_t)stream;
+  int nn = (int)n;
+  Space s = grid_space_free(dim);
+  int rc = JMD_DISPATCH_LAYOUT(
+      dim, position_stride,
+      (bin_and_order<3, 3>(position, nn, g, s, w, nullptr, nullptr, nullptr, st)),
+      (bin_and_order<3, 4>(position, nn, g, s, w, nullptr, nullptr, nullptr, st)),
+      (bin_and_order<2, 2>(position, nn, g, s, w, nullptr, nullptr, nullptr, st)));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_order_out, dim3(ceil_div(nn, 256)), dim3(256), 0, st, w.sorted_id, nn,
+                     order, inverse);
+  JMD_LAUNCH_CHECK();
+  // clear the flag word the scan may have raised (capacity 0: none)
+  hipLaunchKernelGGL(k_nl_finish, dim3(1), dim3(64), 0, st, w.misc, nullptr, nullptr,
+                     (long long)0x7fffffffffffLL, nullptr);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
 }
 
 int jmd_neighbor_list_check(const float* position, int32_t position_stride,

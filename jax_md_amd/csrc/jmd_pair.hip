@@ -32,6 +32,7 @@ struct PairArgs {
   int ns;
   float ro2, rc2, inv_d;  // LJ smoothing: r_on^2, r_c^2, 1/(rc^2-ro^2)^3
   int smooth;             // r_on < r_c
+  float rc;               // interaction range (LJ r_c, soft sphere max sigma)
 };
 
 __device__ __forceinline__ float pick(int mode, float scalar, const float* v, int i, int j,
@@ -41,9 +42,6 @@ __device__ __forceinline__ float pick(int mode, float scalar, const float* v, in
   return v[si * ns + sj];
 }
 
-// Returns U(r) and writes dU/dr / r for one pair given r^2.  Multiply-adds
-// are explicit (contraction off), so every kernel variant rounds alike and the
-// force does not depend on whether the energy is also requested.
 // 1/x from v_rcp_f32 plus one Newton step (~0.5 ulp, no IEEE division).
 __device__ __forceinline__ float fast_rcp(float x) {
 #pragma clang fp contract(off)
@@ -61,39 +59,44 @@ __device__ __forceinline__ float fast_rsqrt(float x) {
   return fmaf(y, e, y);
 }
 
+// U(r) and dU/dr / r for one pair from r^2, without branches; callers mask
+// invalid lanes with a select.  Multiply-adds are explicit (contraction off)
+// so every kernel variant rounds alike.
 template <int KIND>
 __device__ __forceinline__ float pair_eval(float r2, float sig, float eps, float alp,
                                            const PairArgs& a, float* dEr) {
 #pragma clang fp contract(off)
   if (KIND == JMD_LENNARD_JONES) {
-    if (!(r2 < a.rc2)) { *dEr = 0.0f; return 0.0f; }
     float inv_r2 = fast_rcp(r2);
     float s2 = sig * sig * inv_r2;
     float s6 = s2 * s2 * s2;
     float s12 = s6 * s6;
     float V = 4.0f * eps * (s12 - s6);
     float dV = -24.0f * eps * fmaf(2.0f, s12, -s6) * inv_r2;
-    if (a.smooth && r2 >= a.ro2) {
+    float E = V, dE = dV;
+    if (a.smooth) {
       float u = a.rc2 - r2;
       float S = u * u * fmaf(2.0f, r2, fmaf(-3.0f, a.ro2, a.rc2)) * a.inv_d;
       float dS = 12.0f * u * (a.ro2 - r2) * a.inv_d;
-      *dEr = fmaf(S, dV, dS * V);
-      return S * V;
+      bool in_s = r2 >= a.ro2;
+      dE = in_s ? fmaf(S, dV, dS * V) : dV;
+      E = in_s ? S * V : V;
     }
-    *dEr = dV;
-    return V;
+    bool valid = r2 < a.rc2;
+    *dEr = valid ? dE : 0.0f;
+    return valid ? E : 0.0f;
   } else {
     // soft sphere: eps/alpha (1 - r/sigma)^alpha for r < sigma
-    if (!(r2 < sig * sig) || !(r2 > 0.0f)) { *dEr = 0.0f; return 0.0f; }
     float inv_r = fast_rsqrt(r2);
     float r = r2 * inv_r;
     float inv_sig = fast_rcp(sig);
     float base = 1.0f - r * inv_sig;
     float up, um;  // base^alpha, base^(alpha-1)
     if (alp == 2.0f) { um = base; up = base * base; }
-    else { um = powf(base, alp - 1.0f); up = um * base; }
-    *dEr = -eps * inv_sig * um * inv_r;
-    return eps / alp * up;
+    else { um = powf(fmaxf(base, 0.0f), alp - 1.0f); up = um * base; }
+    bool valid = (r2 < sig * sig) && (r2 > 0.0f);
+    *dEr = valid ? -eps * inv_sig * um * inv_r : 0.0f;
+    return valid ? eps / alp * up : 0.0f;
   }
 }
 
@@ -112,6 +115,69 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return (b & 7) * per + (b >> 3);
 }
 
+// Sum over one Dense row slice (entries sub, sub+TPA, ...) into f, e.
+// INTERIOR: every particle of the wavefront lies at least the interaction
+// range from each box face, so for any pair within range the plain
+// difference x_j - x_i is already the minimum image and no wrap is needed
+// (pairs whose plain difference is not the minimum image are then beyond
+// range and contribute zero either way).
+template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, bool ENERGY, bool INTERIOR>
+__device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned n,
+                                        const Space& s, const PairArgs& a,
+                                        const int* __restrict__ row, int width, int sub,
+                                        unsigned i, const float* xi, const float* xim,
+                                        float* f, float* e) {
+#pragma clang fp contract(off)
+  float sig = a.sig, eps = a.eps, alp = a.alp;
+  int si = 0;
+  if (GENERAL && a.species) si = a.species[i];
+#ifndef JMD_PAIR_UNROLL
+#define JMD_PAIR_UNROLL 4
+#endif
+  constexpr int UNR = JMD_PAIR_UNROLL;
+  for (int k0 = sub; k0 < width; k0 += UNR * TPA) {
+    unsigned jj[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      int k = k0 + u * TPA;
+      unsigned v = (unsigned)row[min(k, width - 1)];  // clamped: no branch
+      jj[u] = k < width ? v : n;
+    }
+    if (jj[0] >= n) break;  // rows are left-compacted, padding is n
+    float xj[UNR][3];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      xj[u][0] = xj[u][1] = xj[u][2] = 0.0f;
+      load_pos<DIM, STRIDE>(pos, jj[u] < n ? jj[u] : i, xj[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      float d[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+        d[c] = (INTERIOR || !s.periodic)
+                   ? xj[u][c] - xi[c]
+                   : min_image_disp(xj[u][c], xi[c], xim[c], s.L[c], s.half[c]);
+      float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                          : fmaf(d[1], d[1], d[0] * d[0]);
+      if (GENERAL) {
+        unsigned j = jj[u] < n ? jj[u] : i;
+        int sj = a.species ? a.species[j] : 0;
+        sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
+        eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
+        if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
+      }
+      float dEr;
+      float uu = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr);
+      const bool ok = jj[u] < n;
+      dEr = ok ? dEr : 0.0f;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) f[c] = fmaf(dEr, d[c], f[c]);
+      if (ENERGY) *e += ok ? uu : 0.0f;
+    }
+  }
+}
+
 template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, bool ENERGY, bool KICK>
 __global__ __launch_bounds__(256) void k_pair_dense(
     const float* __restrict__ pos, int n, Space s, PairArgs a, const int* __restrict__ idx,
@@ -124,57 +190,46 @@ __global__ __launch_bounds__(256) void k_pair_dense(
   const int i = gt / TPA;
   const int sub = gt % TPA;
   const bool active = i < n;
-  double f[3] = {0.0, 0.0, 0.0};
-  double e = 0.0;
+  float f[3] = {0.0f, 0.0f, 0.0f};
+  float e = 0.0f;
+  float xi[3] = {0, 0, 0}, xim[3] = {0, 0, 0};
+  bool interior = true;
   if (active) {
-    float xi[3] = {0, 0, 0}, xim[3] = {0, 0, 0};
     load_pos<DIM, STRIDE>(pos, i, xi);
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) xim[c] = xi[c] - s.L[c];
-    float sig = a.sig, eps = a.eps, alp = a.alp;
-    int si = 0;
-    if (GENERAL && a.species) si = a.species[i];
-    const int* row = idx + (long long)i * width;
-    for (int k = sub; k < width; k += TPA) {
-      int j = row[k];
-      if (j >= n) break;  // rows are left-compacted, padding is n
-      float xj[3] = {0, 0, 0}, d[3];
-      load_pos<DIM, STRIDE>(pos, j, xj);
-#pragma unroll
-      for (int c = 0; c < DIM; ++c)
-        d[c] = s.periodic ? min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c])
-                          : xj[c] - xi[c];
-      float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
-                          : fmaf(d[1], d[1], d[0] * d[0]);
-      if (GENERAL) {
-        int sj = a.species ? a.species[j] : 0;
-        sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
-        eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
-        if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
-      }
-      float dEr;
-      float u = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr);
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) f[c] += (double)(dEr * d[c]);
-      if (ENERGY) e += (double)u;
+    for (int c = 0; c < DIM; ++c) {
+      xim[c] = xi[c] - s.L[c];
+      interior = interior && xi[c] >= a.rc && xi[c] <= s.L[c] - a.rc;
     }
   }
+  const bool wave_interior = __all(interior || !active);
+  if (active) {
+    const int* row = idx + (long long)i * width;
+    if (wave_interior)
+      row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, true>(pos, n, s, a, row, width, sub, i,
+                                                             xi, xim, f, &e);
+    else
+      row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false>(pos, n, s, a, row, width, sub, i,
+                                                              xi, xim, f, &e);
+  }
+  double fd[3];
 #pragma unroll
-  for (int c = 0; c < DIM; ++c) f[c] = group_sum<TPA>(f[c]);
-  if (ENERGY) e = group_sum<TPA>(e);
+  for (int c = 0; c < DIM; ++c) fd[c] = group_sum<TPA>((double)f[c]);
+  double ed = 0.0;
+  if (ENERGY) ed = group_sum<TPA>((double)e);
   if (active && sub == 0) {
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-      float fc = (float)f[c];
+      float fc = (float)fd[c];
       if (force) force[(long long)i * DIM + c] = fc;
       if (KICK) mom[(long long)i * DIM + c] += half_dt * fc;
     }
-    if (ENERGY && e_atom) e_atom[i] = (float)(0.5 * e);
+    if (ENERGY && e_atom) e_atom[i] = (float)(0.5 * ed);
   }
   if (ENERGY) {
     // block partial of 0.5 * sum_i e_i in float64
     __shared__ double s_e[256 / kWave];
-    double v = (active && sub == 0) ? 0.5 * e : 0.0;
+    double v = (active && sub == 0) ? 0.5 * ed : 0.0;
     v = wave_reduce_sum(v);
     if (lane_id() == 0) s_e[threadIdx.x / kWave] = v;
     __syncthreads();
@@ -289,7 +344,16 @@ static bool make_pair_args(const jmd_pair_potential* p, PairArgs* a) {
   float ro = p->r_onset, rc = p->r_cutoff;
   a->rc2 = rc * rc;
   a->ro2 = ro * ro;
-  a->smooth = ro < rc;
+  a->rc = rc;
+  if (p->kind == JMD_SOFT_SPHERE) {
+    // Interaction range (used only for the wrap-free interior test): a scalar
+    // sigma, else the caller's r_cutoff bound on sigma (0: unknown, never
+    // interior).
+    ro = rc = 0.0f;
+    a->rc = p->sigma.mode == JMD_PARAM_SCALAR ? p->sigma.value
+                                               : (p->r_cutoff > 0.0f ? p->r_cutoff : 3.0e38f);
+  }
+  a->smooth = p->kind == JMD_LENNARD_JONES && ro < rc;
   a->inv_d = 0.0f;
   if (a->smooth) {
     double dd = (double)a->rc2 - (double)a->ro2;

@@ -160,14 +160,27 @@ class NVELoop:
   half kick.  The rebuild decisions are those of the reference's
   ``update`` before ``apply_fn``; overflow is reported in the list's error
   word and checked by the caller once per block, as the reference does.
+
+  With ``graph`` (default) a block is one HIP graph launch: the first call for
+  a given set of buffers and block length runs eagerly and records the
+  block's kernels into a graph; later calls on the same buffers replay it.
+  Every rebuild decision stays on the device (gated kernels), so the recorded
+  sequence is valid for any trajectory.
   """
 
-  def __init__(self, energy_or_force_fn, shift_fn, dt):
+  def __init__(self, energy_or_force_fn, shift_fn, dt, reorder: bool = True,
+               graph: bool = True):
     self.energy = quantity.native_energy_of(energy_or_force_fn)
     self.shift = native_shift(shift_fn)
     self.dt = f32(dt)
     self.half = f32(self.dt / f32(2))
-    self.force_events = None
+    self.reorder = reorder
+    self.graph = graph
+    self._bufs = None
+    self._bufs_key = None
+    self._graph = None
+    self._graph_key = None
+    self._keep = None
 
   def prime(self, state: NVEState, nbrs) -> None:
     """Sets the gate from the skin criterion of the current positions."""
@@ -183,7 +196,123 @@ class NVELoop:
         int(plan.periodic), nat.f3(plan.side), float(spec.threshold_sq), nat.ptr(plan.gate),
         nat.stream_handle(state.position.device)), 'jmd_neighbor_list_check')
 
+  def _use_reorder(self, state, nbrs) -> bool:
+    from . import partition
+    return bool(self.reorder and nbrs.format is partition.Dense and nbrs.plan.use_cell_list
+                and state.position.shape[0] > 1)
+
+  @staticmethod
+  def _key(state, nbrs, n_steps, reorder):
+    def p(t):
+      return (t.data_ptr(), tuple(t.shape)) if isinstance(t, torch.Tensor) else t
+    return (p(state.position), p(state.position_hi), p(state.momentum), p(state.force),
+            p(state.mass), p(nbrs.idx), p(nbrs.reference_position), p(nbrs.error.code),
+            id(nbrs.plan), int(n_steps), bool(reorder))
+
   def run(self, state: NVEState, nbrs, n_steps: int, events=None) -> NVEState:
+    """Advances ``state`` and ``nbrs`` in place by ``n_steps``.  With
+    ``reorder`` (Dense lists on a cell grid) the block runs with particles in
+    cell order, so that a wavefront's neighbour gathers share cache lines;
+    user order, and the list in user ids, are restored at the end (rows hold
+    the reference's neighbour sets; inside a row the order follows the cell
+    order of the block).  ``events`` (a list) collects a (start, end) timing
+    event pair around every force kernel; it forces the eager path."""
+    if state.position_hi is None:
+      state.position_hi = state.position.to(torch.float64)
+    reorder = self._use_reorder(state, nbrs)
+    if events is not None or not self.graph:
+      self._enqueue(state, nbrs, n_steps, reorder, events)
+      return state
+    key = self._key(state, nbrs, n_steps, reorder)
+    if self._graph is not None and self._graph_key == key:
+      self._graph.replay()
+      return state
+    self._graph, self._graph_key = None, None
+    # eager first block: it also loads every kernel before the capture
+    self._enqueue(state, nbrs, n_steps, reorder, None)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+      self._enqueue(state, nbrs, n_steps, reorder, None)  # recorded, not run
+    self._graph, self._graph_key = g, key
+    return state
+
+  def _enqueue(self, state, nbrs, n_steps, reorder, events):
+    if reorder:
+      self._run_reordered(state, nbrs, n_steps, events)
+    else:
+      self._run_steps(state, nbrs, n_steps, events)
+
+  def _buffers(self, state, nbrs):
+    """Cell-order working copies of the state and list, kept across blocks
+    (fixed addresses for graph replay)."""
+    R = state.position
+    N, dim = R.shape
+    dev = R.device
+    key = (N, dim, str(dev), tuple(nbrs.idx.shape), isinstance(state.mass, torch.Tensor)
+           and state.mass.ndim > 0)
+    if self._bufs_key != key:
+      b = dict(
+          order=torch.empty((N,), dtype=torch.int32, device=dev),
+          inv=torch.empty((N,), dtype=torch.int32, device=dev),
+          xh=torch.empty((N, dim), dtype=torch.float64, device=dev),
+          x=new_positions(N, dim, dev),
+          p=torch.empty((N, dim), dtype=torch.float32, device=dev),
+          F=torch.empty((N, dim), dtype=torch.float32, device=dev),
+          ref=torch.empty((N, dim), dtype=torch.float32, device=dev),
+          idx=torch.empty(tuple(nbrs.idx.shape), dtype=torch.int32, device=dev),
+          mass=(torch.empty((N,), dtype=torch.float32, device=dev) if key[-1] else None),
+          m32=(torch.empty((N,), dtype=torch.float32, device=dev) if key[-1] else None))
+      self._bufs, self._bufs_key = b, key
+    return self._bufs
+
+  def _run_reordered(self, state, nbrs, n_steps, events):
+    lib = nat.load()
+    plan = nbrs.plan
+    R = state.position
+    N, dim = R.shape
+    dev = R.device
+    st = nat.stream_handle(dev)
+    stride = nat.position_stride(R)
+    b = self._buffers(state, nbrs)
+    order, inv = b['order'], b['inv']
+    nat.check(lib.jmd_cell_order(nat.ptr(R), N, dim, stride, nat.i3(plan.cells_per_side),
+                                 nat.f3(plan.cell_size3), nat.ptr(order), nat.ptr(inv),
+                                 plan.workspace.ptr, st), 'jmd_cell_order')
+
+    def permute(src, dst, row_bytes, src_stride, dst_stride, inverse):
+      nat.check(lib.jmd_permute_rows(nat.ptr(src), nat.ptr(dst), nat.ptr(order), N, row_bytes,
+                                     src_stride, dst_stride, inverse, st), 'jmd_permute_rows')
+
+    rb = 4 * dim
+    xh, x, p, F, ref, idx = b['xh'], b['x'], b['p'], b['F'], b['ref'], b['idx']
+    permute(state.position_hi, xh, 8 * dim, 8 * dim, 8 * dim, 0)
+    xs = nat.position_stride(x)
+    permute(R, x, rb, 4 * stride, 4 * xs, 0)
+    permute(state.momentum, p, rb, rb, rb, 0)
+    permute(state.force, F, rb, rb, rb, 0)
+    mass = state.mass
+    if b['mass'] is not None:
+      b['m32'].copy_(mass.to(device=dev, dtype=torch.float32).reshape(-1))
+      permute(b['m32'], b['mass'], 4, 4, 4, 0)
+      mass = b['mass']
+    permute(nbrs.reference_position, ref, rb, rb, rb, 0)
+    width = nbrs.idx.shape[1]
+    nat.check(lib.jmd_remap_neighbor_ids(nat.ptr(nbrs.idx), nat.ptr(idx), nat.ptr(inv), N,
+                                         nat.DENSE, width, st), 'jmd_remap_neighbor_ids')
+    nb = dataclasses.replace(nbrs, idx=idx, reference_position=ref)
+    sp = NVEState(x, p, F, mass, xh)
+    self._run_steps(sp, nb, n_steps, events)
+    # back to user order
+    permute(xh, state.position_hi, 8 * dim, 8 * dim, 8 * dim, 1)
+    permute(x, R, rb, 4 * xs, 4 * stride, 1)
+    permute(p, state.momentum, rb, rb, rb, 1)
+    permute(F, state.force, rb, rb, rb, 1)
+    permute(ref, nbrs.reference_position, rb, rb, rb, 1)
+    nat.check(lib.jmd_remap_neighbor_ids(nat.ptr(idx), nat.ptr(nbrs.idx), nat.ptr(order), N,
+                                         nat.DENSE, width, st), 'jmd_remap_neighbor_ids')
+    return state
+
+  def _run_steps(self, state: NVEState, nbrs, n_steps: int, events=None) -> NVEState:
     lib = nat.load()
     plan = nbrs.plan
     spec = plan.spec
@@ -207,6 +336,7 @@ class NVELoop:
     idx, err, ref = nat.ptr(nbrs.idx), nat.ptr(nbrs.error.code), nat.ptr(nbrs.reference_position)
     occ, ws = nat.ptr(plan.occupancy), plan.workspace.ptr
     pot, keep = self.energy._potential(R, {})
+    self._keep = (pot, keep)  # device parameter arrays stay alive for graph replays
     pws = nat.ptr(self.energy._workspace(dev, N))
     fmt = spec.format_code
     width = nbrs.idx.shape[1]
@@ -231,5 +361,4 @@ class NVELoop:
         e1 = torch.cuda.Event(enable_timing=True)
         e1.record()
         events.append((e0, e1))
-    del keep
     return state

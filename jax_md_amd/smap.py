@@ -94,6 +94,17 @@ class PairEnergy:
     self._ws = {}
 
   # -- helpers ---------------------------------------------------------------
+  def _sigma_bound(self, sig) -> float:
+    if not (isinstance(sig, (torch.Tensor, onp.ndarray)) and sig.ndim > 0):
+      return 0.0
+    if sig is self.params.get('sigma'):
+      if getattr(self, '_sig_max', None) is None:
+        v = sig.detach().max().cpu().item() if isinstance(sig, torch.Tensor) else onp.max(sig)
+        self._sig_max = float(onp.nextafter(onp.float32(v), onp.float32(onp.inf)))
+      return self._sig_max
+    return 0.0
+
+
   def _workspace(self, device, n):
     key = (str(device), int(n))
     ws = self._ws.get(key)
@@ -119,6 +130,11 @@ class PairEnergy:
     pot.alpha = P.param(merged.get('alpha', 2.0), sp)
     pot.r_onset = self.r_onset
     pot.r_cutoff = self.r_cutoff
+    if self.kind == nat.SOFT_SPHERE:
+      # bound on sigma for the wrap-free interior path: known when sigma is
+      # the construction-time value (its max is taken once, on the host)
+      sig = merged.get('sigma', 1.0)
+      pot.r_cutoff = self._sigma_bound(sig)
     ns = 0
     for k in ('sigma', 'epsilon', 'alpha'):
       v = merged.get(k)

@@ -127,3 +127,74 @@ def test_fire_vs_oracle_2d_soft_sphere():
   E64 = O.energy_force_brute(m, sp, ost.position)[0]
   print(f'  E={E:.8e} E64={E64:.8e}')
   assert abs(E - E64) <= 1e-5 * abs(E64) + 1e-9
+
+
+@pytest.mark.parametrize('reorder', [True, False])
+def test_nve_block_loop_matches_drop_in(reorder):
+  """simulate.NVELoop (the native fori_loop body, with or without cell-order
+  execution) against the per-step drop-in path: same trajectory to float32
+  round-off after 60 steps, and a list holding exactly the neighbour sets of a
+  fresh build at the final positions."""
+  print(f'case NVELoop reorder={reorder}', flush=True)
+  jmd, R, P, L, shift, nfn, efn = _lj(12, seed=2)
+  Rt = torch.as_tensor(R, device=DEV)
+  Pt = torch.as_tensor(P, device=DEV)
+  init_fn, apply_fn = jmd.simulate.nve(efn, shift, 0.005)
+  nb1 = nfn.allocate(Rt)
+  s1 = init_fn(0, Rt, kT=1.0, momenta=Pt, neighbor=nb1)
+  for _ in range(60):
+    nb1 = nb1.update(s1.position)
+    s1 = apply_fn(s1, neighbor=nb1)
+  nb2 = nfn.allocate(Rt)
+  s2 = init_fn(0, Rt, kT=1.0, momenta=Pt, neighbor=nb2)
+  loop = jmd.simulate.NVELoop(efn, shift, 0.005, reorder=reorder)
+  loop.prime(s2, nb2)
+  loop.run(s2, nb2, 30)
+  loop.run(s2, nb2, 30)
+  assert not bool(nb2.did_buffer_overflow)
+  sp = O.Space(np.float64(L))
+  dR = sp.displacement(s1.position_hi.cpu().numpy(), s2.position_hi.cpu().numpy())
+  dP = (s1.momentum - s2.momentum).abs().max().item()
+  print(f'  max |dR|={np.abs(dR).max():.3e} max |dP|={dP:.3e}')
+  assert np.abs(dR).max() < 1e-4 and dP < 1e-3
+  assert np.array_equal(s2.position.cpu().numpy(), s2.position_hi.cpu().numpy().astype(F32))
+  fresh = nfn.allocate(s2.position)
+  N = R.shape[0]
+  a = np.sort(np.where(nb2.idx.cpu().numpy() < N, nb2.idx.cpu().numpy(), N), axis=1)
+  # the loop's list was built at its last rebuild: compare at those positions
+  ref_pos = nb2.reference_position
+  fresh = nfn.allocate(ref_pos.clone())
+  b = np.sort(fresh.idx.cpu().numpy(), axis=1)
+  w = min(a.shape[1], b.shape[1])
+  assert np.array_equal(a[:, :w], b[:, :w])
+
+
+@pytest.mark.parametrize('reorder', [True, False])
+def test_nve_loop_graph_replays_match_eager(reorder):
+  """The HIP-graph block (recorded once, replayed) gives bit-identical state,
+  lists and rebuild counts to the eagerly launched block, on every replay,
+  with eager work and synchronisations between the replays."""
+  print(f'case NVELoop graph replays reorder={reorder}', flush=True)
+  jmd, R, P, L, shift, nfn, efn = _lj(12, seed=4)
+  Rt = torch.as_tensor(R, device=DEV)
+  Pt = torch.as_tensor(P, device=DEV)
+  init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
+  runs = []
+  for graph in (False, True):
+    nb = nfn.allocate(Rt)
+    s = init_fn(0, Rt, kT=1.0, momenta=Pt, neighbor=nb)
+    loop = jmd.simulate.NVELoop(efn, shift, 0.005, reorder=reorder, graph=graph)
+    loop.prime(s, nb)
+    snaps = []
+    for blk in range(4):
+      loop.run(s, nb, 15)
+      torch.cuda.synchronize()
+      _ = efn.energy_and_force(s.position, nb)  # eager work between replays
+      snaps.append((s.position_hi.clone(), s.momentum.clone(), s.force.clone(),
+                    nb.idx.clone(), nb.reference_position.clone(), nb.plan.builds))
+    runs.append(snaps)
+  for blk, (a, b) in enumerate(zip(*runs)):
+    for k in range(5):
+      assert torch.equal(a[k], b[k]), (reorder, blk, k)
+    assert a[5] == b[5], (reorder, blk)
+  assert runs[1][-1][5] > runs[1][0][5]  # rebuilds happened inside the replays

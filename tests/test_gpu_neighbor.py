@@ -235,3 +235,66 @@ def test_padded_layout_same_list(fmt):
   gp = nfn.allocate(buf[:, :3])
   _assert_same(gp, o, f'padded fmt={fmt}')
   assert torch.equal(gp.idx, g.idx)
+
+
+
+@pytest.mark.parametrize('case', ['cps3', 'cps4', 'dim2', 'far', 'padded'])
+def test_dense_cells_geometry_cases_with_updates(case):
+  """Dense builds and gated updates against the oracle where the build's
+  fast test needs care: 3 or 4 cells per side (the staged neighbour image is
+  not always the nearest: minimum-image fold), 2-D, coordinates up to three
+  periods outside the box (canonicalisation rounds at |x|: exact test for
+  every candidate), and the padded [N, 4] layout."""
+  jmd = _jmd()
+  rng = np.random.default_rng(41)
+  r_cut, dr = 2.5, 0.5
+  if case in ('padded', 'far'):
+    R, _, box = O.lj_lattice(12, seed=9)
+  elif case == 'cps3':
+    box = F32(9.5)  # list cutoff 3.0 -> 3 cells per side
+    R = _uniform(600, box, 42)
+  elif case == 'cps4':
+    box = np.array([12.5, 13.0, 16.0], F32)  # 4, 4, 5 cells
+    R = _uniform(1500, box, 43)
+  else:
+    box = np.array([20.0, 17.0], F32)
+    R = _uniform(900, box, 44)
+  if case == 'far':
+    R = (R + F32(box) * rng.integers(-3, 4, size=R.shape).astype(F32)).astype(F32)
+  disp, _ = jmd.space.periodic(box)
+  nfn = jmd.partition.neighbor_list(disp, box, r_cut, dr, format=jmd.partition.Dense)
+  osp = O.Space(np.asarray(box, F32))
+  spec = O.NeighborListSpec(osp, np.asarray(box, F32), r_cut, dr, 1.25, False, True, O.DENSE)
+  buf = None
+  if case == 'padded':
+    buf = torch.zeros((R.shape[0], 4), dtype=torch.float32, device=DEV)
+
+  def dev_pos(X):
+    if buf is None:
+      return torch.as_tensor(X, device=DEV)
+    buf[:, :3] = torch.as_tensor(X, device=DEV)
+    return buf[:, :3]
+
+  print(f'case dense geometry {case} N={R.shape[0]}', flush=True)
+  g = nfn.allocate(dev_pos(R))
+  o = O.neighbor_list_build(spec, R)
+  assert o.use_cell_list
+  _assert_same(g, o, f'{case} allocate')
+  Rk = R.copy()
+  for step in range(3):
+    Rk = (Rk + rng.normal(0, 0.05, Rk.shape)).astype(F32)
+    if step == 1:
+      Rk[3] += F32(0.6)  # beyond the skin: forces a rebuild
+    g = g.update(dev_pos(Rk))
+    o = O.neighbor_list_update(spec, Rk, o)
+    _assert_same(g, o, f'{case} update {step}')
+    assert np.array_equal(g.reference_position.cpu().numpy(), o.reference_position)
+
+def test_unwrapped_positions_bit_exact_sparse():
+  """Coordinates several periods outside the box (Sparse phase kernels)."""
+  R, _, L = O.lj_lattice(10, seed=12)
+  rng = np.random.default_rng(13)
+  R = (R + F32(L) * rng.integers(-3, 4, size=R.shape).astype(F32)).astype(F32)
+  print('case unwrapped sparse', flush=True)
+  _, _, g, o = _both(R, L, 2.5, 0.5, O.SPARSE)
+  _assert_same(g, o, 'unwrapped sparse')
