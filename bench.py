@@ -90,6 +90,9 @@ def main():
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--tpa', type=int, default=0, help='threads per row of the pair kernel')
   ap.add_argument('--eager', action='store_true', help='no HIP graphs (host-launched steps)')
+  ap.add_argument('--copy-calibration', action='store_true',
+                  help='only run a plain 256 MiB device copy (the reference kernel that '
+                       'scales the FETCH_SIZE / WRITE_SIZE counters) and exit')
   args = ap.parse_args()
   if args.gpus != 1:
     print(f'bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU '
@@ -105,6 +108,16 @@ def main():
 
   dev = 'cuda:0'
   torch.cuda.set_device(0)
+  if args.copy_calibration:
+    nbytes = 256 << 20
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+      dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_calibration_bytes_read': nbytes, 'bytes_written': nbytes,
+                      'dispatches': 5}), flush=True)
+    return
   if args.tpa:
     nat.check(nat.load().jmd_set_threads_per_atom(args.tpa), 'jmd_set_threads_per_atom')
   R, P, L = O.lj_lattice(args.n_side, seed=0)

@@ -88,12 +88,14 @@ const char* jmd_version(void);
 /* ------------------------------------------------------------------------ */
 
 /* Scratch bytes needed by jmd_cell_list / jmd_neighbor_list_build for n
- * particles on a grid of cell_count cells (cell_count 0: no cell list).
- * The workspace must be zero-filled before its first use; every call leaves
- * it in that state again, except int32 word 4, which counts the completed
- * list builds (gated builds that ran, and ungated ones). */
+ * particles on a grid of cell_count cells (cell_count 0: no cell list) with
+ * cell capacity cell_capacity (the largest capacity later builds pass; 0 for
+ * calls without a list build).  The workspace must be zero-filled before its
+ * first use; every call leaves it in that state again, except int32 word 4,
+ * which counts the completed list builds (gated builds that ran, and
+ * ungated ones). */
 size_t jmd_neighbor_workspace_bytes(int64_t n, int32_t dim, int64_t cell_count,
-                                    int32_t format);
+                                    int32_t cell_capacity, int32_t format);
 
 /* partition.cell_list(...).allocate/update body: cell_list_fn
  * (jax_md/partition.py:349-471).  Bins n particles into the

@@ -43,7 +43,7 @@ _I3 = c_int32 * 3
 # name -> (restype, argtypes)
 _SIGNATURES = {
     'jmd_version': (c_char_p, []),
-    'jmd_neighbor_workspace_bytes': (c_size_t, [c_int64, c_int32, c_int64, c_int32]),
+    'jmd_neighbor_workspace_bytes': (c_size_t, [c_int64, c_int32, c_int64, c_int32, c_int32]),
     'jmd_cell_list': (c_int32, [c_void_p, c_int64, c_int32, c_int32, POINTER(c_int32),
                                 POINTER(c_float), c_int32, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p]),

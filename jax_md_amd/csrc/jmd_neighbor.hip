@@ -33,6 +33,8 @@ enum {
   M_TOTAL = 3,      // Sparse total (scan result)
   M_BUILDS = 4,     // completed gated/ungated builds (never reset)
   M_FAR = 5,        // some position lies >= 2 periods outside the box
+  M_TICKET_BIN = 6, // finished workgroups of k_bin_slots (zero at rest)
+  M_TICKET_NL = 7,  // finished workgroups of k_nl_cells (zero at rest)
   M_WORDS = 16
 };
 
@@ -46,12 +48,15 @@ struct Work {
   float4* sorted_pos;// [n]
   int* row_count;    // [n]
   int* row_offset;   // [n+1]
+  int* slots;        // [ncell * cap] ids per cell in arrival order (k_bin_slots)
   int* misc;         // [M_WORDS] (zero at rest except as documented)
 };
 
+constexpr int kCellBlocks = 2048;  // persistent grid of the cell build
+
 static inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
-static size_t carve(void* base, long long n, long long ncell, Work* w) {
+static size_t carve(void* base, long long n, long long ncell, long long cap, Work* w) {
   size_t off = 0;
   char* b = (char*)base;
   auto take = [&](size_t bytes) -> char* {
@@ -71,6 +76,7 @@ static size_t carve(void* base, long long n, long long ncell, Work* w) {
   t.sorted_pos = (float4*)take(sizeof(float4) * n);
   t.row_count = (int*)take(sizeof(int) * n);
   t.row_offset = (int*)take(sizeof(int) * (n + 1));
+  t.slots = (int*)take(sizeof(int) * (ncell > 0 && cap > 0 ? nc * cap : 1));
   if (w) *w = t;
   return off;
 }
@@ -103,43 +109,98 @@ __device__ __forceinline__ void dev_bin_count(const float* __restrict__ pos, int
 }
 
 // One workgroup: exclusive scan of cell_count into cell_start[0..ncell], max
-// filling, cell-overflow flag, and cell_count reset to zero.
+// filling, cell-overflow flag, and (reset_counts) cell_count reset to zero.
+// The counts go through in chunks of blockDim * 16 cells; each thread holds
+// 16 consecutive counts (four 16-byte loads in flight at once), so a chunk
+// costs one memory round trip.
+constexpr int kScanPer = 16;
+
+__device__ __forceinline__ int coherent_load(const int* p);
+
 __device__ __forceinline__ void dev_bin_scan(int* __restrict__ cell_count,
                                              int* __restrict__ cell_start, int ncell, int cap,
-                                             int* __restrict__ misc) {
-  __shared__ int s_sum[1024];
+                                             int* __restrict__ misc, bool reset_counts = true,
+                                             bool coherent = false) {
+  __shared__ int s_wsum[1024 / kWave];
   __shared__ int s_max[1024 / kWave];
   const int t = threadIdx.x, nt = blockDim.x;
-  int per = (ncell + nt - 1) / nt;
-  int beg = t * per;
-  int end = min(beg + per, ncell);
-  int local = 0, lmax = 0;
-  for (int c = beg; c < end; ++c) {
-    int v = cell_count[c];
-    local += v;
-    lmax = max(lmax, v);
+  const int lane = lane_id(), wv = t / kWave, nw = nt / kWave;
+  int carry = 0, lmax = 0;
+  for (int c0 = 0; c0 < ncell; c0 += nt * kScanPer) {
+    const int beg = c0 + t * kScanPer;
+    int v[kScanPer];
+    if (coherent) {  // counts produced by atomics of the same kernel
+#pragma unroll
+      for (int k = 0; k < kScanPer; ++k)
+        v[k] = beg + k < ncell ? coherent_load(cell_count + beg + k) : 0;
+    } else if (beg + kScanPer <= ncell) {
+      const int4* p = reinterpret_cast<const int4*>(cell_count + beg);
+#pragma unroll
+      for (int k = 0; k < kScanPer / 4; ++k) {
+        int4 q = p[k];
+        v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanPer; ++k) v[k] = beg + k < ncell ? cell_count[beg + k] : 0;
+    }
+    int local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+      local += v[k];
+      lmax = max(lmax, v[k]);
+    }
+    int incl = local;  // inclusive scan over the wavefront
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      int y = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += y;
+    }
+    if (lane == kWave - 1) s_wsum[wv] = incl;
+    __syncthreads();
+    int wbase = 0, ctot = 0;
+    for (int k = 0; k < nw; ++k) {
+      const int ws = s_wsum[k];
+      wbase += k < wv ? ws : 0;
+      ctot += ws;
+    }
+    __syncthreads();  // s_wsum is rewritten by the next chunk
+    int run = carry + wbase + incl - local;
+    if (beg + kScanPer <= ncell) {
+      int4* ps = reinterpret_cast<int4*>(cell_start + beg);
+#pragma unroll
+      for (int k = 0; k < kScanPer / 4; ++k) {
+        int4 q;
+        q.x = run; run += v[4 * k];
+        q.y = run; run += v[4 * k + 1];
+        q.z = run; run += v[4 * k + 2];
+        q.w = run; run += v[4 * k + 3];
+        ps[k] = q;
+      }
+      if (reset_counts) {
+        int4* pc = reinterpret_cast<int4*>(cell_count + beg);
+#pragma unroll
+        for (int k = 0; k < kScanPer / 4; ++k) pc[k] = make_int4(0, 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanPer; ++k) {
+        if (beg + k < ncell) {
+          cell_start[beg + k] = run;
+          if (reset_counts) cell_count[beg + k] = 0;
+        }
+        run += v[k];
+      }
+    }
+    carry += ctot;
   }
-  s_sum[t] = local;
-  int wmax = wave_reduce_max(lmax);
-  if (lane_id() == 0) s_max[t / kWave] = wmax;
+  if (t == 0) cell_start[ncell] = carry;
+  const int wmax = wave_reduce_max(lmax);
+  if (lane == 0) s_max[wv] = wmax;
   __syncthreads();
-  for (int off = 1; off < nt; off <<= 1) {  // Hillis-Steele inclusive scan
-    int v = (t >= off) ? s_sum[t - off] : 0;
-    __syncthreads();
-    s_sum[t] += v;
-    __syncthreads();
-  }
-  int run = (t == 0) ? 0 : s_sum[t - 1];
-  for (int c = beg; c < end; ++c) {
-    int v = cell_count[c];
-    cell_start[c] = run;
-    run += v;
-    cell_count[c] = 0;
-  }
-  if (t == nt - 1) cell_start[ncell] = s_sum[nt - 1];
   if (t == 0) {
     int m = 0;
-    for (int k = 0; k < nt / kWave; ++k) m = max(m, s_max[k]);
+    for (int k = 0; k < nw; ++k) m = max(m, s_max[k]);
     misc[M_FILL] = m;
     if (cap > 0 && m > cap) atomicOr(&misc[M_FLAGS], JMD_CELL_LIST_OVERFLOW);
   }
@@ -315,7 +376,7 @@ struct BuildArgs {
 // MINIMG: some dimension has fewer than 5 cells, where the staged image is
 // not always the nearest one and a minimum-image fold is applied.
 template <int DIM, int STRIDE, int MODE, bool MINIMG>
-__device__ void dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
+__device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
   constexpr int NST = DIM == 3 ? 27 : 9;
   __shared__ int s_cs[NST], s_cc[NST], s_off[NST + 1];
   __shared__ float s_shift[NST][3];
@@ -362,6 +423,8 @@ __device__ void dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
       }
       s_cs[tid] = A.w.cell_start[c];
       s_cc[tid] = A.w.cell_start[c + 1] - s_cs[tid];
+      // k_cell_sort orders at most cap members of a cell (more: overflow)
+      if (g.cap > 0) s_cc[tid] = min(s_cc[tid], g.cap);
     }
     __syncthreads();
     if (tid == 0) {
@@ -472,28 +535,178 @@ __device__ void dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
       __syncthreads();  // s_cnt is reused by the next home group
     }
   }
-  // One atomic per workgroup: same-address atomics from every row would
-  // serialise in the L2.
+  // One device-scope atomic per workgroup (its return value orders the
+  // workgroup's completion ticket after it, see last_workgroup).
+  int ret = 0;
   if (MODE == MODE_DENSE || MODE == MODE_COUNT_MAX) {
     if (tid == 0) s_occ = 0;
     __syncthreads();
     if (lane == 0) atomicMax(&s_occ, occ_local);
     __syncthreads();
-    if (tid == 0 && s_occ > 0) atomicMax(&A.w.misc[M_OCC], s_occ);
+    if (tid == 0) ret = atomicMax(&A.w.misc[M_OCC], s_occ);
+  }
+  return ret;
+}
+
+// ---------------------------------------------------------------------------
+// Three-launch cell build of the neighbour lists:
+//   k_bin_slots:  each particle takes a slot of its cell (wave-aggregated
+//                 atomics); the last workgroup scans the counts into cell
+//                 starts (the sorted-array offsets of the reference).
+//   k_cell_sort:  one wavefront per cell orders the cell's ids (the stable
+//                 argsort order) into sorted_id / sorted_pos.
+//   k_nl_cells:   the build; its last workgroup finishes (error bits,
+//                 occupancy, gate reset).
+// ---------------------------------------------------------------------------
+
+// Completion ticket: returns true (in all threads) in the workgroup that
+// finishes last.  The last workgroup reads only values written by
+// device-scope atomics, which are performed at the device's coherence point;
+// each workgroup issues its ticket after its atomics have returned (`dep` is
+// the value returned by thread 0's last one, and every other wave consumed
+// its own returns before the barrier), so no cache write-back fence is needed
+// (an agent-scope release fence writes back the whole L2 of the XCD).
+__device__ __forceinline__ bool last_workgroup(int* ticket, int dep) {
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned inc = dep == (int)0x80000000 ? 2u : 1u;  // never 2: orders after dep
+    const unsigned t = atomicAdd(reinterpret_cast<unsigned*>(ticket), inc);
+    s_last = (t == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  return s_last != 0;
+}
+
+__device__ __forceinline__ int coherent_load(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int DIM, int STRIDE>
+__global__ __launch_bounds__(256) void k_bin_slots(const float* __restrict__ pos, int n,
+                                                   Grid g, int* __restrict__ cell_count,
+                                                   int* __restrict__ cell_start,
+                                                   int* __restrict__ slots,
+                                                   float* __restrict__ ref_out,
+                                                   int* __restrict__ misc, const int* gate) {
+  if (gate_closed(gate)) return;
+  const int lane = lane_id();
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    const bool valid = i < n;
+    int h = 0;
+    if (valid) {
+      float x[3];
+      load_pos<DIM, STRIDE>(pos, i, x);
+      int ci[3];
+      h = cell_of<DIM>(x, g, ci);
+      if (ref_out) store_pos<DIM>(ref_out, i, x);
+    }
+    // one atomic per distinct cell in the wavefront (few when particles are
+    // stored in cell order), all issued by one instruction: first group the
+    // lanes by cell (leader = lowest lane of the group), then the leaders add
+    // their group sizes, then every lane offsets its leader's result
+    unsigned long long todo = __ballot(valid);
+    int my_leader = lane, my_off = 0, grp = 0;
+    while (todo) {
+      const int leader = __ffsll(todo) - 1;
+      const int hl = __shfl(h, leader);
+      const unsigned long long peers = __ballot(valid && h == hl) & todo;
+      if ((peers >> lane) & 1ull) {
+        my_leader = leader;
+        my_off = (int)__popcll(peers & ((1ull << lane) - 1ull));
+      }
+      if (lane == leader) grp = (int)__popcll(peers);
+      todo &= ~peers;
+    }
+    int b = 0;
+    if (valid && lane == my_leader) b = atomicAdd(&cell_count[h], grp);
+    b = __shfl(b, my_leader);
+    if (valid) {
+      const int rank = b + my_off;
+      if (rank < g.cap) slots[(long long)h * g.cap + rank] = i;
+    }
+  }
+  // every count was added by the device-scope atomics above
+  if (!last_workgroup(&misc[M_TICKET_BIN], 0)) return;
+  if (threadIdx.x == 0) misc[M_TICKET_BIN] = 0;
+  dev_bin_scan(cell_count, cell_start, g.ncell, g.cap, misc, true, /*coherent=*/true);
+}
+
+// One wavefront per cell: ranks by id (a lane's id against every member,
+// broadcast with readlane), canonical float4 position at that rank.
+template <int DIM, int STRIDE>
+__global__ __launch_bounds__(256) void k_cell_sort(const float* __restrict__ pos, Grid g,
+                                                   Space s, const int* __restrict__ cell_start,
+                                                   const int* __restrict__ slots,
+                                                   int* __restrict__ sorted_id,
+                                                   float4* __restrict__ sorted_pos,
+                                                   int* __restrict__ misc, const int* gate) {
+  if (gate_closed(gate)) return;
+  const int lane = lane_id();
+  const int nw = gridDim.x * blockDim.x / kWave;
+  bool far = false;
+  for (int cell = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; cell < g.ncell; cell += nw) {
+    const int start = cell_start[cell];
+    const int cnt = min(cell_start[cell + 1] - start, g.cap);
+    const int* mem = slots + (long long)cell * g.cap;
+    for (int a0 = 0; a0 < cnt; a0 += kWave) {
+      const int a = a0 + lane;
+      const int id = a < cnt ? mem[a] : 0x7fffffff;
+      int rank = 0;
+      for (int b0 = 0; b0 < cnt; b0 += kWave) {
+        const int idb = b0 + lane < cnt ? mem[b0 + lane] : 0x7fffffff;
+        const int m = min(kWave, cnt - b0);
+        for (int k = 0; k < m; ++k) rank += __builtin_amdgcn_readlane(idb, k) < id ? 1 : 0;
+      }
+      if (a < cnt) {
+        float x[3] = {0.0f, 0.0f, 0.0f};
+        load_pos<DIM, STRIDE>(pos, id, x);
+        if (s.periodic) {
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            // 0 <= x < L - cell_size: int(x / cell_size) < cps, no wrap
+            if (x[d] >= 0.0f && x[d] < s.L[d] - g.cell_size[d]) continue;
+            const int q = (int)__fdiv_rn(x[d], g.cell_size[d]);
+            const int wraps = (q - floor_mod_int(q, g.cps[d])) / g.cps[d];
+            if (wraps != 0) x[d] = x[d] - (float)wraps * s.L[d];
+            // x - wraps * L rounds at the scale of |x| once |wraps| >= 2
+            if (wraps > 1 || wraps < -1) far = true;
+          }
+        }
+        sorted_id[start + rank] = id;
+        sorted_pos[start + rank] = make_float4(x[0], x[1], x[2], 0.0f);
+      }
+    }
+  }
+  if (__ballot(far) != 0ull && lane == 0) atomicOr(&misc[M_FAR], 1);
+}
+
+__device__ __forceinline__ void dev_finish(int* __restrict__ misc, int occ,
+                                           int* __restrict__ occupancy,
+                                           uint8_t* __restrict__ error,
+                                           long long max_occupancy, int* gate);
+
+// The list build; with `finish`, its last workgroup completes the build
+// (error bits, occupancy, gate reset).
+template <int DIM, int STRIDE, int MODE, bool MINIMG>
+__global__ __launch_bounds__(256) void k_nl_cells(BuildArgs A, int finish) {
+  if (gate_closed(A.gate)) return;
+  const int dep = dev_nl_cells<DIM, STRIDE, MODE, MINIMG>(A, blockIdx.x, gridDim.x);
+  if (!finish) return;
+  if (!last_workgroup(&A.w.misc[M_TICKET_NL], dep)) return;
+  if (threadIdx.x == 0) {
+    A.w.misc[M_TICKET_NL] = 0;
+    dev_finish(A.w.misc, coherent_load(&A.w.misc[M_OCC]), A.occupancy, A.error,
+               A.error ? A.width : 0x7fffffffffffLL, A.gate);
   }
 }
 
-template <int DIM, int STRIDE, int MODE, bool MINIMG>
-__global__ __launch_bounds__(256) void k_nl_cells(BuildArgs A) {
-  if (gate_closed(A.gate)) return;
-  dev_nl_cells<DIM, STRIDE, MODE, MINIMG>(A, blockIdx.x, gridDim.x);
-}
-
 // Error bits, occupancy output, gate and counter reset (partition.py:1066,1110).
-__device__ __forceinline__ void dev_finish(int* __restrict__ misc, int* __restrict__ occupancy,
+__device__ __forceinline__ void dev_finish(int* __restrict__ misc, int occ,
+                                           int* __restrict__ occupancy,
                                            uint8_t* __restrict__ error,
                                            long long max_occupancy, int* gate) {
-  int occ = misc[M_OCC];
   int flags = misc[M_FLAGS];
   if ((long long)occ > max_occupancy) flags |= JMD_NEIGHBOR_LIST_OVERFLOW;
   if (occupancy) occupancy[0] = occ;
@@ -627,12 +840,12 @@ __global__ __launch_bounds__(256) void k_sparse_pad(int* __restrict__ idx, long 
   }
 }
 
+// Error word, occupancy output and resets after a multi-kernel build.
 __global__ void k_nl_finish(int* __restrict__ misc, int* __restrict__ occupancy,
-                            uint8_t* __restrict__ error, long long max_occupancy,
-                            int* gate) {
+                            uint8_t* __restrict__ error, long long max_occupancy, int* gate) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (gate_closed(gate)) return;
-  dev_finish(misc, occupancy, error, max_occupancy, gate);
+  dev_finish(misc, misc[M_OCC], occupancy, error, max_occupancy, gate);
 }
 
 // Skin criterion (partition.py:1141-1154).
@@ -703,41 +916,77 @@ static int bin_and_order(const float* pos, int n, const Grid& g, const Space& s,
   return JMD_OK;
 }
 
-constexpr int kCellBlocks = 2048;  // persistent grid of the cell build
-
 template <int DIM, int STRIDE, bool MINIMG>
-static int launch_cells(const BuildArgs& A, int mode, hipStream_t st) {
+static void launch_cells(const BuildArgs& A, int mode, int finish, hipStream_t st) {
   dim3 grid(A.g.ncell < kCellBlocks ? A.g.ncell : kCellBlocks), block(256);
   switch (mode) {
     case MODE_DENSE:
-      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_DENSE, MINIMG>), grid, block, 0, st, A);
+      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_DENSE, MINIMG>), grid, block, 0, st, A,
+                         finish);
       break;
     case MODE_SPARSE_COUNT:
       hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_SPARSE_COUNT, MINIMG>), grid, block, 0,
-                         st, A);
+                         st, A, finish);
       break;
     case MODE_SPARSE_WRITE:
       hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_SPARSE_WRITE, MINIMG>), grid, block, 0,
-                         st, A);
+                         st, A, finish);
       break;
     default:
       hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_COUNT_MAX, MINIMG>), grid, block, 0, st,
-                         A);
+                         A, finish);
       break;
   }
+}
+
+template <int DIM, int STRIDE>
+static int cells_dispatch(const BuildArgs& A, int mode, int finish, hipStream_t st) {
+  // The staged image of a stencil cell is the nearest one when every
+  // periodic dimension has at least 5 cells (|d| < 3 cell sizes < L - c).
+  bool minimg = false;
+  if (A.s.periodic)
+    for (int d = 0; d < DIM; ++d) minimg = minimg || A.g.cps[d] < 5;
+  if (minimg) launch_cells<DIM, STRIDE, true>(A, mode, finish, st);
+  else launch_cells<DIM, STRIDE, false>(A, mode, finish, st);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }
 
+constexpr int kBinBlocks = 1024;
+
+// Cell-list build: k_bin_slots, k_cell_sort, k_nl_cells (Sparse adds the
+// row scan, the padding and a second k_nl_cells pass).
 template <int DIM, int STRIDE>
-static int cells_dispatch(const BuildArgs& A, int mode, hipStream_t st) {
-  // The staged image of a stencil cell is the nearest one when every
-  // periodic dimension has at least 5 cells (|d| < 2 cell sizes <= 2L/5).
-  bool minimg = false;
-  if (A.s.periodic)
-    for (int d = 0; d < DIM; ++d) minimg = minimg || A.g.cps[d] < 5;
-  return minimg ? launch_cells<DIM, STRIDE, true>(A, mode, st)
-                : launch_cells<DIM, STRIDE, false>(A, mode, st);
+static int build_cells(const BuildArgs& A, bool sparse, bool count_only, hipStream_t st) {
+  const Work& w = A.w;
+  int nb = ceil_div(A.n > 0 ? A.n : 1, 256);
+  if (nb > kBinBlocks) nb = kBinBlocks;
+  hipLaunchKernelGGL((k_bin_slots<DIM, STRIDE>), dim3(nb), dim3(256), 0, st, A.pos, A.n, A.g,
+                     w.cell_count, w.cell_start, w.slots, A.ref_out, w.misc, A.gate);
+  JMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_cell_sort<DIM, STRIDE>), dim3(ceil_div((long long)A.g.ncell * kWave, 256)),
+                     dim3(256), 0, st, A.pos, A.g, A.s, w.cell_start, w.slots, w.sorted_id,
+                     w.sorted_pos, w.misc, A.gate);
+  JMD_LAUNCH_CHECK();
+  if (!sparse)
+    return cells_dispatch<DIM, STRIDE>(A, count_only ? MODE_COUNT_MAX : MODE_DENSE, 1, st);
+  int rc = cells_dispatch<DIM, STRIDE>(A, MODE_SPARSE_COUNT, 0, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_row_scan, dim3(1), dim3(1024), 0, st, w.row_count, w.row_offset, A.n,
+                     w.misc, A.gate);
+  JMD_LAUNCH_CHECK();
+  if (count_only) {
+    hipLaunchKernelGGL(k_nl_finish, dim3(1), dim3(64), 0, st, w.misc, A.occupancy, nullptr,
+                       (long long)0x7fffffffffffLL, A.gate);
+    JMD_LAUNCH_CHECK();
+    return JMD_OK;
+  }
+  int pb = ceil_div(A.width, 256);
+  pb = pb > 4096 ? 4096 : pb + 1;
+  hipLaunchKernelGGL(k_sparse_pad, dim3(pb), dim3(256), 0, st, A.idx, A.width, A.n, w.misc,
+                     A.gate);
+  JMD_LAUNCH_CHECK();
+  return cells_dispatch<DIM, STRIDE>(A, MODE_SPARSE_WRITE, 1, st);
 }
 
 template <int DIM, int STRIDE>
@@ -748,7 +997,6 @@ static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
   const bool sparse = format != JMD_DENSE;
   const int ordered = format == JMD_ORDERED_SPARSE;
   const bool count_only = error == nullptr;
-  int rc;
   if (use_cells) {
     BuildArgs A;
     A.pos = pos;
@@ -773,67 +1021,45 @@ static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
     A.error = error;
     A.gate = gate;
     A.w = w;
-    rc = bin_and_order<DIM, STRIDE>(pos, n, g, s, w, ref_out, nullptr, gate, st);
-    if (rc) return rc;
-    if (!sparse && count_only) {
-      rc = cells_dispatch<DIM, STRIDE>(A, MODE_COUNT_MAX, st);
-    } else if (!sparse) {
-      rc = cells_dispatch<DIM, STRIDE>(A, MODE_DENSE, st);
-    } else {
-      rc = cells_dispatch<DIM, STRIDE>(A, MODE_SPARSE_COUNT, st);
-      if (rc) return rc;
-      hipLaunchKernelGGL(k_row_scan, dim3(1), dim3(1024), 0, st, w.row_count, w.row_offset,
-                         n, w.misc, gate);
-      JMD_LAUNCH_CHECK();
-      if (!count_only) rc = cells_dispatch<DIM, STRIDE>(A, MODE_SPARSE_WRITE, st);
-    }
-    if (rc) return rc;
-  } else {
-    if (ref_out && n > 0) {
-      int nb = ceil_div(n, 256);
-      hipLaunchKernelGGL((k_copy_gated<DIM, STRIDE>), dim3(nb > 2048 ? 2048 : nb), dim3(256),
-                         0, st, pos, ref_out, n, gate);
-      JMD_LAUNCH_CHECK();
-    }
-    // Dense without cells tests wrap(Rj - Ri) (partition.py:965-971);
-    // Sparse tests wrap(Ri - Rj) (partition.py:990-995).
-    const int test = sparse ? TEST_IJ : TEST_JI;
-    dim3 grid(ceil_div((long long)n * kWave, 256)), block(256);
+    return build_cells<DIM, STRIDE>(A, sparse, count_only, st);
+  }
+  if (ref_out && n > 0) {
+    int nb = ceil_div(n, 256);
+    hipLaunchKernelGGL((k_copy_gated<DIM, STRIDE>), dim3(nb > 2048 ? 2048 : nb), dim3(256),
+                       0, st, pos, ref_out, n, gate);
+    JMD_LAUNCH_CHECK();
+  }
+  // Dense without cells tests wrap(Rj - Ri) (partition.py:965-971);
+  // Sparse tests wrap(Ri - Rj) (partition.py:990-995).
+  const int test = sparse ? TEST_IJ : TEST_JI;
+  dim3 grid(ceil_div((long long)n * kWave, 256)), block(256);
 #define JMD_ALLP(MODE)                                                                      \
   hipLaunchKernelGGL((k_nl_allpairs<DIM, STRIDE, MODE>), grid, block, 0, st, pos, n, s,      \
                      cutoff_sq, test, mask_self, ordered, idx, width, w.row_count,           \
                      w.row_offset, w.misc, gate);                                            \
   JMD_LAUNCH_CHECK();
-    if (n > 0) {
-      if (!sparse && count_only) {
-        JMD_ALLP(MODE_COUNT_MAX)
-      } else if (!sparse) {
-        JMD_ALLP(MODE_DENSE)
-      } else {
-        JMD_ALLP(MODE_SPARSE_COUNT)
-        hipLaunchKernelGGL(k_row_scan, dim3(1), dim3(1024), 0, st, w.row_count,
-                           w.row_offset, n, w.misc, gate);
-        JMD_LAUNCH_CHECK();
-        if (!count_only) { JMD_ALLP(MODE_SPARSE_WRITE) }
-      }
+  if (n > 0) {
+    if (!sparse && count_only) {
+      JMD_ALLP(MODE_COUNT_MAX)
+    } else if (!sparse) {
+      JMD_ALLP(MODE_DENSE)
+    } else {
+      JMD_ALLP(MODE_SPARSE_COUNT)
+      hipLaunchKernelGGL(k_row_scan, dim3(1), dim3(1024), 0, st, w.row_count, w.row_offset, n,
+                         w.misc, gate);
+      JMD_LAUNCH_CHECK();
+      if (!count_only) { JMD_ALLP(MODE_SPARSE_WRITE) }
     }
+  }
 #undef JMD_ALLP
-  }
-  if (count_only) {
-    // occupancy only: no error bits
-    hipLaunchKernelGGL(k_nl_finish, dim3(1), dim3(64), 0, st, w.misc, occupancy, nullptr,
-                       (long long)0x7fffffffffffLL, gate);
-    JMD_LAUNCH_CHECK();
-    return JMD_OK;
-  }
-  if (sparse) {
+  if (sparse && !count_only) {
     int pb = ceil_div(width, 256);
     pb = pb > 4096 ? 4096 : pb + 1;
     hipLaunchKernelGGL(k_sparse_pad, dim3(pb), dim3(256), 0, st, idx, width, n, w.misc, gate);
     JMD_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_nl_finish, dim3(1), dim3(64), 0, st, w.misc, occupancy, error,
-                     width, gate);
+                     count_only ? (long long)0x7fffffffffffLL : width, gate);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }
@@ -861,10 +1087,11 @@ extern "C" {
 const char* jmd_version(void) { return "jaxmd_native 0.3.0 (gfx950)"; }
 
 size_t jmd_neighbor_workspace_bytes(int64_t n, int32_t dim, int64_t cell_count,
-                                    int32_t format) {
+                                    int32_t cell_capacity, int32_t format) {
   (void)dim;
   (void)format;
-  return carve(nullptr, n, cell_count, nullptr);
+  if (n < 0 || cell_count < 0 || cell_capacity < 0) return 0;
+  return carve(nullptr, n, cell_count, cell_capacity, nullptr);
 }
 
 int jmd_cell_filling_max(const float* position, int64_t n, int32_t dim,
@@ -878,7 +1105,7 @@ int jmd_cell_filling_max(const float* position, int64_t n, int32_t dim,
   Grid g;
   if (!make_grid(dim, cells_per_side, cell_size, 0, &g)) return JMD_ERR_INVALID;
   Work w;
-  carve(workspace, n, g.ncell, &w);
+  carve(workspace, n, g.ncell, 0, &w);
   hipStream_t st = (hipStream_t)stream;
   int nn = (int)n;
   if (nn > 0) {
@@ -915,7 +1142,7 @@ int jmd_cell_list(const float* position, int64_t n, int32_t dim, int32_t positio
   Grid g;
   if (!make_grid(dim, cells_per_side, cell_size, cell_capacity, &g)) return JMD_ERR_INVALID;
   Work w;
-  carve(workspace, n, g.ncell, &w);
+  carve(workspace, n, g.ncell, 0, &w);
   hipStream_t st = (hipStream_t)stream;
   int nn = (int)n;
   long long slots = (long long)g.ncell * cell_capacity;
@@ -983,7 +1210,7 @@ int jmd_neighbor_list_build(const float* position, int64_t n, int32_t dim,
   }
   Space s = make_space(dim, periodic, side);
   Work w;
-  carve(workspace, n, use_cell_list ? g.ncell : 1, &w);
+  carve(workspace, n, use_cell_list ? g.ncell : 1, use_cell_list ? cell_capacity : 0, &w);
   hipStream_t st = (hipStream_t)stream;
   int nn = (int)n;
   return JMD_DISPATCH_LAYOUT(
@@ -1007,7 +1234,7 @@ int jmd_cell_order(const float* position, int64_t n, int32_t dim, int32_t positi
   if (!make_grid(dim, cells_per_side, cell_size, 0, &g)) return JMD_ERR_INVALID;
   if (n == 0) return JMD_OK;
   Work w;
-  carve(workspace, n, g.ncell, &w);
+  carve(workspace, n, g.ncell, 0, &w);
   hipStream_t st = (hipStream_t)stream;
   int nn = (int)n;
   Space s = grid_space_free(dim);

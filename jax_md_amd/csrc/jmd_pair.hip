@@ -461,7 +461,7 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
       }
       return JMD_OK;
     }
-    int tpa = g_tpa_override ? g_tpa_override : (dim == 3 ? 4 : 2);
+    int tpa = g_tpa_override ? g_tpa_override : (dim == 3 ? 8 : 2);
     long long threads = (long long)nn * tpa;
     int nb = ceil_div(threads, 256);
     nb = (nb + 7) & ~7;

@@ -169,7 +169,7 @@ def _filling_max(position, cps, cell_size):
   lib = nat.load()
   N, dim = position.shape
   ncell = reduce(mul, [int(c) for c in cps], 1)
-  ws = _Workspace(position.device, lib.jmd_neighbor_workspace_bytes(N, dim, ncell, 0))
+  ws = _Workspace(position.device, lib.jmd_neighbor_workspace_bytes(N, dim, ncell, 0, 0))
   out = torch.zeros((1,), dtype=torch.int32, device=position.device)
   nat.check(lib.jmd_cell_filling_max(nat.ptr(position), N, dim, _check_position(position),
                                      nat.i3(cps), nat.f3(cell_size),
@@ -242,7 +242,7 @@ def cell_list(box_size, minimum_cell_size, buffer_size_multiplier: float = 1.25)
     id_buf = torch.empty((ncell * capacity,), dtype=torch.int32, device=dev)
     pcid = torch.empty((N, dim), dtype=torch.int32, device=dev)
     ovf = torch.zeros((), dtype=torch.uint8, device=dev)
-    ws = _Workspace(dev, lib.jmd_neighbor_workspace_bytes(N, dim, ncell, 0))
+    ws = _Workspace(dev, lib.jmd_neighbor_workspace_bytes(N, dim, ncell, 0, 0))
     nat.check(lib.jmd_cell_list(nat.ptr(position), N, dim, stride, nat.i3(cps3), nat.f3(cs3),
                                 capacity,
                                 nat.ptr(pos_buf), nat.ptr(id_buf), nat.ptr(pcid), None,
@@ -423,7 +423,7 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
       cap = int(onp.int32(fill) * capacity_multiplier) + extra_capacity
       cell_size = spec.cutoff
     ws = _Workspace(dev, lib.jmd_neighbor_workspace_bytes(N, dim, ncell if use_cells else 0,
-                                                          fcode))
+                                                          int(cap or 0), fcode))
     plan = _Plan(ws, torch.zeros((1,), dtype=torch.int32, device=dev),
                  torch.zeros((1,), dtype=torch.int32, device=dev), use_cells, cps3, cs3, side,
                  bool(geo.periodic), dim, spec)

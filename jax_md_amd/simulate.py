@@ -131,7 +131,8 @@ def nve(energy_or_force_fn, shift_fn, dt=1e-3, **sim_kwargs):
     if momenta is None:
       P = _momenta(key, R, kT, mass)
     else:
-      P = momenta.to(device=R.device, dtype=torch.float32).contiguous()
+      # a private copy: NVELoop advances the state in place
+      P = momenta.to(device=R.device, dtype=torch.float32).clone().contiguous()
     if isinstance(mass, torch.Tensor) and mass.ndim == 2:
       mass = mass.reshape(-1)
     return NVEState(R, P, F, mass, R.to(torch.float64))

@@ -38,7 +38,10 @@ def test_host_only_queries():
     pytest.skip('libjaxmd_native.so not built')
   lib = _native.load()
   assert lib.jmd_version().startswith(b'jaxmd_native')
-  assert lib.jmd_neighbor_workspace_bytes(1000, 3, 125, 0) > 0
+  assert lib.jmd_neighbor_workspace_bytes(1000, 3, 125, 0, 0) > 0
+  # the slot array grows with the cell capacity
+  assert (lib.jmd_neighbor_workspace_bytes(1000, 3, 125, 40, 0)
+          >= lib.jmd_neighbor_workspace_bytes(1000, 3, 125, 0, 0) + 125 * 40 * 4 - 256)
   assert lib.jmd_pair_workspace_bytes(1000) >= 8 * 64
   assert lib.jmd_set_threads_per_atom(3) == _native.JMD_ERR_INVALID
   assert lib.jmd_set_threads_per_atom(0) == _native.JMD_OK
