@@ -79,6 +79,150 @@ def counter_traffic(N):
     return None
 
 
+def adjoint_section(jmd, O, check=True):
+  """Adjoint workload: LJ N = 32,768 (32^3 lattice), gradient of E(x_T) after
+  200 NVE steps (dt = 0.005) with respect to x_0, sigma and epsilon, with
+  checkpoints every 20 steps; time against the same forward run."""
+  import numpy as np
+  import torch
+  dev = 'cuda:0'
+  R, P, L = O.lj_lattice(32, seed=1)
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense)
+  Rt = torch.as_tensor(R, device=dev)
+  Pt = torch.as_tensor(P, device=dev)
+  T, K, dt = 200, 20, 0.005
+  # forward alone: the same kernels, no checkpoints
+  nb = nfn.allocate(Rt)
+  init_fn, _ = jmd.simulate.nve(efn, shift, dt)
+  st = init_fn(0, Rt, kT=0.0, momenta=Pt, neighbor=nb)
+  loop = jmd.simulate.NVELoop(efn, shift, dt, reorder=False, graph=False)
+  loop.prime(st, nb)
+  loop._run_steps(st, nb, 20)  # warm
+  st = init_fn(0, Rt, kT=0.0, momenta=Pt, neighbor=nb)
+  nb = nfn.allocate(Rt)
+  loop.prime(st, nb)
+  torch.cuda.synchronize()
+  t = time.perf_counter()
+  loop._run_steps(st, nb, T)
+  E_fwd, _ = efn.energy_and_force(st.position, nb)
+  torch.cuda.synchronize()
+  t_fwd = time.perf_counter() - t
+  jmd.adjoint.nve_energy_gradient(efn, shift, nfn, Rt, Pt, dt, 20, checkpoint_every=K)  # warm
+  torch.cuda.synchronize()
+  t = time.perf_counter()
+  g = jmd.adjoint.nve_energy_gradient(efn, shift, nfn, Rt, Pt, dt, T, checkpoint_every=K)
+  torch.cuda.synchronize()
+  t_grad = time.perf_counter() - t
+  out = {'workload': 'LJ N=32768 (32^3 lattice, rho=0.8442, jitter 0.05, kT=1), 200 NVE '
+                     'steps dt=0.005, dE(x_T)/d(x_0, sigma, epsilon), checkpoints every 20',
+         'forward_s': t_fwd, 'gradient_s': t_grad, 'ratio_vs_forward': t_grad / t_fwd,
+         'checkpoint_bytes': g.checkpoint_bytes, 'energy_T': g.energy}
+  if check:
+    try:
+      from oracle import c_oracle
+      h = 1e-4
+      rng = np.random.default_rng(5)
+      u = rng.normal(size=R.shape)
+      u /= np.linalg.norm(u)
+      dt32 = float(np.float32(dt))
+
+      def e_after(R0, sigma=1.0, eps=1.0):
+        sysc = c_oracle.LJSystem(R0, float(L), sigma=sigma, eps=eps)
+        sysc.nve(P.astype(np.float64), dt32, T)
+        return sysc.forces()[0]
+
+      R64 = R.astype(np.float64)
+      fd_x = (e_after(R64 + h * u) - e_after(R64 - h * u)) / (2 * h)
+      ad_x = float(np.sum(g.position.cpu().numpy() * u))
+      fd_s = (e_after(R64, sigma=1 + h) - e_after(R64, sigma=1 - h)) / (2 * h)
+      fd_e = (e_after(R64, eps=1 + h) - e_after(R64, eps=1 - h)) / (2 * h)
+      gn = float(np.linalg.norm(g.position.cpu().numpy()))
+      out['fd_check'] = {
+          'h': h, 'direction_x': {'adjoint': ad_x, 'fd': fd_x,
+                                  'rel_err': abs(ad_x - fd_x) / max(abs(fd_x), 1e-30),
+                                  'err_over_grad_norm': abs(ad_x - fd_x) / gn},
+          'sigma': {'adjoint': g.sigma, 'fd': fd_s, 'rel_err': abs(g.sigma - fd_s) / abs(fd_s)},
+          'epsilon': {'adjoint': g.epsilon, 'fd': fd_e,
+                      'rel_err': abs(g.epsilon - fd_e) / abs(fd_e)},
+          'reference': 'C FP64 restatement (oracle/md_oracle.c), central differences'}
+    except Exception as e:  # the checker needs gcc-built oracle/libmd_oracle.so
+      out['fd_check'] = f'skipped: {e}'
+  log(f'adjoint: forward {t_fwd * 1e3:.1f} ms, gradient {t_grad * 1e3:.1f} ms, '
+      f'ratio {t_grad / t_fwd:.2f}')
+  return out
+
+
+def batched_section(jmd, O, n_systems=512, n_steps=500, plain_sample=8, plain_steps=50):
+  """Batched workload: 512 bidisperse 2-D soft-sphere systems (N log-uniform in
+  [64, 4096], seed 0), 500 FIRE steps (dt_start 0.01, dt_max 0.1) in one
+  launch, against per-system plain launches on a sample of the systems."""
+  import numpy as np
+  import torch
+  dev = 'cuda:0'
+  rng = np.random.default_rng(0)
+  sizes = np.floor(np.exp(rng.uniform(np.log(64), np.log(4096), n_systems))).astype(int)
+  systems = [O.bidisperse_2d(int(n), seed=1000 + k) for k, n in enumerate(sizes)]
+  sigma = np.array([[1.0, 1.2], [1.2, 1.4]], np.float32)
+  fire = dict(dt_start=0.01, dt_max=0.1)
+
+  def make():
+    return jmd.ensemble.SoftSphereFireEnsemble([s[0] for s in systems], [s[2] for s in systems],
+                                               [s[1] for s in systems], sigma=sigma, **fire)
+
+  ens = make().allocate()
+  ens.run(5)  # warm
+  ens = make().allocate()
+  torch.cuda.synchronize()
+  t = time.perf_counter()
+  ens.run(n_steps)
+  torch.cuda.synchronize()
+  t_ens = time.perf_counter() - t
+  total = int(sizes.sum())
+  value = total * n_steps / t_ens
+  overflow = int((ens.did_buffer_overflow != 0).sum().item())
+  # plain per-system path on a sample spread over the sizes
+  pick = np.argsort(sizes)[np.linspace(0, n_systems - 1, plain_sample).astype(int)]
+  t_plain, steps_atoms = 0.0, 0
+  for k in pick:
+    R, sp, L = systems[k]
+    disp, shift = jmd.space.periodic(L)
+    nfn, efn = jmd.energy.soft_sphere_neighbor_list(
+        disp, L, species=torch.as_tensor(sp, device=dev),
+        sigma=torch.as_tensor(sigma, device=dev), format=jmd.partition.Dense)
+    init_fn, apply_fn = jmd.minimize.fire_descent(efn, shift, **fire)
+    Rt = torch.as_tensor(R, device=dev)
+    nb = nfn.allocate(Rt)
+    st = init_fn(Rt, neighbor=nb)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(plain_steps):
+      nb = nb.update(st.position)
+      st = apply_fn(st, neighbor=nb)
+    torch.cuda.synchronize()
+    t_plain += time.perf_counter() - t
+    steps_atoms += len(R) * plain_steps
+  plain = steps_atoms / t_plain
+  # energies at the final positions against FP64 on a few systems
+  E = ens.energy.cpu().numpy()
+  errs = []
+  for k in pick[:4]:
+    R, sp, L = systems[k]
+    model = O.PairModel('soft_sphere', sigma=sigma.astype(np.float64), species=sp)
+    pos = ens.position[ens.system_slice(int(k))].cpu().numpy().astype(np.float64)
+    E64 = O.energy_force_brute(model, O.Space(np.float64(L)), pos)[0]
+    errs.append(abs(E[k] - E64) / max(abs(E64), 1e-30))
+  log(f'batched: {value:.3e} atom-steps/s in {t_ens * 1e3:.1f} ms, plain {plain:.3e}, '
+      f'x{value / plain:.1f}')
+  return {'workload': f'{n_systems} bidisperse 2-D soft-sphere systems (N log-uniform '
+                      f'64..4096, total {total}), {n_steps} FIRE steps, one launch',
+          'value': value, 'unit': 'atom-steps/s', 'seconds': t_ens,
+          'plain_per_system_value': plain,
+          'plain_sample': f'{plain_sample} systems spread over the sizes, {plain_steps} steps each',
+          'speedup_vs_plain': value / plain, 'systems_overflowed': overflow,
+          'energy_rel_err_vs_fp64_max': float(max(errs)) if errs else None}
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument('--gpus', type=int, default=1)
@@ -90,6 +234,8 @@ def main():
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--tpa', type=int, default=0, help='threads per row of the pair kernel')
   ap.add_argument('--eager', action='store_true', help='no HIP graphs (host-launched steps)')
+  ap.add_argument('--no-extra', action='store_true',
+                  help='skip the batched-ensemble and adjoint measurements')
   ap.add_argument('--copy-calibration', action='store_true',
                   help='only run a plain 256 MiB device copy (the reference kernel that '
                        'scales the FETCH_SIZE / WRITE_SIZE counters) and exit')
@@ -238,6 +384,14 @@ def main():
       'energy': {'E_total_start': float(E_start) + float(KE_start),
                  'E_total_end': float(E_end) + float(KE_end)},
   }
+  if not args.no_extra:
+    for key, fn in (('batched', lambda: batched_section(jmd, O)),
+                    ('adjoint', lambda: adjoint_section(jmd, O, check=not args.no_cpu))):
+      try:
+        out[key] = fn()
+      except Exception as e:  # reported, never silently dropped
+        out[key] = f'failed: {type(e).__name__}: {e}'
+        log(f'{key} section failed: {e}')
   print(json.dumps(out), flush=True)
 
 

@@ -224,6 +224,101 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
                           void* workspace, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Adjoint of the pair force                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Scratch bytes for jmd_pair_force_vjp. */
+size_t jmd_adjoint_workspace_bytes(int64_t n, int32_t dim);
+
+/* Vector-Jacobian product of the force F(R) = -dE/dR of
+ * jmd_pair_energy_force (Dense lists), the gradient the reference obtains
+ * from jax.grad through smap.pair_neighbor_list (quantity.py:58-60,
+ * smap.py:856-979).  For a cotangent v [n, dim]:
+ *   position_cotangent [n, dim] = (dF/dR)^T v  (dF/dR is symmetric)
+ *   param_cotangent    [2] float64 = v^T dF/dsigma, v^T dF/depsilon
+ *   energy_param_grad  [2] float64 = dE/dsigma, dE/depsilon
+ * (the last two for scalar sigma and epsilon only; NULL skips them;
+ * r_onset / r_cutoff are constants, energy.py:319-320).  Pair terms float32,
+ * row sums float64, fixed-order reductions. */
+int jmd_pair_force_vjp(const float* position, int64_t n, int32_t dim,
+                       int32_t position_stride, int32_t periodic, const float* side,
+                       const jmd_pair_potential* potential, const int32_t* idx,
+                       int64_t idx_width, int32_t format, const float* cotangent,
+                       float* position_cotangent, double* param_cotangent,
+                       double* energy_param_grad, void* workspace, void* stream);
+
+/* Doubles in the per-workgroup parameter accumulator of
+ * jmd_nve_adjoint_step (zero-filled by the caller). */
+size_t jmd_adjoint_param_acc_doubles(int64_t n, int32_t dim);
+
+/* One backward step of the velocity-Verlet adjoint (simulate.py:227-243
+ * differentiated as jax.grad would through lax.fori_loop): with h = J(x) ph_in,
+ *   x_adjoint += coef_x h        (float64, [n, dim])
+ *   ph_out     = ph_in + coef_p x_adjoint   (float32; must not alias ph_in)
+ *   param_acc[2 b + k] += coef_param (ph_in^T dF/dtheta_k), k = sigma, eps
+ * param_acc may be NULL.  No counterpart entry point in the reference. */
+int jmd_nve_adjoint_step(const float* position, int64_t n, int32_t dim,
+                         int32_t position_stride, int32_t periodic, const float* side,
+                         const jmd_pair_potential* potential, const int32_t* idx,
+                         int64_t idx_width, const float* ph_in, float* ph_out,
+                         double* x_adjoint, float coef_x, float coef_p, double* param_acc,
+                         float coef_param, void* stream);
+
+/* out[0..1] = fixed-order sums of the accumulator's two columns. */
+int jmd_adjoint_param_sum(const double* param_acc, int64_t n, int32_t dim, double* out,
+                          void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Ragged ensembles of 2-D systems (one persistent launch)                   */
+/* ------------------------------------------------------------------------ */
+
+/* One system of an ensemble: particles [offset, offset + n) of the
+ * concatenated arrays, its periodic box, the cell grid of its neighbour list
+ * (cells_per_side = floor(side / cutoff), cell_size = side / cells_per_side,
+ * partition.py:146-188), the grid its cell capacity is counted on
+ * (partition.py:249-250 derives it a second time), and its capacities. */
+typedef struct {
+  int32_t offset, n;
+  int32_t cells_per_side[2];
+  float cell_size[2];
+  int32_t fill_cells_per_side[2];
+  float fill_cell_size[2];
+  float side[2];
+  int32_t cell_capacity, max_occupancy;
+} jmd_system2d;
+
+/* Largest cell count (either grid) a system may have. */
+int jmd_fire_ensemble_max_cells(void);
+
+/* Soft-sphere FIRE minimisation of every system of a ragged ensemble, one
+ * workgroup per system (systems[order[b]] for workgroup b): the reference's
+ * loop `nbrs = nbrs.update(R); state = apply_fn(state, neighbor=nbrs)` of
+ * minimize.fire_descent (minimize.py:124-226) over
+ * energy.soft_sphere_neighbor_list with a Dense list (energy.py:200-243,
+ * partition.py:801-1164), for every system at once.  mode:
+ *   0  out_stat[s] = largest cell filling on the fill grid (for capacities)
+ *   1  out_stat[s] = largest row count of a list built now (for widths)
+ *   2  allocate: build the list (idx [n_total, width], global ids, padding
+ *      n_total), F = F(R), P = 0, dt = dt_start, alpha = alpha_start,
+ *      energy[s] = E(R)
+ *   3  n_steps FIRE steps; energy[s] = E at the end
+ * fire_params = static_cast float16 values (dt_start, dt_max, n_min, f_inc,
+ * f_dec, alpha_start, f_alpha); fire_state [n_systems, 2] = (dt, alpha),
+ * fire_n_pos [n_systems]; error [n_systems] sticky overflow bits; builds
+ * [n_systems] counts list builds; scratch holds 3 * n_total int32.  The
+ * potential's parameters index particles by their global (concatenated) id.
+ * No counterpart entry point in the reference, which runs one system at a
+ * time. */
+int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int32_t n_systems,
+                         int64_t n_total, int32_t mode, int32_t n_steps, double* position_hi,
+                         float* position, float* momentum, float* force,
+                         float* reference_position, int32_t* idx, int32_t width,
+                         int32_t* scratch, const jmd_pair_potential* potential,
+                         float cutoff_sq, float threshold_sq, const float* fire_params,
+                         float* fire_state, int32_t* fire_n_pos, uint8_t* error,
+                         int32_t* out_stat, double* energy, int32_t* builds, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Integrators                                                               */
 /* ------------------------------------------------------------------------ */
 

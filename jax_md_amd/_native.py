@@ -73,6 +73,23 @@ _SIGNATURES = {
                                    c_int32, c_int32, c_void_p]),
     'jmd_remap_neighbor_ids': (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int32,
                                          c_int64, c_void_p]),
+    'jmd_adjoint_workspace_bytes': (c_size_t, [c_int64, c_int32]),
+    'jmd_pair_force_vjp': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
+                                     POINTER(c_float), POINTER(JmdPairPotential), c_void_p,
+                                     c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
+    'jmd_adjoint_param_acc_doubles': (c_size_t, [c_int64, c_int32]),
+    'jmd_nve_adjoint_step': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
+                                       POINTER(c_float), POINTER(JmdPairPotential), c_void_p,
+                                       c_int64, c_void_p, c_void_p, c_void_p, c_float, c_float,
+                                       c_void_p, c_float, c_void_p]),
+    'jmd_adjoint_param_sum': (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    'jmd_fire_ensemble_max_cells': (c_int32, []),
+    'jmd_fire_ensemble_2d': (c_int32, [c_void_p, c_void_p, c_int32, c_int64, c_int32, c_int32,
+                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_int32, c_void_p, POINTER(JmdPairPotential),
+                                       c_float, c_float, POINTER(c_float), c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'jmd_fire_workspace_bytes': (c_size_t, [c_int64]),
     'jmd_fire_update': (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p,
                                   POINTER(c_float), c_void_p, c_void_p, c_void_p]),

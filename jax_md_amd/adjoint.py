@@ -1,0 +1,230 @@
+"""Gradients through the pair force and through NVE trajectories on gfx950.
+
+The reference obtains every gradient from ``jax.grad``: of the energy for the
+force (quantity.py:58-60), of that force for the Hessian, and through
+``lax.fori_loop`` over ``simulate.nve``'s step for a trajectory
+(simulate.py:227-317).  Here each is a kernel of its own:
+
+* ``force_vjp(energy_fn, R, neighbor, v)``: the VJP of F(R; sigma, epsilon) =
+  -dE/dR, that is J v with J = dF/dR (symmetric), v^T dF/dsigma and
+  v^T dF/depsilon, and dE/dsigma, dE/depsilon, from one pass over the list;
+* ``ForceFunction``: the force as a ``torch.autograd.Function`` whose backward
+  is that kernel (the counterpart of a ``jax.custom_vjp`` on the force);
+* ``nve_energy_gradient``: d E(x_T) / d(x_0, p_0, sigma, epsilon) after T
+  velocity-Verlet steps, by checkpointed recomputation: the forward run keeps
+  the state and list every ``checkpoint_every`` steps; the backward pass
+  replays one segment at a time (same kernels, same list decisions, so the
+  replay is bit-identical) and runs one fused adjoint kernel per step.
+
+Only Dense lists and scalar sigma / epsilon parameters are differentiated
+with respect to parameters; r_onset / r_cutoff stay fixed as the reference
+fixes them at construction (energy.py:319-320).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import time
+
+import numpy as onp
+import torch
+
+from . import _native as nat
+from . import partition
+from . import quantity
+from . import simulate
+
+f32 = onp.float32
+
+
+def _energy(energy_fn):
+  e = quantity.native_energy_of(energy_fn)
+  return e
+
+
+def _workspace(device, n, dim):
+  lib = nat.load()
+  return torch.empty((lib.jmd_adjoint_workspace_bytes(n, dim),), dtype=torch.uint8,
+                     device=device)
+
+
+def force_vjp(energy_fn, R, neighbor, cotangent, params: bool = True, **dynamic_kwargs):
+  """(J v [N, dim] float32, (v.dF/dsigma, v.dF/deps) float64 [2] or None,
+  (dE/dsigma, dE/deps) float64 [2] or None) for the pair energy ``energy_fn``
+  over the Dense list ``neighbor``."""
+  energy = _energy(energy_fn)
+  if neighbor.format is not partition.Dense:
+    raise NotImplementedError('the force adjoint runs over Dense neighbour lists')
+  lib = nat.load()
+  N, dim = R.shape
+  dev = R.device
+  v = cotangent.to(device=dev, dtype=torch.float32).contiguous()
+  out = torch.empty((N, dim), dtype=torch.float32, device=dev)
+  pc = torch.empty((2,), dtype=torch.float64, device=dev) if params else None
+  eg = torch.empty((2,), dtype=torch.float64, device=dev) if params else None
+  pot, keep = energy._potential(R, dynamic_kwargs)
+  geo = energy.geometry
+  ws = _workspace(dev, N, dim)
+  nat.check(lib.jmd_pair_force_vjp(
+      nat.ptr(R), N, dim, nat.position_stride(R, 'R'), int(geo.periodic),
+      nat.f3(geo.side_values(dim)), nat.ctypes.byref(pot), nat.ptr(neighbor.idx),
+      neighbor.idx.shape[1], nat.DENSE, nat.ptr(v), nat.ptr(out), nat.ptr(pc), nat.ptr(eg),
+      nat.ptr(ws), nat.stream_handle(dev)), 'jmd_pair_force_vjp')
+  del keep
+  return out, pc, eg
+
+
+class ForceFunction(torch.autograd.Function):
+  """F = -dE/dR as a differentiable function of (R, sigma, epsilon); the
+  backward pass is ``force_vjp``.  Use through ``differentiable_force``."""
+
+  @staticmethod
+  def forward(ctx, R, sigma, epsilon, energy_fn, neighbor):
+    kw = {}
+    if sigma is not None:
+      kw['sigma'] = float(sigma)
+    if epsilon is not None:
+      kw['epsilon'] = float(epsilon)
+    F = _energy(energy_fn).force(R.detach(), neighbor, **kw)
+    ctx.save_for_backward(R.detach())
+    ctx.energy_fn, ctx.neighbor, ctx.kw = energy_fn, neighbor, kw
+    ctx.has = (sigma is not None, epsilon is not None)
+    return F
+
+  @staticmethod
+  def backward(ctx, g):
+    (R,) = ctx.saved_tensors
+    want = ctx.needs_input_grad[1] or ctx.needs_input_grad[2]
+    Jv, pc, _ = force_vjp(ctx.energy_fn, R, ctx.neighbor, g, params=want, **ctx.kw)
+    gs = ge = None
+    if want:
+      gs = pc[0].to(torch.float32) if ctx.has[0] else None
+      ge = pc[1].to(torch.float32) if ctx.has[1] else None
+    return Jv, gs, ge, None, None
+
+
+def differentiable_force(energy_fn):
+  """force(R, neighbor, sigma=None, epsilon=None) differentiable in R and in
+  the (0-d tensor) parameters passed."""
+  def force(R, neighbor, sigma=None, epsilon=None):
+    return ForceFunction.apply(R, sigma, epsilon, energy_fn, neighbor)
+  return force
+
+
+@dataclasses.dataclass
+class TrajectoryGradient:
+  energy: float                 # E(x_T), float64
+  position: torch.Tensor        # dE/dx_0 [N, dim] float64
+  momentum: torch.Tensor        # dE/dp_0 [N, dim] float32
+  sigma: float                  # dE/dsigma (through the trajectory and directly)
+  epsilon: float
+  seconds_forward: float        # forward run with checkpointing
+  seconds_backward: float       # replay + adjoint
+  checkpoint_bytes: int         # device memory held by checkpoints and a segment
+
+
+def _snapshot(state, nbrs):
+  return dict(x_hi=state.position_hi.clone(), x=state.position.clone(),
+              p=state.momentum.clone(), F=state.force.clone(), idx=nbrs.idx.clone(),
+              ref=nbrs.reference_position.clone(), err=nbrs.error.code.clone(),
+              gate=nbrs.plan.gate.clone())
+
+
+def _restore(snap, state, nbrs):
+  state.position_hi.copy_(snap['x_hi'])
+  state.position.copy_(snap['x'])
+  state.momentum.copy_(snap['p'])
+  state.force.copy_(snap['F'])
+  nbrs.idx.copy_(snap['idx'])
+  nbrs.reference_position.copy_(snap['ref'])
+  nbrs.error.code.copy_(snap['err'])
+  nbrs.plan.gate.copy_(snap['gate'])
+
+
+def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int,
+                        checkpoint_every: int = 20, mass=1.0,
+                        neighbor=None) -> TrajectoryGradient:
+  """Gradient of E(x_T) after ``n_steps`` velocity-Verlet steps from (R, P)
+  with respect to R, P, sigma and epsilon.
+
+  Backward recursion (one fused kernel per step): with adjoints X of x and PH
+  of the half-step momentum, X_T = -F(x_T), PH = (dt/m) X_T, and for
+  k = T-1 .. 0:  X += c_k J(x_k) PH;  theta += c_k PH^T dF(x_k)/dtheta;
+  PH <- PH + (dt/m) X, where c_k = dt for k >= 1 and dt/2 for k = 0 (the
+  product at x_k enters steps k and k-1); dE/dp_0 is the last PH.
+  """
+  energy = _energy(energy_fn)
+  if not isinstance(mass, (int, float)):
+    raise NotImplementedError('the trajectory adjoint takes a scalar mass')
+  lib = nat.load()
+  dev = R.device
+  N, dim = R.shape
+  dt32 = f32(dt)
+  m = float(f32(mass))
+  nbrs = neighbor if neighbor is not None else neighbor_fn.allocate(R)
+  if nbrs.format is not partition.Dense:
+    raise NotImplementedError('the trajectory adjoint runs over Dense neighbour lists')
+  init_fn, _ = simulate.nve(energy, shift_fn, dt)
+  state = init_fn(0, R, kT=0.0, mass=mass, momenta=P, neighbor=nbrs)
+  loop = simulate.NVELoop(energy, shift_fn, dt, reorder=False, graph=False)
+  loop.prime(state, nbrs)
+  K = max(1, int(checkpoint_every))
+  torch.cuda.synchronize(dev)
+  t0 = time.perf_counter()
+  ckpts = []
+  done = 0
+  while done < n_steps:
+    k = min(K, n_steps - done)
+    ckpts.append((done, k, _snapshot(state, nbrs)))
+    loop._run_steps(state, nbrs, k)
+    done += k
+  torch.cuda.synchronize(dev)
+  t1 = time.perf_counter()
+
+  # final energy, X_T = -F(x_T), direct parameter gradient
+  E_T, F_T = energy.energy_and_force(state.position, nbrs)
+  zero = torch.zeros((N, dim), dtype=torch.float32, device=dev)
+  _, _, direct = force_vjp(energy, state.position, nbrs, zero, params=True)
+  X = (-F_T).to(torch.float64).contiguous()
+  PH = ((dt32 / m) * X).to(torch.float32).contiguous()
+  PH2 = torch.empty_like(PH)
+  acc = torch.zeros((lib.jmd_adjoint_param_acc_doubles(N, dim),), dtype=torch.float64,
+                    device=dev)
+  pot, keep = energy._potential(state.position, {})
+  geo = energy.geometry
+  side = nat.f3(geo.side_values(dim))
+  st = nat.stream_handle(dev)
+  if n_steps == 0:
+    PH.zero_()  # dE/dp_0 = 0 when x_T = x_0
+  xs = [simulate.new_positions(N, dim, dev) for _ in range(K)]
+  idxs = [torch.empty_like(nbrs.idx) for _ in range(K)]
+  width = nbrs.idx.shape[1]
+  ckpt_bytes = sum(sum(t.numel() * t.element_size() for t in c[2].values()) for c in ckpts)
+  ckpt_bytes += sum(t.numel() * t.element_size() for t in xs + idxs)
+
+  for start, k, snap in reversed(ckpts):
+    _restore(snap, state, nbrs)
+
+    def hook(q, start=start):
+      xs[q].copy_(state.position)
+      idxs[q].copy_(nbrs.idx)
+
+    loop._run_steps(state, nbrs, k, hook=hook)
+    for q in range(k - 1, -1, -1):
+      step = start + q
+      c = float(dt32) if step >= 1 else float(f32(dt32 / f32(2)))
+      nat.check(lib.jmd_nve_adjoint_step(
+          nat.ptr(xs[q]), N, dim, nat.position_stride(xs[q]), int(geo.periodic), side,
+          nat.ctypes.byref(pot), nat.ptr(idxs[q]), width, nat.ptr(PH), nat.ptr(PH2),
+          nat.ptr(X), c, float(dt32) / m, nat.ptr(acc), c, st), 'jmd_nve_adjoint_step')
+      if step >= 1:
+        PH, PH2 = PH2, PH
+  out = torch.empty((2,), dtype=torch.float64, device=dev)
+  nat.check(lib.jmd_adjoint_param_sum(nat.ptr(acc), N, dim, nat.ptr(out), st),
+            'jmd_adjoint_param_sum')
+  del keep
+  torch.cuda.synchronize(dev)
+  t2 = time.perf_counter()
+  tot = (out + direct).cpu().numpy()
+  return TrajectoryGradient(float(E_T), X, PH.clone(), float(tot[0]), float(tot[1]),
+                            t1 - t0, t2 - t1, int(ckpt_bytes))

@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libjaxmd_native.so')
 
-SOURCES = ['jmd_neighbor.hip', 'jmd_pair.hip', 'jmd_integrate.hip', 'jmd_layout.hip']
+SOURCES = ['jmd_neighbor.hip', 'jmd_pair.hip', 'jmd_adjoint.hip', 'jmd_ensemble.hip',
+           'jmd_integrate.hip', 'jmd_layout.hip']
 
 FLAGS = [
     '--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',

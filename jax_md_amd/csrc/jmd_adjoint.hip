@@ -1,0 +1,407 @@
+// Adjoint of the pair force over a Dense neighbour list for gfx950.
+//
+// The reference differentiates through the force with jax.grad
+// (quantity.py:58-60, simulate.py:227-243).  Here the vector-Jacobian product
+// of F(R; sigma, epsilon) = -dE/dR is a kernel of its own: for a cotangent v,
+//
+//   (J v)_i = sum_j [ g w + 2 g' (d . w) d ],     d = R_j - R_i,  w = v_j - v_i,
+//
+// with g(s) = U'(r)/r and g' = dg/ds as functions of s = r^2 (J = dF/dR is
+// symmetric, so the VJP is this product), and the parameter cotangents
+//
+//   v^T dF/dtheta = sum_i sum_j (dg_ij/dtheta) (v_i . d_ij)
+//
+// for scalar sigma and epsilon (r_onset and r_cutoff stay fixed, as the
+// reference fixes them at construction, energy.py:319-320).  Rows are owned by
+// TPA lanes exactly as in the force kernel: register accumulation, shuffle
+// reduction, no atomics, deterministic.
+//
+// For a velocity-Verlet trajectory (x_{k+1} = x_k + dt ph_k / m,
+// ph_k = p_k + dt/2 F(x_k), p_{k+1} = ph_k + dt/2 F(x_{k+1})) the backward
+// step needs one product J(x_k) PH_k per step: the product at x_{k+1} with
+// P_{k+1} = PH_{k+1} serves both adjacent steps.  k_pair_adjoint's step mode
+// fuses that product with the elementwise recursion
+//   X_k = X_{k+1} + c_x J(x_k) PH_k,   PH_{k-1} = PH_k + (dt/m) X_k,
+// and accumulates the parameter cotangents per workgroup.
+#include "jmd_pair_common.hpp"
+
+namespace jmd {
+
+// Second-order pair terms from s = r^2 (float32, contraction off):
+//   g = U'(r)/r, gp = dg/ds, gs = dg/dsigma, ge = dg/depsilon,
+//   us = dU/dsigma, ue = dU/depsilon.
+struct PairD2 {
+  float g, gp, gs, ge, us, ue;
+};
+
+template <int KIND>
+__device__ __forceinline__ PairD2 pair_eval2(float s, float sig, float eps, float alp,
+                                             const PairArgs& a) {
+#pragma clang fp contract(off)
+  PairD2 o;
+  if (KIND == JMD_LENNARD_JONES) {
+    // V = 4 eps (q^6 - q^3), q = sigma^2 / s:
+    //   g_V = 2 dV/ds = -24 eps (2 q^6 - q^3) / s
+    //   dg_V/ds = 48 eps (7 q^6 - 2 q^3) / s^2
+    //   dV/dsigma = 24 eps (2 q^6 - q^3) / sigma,  dg_V/dsigma = -144 eps (4 q^6 - q^3) / (sigma s)
+    const float inv = fast_rcp(s);
+    const float inv_sig = fast_rcp(sig);
+    const float q = sig * sig * inv;
+    const float s6 = q * q * q;
+    const float s12 = s6 * s6;
+    const float c2 = fmaf(2.0f, s12, -s6);
+    const float V = 4.0f * eps * (s12 - s6);
+    const float gV = -24.0f * eps * c2 * inv;
+    const float gVp = 48.0f * eps * fmaf(7.0f, s12, -2.0f * s6) * inv * inv;
+    const float V_sig = 24.0f * eps * c2 * inv_sig;
+    const float gV_sig = -144.0f * eps * fmaf(4.0f, s12, -s6) * inv_sig * inv;
+    const float V_eps = 4.0f * (s12 - s6);
+    const float gV_eps = -24.0f * c2 * inv;
+    o.g = gV;
+    o.gp = gVp;
+    o.gs = gV_sig;
+    o.ge = gV_eps;
+    o.us = V_sig;
+    o.ue = V_eps;
+    if (a.smooth && s >= a.ro2) {
+      // U = V S(s), S = (rc^2 - s)^2 (rc^2 + 2 s - 3 ro^2) / (rc^2 - ro^2)^3
+      const float u = a.rc2 - s;
+      const float S = u * u * fmaf(2.0f, s, fmaf(-3.0f, a.ro2, a.rc2)) * a.inv_d;
+      const float Ss = 6.0f * u * (a.ro2 - s) * a.inv_d;
+      const float Sss = 6.0f * (2.0f * s - a.ro2 - a.rc2) * a.inv_d;
+      o.g = fmaf(S, gV, 2.0f * Ss * V);
+      o.gp = fmaf(gVp, S, fmaf(2.0f * gV, Ss, 2.0f * V * Sss));
+      o.gs = fmaf(S, gV_sig, 2.0f * Ss * V_sig);
+      o.ge = fmaf(S, gV_eps, 2.0f * Ss * V_eps);
+      o.us = S * V_sig;
+      o.ue = S * V_eps;
+    }
+    if (!(s < a.rc2)) o = PairD2{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  } else {
+    // U = eps / alpha b^alpha, b = 1 - r / sigma, for 0 < r < sigma:
+    //   g = -eps b^(alpha-1) / (sigma r)
+    //   dg/ds = eps / (2 sigma r) [(alpha-1) b^(alpha-2) / (sigma r) + b^(alpha-1) / r^2]
+    //   dU/dsigma = eps b^(alpha-1) r / sigma^2
+    //   dg/dsigma = -eps / (r sigma^2) [(alpha-1) b^(alpha-2) r / sigma - b^(alpha-1)]
+    const float inv_r = fast_rsqrt(s);
+    const float r = s * inv_r;
+    const float inv_sig = fast_rcp(sig);
+    const float b = 1.0f - r * inv_sig;
+    float bm1, bm2;
+    if (alp == 2.0f) {
+      bm1 = b;
+      bm2 = 1.0f;
+    } else {
+      bm2 = powf(fmaxf(b, 0.0f), alp - 2.0f);
+      bm1 = bm2 * b;
+    }
+    const bool valid = (s < sig * sig) && (s > 0.0f);
+    o.g = -eps * inv_sig * bm1 * inv_r;
+    o.gp = 0.5f * eps * inv_sig * inv_r *
+           fmaf((alp - 1.0f) * bm2, inv_sig * inv_r, bm1 * inv_r * inv_r);
+    o.ge = -inv_sig * bm1 * inv_r;
+    o.ue = bm1 * b / alp;
+    o.us = eps * bm1 * r * inv_sig * inv_sig;
+    o.gs = -eps * inv_r * inv_sig * inv_sig * fmaf((alp - 1.0f) * bm2, r * inv_sig, -bm1);
+    if (!valid) o = PairD2{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  }
+  return o;
+}
+
+enum { ADJ_VJP = 0, ADJ_STEP = 1 };
+
+// One launch over all rows.  ADJ_VJP: hout = J v, part[4 b + k] = block sums
+// of (v^T dF/dsigma, v^T dF/deps, dE/dsigma, dE/deps).  ADJ_STEP:
+// X += c_x (J v) (float64), vout = v + c_p X, part[2 b + k] += c_t (v^T dF/dtheta).
+template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, int MODE>
+__global__ __launch_bounds__(256) void k_pair_adjoint(
+    const float* __restrict__ pos, int n, Space s, PairArgs a, const int* __restrict__ idx,
+    int width, const float* __restrict__ vin, float* __restrict__ hout,
+    double* __restrict__ part, int want_params, double* __restrict__ xadj,
+    float* __restrict__ vout, float cx, float cp, float ct) {
+#pragma clang fp contract(off)
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int gt = tile * blockDim.x + threadIdx.x;
+  const int i = gt / TPA;
+  const int sub = gt % TPA;
+  const bool active = i < n;
+  float h[3] = {0.0f, 0.0f, 0.0f};
+  float ts = 0.0f, te = 0.0f, us = 0.0f, ue = 0.0f;
+  float xi[3] = {0, 0, 0}, xim[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+  if (active) {
+    load_pos<DIM, STRIDE>(pos, i, xi);
+    load_pos<DIM>(vin, i, vi);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) xim[c] = xi[c] - s.L[c];
+  }
+  if (active && width > 0) {
+    const int* row = idx + (long long)i * width;
+    float sig = a.sig, eps = a.eps, alp = a.alp;
+    int si = 0;
+    if (GENERAL && a.species) si = a.species[i];
+    for (int k = sub; k < width; k += TPA) {
+      const unsigned jr = (unsigned)row[k];
+      if (jr >= (unsigned)n) break;  // left-compacted rows, padding n
+      const unsigned j = jr;
+      float xj[3] = {0, 0, 0}, vj[3] = {0, 0, 0};
+      load_pos<DIM, STRIDE>(pos, j, xj);
+      load_pos<DIM>(vin, j, vj);
+      float d[3] = {0.0f, 0.0f, 0.0f}, w[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        d[c] = s.periodic ? min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c])
+                          : xj[c] - xi[c];
+        w[c] = vj[c] - vi[c];
+      }
+      const float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                                : fmaf(d[1], d[1], d[0] * d[0]);
+      if (GENERAL) {
+        const int sj = a.species ? a.species[j] : 0;
+        sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
+        eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
+        if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
+      }
+      const PairD2 D = pair_eval2<KIND>(r2, sig, eps, alp, a);
+      const float dw = DIM == 3 ? fmaf(d[2], w[2], fmaf(d[1], w[1], d[0] * w[0]))
+                                : fmaf(d[1], w[1], d[0] * w[0]);
+      const float t = 2.0f * D.gp * dw;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) h[c] = fmaf(D.g, w[c], fmaf(t, d[c], h[c]));
+      if (want_params) {
+        const float lam = DIM == 3 ? fmaf(vi[2], d[2], fmaf(vi[1], d[1], vi[0] * d[0]))
+                                   : fmaf(vi[1], d[1], vi[0] * d[0]);
+        ts = fmaf(D.gs, lam, ts);
+        te = fmaf(D.ge, lam, te);
+        us += D.us;
+        ue += D.ue;
+      }
+    }
+  }
+  double hd[3];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) hd[c] = group_sum<TPA>((double)h[c]);
+  double pd[4] = {0.0, 0.0, 0.0, 0.0};
+  if (want_params) {
+    pd[0] = group_sum<TPA>((double)ts);
+    pd[1] = group_sum<TPA>((double)te);
+    pd[2] = group_sum<TPA>((double)us);
+    pd[3] = group_sum<TPA>((double)ue);
+  }
+  const bool owner = active && sub == 0;
+  if (owner) {
+    if (MODE == ADJ_VJP) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) hout[(long long)i * DIM + c] = (float)hd[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        const long long q = (long long)i * DIM + c;
+        const double X = xadj[q] + (double)cx * hd[c];
+        xadj[q] = X;
+        vout[q] = (float)((double)vin[q] + (double)cp * X);
+      }
+    }
+  }
+  if (want_params) {
+    __shared__ double s_p[4][256 / kWave];
+    const int nv = MODE == ADJ_VJP ? 4 : 2;
+    for (int k = 0; k < nv; ++k) {
+      double v = owner ? pd[k] : 0.0;
+      v = wave_reduce_sum(v);
+      if (lane_id() == 0) s_p[k][threadIdx.x / kWave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < nv; ++k) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_p[k][w];
+        if (MODE == ADJ_VJP) part[4 * tile + k] = k < 2 ? t : 0.5 * t;  // E = sum / 2
+        else part[2 * tile + k] += (double)ct * t;
+      }
+    }
+  }
+}
+
+// Fixed-order column sums of block partials (one workgroup):
+// sums[k] = sum_b part[ncol b + k]; then out_a[0..1] = sums[0..1],
+// out_b[0..1] = sums[2..3] (either may be null).
+__global__ __launch_bounds__(256) void k_sum_columns(const double* __restrict__ part, int nb,
+                                                     int ncol, double* __restrict__ out_a,
+                                                     double* __restrict__ out_b) {
+  __shared__ double s[256];
+  for (int k = 0; k < ncol; ++k) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) v += part[(long long)ncol * b + k];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      if (k < 2 && out_a) out_a[k] = s[0];
+      if (k >= 2 && out_b) out_b[k - 2] = s[0];
+    }
+    __syncthreads();
+  }
+}
+
+constexpr int kAdjTPA = 8;  // lanes per row (3-D); 2-D uses 4
+
+static int adj_tpa(int dim) { return dim == 3 ? kAdjTPA : 4; }
+
+static int adj_blocks(long long n, int dim) {
+  int nb = ceil_div(n * adj_tpa(dim), 256);
+  return (nb + 7) & ~7;
+}
+
+struct AdjLaunch {
+  dim3 grid;
+  hipStream_t st;
+  const float* pos;
+  int n;
+  Space s;
+  PairArgs a;
+  const int* idx;
+  int width;
+  const float* vin;
+  float* hout;
+  double* part;
+  int want_params;
+  double* xadj;
+  float* vout;
+  float cx, cp, ct;
+};
+
+template <int DIM, int STRIDE, int KIND, int TPA, int MODE>
+static void launch_adj(const AdjLaunch& L) {
+  if (L.a.general)
+    hipLaunchKernelGGL((k_pair_adjoint<DIM, STRIDE, KIND, TPA, true, MODE>), L.grid, dim3(256), 0,
+                       L.st, L.pos, L.n, L.s, L.a, L.idx, L.width, L.vin, L.hout, L.part,
+                       L.want_params, L.xadj, L.vout, L.cx, L.cp, L.ct);
+  else
+    hipLaunchKernelGGL((k_pair_adjoint<DIM, STRIDE, KIND, TPA, false, MODE>), L.grid, dim3(256),
+                       0, L.st, L.pos, L.n, L.s, L.a, L.idx, L.width, L.vin, L.hout, L.part,
+                       L.want_params, L.xadj, L.vout, L.cx, L.cp, L.ct);
+}
+
+template <int MODE>
+static void dispatch_adj(int dim, int stride, const AdjLaunch& L) {
+  if (dim == 3) {
+    if (L.a.kind == JMD_LENNARD_JONES) {
+      if (stride == 4) launch_adj<3, 4, JMD_LENNARD_JONES, kAdjTPA, MODE>(L);
+      else launch_adj<3, 3, JMD_LENNARD_JONES, kAdjTPA, MODE>(L);
+    } else {
+      if (stride == 4) launch_adj<3, 4, JMD_SOFT_SPHERE, kAdjTPA, MODE>(L);
+      else launch_adj<3, 3, JMD_SOFT_SPHERE, kAdjTPA, MODE>(L);
+    }
+  } else {
+    if (L.a.kind == JMD_LENNARD_JONES) launch_adj<2, 2, JMD_LENNARD_JONES, 4, MODE>(L);
+    else launch_adj<2, 2, JMD_SOFT_SPHERE, 4, MODE>(L);
+  }
+}
+
+static int adj_check(const float* position, int64_t n, int32_t dim, int32_t position_stride,
+                     int32_t periodic, const float* side, const jmd_pair_potential* potential,
+                     const int32_t* idx, int64_t idx_width, PairArgs* a) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || idx_width < 0 ||
+      idx_width > 0x7fffffff || (n > 0 && !position) || (idx_width > 0 && n > 0 && !idx) ||
+      (periodic && !side))
+    return JMD_ERR_INVALID;
+  if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  if ((long long)n * idx_width > 0x7fffffffffffLL) return JMD_ERR_INVALID;
+  if (!make_pair_args(potential, a)) return JMD_ERR_INVALID;
+  return JMD_OK;
+}
+
+}  // namespace jmd
+
+using namespace jmd;
+
+extern "C" {
+
+size_t jmd_adjoint_workspace_bytes(int64_t n, int32_t dim) {
+  if (n < 0 || (dim != 2 && dim != 3)) return 0;
+  return (size_t)adj_blocks(n, dim) * 4 * sizeof(double) + 256;
+}
+
+int jmd_pair_force_vjp(const float* position, int64_t n, int32_t dim, int32_t position_stride,
+                       int32_t periodic, const float* side, const jmd_pair_potential* potential,
+                       const int32_t* idx, int64_t idx_width, int32_t format,
+                       const float* cotangent, float* position_cotangent,
+                       double* param_cotangent, double* energy_param_grad, void* workspace,
+                       void* stream) {
+  PairArgs a;
+  int rc = adj_check(position, n, dim, position_stride, periodic, side, potential, idx,
+                     idx_width, &a);
+  if (rc) return rc;
+  if (format != JMD_DENSE) return JMD_ERR_UNSUPPORTED;
+  const bool params = param_cotangent != nullptr || energy_param_grad != nullptr;
+  if (params && a.general) return JMD_ERR_UNSUPPORTED;  // scalar sigma / epsilon only
+  if (n > 0 && !cotangent) return JMD_ERR_INVALID;
+  if (params && !workspace) return JMD_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = adj_blocks(n, dim);
+  double* part = (double*)workspace;
+  if (n == 0) {
+    double* outs[2] = {param_cotangent, energy_param_grad};
+    for (double* o : outs)
+      if (o) {
+        rc = launch_fill<double>(o, 2, 0.0, st);
+        if (rc) return rc;
+      }
+    return JMD_OK;
+  }
+  float* hout = position_cotangent;
+  if (!hout) return JMD_ERR_INVALID;  // the product is the kernel's main output
+  Space s = make_space(dim, periodic, side);
+  AdjLaunch L{dim3(nb), st, position, (int)n, s, a, idx, (int)idx_width, cotangent, hout,
+              part, params ? 1 : 0, nullptr, nullptr, 0.0f, 0.0f, 0.0f};
+  dispatch_adj<ADJ_VJP>(dim, position_stride, L);
+  JMD_LAUNCH_CHECK();
+  if (params) {
+    hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(256), 0, st, part, nb, 4, param_cotangent,
+                       energy_param_grad);
+    JMD_LAUNCH_CHECK();
+  }
+  return JMD_OK;
+}
+
+size_t jmd_adjoint_param_acc_doubles(int64_t n, int32_t dim) {
+  if (n < 0 || (dim != 2 && dim != 3)) return 0;
+  return (size_t)adj_blocks(n, dim) * 2;
+}
+
+int jmd_nve_adjoint_step(const float* position, int64_t n, int32_t dim, int32_t position_stride,
+                         int32_t periodic, const float* side,
+                         const jmd_pair_potential* potential, const int32_t* idx,
+                         int64_t idx_width, const float* ph_in, float* ph_out,
+                         double* x_adjoint, float coef_x, float coef_p, double* param_acc,
+                         float coef_param, void* stream) {
+  PairArgs a;
+  int rc = adj_check(position, n, dim, position_stride, periodic, side, potential, idx,
+                     idx_width, &a);
+  if (rc) return rc;
+  if (n == 0) return JMD_OK;
+  if (!ph_in || !ph_out || !x_adjoint || ph_in == ph_out) return JMD_ERR_INVALID;
+  if (param_acc && a.general) return JMD_ERR_UNSUPPORTED;
+  hipStream_t st = (hipStream_t)stream;
+  Space s = make_space(dim, periodic, side);
+  AdjLaunch L{dim3(adj_blocks(n, dim)), st, position, (int)n, s, a, idx, (int)idx_width, ph_in,
+              nullptr, param_acc, param_acc ? 1 : 0, x_adjoint, ph_out, coef_x, coef_p,
+              coef_param};
+  dispatch_adj<ADJ_STEP>(dim, position_stride, L);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+int jmd_adjoint_param_sum(const double* param_acc, int64_t n, int32_t dim, double* out,
+                          void* stream) {
+  if (n < 0 || (dim != 2 && dim != 3) || !param_acc || !out) return JMD_ERR_INVALID;
+  hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(256), 0, (hipStream_t)stream, param_acc,
+                     adj_blocks(n, dim), 2, out, nullptr);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+}  // extern "C"

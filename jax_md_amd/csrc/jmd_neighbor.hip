@@ -326,7 +326,10 @@ __global__ __launch_bounds__(256) void k_cell_emplace(
 enum { MODE_DENSE = 0, MODE_SPARSE_COUNT = 1, MODE_SPARSE_WRITE = 2, MODE_COUNT_MAX = 3 };
 enum { TEST_IJ = 0, TEST_JI = 1, TEST_BOTH = 2 };
 
-constexpr int kStage = 2048;   // candidates staged in LDS per pass
+#ifndef JMD_NL_STAGE
+#define JMD_NL_STAGE 2048
+#endif
+constexpr int kStage = JMD_NL_STAGE;  // candidates staged in LDS per pass
 constexpr int kHomeGroup = 512;
 
 template <int DIM>

@@ -16,104 +16,9 @@
 //   S = (rc^2 - r^2)^2 (rc^2 + 2 r^2 - 3 ro^2) / (rc^2 - ro^2)^3,
 //   S'/r = 12 (rc^2 - r^2)(ro^2 - r^2) / (rc^2 - ro^2)^3   on [r_on, r_c),
 //   F_i += (S V'/r + S'/r V) (R_j - R_i).
-#include "jmd_common.hpp"
+#include "jmd_pair_common.hpp"
 
 namespace jmd {
-
-struct PairArgs {
-  int kind;
-  int general;          // any non-scalar parameter
-  int sig_mode, eps_mode, alp_mode;
-  float sig, eps, alp;
-  const float* sig_v;
-  const float* eps_v;
-  const float* alp_v;
-  const int* species;
-  int ns;
-  float ro2, rc2, inv_d;  // LJ smoothing: r_on^2, r_c^2, 1/(rc^2-ro^2)^3
-  int smooth;             // r_on < r_c
-  float rc;               // interaction range (LJ r_c, soft sphere max sigma)
-};
-
-__device__ __forceinline__ float pick(int mode, float scalar, const float* v, int i, int j,
-                                      int si, int sj, int ns) {
-  if (mode == JMD_PARAM_SCALAR) return scalar;
-  if (mode == JMD_PARAM_PER_PARTICLE) return 0.5f * (v[i] + v[j]);
-  return v[si * ns + sj];
-}
-
-// 1/x from v_rcp_f32 plus one Newton step (~0.5 ulp, no IEEE division).
-__device__ __forceinline__ float fast_rcp(float x) {
-#pragma clang fp contract(off)
-  float r = __builtin_amdgcn_rcpf(x);
-  float e = fmaf(-x, r, 1.0f);
-  return fmaf(r, e, r);
-}
-
-// 1/sqrt(x) from v_rsq_f32 plus one Newton step.
-__device__ __forceinline__ float fast_rsqrt(float x) {
-#pragma clang fp contract(off)
-  float y = __builtin_amdgcn_rsqf(x);
-  float h = 0.5f * x * y;
-  float e = fmaf(-h, y, 0.5f);
-  return fmaf(y, e, y);
-}
-
-// U(r) and dU/dr / r for one pair from r^2, without branches; callers mask
-// invalid lanes with a select.  Multiply-adds are explicit (contraction off)
-// so every kernel variant rounds alike.
-template <int KIND>
-__device__ __forceinline__ float pair_eval(float r2, float sig, float eps, float alp,
-                                           const PairArgs& a, float* dEr) {
-#pragma clang fp contract(off)
-  if (KIND == JMD_LENNARD_JONES) {
-    float inv_r2 = fast_rcp(r2);
-    float s2 = sig * sig * inv_r2;
-    float s6 = s2 * s2 * s2;
-    float s12 = s6 * s6;
-    float V = 4.0f * eps * (s12 - s6);
-    float dV = -24.0f * eps * fmaf(2.0f, s12, -s6) * inv_r2;
-    float E = V, dE = dV;
-    if (a.smooth) {
-      float u = a.rc2 - r2;
-      float S = u * u * fmaf(2.0f, r2, fmaf(-3.0f, a.ro2, a.rc2)) * a.inv_d;
-      float dS = 12.0f * u * (a.ro2 - r2) * a.inv_d;
-      bool in_s = r2 >= a.ro2;
-      dE = in_s ? fmaf(S, dV, dS * V) : dV;
-      E = in_s ? S * V : V;
-    }
-    bool valid = r2 < a.rc2;
-    *dEr = valid ? dE : 0.0f;
-    return valid ? E : 0.0f;
-  } else {
-    // soft sphere: eps/alpha (1 - r/sigma)^alpha for r < sigma
-    float inv_r = fast_rsqrt(r2);
-    float r = r2 * inv_r;
-    float inv_sig = fast_rcp(sig);
-    float base = 1.0f - r * inv_sig;
-    float up, um;  // base^alpha, base^(alpha-1)
-    if (alp == 2.0f) { um = base; up = base * base; }
-    else { um = powf(fmaxf(base, 0.0f), alp - 1.0f); up = um * base; }
-    bool valid = (r2 < sig * sig) && (r2 > 0.0f);
-    *dEr = valid ? -eps * inv_sig * um * inv_r : 0.0f;
-    return valid ? eps / alp * up : 0.0f;
-  }
-}
-
-template <int TPA, typename T>
-__device__ __forceinline__ T group_sum(T v) {
-#pragma unroll
-  for (int off = TPA / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
-}
-
-// XCD-aware block order: consecutive logical tiles go to the same XCD so each
-// XCD's L2 holds a compact slab of the position table.  The grid is padded to
-// a multiple of 8 by the caller.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  int per = nb >> 3;
-  return (b & 7) * per + (b >> 3);
-}
 
 // Sum over one Dense row slice (entries sub, sub+TPA, ...) into f, e.
 // INTERIOR: every particle of the wavefront lies at least the interaction
@@ -135,21 +40,35 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
 #define JMD_PAIR_UNROLL 4
 #endif
   constexpr int UNR = JMD_PAIR_UNROLL;
+  if (width <= 0) return;
+  // Entries are read UNR at a time, and the next block's entries are loaded
+  // before this block's pair terms are computed (software pipeline), so a
+  // block costs one dependent memory round trip (the position gather).
+  // Loads are clamped to the row (always in bounds); entries past the row
+  // end count as padding.
+  unsigned nxt[UNR];
+#pragma unroll
+  for (int u = 0; u < UNR; ++u) nxt[u] = (unsigned)row[min(sub + u * TPA, width - 1)];
   for (int k0 = sub; k0 < width; k0 += UNR * TPA) {
     unsigned jj[UNR];
+    unsigned jmin = n;
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-      int k = k0 + u * TPA;
-      unsigned v = (unsigned)row[min(k, width - 1)];  // clamped: no branch
-      jj[u] = k < width ? v : n;
+      jj[u] = k0 + u * TPA < width ? nxt[u] : n;
+      jmin = min(jmin, jj[u]);
     }
-    if (jj[0] >= n) break;  // rows are left-compacted, padding is n
+    // rows are left-compacted with padding n: the block is past the row end
+    // iff its first entry is padding, iff all its entries are
+    if (jmin >= n) break;
     float xj[UNR][3];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       xj[u][0] = xj[u][1] = xj[u][2] = 0.0f;
       load_pos<DIM, STRIDE>(pos, jj[u] < n ? jj[u] : i, xj[u]);
     }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      nxt[u] = (unsigned)row[min(k0 + (UNR + u) * TPA, width - 1)];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       float d[3] = {0.0f, 0.0f, 0.0f};
@@ -313,53 +232,6 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = s[0];
-}
-
-static bool make_pair_args(const jmd_pair_potential* p, PairArgs* a) {
-  if (!p) return false;
-  if (p->kind != JMD_LENNARD_JONES && p->kind != JMD_SOFT_SPHERE) return false;
-  a->kind = p->kind;
-  a->sig_mode = p->sigma.mode;
-  a->eps_mode = p->epsilon.mode;
-  a->alp_mode = p->kind == JMD_SOFT_SPHERE ? p->alpha.mode : JMD_PARAM_SCALAR;
-  for (int m : {a->sig_mode, a->eps_mode, a->alp_mode})
-    if (m < JMD_PARAM_SCALAR || m > JMD_PARAM_SPECIES) return false;
-  a->sig = p->sigma.value;
-  a->eps = p->epsilon.value;
-  a->alp = p->alpha.value;
-  a->sig_v = p->sigma.values;
-  a->eps_v = p->epsilon.values;
-  a->alp_v = p->alpha.values;
-  a->species = p->species;
-  a->ns = p->n_species;
-  bool need_species = false;
-  const jmd_param* ps[3] = {&p->sigma, &p->epsilon, &p->alpha};
-  for (int q = 0; q < 3; ++q) {
-    if (q == 2 && p->kind != JMD_SOFT_SPHERE) continue;
-    if (ps[q]->mode != JMD_PARAM_SCALAR && !ps[q]->values) return false;
-    if (ps[q]->mode == JMD_PARAM_SPECIES) need_species = true;
-  }
-  if (need_species && (!p->species || p->n_species < 1)) return false;
-  a->general = (a->sig_mode | a->eps_mode | a->alp_mode) != 0;
-  float ro = p->r_onset, rc = p->r_cutoff;
-  a->rc2 = rc * rc;
-  a->ro2 = ro * ro;
-  a->rc = rc;
-  if (p->kind == JMD_SOFT_SPHERE) {
-    // Interaction range (used only for the wrap-free interior test): a scalar
-    // sigma, else the caller's r_cutoff bound on sigma (0: unknown, never
-    // interior).
-    ro = rc = 0.0f;
-    a->rc = p->sigma.mode == JMD_PARAM_SCALAR ? p->sigma.value
-                                               : (p->r_cutoff > 0.0f ? p->r_cutoff : 3.0e38f);
-  }
-  a->smooth = p->kind == JMD_LENNARD_JONES && ro < rc;
-  a->inv_d = 0.0f;
-  if (a->smooth) {
-    double dd = (double)a->rc2 - (double)a->ro2;
-    a->inv_d = (float)(1.0 / (dd * dd * dd));
-  }
-  return true;
 }
 
 struct DenseLaunch {

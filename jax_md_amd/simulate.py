@@ -313,7 +313,10 @@ class NVELoop:
                                          nat.DENSE, width, st), 'jmd_remap_neighbor_ids')
     return state
 
-  def _run_steps(self, state: NVEState, nbrs, n_steps: int, events=None) -> NVEState:
+  def _run_steps(self, state: NVEState, nbrs, n_steps: int, events=None,
+                 hook=None) -> NVEState:
+    """The per-step kernel sequence; ``hook(k)``, when given, runs after step
+    k's (gated) list update, when the state holds x_k and a list valid for it."""
     lib = nat.load()
     plan = nbrs.plan
     spec = plan.spec
@@ -343,11 +346,13 @@ class NVELoop:
     width = nbrs.idx.shape[1]
     geo = self.energy.geometry
     esides = nat.f3(geo.side_values(dim))
-    for _ in range(n_steps):
+    for k in range(n_steps):
       nat.check(lib.jmd_neighbor_list_build(
           x, N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq), int(plan.use_cell_list),
           cps, cs, int(nbrs.cell_list_capacity or 0), fmt, int(spec.mask_self), idx,
           int(nbrs.max_occupancy), occ, err, ref, gate, ws, st), 'jmd_neighbor_list_build')
+      if hook is not None:
+        hook(k)
       nat.check(lib.jmd_nve_kick_drift(
           x_hi, p, F, mv, mass_s if mass_v is None else 0.0, N, dim, wraps, side,
           float(self.dt), float(self.half), None, x_hi, x, stride, p, ref, thr, gate, st),

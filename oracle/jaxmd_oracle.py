@@ -575,6 +575,7 @@ class PairModel:
   species: Optional[np.ndarray] = None
   r_onset: float = 2.0
   r_cutoff: float = 2.5
+  fixed_smoothing: Optional[tuple] = None
 
   def pair_params(self, i, j, name):
     p = np.asarray(getattr(self, name), F64)
@@ -588,6 +589,8 @@ class PairModel:
     return p[i, j]
 
   def smoothing(self):
+    if self.fixed_smoothing is not None:  # absolute (r_on, r_c), e.g. for d/dsigma
+      return F64(self.fixed_smoothing[0]), F64(self.fixed_smoothing[1])
     smax = F32(np.max(np.asarray(self.sigma, F32)))
     r_on = F64(F32(F32(self.r_onset) * smax))
     r_c = F64(F32(F32(self.r_cutoff) * smax))
@@ -678,6 +681,101 @@ def energy_force_brute(model, space, R):
     cutoff = float(np.max(np.asarray(model.sigma, F64)))
   i, j = all_pairs(space, R, cutoff)
   return pair_energy_force(model, space, R, i, j, 0.5)
+
+
+# ---------------------------------------------------------------------------
+# Second derivatives: the gradient of the force, which the reference takes
+# with jax.grad through smap.pair_neighbor_list (quantity.py:58-60).  Pinned by
+# central differences of pair_energy_force (tests/test_oracle.py).
+# ---------------------------------------------------------------------------
+
+
+def pair_d2(model, r2, sig, eps, alp=2.0):
+  """FP64 (g, g', dg/dsigma, dg/depsilon, dU/dsigma, dU/depsilon) of one pair
+  as functions of s = r^2, where g = U'(r)/r (the force is sum_j g d_ij) and
+  g' = dg/ds.  LJ: U = S(s) V(s) with the smoothing constants fixed."""
+  s = np.asarray(r2, F64)
+  sig = np.asarray(sig, F64)
+  eps = np.asarray(eps, F64)
+  if model.kind == 'lj':
+    r_on, r_c = model.smoothing()
+    q = sig * sig / s
+    q3 = q ** 3
+    q6 = q3 * q3
+    V = 4 * eps * (q6 - q3)
+    gV = -24 * eps * (2 * q6 - q3) / s
+    gVp = 48 * eps * (7 * q6 - 2 * q3) / s ** 2
+    V_s = 24 * eps * (2 * q6 - q3) / sig
+    gV_s = -144 * eps * (4 * q6 - q3) / (sig * s)
+    V_e = 4 * (q6 - q3)
+    gV_e = -24 * (2 * q6 - q3) / s
+    rc2, ro2 = r_c * r_c, r_on * r_on
+    if rc2 > ro2:
+      D = (rc2 - ro2) ** 3
+      inner = (s >= ro2) & (s < rc2)
+      u = rc2 - s
+      S = np.where(s < ro2, 1.0, np.where(s < rc2, u * u * (rc2 + 2 * s - 3 * ro2) / D, 0.0))
+      Ss = np.where(inner, 6 * u * (ro2 - s) / D, 0.0)
+      Sss = np.where(inner, 6 * (2 * s - ro2 - rc2) / D, 0.0)
+    else:
+      S = np.where(s < rc2, 1.0, 0.0)
+      Ss = Sss = np.zeros_like(s)
+    g = S * gV + 2 * Ss * V
+    gp = gVp * S + 2 * gV * Ss + 2 * V * Sss
+    gs = S * gV_s + 2 * Ss * V_s
+    ge = S * gV_e + 2 * Ss * V_e
+    us = S * V_s
+    ue = S * V_e
+    zero = ~(s < rc2)
+    return tuple(np.where(zero, 0.0, x) for x in (g, gp, gs, ge, us, ue))
+  alp = np.asarray(alp, F64)
+  r = np.sqrt(s)
+  b = 1 - r / sig
+  inside = (r < sig) & (s > 0)
+  with np.errstate(divide='ignore', invalid='ignore'):
+    bm1 = np.where(inside, np.abs(b) ** (alp - 1), 0.0)
+    bm2 = np.where(inside, np.abs(b) ** (alp - 2), 0.0)
+    g = -eps * bm1 / (sig * r)
+    gp = eps / (2 * sig * r) * ((alp - 1) * bm2 / (sig * r) + bm1 / r ** 2)
+    ge = -bm1 / (sig * r)
+    ue = bm1 * b / alp
+    us = eps * bm1 * r / sig ** 2
+    gs = -eps / (r * sig ** 2) * ((alp - 1) * bm2 * r / sig - bm1)
+  return tuple(np.where(inside, x, 0.0) for x in (g, gp, gs, ge, us, ue))
+
+
+def force_vjp(model, space, R, v, i, j):
+  """FP64 VJP of F = -dE/dR over the ordered pairs (i, j) of a list holding
+  both orders: (J v)_i = sum_j g w + 2 g' (d.w) d with d = disp(R_j, R_i),
+  w = v_j - v_i; v^T dF/dsigma, v^T dF/depsilon (scalar parameters); and
+  dE/dsigma, dE/depsilon.  Returns (Jv, (ts, te), (Es, Ee))."""
+  R = np.asarray(R, F64)
+  v = np.asarray(v, F64)
+  d = space.displacement(R[j], R[i]).astype(F64)
+  s = np.sum(d * d, -1)
+  sig = model.pair_params(i, j, 'sigma')
+  eps = model.pair_params(i, j, 'epsilon')
+  alp = model.pair_params(i, j, 'alpha') if model.kind != 'lj' else 2.0
+  g, gp, gs, ge, us, ue = pair_d2(model, s, sig, eps, alp)
+  w = v[j] - v[i]
+  dw = np.sum(d * w, -1)
+  h = g[:, None] * w + (2 * gp * dw)[:, None] * d
+  Jv = np.zeros_like(R)
+  np.add.at(Jv, i, h)
+  lam = np.sum(v[i] * d, -1)
+  return Jv, (float(np.sum(gs * lam)), float(np.sum(ge * lam))), \
+      (0.5 * float(np.sum(us)), 0.5 * float(np.sum(ue)))
+
+
+def nve_energy_after(model, space, R, P, dt, n_steps):
+  """E(x_T) after n_steps of FP64 velocity Verlet with brute-force forces:
+  the function whose finite differences pin the trajectory adjoint."""
+  def force(X):
+    return energy_force_brute(model, space, X)[2]
+  st = NVEState(np.asarray(R, F64), np.asarray(P, F64), force(R))
+  for _ in range(n_steps):
+    st = nve_step(st, force, space, dt)
+  return energy_force_brute(model, space, st.position)[0]
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,155 @@
+"""Adjoint kernels on the GPU against the FP64 oracle and finite differences.
+
+Bars: the force VJP (J v and the scalar-parameter cotangents) within 1e-5
+norm-wise of the FP64 second-derivative oracle on the same pairs; the
+checkpointed NVE trajectory gradient within 1e-4 of central finite
+differences (h = 1e-4, FP64 oracle trajectory with brute-force forces).
+"""
+
+import dataclasses
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import jaxmd_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = 'cuda:0'
+F32 = np.float32
+
+
+def _jmd():
+  import jax_md_amd
+  return jax_md_amd
+
+
+def _lj_system(n_side, seed, jitter=0.05):
+  jmd = _jmd()
+  R, P, L = O.lj_lattice(n_side, seed=seed, jitter=jitter)
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense)
+  return jmd, R, P, L, shift, nfn, efn
+
+
+@pytest.mark.parametrize('n_side', [8, 12])
+def test_lj_force_vjp_matches_oracle(n_side):
+  print(f'case LJ force vjp n_side={n_side}', flush=True)
+  jmd, R, _, L, _, nfn, efn = _lj_system(n_side, seed=11)
+  Rt = torch.as_tensor(R, device=DEV)
+  nb = nfn.allocate(Rt)
+  v = np.random.default_rng(3).normal(size=R.shape).astype(F32)
+  Jv, pc, eg = jmd.adjoint.force_vjp(efn, Rt, nb, torch.as_tensor(v, device=DEV))
+  i, j = O.pairs_from_dense(nb.idx.cpu().numpy())
+  model = O.PairModel('lj', fixed_smoothing=(2.0, 2.5))
+  want, (ts, te), (es, ee) = O.force_vjp(model, O.Space(np.float64(L)), R, v, i, j)
+  got = Jv.cpu().numpy().astype(np.float64)
+  rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+  print(f'  |Jv| rel err {rel:.2e}')
+  assert rel <= 1e-5
+  pc = pc.cpu().numpy()
+  eg = eg.cpu().numpy()
+  scale_t = np.sqrt(np.sum(v.astype(np.float64) ** 2)) * np.sqrt(len(i))
+  assert abs(pc[0] - ts) <= 1e-5 * (abs(ts) + scale_t)
+  assert abs(pc[1] - te) <= 1e-5 * (abs(te) + scale_t)
+  assert abs(eg[0] - es) <= 1e-5 * abs(es) + 1e-6
+  assert abs(eg[1] - ee) <= 1e-5 * abs(ee) + 1e-6
+
+
+def test_soft_sphere_force_vjp_2d():
+  """Soft spheres (alpha = 2) in 2-D: J v against the oracle; the parameter
+  cotangents are refused for per-particle diameters and given for scalars."""
+  jmd = _jmd()
+  rng = np.random.default_rng(8)
+  L = F32(14.0)
+  R = (L * rng.uniform(size=(300, 2))).astype(F32)
+  disp, _ = jmd.space.periodic(L)
+  v = rng.normal(size=R.shape).astype(F32)
+  sp = O.Space(np.float64(L))
+  for sigma in (F32(1.2), (1.0 + 0.4 * (rng.uniform(size=300) < 0.5)).astype(F32)):
+    scalar = np.ndim(sigma) == 0
+    print(f'case SS vjp scalar_sigma={scalar}', flush=True)
+    sig_t = float(sigma) if scalar else torch.as_tensor(sigma, device=DEV)
+    nfn, efn = jmd.energy.soft_sphere_neighbor_list(disp, L, sigma=sig_t,
+                                                      format=jmd.partition.Dense)
+    Rt = torch.as_tensor(R, device=DEV)
+    nb = nfn.allocate(Rt)
+    i, j = O.pairs_from_dense(nb.idx.cpu().numpy())
+    model = O.PairModel('soft_sphere', sigma=np.float64(sigma) if scalar
+                        else sigma.astype(np.float64))
+    want, (ts, te), _ = O.force_vjp(model, sp, R, v, i, j)
+    if scalar:
+      Jv, pc, _ = jmd.adjoint.force_vjp(efn, Rt, nb, torch.as_tensor(v, device=DEV))
+      pc = pc.cpu().numpy()
+      scale = np.sqrt(np.sum(v.astype(np.float64) ** 2)) * np.sqrt(len(i))
+      assert abs(pc[0] - ts) <= 1e-5 * (abs(ts) + scale)
+      assert abs(pc[1] - te) <= 1e-5 * (abs(te) + scale)
+    else:
+      Jv, _, _ = jmd.adjoint.force_vjp(efn, Rt, nb, torch.as_tensor(v, device=DEV),
+                                       params=False)
+      with pytest.raises(Exception):
+        jmd.adjoint.force_vjp(efn, Rt, nb, torch.as_tensor(v, device=DEV), params=True)
+    got = Jv.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(got - want) <= 1e-5 * np.linalg.norm(want)
+
+
+def test_force_function_autograd():
+  """ForceFunction (the custom-VJP force): backward of <g, F> gives J g and
+  the parameter cotangents of the kernel."""
+  jmd, R, _, L, _, nfn, efn = _lj_system(8, seed=12)
+  Rt = torch.as_tensor(R, device=DEV).requires_grad_(True)
+  nb = nfn.allocate(Rt.detach())
+  sig = torch.tensor(1.0, device=DEV, requires_grad=True)
+  eps = torch.tensor(1.0, device=DEV, requires_grad=True)
+  force = jmd.adjoint.differentiable_force(efn)
+  F = force(Rt, nb, sig, eps)
+  g = torch.as_tensor(np.random.default_rng(4).normal(size=R.shape).astype(F32), device=DEV)
+  (F * g).sum().backward()
+  Jv, pc, _ = jmd.adjoint.force_vjp(efn, Rt.detach(), nb, g)
+  assert torch.equal(Rt.grad, Jv)
+  assert abs(sig.grad.item() - float(pc[0])) <= 1e-6 * abs(float(pc[0])) + 1e-6
+  assert abs(eps.grad.item() - float(pc[1])) <= 1e-6 * abs(float(pc[1])) + 1e-6
+
+
+@pytest.mark.parametrize('n_steps,every', [(30, 10), (37, 8)])
+def test_nve_trajectory_gradient_vs_finite_differences(n_steps, every):
+  """d E(x_T) / d(x_0, p_0, sigma, epsilon) against central differences of the
+  FP64 oracle trajectory (h = 1e-4 along random unit directions)."""
+  print(f'case trajectory gradient T={n_steps} K={every}', flush=True)
+  jmd, R, P, L, shift, nfn, efn = _lj_system(6, seed=13)
+  Rt = torch.as_tensor(R, device=DEV)
+  Pt = torch.as_tensor(P, device=DEV)
+  dt = 0.005
+  res = jmd.adjoint.nve_energy_gradient(efn, shift, nfn, Rt, Pt, dt, n_steps,
+                                        checkpoint_every=every)
+  sp = O.Space(np.float64(L))
+  model = O.PairModel('lj', fixed_smoothing=(2.0, 2.5))
+  R64, P64 = R.astype(np.float64), P.astype(np.float64)
+  E64 = O.nve_energy_after(model, sp, R64, P64, dt, n_steps)
+  assert abs(res.energy - E64) <= 1e-5 * abs(E64)
+  h = 1e-4
+  rng = np.random.default_rng(21)
+  for name in ('position', 'momentum'):
+    u = rng.normal(size=R.shape)
+    u /= np.linalg.norm(u)
+    if name == 'position':
+      ep = O.nve_energy_after(model, sp, R64 + h * u, P64, dt, n_steps)
+      em = O.nve_energy_after(model, sp, R64 - h * u, P64, dt, n_steps)
+    else:
+      ep = O.nve_energy_after(model, sp, R64, P64 + h * u, dt, n_steps)
+      em = O.nve_energy_after(model, sp, R64, P64 - h * u, dt, n_steps)
+    fd = (ep - em) / (2 * h)
+    g = getattr(res, name).cpu().numpy().astype(np.float64)
+    ad = float(np.sum(g * u))
+    scale = np.linalg.norm(g)
+    print(f'  d/d{name}: adjoint {ad:.6e} fd {fd:.6e} |grad| {scale:.3e}')
+    assert abs(ad - fd) <= 1e-4 * max(abs(fd), scale)
+  for name in ('sigma', 'epsilon'):
+    mp = dataclasses.replace(model, **{name: 1.0 + h})
+    mm = dataclasses.replace(model, **{name: 1.0 - h})
+    fd = (O.nve_energy_after(mp, sp, R64, P64, dt, n_steps)
+          - O.nve_energy_after(mm, sp, R64, P64, dt, n_steps)) / (2 * h)
+    ad = getattr(res, name)
+    print(f'  d/d{name}: adjoint {ad:.6e} fd {fd:.6e}')
+    assert abs(ad - fd) <= 1e-4 * abs(fd)

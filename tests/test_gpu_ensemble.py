@@ -1,0 +1,115 @@
+"""Ragged FIRE ensembles (one launch) against the plain per-system path.
+
+Bars: per system, the allocated capacities and the Dense list equal the
+plain ``neighbor_fn.allocate`` (and the oracle) bit for bit; forces at
+allocation equal the plain Dense kernel's bit for bit (same lane grouping);
+after FIRE steps the FIRE scalars match the plain path and positions agree
+to 1e-5 (the float64 FIRE reductions are summed in a different order);
+final energies match the FP64 oracle at the final positions to 1e-6.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import jaxmd_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = 'cuda:0'
+F32 = np.float32
+SIGMA = np.array([[1.0, 1.2], [1.2, 1.4]], F32)
+
+
+def _jmd():
+  import jax_md_amd
+  return jax_md_amd
+
+
+def _systems(sizes, seed=0):
+  out = []
+  for k, n in enumerate(sizes):
+    R, sp, L = O.bidisperse_2d(n, seed=seed + k)
+    out.append((R, sp, L))
+  return out
+
+
+def _plain(jmd, R, sp, L, **fire):
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.soft_sphere_neighbor_list(
+      disp, L, species=torch.as_tensor(sp, device=DEV), sigma=torch.as_tensor(SIGMA, device=DEV),
+      format=jmd.partition.Dense)
+  init_fn, apply_fn = jmd.minimize.fire_descent(efn, shift, **fire)
+  return nfn, efn, init_fn, apply_fn
+
+
+FIRE = dict(dt_start=0.01, dt_max=0.1)
+
+
+def test_ensemble_allocate_matches_plain_lists_and_forces():
+  jmd = _jmd()
+  sizes = [64, 300, 97, 1024, 150]
+  syst = _systems(sizes, seed=3)
+  print('case ensemble allocate', flush=True)
+  ens = jmd.ensemble.SoftSphereFireEnsemble([s[0] for s in syst], [s[2] for s in syst],
+                                            [s[1] for s in syst], sigma=SIGMA, **FIRE)
+  ens.allocate()
+  assert int(ens.did_buffer_overflow.max()) == 0
+  for s, (R, sp, L) in enumerate(syst):
+    nfn, efn, init_fn, _ = _plain(jmd, R, sp, L, **FIRE)
+    Rt = torch.as_tensor(R, device=DEV)
+    nb = nfn.allocate(Rt)
+    assert ens.capacities(s) == (nb.cell_list_capacity, nb.max_occupancy), s
+    assert torch.equal(ens.system_neighbors(s), nb.idx), s
+    st = init_fn(Rt, neighbor=nb)
+    assert torch.equal(ens.force[ens.system_slice(s)], st.force), s
+    # oracle list
+    spec = O.NeighborListSpec(O.Space(np.float32(L)), L, F32(1.4), 0.2, 1.25, False, True,
+                              O.DENSE)
+    o = O.neighbor_list_build(spec, R)
+    assert np.array_equal(nb.idx.cpu().numpy(), o.idx), s
+
+
+def test_ensemble_fire_matches_plain_fire():
+  jmd = _jmd()
+  sizes = [128, 257, 64, 512]
+  syst = _systems(sizes, seed=11)
+  n_steps = 25
+  print('case ensemble fire', flush=True)
+  ens = jmd.ensemble.SoftSphereFireEnsemble([s[0] for s in syst], [s[2] for s in syst],
+                                            [s[1] for s in syst], sigma=SIGMA, **FIRE)
+  ens.allocate().run(n_steps)
+  assert int(ens.did_buffer_overflow.max()) == 0
+  E = ens.energy.cpu().numpy()
+  for s, (R, sp, L) in enumerate(syst):
+    nfn, efn, init_fn, apply_fn = _plain(jmd, R, sp, L, **FIRE)
+    Rt = torch.as_tensor(R, device=DEV)
+    nb = nfn.allocate(Rt)
+    st = init_fn(Rt, neighbor=nb)
+    for _ in range(n_steps):
+      nb = nb.update(st.position)
+      st = apply_fn(st, neighbor=nb)
+    sl = ens.system_slice(s)
+    dx = (ens.position_hi[sl] - st.position_hi).abs().max().item()
+    assert dx <= 1e-5, (s, dx)
+    assert float(ens.fire_state[s, 0]) == float(st.fire_scalars[0]), s
+    assert float(ens.fire_state[s, 1]) == float(st.fire_scalars[1]), s
+    assert int(ens.fire_n_pos[s]) == int(st.fire_n_pos[0]), s
+    # energy against FP64 at the ensemble's final positions
+    model = O.PairModel('soft_sphere', sigma=SIGMA.astype(np.float64), species=sp)
+    E64 = O.energy_force_brute(model, O.Space(np.float64(L)),
+                               ens.position[sl].cpu().numpy().astype(np.float64))[0]
+    assert abs(E[s] - E64) <= 1e-6 * abs(E64) + 1e-9, (s, E[s], E64)
+
+
+def test_ensemble_rebuilds_and_overflow_flag():
+  """Systems rebuild their lists independently (build counts differ), and a
+  forced small width raises the overflow bit of the affected systems only."""
+  jmd = _jmd()
+  syst = _systems([200, 90, 400], seed=21)
+  ens = jmd.ensemble.SoftSphereFireEnsemble([s[0] for s in syst], [s[2] for s in syst],
+                                            [s[1] for s in syst], sigma=SIGMA, **FIRE)
+  ens.allocate().run(60)
+  b = ens.builds.cpu().numpy()
+  assert np.all(b >= 2), b
+  assert int(ens.did_buffer_overflow.max()) == 0

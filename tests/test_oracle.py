@@ -390,3 +390,65 @@ def test_fire_descends_quadratic():
     E_new = energy(st.position)
     assert E_new < E
     E = E_new
+
+
+# ---------------------------------------------------------------------------
+# second derivatives (the adjoint's oracle), pinned by central differences
+# ---------------------------------------------------------------------------
+
+
+def _d2_system(kind, seed):
+  rng = np.random.default_rng(seed)
+  if kind == 'lj':
+    R, _, L = O.lj_lattice(4, seed=seed)
+    R = (R + rng.normal(0, 0.08, R.shape)).astype(np.float64) % L
+    model = O.PairModel('lj', sigma=1.0, epsilon=1.0, fixed_smoothing=(2.0, 2.5))
+    cutoff = 2.5
+  else:
+    L = 6.0
+    R = rng.uniform(0, L, (40, 2))
+    model = O.PairModel('soft_sphere', sigma=1.3, epsilon=1.0, alpha=2.0)
+    cutoff = 1.3
+  return R, L, model, cutoff
+
+
+@pytest.mark.parametrize('kind', ['lj', 'soft_sphere'])
+def test_force_vjp_matches_central_differences(kind):
+  R, L, model, cutoff = _d2_system(kind, 3)
+  sp = O.Space(np.float64(L))
+  rng = np.random.default_rng(5)
+  v = rng.normal(size=R.shape)
+  i, j = O.all_pairs(sp, R, cutoff + 0.3)
+  Jv, (ts, te), (es, ee) = O.force_vjp(model, sp, R, v, i, j)
+  h = 1e-5
+
+  def F(X, m=model):
+    ii, jj = O.all_pairs(sp, X, cutoff + 0.3)
+    return O.pair_energy_force(m, sp, X, ii, jj, 0.5)
+
+  fd = (F(R + h * v)[2] - F(R - h * v)[2]) / (2 * h)
+  assert np.linalg.norm(Jv - fd) <= 1e-6 * np.linalg.norm(fd)
+  import dataclasses
+  for name, t, e in (('sigma', ts, es), ('epsilon', te, ee)):
+    base = float(getattr(model, name))
+    mp = dataclasses.replace(model, **{name: base + h})
+    mm = dataclasses.replace(model, **{name: base - h})
+    Ep, _, Fp = F(R, mp)
+    Em, _, Fm = F(R, mm)
+    dF = (Fp - Fm) / (2 * h)
+    assert abs(np.sum(v * dF) - t) <= 1e-6 * (abs(t) + np.linalg.norm(dF) * np.linalg.norm(v))
+    assert abs((Ep - Em) / (2 * h) - e) <= 1e-6 * (abs(e) + 1.0)
+
+
+def test_force_vjp_dropped_term_fails():
+  """The pin is sharp: dropping the 2 g' (d.w) d term breaks the match."""
+  R, L, model, cutoff = _d2_system('lj', 4)
+  sp = O.Space(np.float64(L))
+  v = np.random.default_rng(6).normal(size=R.shape)
+  i, j = O.all_pairs(sp, R, cutoff + 0.3)
+  Jv, _, _ = O.force_vjp(model, sp, R, v, i, j)
+  d = sp.displacement(R[j], R[i])
+  g = O.pair_d2(model, np.sum(d * d, -1), 1.0, 1.0)[0]
+  partial = np.zeros_like(R)
+  np.add.at(partial, i, g[:, None] * (v[j] - v[i]))
+  assert np.linalg.norm(partial - Jv) > 1e-2 * np.linalg.norm(Jv)
