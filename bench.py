@@ -12,9 +12,9 @@ replay, with a host overflow check per block; an overflowing block is redone
 from its saved start state after re-allocating the list (the reference's
 protocol, examples/nve_neighbor_list.py:186-201).
 
-The force kernel's duration is measured with HIP events around a graph of
-20 back-to-back force launches on the loop's own cell-ordered buffers (the
-state the timed region ended in), on the stream they run on.  HBM traffic per
+The force kernel's duration is measured with HIP events around each force
+launch of one more (eager) 100-step block after the timed region, on the
+stream the kernels run on, with the device kept behind the host.  HBM traffic per
 launch comes from the committed counter summary of the same kernel and
 workload (profiles/k_pair_dense_lj_n262144_pmc.json), when present.
 
@@ -38,34 +38,23 @@ def log(*a):
   print(*a, file=sys.stderr, flush=True)
 
 
-def force_kernel_us(jmd, loop, state, nbrs, efn, reps=20):
-  """Average duration of the force + half-kick kernel, from HIP events around
-  a graph of `reps` launches on the loop's cell-ordered buffers."""
+def force_kernel_us(jmd, loop, state, nbrs, efn, steps=100):
+  """Average duration of the force + half-kick kernel inside the step loop:
+  one eager block of `steps` steps with HIP events around each force launch,
+  on the stream the kernels run on.  A spin kernel queued first keeps the
+  device behind the host, so no event interval contains a host gap; the
+  kernel runs where it runs in the timed region (right after the drift, in
+  cell order)."""
   import torch
-  b = loop._bufs
-  if b is not None:
-    import dataclasses
-    x, idx = b['x'], b['idx']
-    nb = dataclasses.replace(nbrs, idx=idx)
-  else:
-    x, nb = state.position, nbrs
-  p = state.momentum.clone() if b is None else b['p'].clone()
-  half = float(loop.half)
-  efn.force_and_kick(x, nb, p, half_dt=half)  # warm
-  g = torch.cuda.CUDAGraph()
-  with torch.cuda.graph(g):
-    for _ in range(reps):
-      efn.force_and_kick(x, nb, p, half_dt=half)
-  s = torch.cuda.current_stream()
-  e0 = torch.cuda.Event(enable_timing=True)
-  e1 = torch.cuda.Event(enable_timing=True)
-  g.replay()
-  e0.record(s)
-  for _ in range(5):
-    g.replay()
-  e1.record(s)
+  ev = []
+  loop.prime(state, nbrs)
+  try:
+    torch.cuda._sleep(int(5e8))  # ~0.2 s of device time queued ahead
+  except Exception:
+    pass
+  loop.run(state, nbrs, steps, ev)
   torch.cuda.synchronize()
-  return e0.elapsed_time(e1) * 1e3 / (5 * reps)
+  return 1e3 * sum(a.elapsed_time(b) for a, b in ev) / len(ev)
 
 
 def counter_traffic(N):
@@ -320,6 +309,7 @@ def main():
   builds = nbrs.plan.builds - builds0
   E_end, F_gpu = efn.energy_and_force(state.position, nbrs)
   KE_end = jmd.quantity.kinetic_energy(momentum=state.momentum)
+  X_end = state.position.cpu().numpy().astype(np.float64)  # where F_gpu was taken
   value = N * args.steps / elapsed
   ms = 1e3 * elapsed / args.steps
   idx = nbrs.idx
@@ -338,7 +328,7 @@ def main():
   force_err = None
   try:
     from oracle import c_oracle
-    sysc = c_oracle.LJSystem(state.position.cpu().numpy().astype(np.float64), float(L))
+    sysc = c_oracle.LJSystem(X_end, float(L))
     E64, F64 = sysc.forces()
     Fg = F_gpu.cpu().numpy().astype(np.float64)
     rel = float(np.linalg.norm(Fg - F64) / np.linalg.norm(F64))

@@ -327,10 +327,14 @@ enum { MODE_DENSE = 0, MODE_SPARSE_COUNT = 1, MODE_SPARSE_WRITE = 2, MODE_COUNT_
 enum { TEST_IJ = 0, TEST_JI = 1, TEST_BOTH = 2 };
 
 #ifndef JMD_NL_STAGE
-#define JMD_NL_STAGE 2048
+#define JMD_NL_STAGE 1536
 #endif
 constexpr int kStage = JMD_NL_STAGE;  // candidates staged in LDS per pass
 constexpr int kHomeGroup = 512;
+#ifndef JMD_NL_HOME_PASS
+#define JMD_NL_HOME_PASS 4
+#endif
+constexpr int kHomePass = JMD_NL_HOME_PASS;  // home particles per candidate pass
 
 template <int DIM>
 __device__ __forceinline__ bool pair_test(const float* xi, const float* xj,
@@ -471,54 +475,81 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
                                  xp.z + s_shift[t][2], __int_as_float(A.w.sorted_id[gi]));
         }
         __syncthreads();
-        for (int a = wave; a < gcount; a += nwave) {
-          const int hi = hstart + g0 + a;
-          const float4 xp = A.w.sorted_pos[hi];
-          const int i = A.w.sorted_id[hi];
-          const float xi[3] = {xp.x, xp.y, xp.z};
-          int cnt = s_cnt[a];
-          long long base = 0;
-          if (MODE == MODE_DENSE) base = (long long)i * width;
-          if (MODE == MODE_SPARSE_WRITE) base = A.w.row_offset[i];
+        // Each wavefront takes kHomePass home particles at a time: one LDS
+        // read of 64 candidates serves all of them, and the loop overhead is
+        // shared.  Home particles are wavefront-uniform.
+        for (int a0 = wave; a0 < gcount; a0 += kHomePass * nwave) {
+          float xi[kHomePass][3];
+          int ih[kHomePass], cnt[kHomePass];
+          long long base[kHomePass];
+#pragma unroll
+          for (int h = 0; h < kHomePass; ++h) {
+            const int a = a0 + h * nwave;
+            xi[h][0] = xi[h][1] = xi[h][2] = 0.0f;
+            ih[h] = -1;
+            cnt[h] = 0;
+            base[h] = 0;
+            if (a < gcount) {
+              const int hi = hstart + g0 + a;
+              const float4 xp = A.w.sorted_pos[hi];
+              xi[h][0] = xp.x;
+              xi[h][1] = xp.y;
+              xi[h][2] = xp.z;
+              ih[h] = A.w.sorted_id[hi];
+              cnt[h] = s_cnt[a];
+              if (MODE == MODE_DENSE) base[h] = (long long)ih[h] * width;
+              if (MODE == MODE_SPARSE_WRITE) base[h] = A.w.row_offset[ih[h]];
+            }
+          }
           for (int q0 = 0; q0 < cpad; q0 += kWave) {
-            const int q = q0 + lane;
-            const float4 c = s_pos[q];
+            const float4 c = s_pos[q0 + lane];
             const int j = __float_as_int(c.w);
             const float cj[3] = {c.x, c.y, c.z};
-            float d[3] = {0.0f, 0.0f, 0.0f};
+            const bool jok = j < n;
 #pragma unroll
-            for (int e = 0; e < DIM; ++e)
-              d[e] = MINIMG ? -min_image_disp(cj[e], xi[e], xi[e] - s.L[e], s.L[e], s.half[e])
-                            : xi[e] - cj[e];
-            const float d2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
-                                      : fmaf(d[1], d[1], d[0] * d[0]);
-            const bool ok = (!A.mask_self || j != i) && (!A.ordered || j < i) && j < n;
-            bool pass = ok && d2 < lo_thr;
-            const bool near = ok && d2 >= lo_thr && d2 <= hi_thr;
-            if (__any(near)) {  // rare: reference-exact test on original positions
-              if (near) {
-                float xo[3] = {0.0f, 0.0f, 0.0f}, xj[3] = {0.0f, 0.0f, 0.0f};
-                load_pos<DIM, STRIDE>(A.pos, i, xo);
-                load_pos<DIM, STRIDE>(A.pos, j, xj);
-                pass = pair_test<DIM>(xo, xj, s, A.cutoff_sq, 0.0f, A.test);
-              }
-            }
-            const unsigned long long bal = __ballot(pass);
-            if ((MODE == MODE_DENSE || MODE == MODE_SPARSE_WRITE) && pass) {
-              const long long slot = cnt + wave_prefix_count(bal);
-              if (MODE == MODE_DENSE) {
-                if (slot < width) A.idx[base + slot] = j;
-              } else {
-                const long long at = base + slot;
-                if (at < width) {
-                  A.idx[at] = j;          // receiver
-                  A.idx[width + at] = i;  // sender
+            for (int h = 0; h < kHomePass; ++h) {
+              if (ih[h] < 0) continue;  // wavefront-uniform
+              const int i = ih[h];
+              float d[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+              for (int e = 0; e < DIM; ++e)
+                d[e] = MINIMG ? -min_image_disp(cj[e], xi[h][e], xi[h][e] - s.L[e], s.L[e],
+                                                s.half[e])
+                              : xi[h][e] - cj[e];
+              const float d2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                                        : fmaf(d[1], d[1], d[0] * d[0]);
+              const bool ok = jok && (!A.mask_self || j != i) && (!A.ordered || j < i);
+              bool pass = ok && d2 < lo_thr;
+              const bool near = ok && d2 >= lo_thr && d2 <= hi_thr;
+              if (__any(near)) {  // rare: reference-exact test on original positions
+                if (near) {
+                  float xo[3] = {0.0f, 0.0f, 0.0f}, xj[3] = {0.0f, 0.0f, 0.0f};
+                  load_pos<DIM, STRIDE>(A.pos, i, xo);
+                  load_pos<DIM, STRIDE>(A.pos, j, xj);
+                  pass = pair_test<DIM>(xo, xj, s, A.cutoff_sq, 0.0f, A.test);
                 }
               }
+              const unsigned long long bal = __ballot(pass);
+              if ((MODE == MODE_DENSE || MODE == MODE_SPARSE_WRITE) && pass) {
+                const long long slot = cnt[h] + wave_prefix_count(bal);
+                if (MODE == MODE_DENSE) {
+                  if (slot < width) A.idx[base[h] + slot] = j;
+                } else {
+                  const long long at = base[h] + slot;
+                  if (at < width) {
+                    A.idx[at] = j;          // receiver
+                    A.idx[width + at] = i;  // sender
+                  }
+                }
+              }
+              cnt[h] += __popcll(bal);
             }
-            cnt += __popcll(bal);
           }
-          if (lane == 0) s_cnt[a] = cnt;
+#pragma unroll
+          for (int h = 0; h < kHomePass; ++h) {
+            const int a = a0 + h * nwave;
+            if (a < gcount && lane == 0) s_cnt[a] = cnt[h];
+          }
         }
       }
       __syncthreads();
