@@ -26,50 +26,6 @@ namespace jmd {
 // difference x_j - x_i is already the minimum image and no wrap is needed
 // (pairs whose plain difference is not the minimum image are then beyond
 // range and contribute zero either way).
-// One pair term of row i accumulated into f (and e).  MASKED: the entry may
-// be padding (ok false): its term is dropped by a select.
-template <int DIM, int KIND, bool GENERAL, bool ENERGY, bool INTERIOR, bool MASKED>
-__device__ __forceinline__ void pair_accum(const float* xj, unsigned j, bool ok, const Space& s,
-                                           const PairArgs& a, unsigned i, int si,
-                                           const float* xi, const float* xim, float* f,
-                                           float* e) {
-#pragma clang fp contract(off)
-  float d[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-  for (int c = 0; c < DIM; ++c)
-    d[c] = (INTERIOR || !s.periodic) ? xj[c] - xi[c]
-                                     : min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c]);
-  const float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
-                            : fmaf(d[1], d[1], d[0] * d[0]);
-  float sig = a.sig, eps = a.eps, alp = a.alp;
-  if (GENERAL) {
-    const int sj = a.species ? a.species[j] : 0;
-    sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
-    eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
-    if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
-  }
-  float dEr;
-  const float uu = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr);
-  if (MASKED) dEr = ok ? dEr : 0.0f;
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) f[c] = fmaf(dEr, d[c], f[c]);
-  if (ENERGY) *e += (MASKED && !ok) ? 0.0f : uu;
-}
-
-// Sum over one Dense row slice (entries sub, sub+TPA, ...) into f, e.
-// INTERIOR: every particle of the wavefront lies at least the interaction
-// range from each box face, so for any pair within range the plain
-// difference x_j - x_i is already the minimum image and no wrap is needed
-// (pairs whose plain difference is not the minimum image are then beyond
-// range and contribute zero either way).
-//
-// Two phases.  Blocks of UNR entries per lane, loaded one block ahead of the
-// position gathers (software pipeline), run without padding masks while
-// every entry of the block is valid in every row of the wavefront.  From the
-// first block in which some row ends, the rest runs one entry per lane at a
-// time with masks, so a wavefront's rows cost their longest length rounded
-// to TPA entries, not to UNR * TPA.  Each lane's entries come in the same
-// order in both phases, so the sums equal a single masked loop's.
 template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, bool ENERGY, bool INTERIOR>
 __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned n,
                                         const Space& s, const PairArgs& a,
@@ -77,6 +33,7 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
                                         unsigned i, const float* xi, const float* xim,
                                         float* f, float* e) {
 #pragma clang fp contract(off)
+  float sig = a.sig, eps = a.eps, alp = a.alp;
   int si = 0;
   if (GENERAL && a.species) si = a.species[i];
 #ifndef JMD_PAIR_UNROLL
@@ -84,47 +41,59 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
 #endif
   constexpr int UNR = JMD_PAIR_UNROLL;
   if (width <= 0) return;
-  // loads are clamped to the row (always in bounds); entries past the row
-  // end count as padding
+  // Entries are read UNR at a time, and the next block's entries are loaded
+  // before this block's pair terms are computed (software pipeline), so a
+  // block costs one dependent memory round trip (the position gather).
+  // Loads are clamped to the row (always in bounds); entries past the row
+  // end count as padding.
   unsigned nxt[UNR];
 #pragma unroll
   for (int u = 0; u < UNR; ++u) nxt[u] = (unsigned)row[min(sub + u * TPA, width - 1)];
-  int k0 = sub;
-  for (; k0 < width; k0 += UNR * TPA) {
+  for (int k0 = sub; k0 < width; k0 += UNR * TPA) {
     unsigned jj[UNR];
-    unsigned jmax = 0;
+    unsigned jmin = n;
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       jj[u] = k0 + u * TPA < width ? nxt[u] : n;
-      jmax = max(jmax, jj[u]);
+      jmin = min(jmin, jj[u]);
     }
-    if (!__all(jmax < n)) break;  // some row of the wavefront ends here
+    // rows are left-compacted with padding n: the block is past the row end
+    // iff its first entry is padding, iff all its entries are
+    if (jmin >= n) break;
     float xj[UNR][3];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       xj[u][0] = xj[u][1] = xj[u][2] = 0.0f;
-      load_pos<DIM, STRIDE>(pos, jj[u], xj[u]);
+      load_pos<DIM, STRIDE>(pos, jj[u] < n ? jj[u] : i, xj[u]);
     }
 #pragma unroll
     for (int u = 0; u < UNR; ++u)
       nxt[u] = (unsigned)row[min(k0 + (UNR + u) * TPA, width - 1)];
 #pragma unroll
-    for (int u = 0; u < UNR; ++u)
-      pair_accum<DIM, KIND, GENERAL, ENERGY, INTERIOR, false>(xj[u], jj[u], true, s, a, i, si,
-                                                              xi, xim, f, e);
-  }
-  // tail, one entry per lane per step; nxt[0] holds the entry at k0
-  unsigned jn = k0 < width ? nxt[0] : n;
-  for (;;) {
-    const unsigned j = jn;
-    const bool ok = j < n;
-    if (!__any(ok)) break;
-    float xj[3] = {0.0f, 0.0f, 0.0f};
-    load_pos<DIM, STRIDE>(pos, ok ? j : i, xj);
-    k0 += TPA;
-    jn = k0 < width ? (unsigned)row[min(k0, width - 1)] : n;
-    pair_accum<DIM, KIND, GENERAL, ENERGY, INTERIOR, true>(xj, ok ? j : i, ok, s, a, i, si, xi,
-                                                           xim, f, e);
+    for (int u = 0; u < UNR; ++u) {
+      float d[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+        d[c] = (INTERIOR || !s.periodic)
+                   ? xj[u][c] - xi[c]
+                   : min_image_disp(xj[u][c], xi[c], xim[c], s.L[c], s.half[c]);
+      float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                          : fmaf(d[1], d[1], d[0] * d[0]);
+      if (GENERAL) {
+        unsigned j = jj[u] < n ? jj[u] : i;
+        int sj = a.species ? a.species[j] : 0;
+        sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
+        eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
+        if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
+      }
+      float dEr;
+      float uu = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr);
+      const bool ok = jj[u] < n;
+      dEr = ok ? dEr : 0.0f;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) f[c] = fmaf(dEr, d[c], f[c]);
+      if (ENERGY) *e += ok ? uu : 0.0f;
+    }
   }
 }
 
