@@ -276,7 +276,9 @@ int jmd_adjoint_param_sum(const double* param_acc, int64_t n, int32_t dim, doubl
  * concatenated arrays, its periodic box, the cell grid of its neighbour list
  * (cells_per_side = floor(side / cutoff), cell_size = side / cells_per_side,
  * partition.py:146-188), the grid its cell capacity is counted on
- * (partition.py:249-250 derives it a second time), and its capacities. */
+ * (partition.py:249-250 derives it a second time), its capacities and list
+ * radii, and where its species parameters start (a potential's species
+ * matrices may hold one block per system). */
 typedef struct {
   int32_t offset, n;
   int32_t cells_per_side[2];
@@ -285,6 +287,10 @@ typedef struct {
   float fill_cell_size[2];
   float side[2];
   int32_t cell_capacity, max_occupancy;
+  float cutoff_sq;      /* float32 (r_cutoff + dr_threshold)^2 of its list */
+  float threshold_sq;   /* float32 (dr_threshold / 2)^2 */
+  int32_t param_offset; /* offset of its species-parameter block (0: shared) */
+  int32_t reserved;
 } jmd_system2d;
 
 /* Largest cell count (either grid) a system may have. */
@@ -314,9 +320,35 @@ int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int3
                          float* position, float* momentum, float* force,
                          float* reference_position, int32_t* idx, int32_t width,
                          int32_t* scratch, const jmd_pair_potential* potential,
-                         float cutoff_sq, float threshold_sq, const float* fire_params,
-                         float* fire_state, int32_t* fire_n_pos, uint8_t* error,
-                         int32_t* out_stat, double* energy, int32_t* builds, void* stream);
+                         const float* fire_params, float* fire_state, int32_t* fire_n_pos,
+                         uint8_t* error, int32_t* out_stat, double* energy, int32_t* builds,
+                         float* tape_x, float* tape_force, float* tape_q, int32_t* tape_idx,
+                         float* tape_scalars, void* stream);
+
+/* Adjoint of jmd_fire_ensemble_2d's trajectory (no counterpart entry point in
+ * the reference, which would take jax.grad through each system's
+ * lax.fori_loop over fire_descent's apply_fn, minimize.py:124-226).  The
+ * loss is scale * sum_s E_s(x_T); theta[s] is its derivative with respect to
+ * system s's parameter theta_s, through which the species matrix sigma
+ * depends with coefficients dsigma (d sigma_ab / d theta_s, laid out like
+ * sigma).  FIRE's float16 dt / alpha / reset decisions are piecewise
+ * constant and carry no derivative.  Mode 3 of jmd_fire_ensemble_2d with
+ * tape_* set records a segment (per node k = 0..n_steps: float32 positions,
+ * forces [n+1][n_total][2], the lists [n+1][n_total][width]; per step:
+ * momenta before the mixing [n][n_total][2] and five scalars per system
+ * [n][n_systems][5]).  mode: 0 initialises x_adjoint = -scale F,
+ * p_adjoint = pending = 0, theta = scale dE/dtheta at (position, idx);
+ * 1 runs n_steps backward steps through a tape; 2 applies the pending
+ * vector at the tape's first node (call once, after the earliest segment). */
+int jmd_fire_ensemble_adjoint_2d(const jmd_system2d* systems, int32_t n_systems,
+                                 int64_t n_total, int32_t mode, int32_t n_steps,
+                                 const float* position, const float* force, const int32_t* idx,
+                                 int32_t width, const jmd_pair_potential* potential,
+                                 const float* dsigma, const float* tape_x,
+                                 const float* tape_force, const float* tape_q,
+                                 const int32_t* tape_idx, const float* tape_scalars,
+                                 double* x_adjoint, double* p_adjoint, float* pending,
+                                 float* scratch, double* theta, float scale, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Integrators                                                               */

@@ -88,8 +88,15 @@ _SIGNATURES = {
     'jmd_fire_ensemble_2d': (c_int32, [c_void_p, c_void_p, c_int32, c_int64, c_int32, c_int32,
                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_void_p, c_int32, c_void_p, POINTER(JmdPairPotential),
-                                       c_float, c_float, POINTER(c_float), c_void_p, c_void_p,
-                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+                                       POINTER(c_float), c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p]),
+    'jmd_fire_ensemble_adjoint_2d': (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_int32,
+                                               c_void_p, c_void_p, c_void_p, c_int32,
+                                               POINTER(JmdPairPotential), c_void_p, c_void_p,
+                                               c_void_p, c_void_p, c_void_p, c_void_p,
+                                               c_void_p, c_void_p, c_void_p, c_void_p,
+                                               c_void_p, c_float, c_void_p]),
     'jmd_fire_workspace_bytes': (c_size_t, [c_int64]),
     'jmd_fire_update': (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p,
                                   POINTER(c_float), c_void_p, c_void_p, c_void_p]),

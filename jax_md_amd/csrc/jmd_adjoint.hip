@@ -27,87 +27,6 @@
 
 namespace jmd {
 
-// Second-order pair terms from s = r^2 (float32, contraction off):
-//   g = U'(r)/r, gp = dg/ds, gs = dg/dsigma, ge = dg/depsilon,
-//   us = dU/dsigma, ue = dU/depsilon.
-struct PairD2 {
-  float g, gp, gs, ge, us, ue;
-};
-
-template <int KIND>
-__device__ __forceinline__ PairD2 pair_eval2(float s, float sig, float eps, float alp,
-                                             const PairArgs& a) {
-#pragma clang fp contract(off)
-  PairD2 o;
-  if (KIND == JMD_LENNARD_JONES) {
-    // V = 4 eps (q^6 - q^3), q = sigma^2 / s:
-    //   g_V = 2 dV/ds = -24 eps (2 q^6 - q^3) / s
-    //   dg_V/ds = 48 eps (7 q^6 - 2 q^3) / s^2
-    //   dV/dsigma = 24 eps (2 q^6 - q^3) / sigma,  dg_V/dsigma = -144 eps (4 q^6 - q^3) / (sigma s)
-    const float inv = fast_rcp(s);
-    const float inv_sig = fast_rcp(sig);
-    const float q = sig * sig * inv;
-    const float s6 = q * q * q;
-    const float s12 = s6 * s6;
-    const float c2 = fmaf(2.0f, s12, -s6);
-    const float V = 4.0f * eps * (s12 - s6);
-    const float gV = -24.0f * eps * c2 * inv;
-    const float gVp = 48.0f * eps * fmaf(7.0f, s12, -2.0f * s6) * inv * inv;
-    const float V_sig = 24.0f * eps * c2 * inv_sig;
-    const float gV_sig = -144.0f * eps * fmaf(4.0f, s12, -s6) * inv_sig * inv;
-    const float V_eps = 4.0f * (s12 - s6);
-    const float gV_eps = -24.0f * c2 * inv;
-    o.g = gV;
-    o.gp = gVp;
-    o.gs = gV_sig;
-    o.ge = gV_eps;
-    o.us = V_sig;
-    o.ue = V_eps;
-    if (a.smooth && s >= a.ro2) {
-      // U = V S(s), S = (rc^2 - s)^2 (rc^2 + 2 s - 3 ro^2) / (rc^2 - ro^2)^3
-      const float u = a.rc2 - s;
-      const float S = u * u * fmaf(2.0f, s, fmaf(-3.0f, a.ro2, a.rc2)) * a.inv_d;
-      const float Ss = 6.0f * u * (a.ro2 - s) * a.inv_d;
-      const float Sss = 6.0f * (2.0f * s - a.ro2 - a.rc2) * a.inv_d;
-      o.g = fmaf(S, gV, 2.0f * Ss * V);
-      o.gp = fmaf(gVp, S, fmaf(2.0f * gV, Ss, 2.0f * V * Sss));
-      o.gs = fmaf(S, gV_sig, 2.0f * Ss * V_sig);
-      o.ge = fmaf(S, gV_eps, 2.0f * Ss * V_eps);
-      o.us = S * V_sig;
-      o.ue = S * V_eps;
-    }
-    if (!(s < a.rc2)) o = PairD2{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  } else {
-    // U = eps / alpha b^alpha, b = 1 - r / sigma, for 0 < r < sigma:
-    //   g = -eps b^(alpha-1) / (sigma r)
-    //   dg/ds = eps / (2 sigma r) [(alpha-1) b^(alpha-2) / (sigma r) + b^(alpha-1) / r^2]
-    //   dU/dsigma = eps b^(alpha-1) r / sigma^2
-    //   dg/dsigma = -eps / (r sigma^2) [(alpha-1) b^(alpha-2) r / sigma - b^(alpha-1)]
-    const float inv_r = fast_rsqrt(s);
-    const float r = s * inv_r;
-    const float inv_sig = fast_rcp(sig);
-    const float b = 1.0f - r * inv_sig;
-    float bm1, bm2;
-    if (alp == 2.0f) {
-      bm1 = b;
-      bm2 = 1.0f;
-    } else {
-      bm2 = powf(fmaxf(b, 0.0f), alp - 2.0f);
-      bm1 = bm2 * b;
-    }
-    const bool valid = (s < sig * sig) && (s > 0.0f);
-    o.g = -eps * inv_sig * bm1 * inv_r;
-    o.gp = 0.5f * eps * inv_sig * inv_r *
-           fmaf((alp - 1.0f) * bm2, inv_sig * inv_r, bm1 * inv_r * inv_r);
-    o.ge = -inv_sig * bm1 * inv_r;
-    o.ue = bm1 * b / alp;
-    o.us = eps * bm1 * r * inv_sig * inv_sig;
-    o.gs = -eps * inv_r * inv_sig * inv_sig * fmaf((alp - 1.0f) * bm2, r * inv_sig, -bm1);
-    if (!valid) o = PairD2{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  }
-  return o;
-}
-
 enum { ADJ_VJP = 0, ADJ_STEP = 1 };
 
 // One launch over all rows.  ADJ_VJP: hout = J v, part[4 b + k] = block sums

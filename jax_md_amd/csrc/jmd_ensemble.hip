@@ -31,7 +31,29 @@ struct EnsSys {
   float fill_cell_size[2];
   float side[2];
   int cap, max_occ;
+  float cutoff_sq, threshold_sq;  // list cutoff^2, (skin / 2)^2
+  int param_offset;               // this system's block of species parameters
+  int reserved;
 };
+
+// Per-step record of a forward segment for the adjoint (all system-major by
+// particle, node k = state after k steps of the segment, k = 0..K):
+//   x[k], F[k]: float32 positions and forces at node k ([K+1][ntot][2])
+//   q[k]: momentum of step k after the second half kick, before the FIRE
+//         mixing ([K][ntot][2])
+//   idx[k]: the Dense list the force at node k was computed with
+//         ([K+1][ntot][width])
+//   sc[k][s]: step k's scalars of system s: dt, alpha, keep, sqrt(pp),
+//         sqrt(ff + 1e-6)  ([K][nsys][kTapeScalars])
+struct EnsTape {
+  float* x;
+  float* F;
+  float* q;
+  int* idx;
+  float* sc;
+  int K;
+};
+constexpr int kTapeScalars = 5;
 
 struct FireP {
   float dt_start, dt_max;
@@ -238,10 +260,10 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     const EnsSys* __restrict__ systems, const int* __restrict__ order, int nsys, int ntot,
     int mode, int n_steps, double* __restrict__ x_hi, float* __restrict__ x,
     float* __restrict__ p, float* __restrict__ F, float* __restrict__ x_ref,
-    int* __restrict__ idx, int width, int* __restrict__ scratch, PairArgs a, float cutoff_sq,
-    float thr_sq, FireP prm, float* __restrict__ fire_state, int* __restrict__ fire_npos,
-    uint8_t* __restrict__ error, int* __restrict__ out_stat, double* __restrict__ energy,
-    int* __restrict__ builds) {
+    int* __restrict__ idx, int width, int* __restrict__ scratch, PairArgs a0, FireP prm,
+    float* __restrict__ fire_state, int* __restrict__ fire_npos, uint8_t* __restrict__ error,
+    int* __restrict__ out_stat, double* __restrict__ energy, int* __restrict__ builds,
+    EnsTape tape) {
   __shared__ int s_cnt[kEnsMaxCells];
   __shared__ int s_start[kEnsMaxCells + 1];
   __shared__ int s_red[kEnsThreads / kWave];
@@ -252,6 +274,11 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   if (sid < 0 || sid >= nsys) return;
   const EnsSys S = systems[sid];
   const int t = threadIdx.x, nt = blockDim.x;
+  const float cutoff_sq = S.cutoff_sq, thr_sq = S.threshold_sq;
+  PairArgs a = a0;  // this system's parameter block
+  if (a.sig_mode == JMD_PARAM_SPECIES) a.sig_v += S.param_offset;
+  if (a.eps_mode == JMD_PARAM_SPECIES) a.eps_v += S.param_offset;
+  if (a.alp_mode == JMD_PARAM_SPECIES) a.alp_v += S.param_offset;
   int* cellp = scratch + S.offset;
   int* rankp = scratch + ntot + S.offset;
   int* sorted = scratch + 2LL * ntot + S.offset;
@@ -343,6 +370,19 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   }
   __syncthreads();
   int n_pos = s_flag;
+  const long long n2 = 2LL * ntot;
+  auto record_node = [&](int k) {  // x, F, list at node k of the segment
+    for (int q = t; q < 2 * S.n; q += nt) {
+      const long long g = 2LL * S.offset + q;
+      tape.x[k * n2 + g] = x[g];
+      tape.F[k * n2 + g] = F[g];
+    }
+    const long long rows = (long long)S.n * width;
+    int* dst = tape.idx + (long long)k * ntot * width + (long long)S.offset * width;
+    const int* src = idx + (long long)S.offset * width;
+    for (long long q = t; q < rows; q += nt) dst[q] = src[q];
+  };
+  if (tape.x) record_node(0);
   for (int step = 0; step < n_steps; ++step) {
     // neighbour update: rebuild when any particle moved beyond half the skin
     bool moved = false;
@@ -354,6 +394,12 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     }
     if (__syncthreads_or(moved)) rebuild(true);
     __syncthreads();
+    if (tape.x) {  // the list the force at node step + 1 will use
+      const long long rows = (long long)S.n * width;
+      int* dst = tape.idx + (long long)(step + 1) * ntot * width + (long long)S.offset * width;
+      const int* src = idx + (long long)S.offset * width;
+      for (long long q = t; q < rows; q += nt) dst[q] = src[q];
+    }
     // velocity-Verlet sub-step at the FIRE time step (float16 value)
     const float dt = s_scal[0];
     const float half = dt / 2.0f;
@@ -384,12 +430,25 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
       ff += fv * fv;
       pp += pv * pv;
       fp += fv * pv;
+      if (tape.x) {
+        tape.q[step * n2 + q] = pk;
+        tape.x[(step + 1) * n2 + q] = x[q];
+        tape.F[(step + 1) * n2 + q] = F[q];
+      }
     }
     wg_sum3(ff, pp, fp, s_d);
     // FIRE bookkeeping (minimize.py:190-219) with float16 dt / alpha
     const double ratio = sqrt(pp) / sqrt(ff + 1e-6);
     const float alpha_mix = s_scal[1];
     const double keep = fp >= 0.0 ? 1.0 : 0.0;
+    if (tape.x && t == 0) {
+      float* sc = tape.sc + ((long long)step * nsys + sid) * kTapeScalars;
+      sc[0] = dt;
+      sc[1] = alpha_mix;
+      sc[2] = (float)keep;
+      sc[3] = (float)sqrt(pp);
+      sc[4] = (float)sqrt(ff + 1e-6);
+    }
     n_pos = fp >= 0.0 ? n_pos + 1 : 0;
     __syncthreads();
     if (t == 0) {
@@ -425,6 +484,193 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the FIRE trajectory (per system, one workgroup each)
+//
+// Forward step k (dt, alpha, keep from the tape; float16 bookkeeping is
+// piecewise constant and carries no derivative):
+//   ph = p_k + h F_k,  x_{k+1} = x_k + dt ph,  F' = F(x_{k+1}),  q = ph + h F',
+//   p_{k+1} = keep ((1 - alpha) q + alpha |q| / sqrt(|F'|^2 + 1e-6) F').
+// Backward, with X, P the adjoints of x, p and W the pending vector of node
+// k + 1 (W = h PH of the step after):
+//   G = keep P,  s1 = F'.G,
+//   Q  = (1 - alpha) G + alpha s1 q / (|q| sqrt(ff)),
+//   Fb = alpha |q|/sqrt(ff) G - alpha s1 |q| F' / sqrt(ff)^3 + h Q,
+//   X += J(x_{k+1}) (Fb + W),  theta += (Fb + W)^T dF(x_{k+1})/dtheta,
+//   P = Q + dt X,  W = h P.
+// One product with the force Jacobian per step.  The loss is
+// scale * sum_s E_s(x_T), theta the derivative with respect to each system's
+// parameter (the species matrix's derivative coefficients dsigma/dtheta).
+// ---------------------------------------------------------------------------
+
+enum { BACK_INIT = 0, BACK_STEPS = 1, BACK_FINISH = 2 };
+
+// h_i = (J v)_i over the system's rows at positions x with list idx; returns
+// this thread's share of v^T dF/dtheta (rows it owns).
+__device__ double ens_hvp_rows(const EnsSys& S, const Space& sp, const PairArgs& a,
+                               const float* __restrict__ dsig, const float* __restrict__ x,
+                               const int* __restrict__ idx, int width, int pad,
+                               const float* __restrict__ v, double* __restrict__ X) {
+#pragma clang fp contract(off)
+  double tacc = 0.0;
+  for (int q = threadIdx.x; q < S.n; q += blockDim.x) {
+    const int i = S.offset + q;
+    float xi[2], xim[2], vi[2];
+    load_pos<2>(x, i, xi);
+    load_pos<2>(v, i, vi);
+    xim[0] = xi[0] - sp.L[0];
+    xim[1] = xi[1] - sp.L[1];
+    const int si = a.species ? a.species[i] : 0;
+    float h0 = 0.0f, h1 = 0.0f, tp = 0.0f;
+    const int* row = idx + (long long)i * width;
+    for (int k = 0; k < width; ++k) {
+      const int j = row[k];
+      if (j == pad) break;
+      float xj[2], vj[2];
+      load_pos<2>(x, j, xj);
+      load_pos<2>(v, j, vj);
+      float d[2], w[2];
+      d[0] = min_image_disp(xj[0], xi[0], xim[0], sp.L[0], sp.half[0]);
+      d[1] = min_image_disp(xj[1], xi[1], xim[1], sp.L[1], sp.half[1]);
+      w[0] = vj[0] - vi[0];
+      w[1] = vj[1] - vi[1];
+      const float r2 = fmaf(d[1], d[1], d[0] * d[0]);
+      const int sj = a.species ? a.species[j] : 0;
+      const float sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
+      const float eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
+      const float alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
+      const PairD2 D = pair_eval2<JMD_SOFT_SPHERE>(r2, sig, eps, alp, a);
+      const float dw = fmaf(d[1], w[1], d[0] * w[0]);
+      const float tt = 2.0f * D.gp * dw;
+      h0 = fmaf(D.g, w[0], fmaf(tt, d[0], h0));
+      h1 = fmaf(D.g, w[1], fmaf(tt, d[1], h1));
+      if (dsig) {
+        const float c = dsig[si * a.ns + sj];
+        const float lam = fmaf(vi[1], d[1], vi[0] * d[0]);
+        tp = fmaf(c * D.gs, lam, tp);
+      }
+    }
+    X[2LL * i] += (double)h0;
+    X[2LL * i + 1] += (double)h1;
+    tacc += (double)tp;
+  }
+  return tacc;
+}
+
+__global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
+    const EnsSys* __restrict__ systems, int nsys, int ntot, int mode, int K,
+    const float* __restrict__ x, const float* __restrict__ F, const int* __restrict__ idx,
+    int width, PairArgs a0, const float* __restrict__ dsig0, EnsTape tape,
+    double* __restrict__ X, double* __restrict__ P, float* __restrict__ W,
+    float* __restrict__ V, double* __restrict__ theta, float scale) {
+#pragma clang fp contract(off)
+  __shared__ double s_d[3 * (kEnsThreads / kWave)];
+  const int sid = blockIdx.x;
+  if (sid >= nsys) return;
+  const EnsSys S = systems[sid];
+  const int t = threadIdx.x, nt = blockDim.x;
+  PairArgs a = a0;
+  if (a.sig_mode == JMD_PARAM_SPECIES) a.sig_v += S.param_offset;
+  if (a.eps_mode == JMD_PARAM_SPECIES) a.eps_v += S.param_offset;
+  if (a.alp_mode == JMD_PARAM_SPECIES) a.alp_v += S.param_offset;
+  const float* dsig = dsig0 ? dsig0 + S.param_offset : nullptr;
+  Space sp;
+  sp.dim = 2;
+  sp.periodic = 1;
+  for (int d = 0; d < 3; ++d) {
+    sp.L[d] = d < 2 ? S.side[d] : 1.0f;
+    sp.half[d] = sp.L[d] * 0.5f;
+  }
+  const int pad = ntot;
+  const long long n2 = 2LL * ntot;
+  const long long b2 = 2LL * S.offset;
+
+  if (mode == BACK_INIT) {  // X = -scale F, P = W = 0, theta = scale dE/dtheta
+    double e = 0.0, z1 = 0.0, z2 = 0.0;
+    for (int k = t; k < 2 * S.n; k += nt) {
+      X[b2 + k] = -(double)scale * (double)F[b2 + k];
+      P[b2 + k] = 0.0;
+      W[b2 + k] = 0.0f;
+    }
+    if (dsig) {
+      for (int q = t; q < S.n; q += nt) {
+        const int i = S.offset + q;
+        float xi[2], xim[2];
+        load_pos<2>(x, i, xi);
+        xim[0] = xi[0] - sp.L[0];
+        xim[1] = xi[1] - sp.L[1];
+        const int si = a.species ? a.species[i] : 0;
+        const int* row = idx + (long long)i * width;
+        for (int k = 0; k < width; ++k) {
+          const int j = row[k];
+          if (j == pad) break;
+          float xj[2], d[2];
+          load_pos<2>(x, j, xj);
+          d[0] = min_image_disp(xj[0], xi[0], xim[0], sp.L[0], sp.half[0]);
+          d[1] = min_image_disp(xj[1], xi[1], xim[1], sp.L[1], sp.half[1]);
+          const float r2 = fmaf(d[1], d[1], d[0] * d[0]);
+          const int sj = a.species ? a.species[j] : 0;
+          const float sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
+          const float eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
+          const float alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
+          const PairD2 D = pair_eval2<JMD_SOFT_SPHERE>(r2, sig, eps, alp, a);
+          e += 0.5 * (double)(dsig[si * a.ns + sj] * D.us);
+        }
+      }
+    }
+    wg_sum3(e, z1, z2, s_d);
+    if (t == 0) theta[sid] = (double)scale * e;
+    return;
+  }
+  if (mode == BACK_FINISH) {  // pending W applied at node 0 of the tape
+    const double tp = ens_hvp_rows(S, sp, a, dsig, tape.x, tape.idx, width, pad, W, X);
+    double e = tp, z1 = 0.0, z2 = 0.0;
+    wg_sum3(e, z1, z2, s_d);
+    if (t == 0) theta[sid] += e;
+    return;
+  }
+  double tsum = 0.0;
+  for (int k = K - 1; k >= 0; --k) {
+    const float* sc = tape.sc + ((long long)k * nsys + sid) * kTapeScalars;
+    const float dt = sc[0], alpha = sc[1], keep = sc[2];
+    const double spp = sc[3], sff = sc[4];
+    const float h = dt / 2.0f;
+    const float* Fn = tape.F + (long long)(k + 1) * n2;
+    const float* qk = tape.q + (long long)k * n2;
+    // (1) FIRE mixing, backward
+    double s1 = 0.0, z1 = 0.0, z2 = 0.0;
+    for (int m = t; m < 2 * S.n; m += nt) s1 += (double)Fn[b2 + m] * (double)keep * P[b2 + m];
+    wg_sum3(s1, z1, z2, s_d);
+    const double ratio = spp / sff;
+    const double cq = spp > 0.0 ? (double)alpha * s1 / (spp * sff) : 0.0;
+    const double cf = (double)alpha * s1 * spp / (sff * sff * sff);
+    for (int m = t; m < 2 * S.n; m += nt) {
+      const long long g = b2 + m;
+      const double G = (double)keep * P[g];
+      const double Q = (1.0 - (double)alpha) * G + cq * (double)qk[g];
+      const double Fb = (double)alpha * ratio * G - cf * (double)Fn[g] + (double)h * Q;
+      P[g] = Q;  // PH until step (3)
+      V[g] = (float)(Fb + (double)W[g]);
+    }
+    __syncthreads();
+    // (2) product with the force Jacobian at node k + 1
+    tsum += ens_hvp_rows(S, sp, a, dsig, tape.x + (long long)(k + 1) * n2,
+                         tape.idx + (long long)(k + 1) * ntot * width, width, pad, V, X);
+    __syncthreads();
+    // (3) x_{k+1} = x_k + dt ph, ph = p_k + h F_k
+    for (int m = t; m < 2 * S.n; m += nt) {
+      const long long g = b2 + m;
+      const double PH = P[g] + (double)dt * X[g];
+      P[g] = PH;
+      W[g] = (float)((double)h * PH);
+    }
+    __syncthreads();
+  }
+  double e = tsum, z1 = 0.0, z2 = 0.0;
+  wg_sum3(e, z1, z2, s_d);
+  if (t == 0) theta[sid] += e;
+}
+
 }  // namespace jmd
 
 using namespace jmd;
@@ -436,9 +682,10 @@ int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int3
                          float* position, float* momentum, float* force,
                          float* reference_position, int32_t* idx, int32_t width,
                          int32_t* scratch, const jmd_pair_potential* potential,
-                         float cutoff_sq, float threshold_sq, const float* fire_params,
-                         float* fire_state, int32_t* fire_n_pos, uint8_t* error,
-                         int32_t* out_stat, double* energy, int32_t* builds, void* stream) {
+                         const float* fire_params, float* fire_state, int32_t* fire_n_pos,
+                         uint8_t* error, int32_t* out_stat, double* energy, int32_t* builds,
+                         float* tape_x, float* tape_force, float* tape_q, int32_t* tape_idx,
+                         float* tape_scalars, void* stream) {
   if (n_systems < 0 || n_total < 0 || n_total >= 0x7fffffff || width < 0 || n_steps < 0 ||
       mode < ENS_FILL || mode > ENS_RUN || !fire_params)
     return JMD_ERR_INVALID;
@@ -454,16 +701,51 @@ int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int3
   FireP prm{fire_params[0], fire_params[1], (int)fire_params[2], fire_params[3],
             fire_params[4], fire_params[5], fire_params[6]};
   static_assert(sizeof(jmd_system2d) == sizeof(EnsSys), "system record layout");
+  const bool taped = tape_x != nullptr;
+  if (taped && (mode != ENS_RUN || !tape_force || !tape_q || !tape_idx || !tape_scalars))
+    return JMD_ERR_INVALID;
+  EnsTape tape{tape_x, tape_force, tape_q, tape_idx, tape_scalars, n_steps};
   hipLaunchKernelGGL(k_fire_ensemble, dim3(n_systems), dim3(kEnsThreads), 0,
                      (hipStream_t)stream, reinterpret_cast<const EnsSys*>(systems), order,
                      n_systems, (int)n_total, mode, n_steps, position_hi, position, momentum,
-                     force, reference_position, idx, width, scratch, a, cutoff_sq,
-                     threshold_sq, prm, fire_state, fire_n_pos, error, out_stat, energy,
-                     builds);
+                     force, reference_position, idx, width, scratch, a, prm, fire_state,
+                     fire_n_pos, error, out_stat, energy, builds, tape);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }
 
 int jmd_fire_ensemble_max_cells(void) { return kEnsMaxCells; }
+
+int jmd_fire_ensemble_adjoint_2d(const jmd_system2d* systems, int32_t n_systems,
+                                 int64_t n_total, int32_t mode, int32_t n_steps,
+                                 const float* position, const float* force, const int32_t* idx,
+                                 int32_t width, const jmd_pair_potential* potential,
+                                 const float* dsigma, const float* tape_x,
+                                 const float* tape_force, const float* tape_q,
+                                 const int32_t* tape_idx, const float* tape_scalars,
+                                 double* x_adjoint, double* p_adjoint, float* pending,
+                                 float* scratch, double* theta, float scale, void* stream) {
+  if (n_systems < 0 || n_total < 0 || n_total >= 0x7fffffff || width < 1 || n_steps < 0 ||
+      mode < BACK_INIT || mode > BACK_FINISH)
+    return JMD_ERR_INVALID;
+  if (n_systems == 0) return JMD_OK;
+  if (!systems || !x_adjoint || !p_adjoint || !pending || !scratch || !theta)
+    return JMD_ERR_INVALID;
+  if (mode == BACK_INIT && (!position || !force || !idx)) return JMD_ERR_INVALID;
+  if (mode != BACK_INIT && (!tape_x || !tape_force || !tape_q || !tape_idx || !tape_scalars))
+    return JMD_ERR_INVALID;
+  PairArgs a;
+  if (!make_pair_args(potential, &a) || a.kind != JMD_SOFT_SPHERE) return JMD_ERR_INVALID;
+  if (dsigma && a.sig_mode != JMD_PARAM_SPECIES) return JMD_ERR_UNSUPPORTED;
+  EnsTape tape{const_cast<float*>(tape_x), const_cast<float*>(tape_force),
+               const_cast<float*>(tape_q), const_cast<int*>(tape_idx),
+               const_cast<float*>(tape_scalars), n_steps};
+  hipLaunchKernelGGL(k_fire_ensemble_back, dim3(n_systems), dim3(kEnsThreads), 0,
+                     (hipStream_t)stream, reinterpret_cast<const EnsSys*>(systems), n_systems,
+                     (int)n_total, mode, n_steps, position, force, idx, width, a, dsigma, tape,
+                     x_adjoint, p_adjoint, pending, scratch, theta, scale);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
 
 }  // extern "C"

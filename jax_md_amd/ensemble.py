@@ -37,16 +37,20 @@ class _System2D(ctypes.Structure):
               ('cells_per_side', ctypes.c_int32 * 2), ('cell_size', ctypes.c_float * 2),
               ('fill_cells_per_side', ctypes.c_int32 * 2),
               ('fill_cell_size', ctypes.c_float * 2), ('side', ctypes.c_float * 2),
-              ('cell_capacity', ctypes.c_int32), ('max_occupancy', ctypes.c_int32)]
+              ('cell_capacity', ctypes.c_int32), ('max_occupancy', ctypes.c_int32),
+              ('cutoff_sq', ctypes.c_float), ('threshold_sq', ctypes.c_float),
+              ('param_offset', ctypes.c_int32), ('reserved', ctypes.c_int32)]
 
 
 class SoftSphereFireEnsemble:
   """FIRE minimisation of many periodic 2-D soft-sphere systems.
 
   positions: sequence of [N_s, 2] arrays; boxes: per-system side lengths;
-  species: sequence of [N_s] integer arrays; sigma: a scalar or a species
-  matrix; epsilon, alpha: scalars.  FIRE constants as in
-  ``minimize.fire_descent`` (static_cast to float16).
+  species: sequence of [N_s] integer arrays; sigma: a scalar, a species
+  matrix shared by all systems, or one species matrix per system
+  ([S, n_species, n_species]); epsilon, alpha: scalars.  Each system's list
+  cutoff is its largest sigma plus dr_threshold (energy.py:207-219).  FIRE
+  constants as in ``minimize.fire_descent`` (static_cast to float16).
   """
 
   def __init__(self, positions: Sequence, boxes: Sequence, species: Optional[Sequence] = None,
@@ -66,19 +70,22 @@ class SoftSphereFireEnsemble:
       raise ValueError('too many particles for int32 ids')
     R = onp.concatenate([onp.asarray(p, f32).reshape(-1, 2) for p in positions])
     sig = onp.asarray(sigma, f32)
-    self.cutoff_sigma = float(f32(onp.max(sig)))
+    per_system = sig.ndim == 3
+    if per_system and sig.shape[0] != S:
+      raise ValueError('per-system sigma needs one species matrix per system')
+    sig_max = [float(f32(onp.max(sig[s] if per_system else sig))) for s in range(S)]
+    self.cutoff_sigma = max(sig_max)
     specs = []
-    for L in boxes:
+    for s, L in enumerate(boxes):
       disp, _ = space_mod.periodic(f32(L))
-      specs.append(partition.NeighborListSpec(disp, f32(L), self.cutoff_sigma, dr_threshold,
+      specs.append(partition.NeighborListSpec(disp, f32(L), sig_max[s], dr_threshold,
                                               capacity_multiplier, False, True,
                                               partition.Dense))
     self.specs = specs
-    self.cutoff_sq = float(specs[0].cutoff_sq)
-    self.threshold_sq = float(specs[0].threshold_sq)
     self.capacity_multiplier = capacity_multiplier
     recs = (_System2D * S)()
     max_cells = lib.jmd_fire_ensemble_max_cells()
+    block = int(sig[0].size) if per_system else 0
     for s, (L, sp) in enumerate(zip(boxes, specs)):
       if not sp.use_cell_list():
         raise NotImplementedError('every system needs a cell list (cutoff < box / 3)')
@@ -96,6 +103,9 @@ class SoftSphereFireEnsemble:
         r.fill_cells_per_side[d] = int(cps2_l[d])
         r.fill_cell_size[d] = float(cs2_l[d])
         r.side[d] = float(f32(L))
+      r.cutoff_sq = float(sp.cutoff_sq)
+      r.threshold_sq = float(sp.threshold_sq)
+      r.param_offset = s * block
     self._recs = recs
     dev = self.device
     order = onp.argsort(-onp.asarray(ns), kind='stable').astype(onp.int32)  # largest first
@@ -118,7 +128,7 @@ class SoftSphereFireEnsemble:
       species = [onp.zeros(n, onp.int32) for n in ns]
     sp_all = onp.concatenate([onp.asarray(x, onp.int32).reshape(-1) for x in species])
     self.species = torch.as_tensor(sp_all, device=dev)
-    self._sig = torch.as_tensor(onp.atleast_2d(sig) if sig.ndim else sig, device=dev)
+    self._sig = torch.as_tensor(onp.ascontiguousarray(sig).reshape(-1), device=dev)
     pot = nat.JmdPairPotential()
     pot.kind = nat.SOFT_SPHERE
     if sig.ndim == 0:
@@ -126,7 +136,7 @@ class SoftSphereFireEnsemble:
       pot.n_species = 0
     else:
       pot.sigma = nat.JmdParam(nat.PARAM_SPECIES, 0.0, ctypes.c_void_p(self._sig.data_ptr()))
-      pot.n_species = int(sig.shape[0])
+      pot.n_species = int(sig.shape[-1])
     pot.epsilon = nat.JmdParam(nat.PARAM_SCALAR, float(f32(epsilon)), None)
     pot.alpha = nat.JmdParam(nat.PARAM_SCALAR, float(f32(alpha)), None)
     pot.r_onset = 0.0
@@ -138,16 +148,19 @@ class SoftSphereFireEnsemble:
     self.width = 0
 
   # -- native call -------------------------------------------------------------
-  def _launch(self, mode, n_steps=0):
+  def _launch(self, mode, n_steps=0, tape=None):
     lib = nat.load()
+    tp = tape or {}
     nat.check(lib.jmd_fire_ensemble_2d(
         nat.ptr(self._systems), nat.ptr(self.order), len(self.sizes), self.n_total, mode,
         n_steps, nat.ptr(self.position_hi), nat.ptr(self.position), nat.ptr(self.momentum),
         nat.ptr(self.force), nat.ptr(self.reference_position), nat.ptr(self.idx),
-        self.width, nat.ptr(self.scratch), ctypes.byref(self._pot), self.cutoff_sq,
-        self.threshold_sq, self._fire, nat.ptr(self.fire_state), nat.ptr(self.fire_n_pos),
-        nat.ptr(self.error), nat.ptr(self._stat), nat.ptr(self.energy), nat.ptr(self.builds),
-        nat.stream_handle(self.device)), 'jmd_fire_ensemble_2d')
+        self.width, nat.ptr(self.scratch), ctypes.byref(self._pot), self._fire,
+        nat.ptr(self.fire_state), nat.ptr(self.fire_n_pos), nat.ptr(self.error),
+        nat.ptr(self._stat), nat.ptr(self.energy), nat.ptr(self.builds),
+        nat.ptr(tp.get('x')), nat.ptr(tp.get('F')), nat.ptr(tp.get('q')),
+        nat.ptr(tp.get('idx')), nat.ptr(tp.get('sc')), nat.stream_handle(self.device)),
+        'jmd_fire_ensemble_2d')
 
   def _upload(self):
     host = torch.frombuffer(bytearray(bytes(self._recs)), dtype=torch.uint8)
@@ -206,3 +219,82 @@ class SoftSphereFireEnsemble:
   def capacities(self, s: int):
     r = self._recs[s]
     return int(r.cell_capacity), int(r.max_occupancy)
+
+  # -- gradients -------------------------------------------------------------
+  _STATE = ('position_hi', 'position', 'momentum', 'force', 'reference_position', 'fire_state',
+            'fire_n_pos', 'idx', 'error')
+
+  def _snapshot(self):
+    return {k: getattr(self, k).clone() for k in self._STATE}
+
+  def _restore(self, snap):
+    for k in self._STATE:
+      getattr(self, k).copy_(snap[k])
+
+  def energy_gradient(self, n_steps: int, dsigma, checkpoint_every: int = 25,
+                      scale: Optional[float] = None):
+    """FIRE for n_steps from the allocated state, then the gradient of
+    scale * sum_s E_s(x_T) (default scale 1/S: the mean final energy) with
+    respect to each system's parameter theta_s, where the species matrix
+    depends on theta_s with coefficients dsigma (a matrix shared by all
+    systems, or one per system).  The state and list are kept every
+    ``checkpoint_every`` steps; each segment is replayed once with a tape and
+    run backward (one force-Jacobian product per step).  Returns
+    (E_T float64 [S], dL/dtheta float64 [S], dL/dx_0 float64 [N_total, 2]).
+    The ensemble is left in its final state.  Boxes stay fixed: a theta that
+    should also move a box is not followed here."""
+    if self.idx is None:
+      raise RuntimeError('allocate() first')
+    lib = nat.load()
+    S, dev = len(self.sizes), self.device
+    scale = 1.0 / S if scale is None else float(scale)
+    ds = onp.asarray(dsigma, f32)
+    if ds.ndim == 2:
+      ds = onp.broadcast_to(ds, (S,) + ds.shape) if self._sig.numel() > ds.size else ds
+    dsig = torch.as_tensor(onp.ascontiguousarray(ds).reshape(-1), device=dev)
+    if dsig.numel() != self._sig.numel():
+      raise ValueError('dsigma must be laid out like sigma')
+    K = max(1, int(checkpoint_every))
+    snaps = []
+    done = 0
+    while done < n_steps:
+      k = min(K, n_steps - done)
+      snaps.append((k, self._snapshot()))
+      self.run(k)
+      done += k
+    E_T = self.energy.clone()
+    final = self._snapshot()
+    final_builds = self.builds.clone()
+    n2 = (self.n_total, 2)
+    X = torch.zeros(n2, dtype=torch.float64, device=dev)
+    P = torch.zeros(n2, dtype=torch.float64, device=dev)
+    W = torch.zeros(n2, dtype=torch.float32, device=dev)
+    V = torch.zeros(n2, dtype=torch.float32, device=dev)
+    theta = torch.zeros((S,), dtype=torch.float64, device=dev)
+    tape = dict(x=torch.empty((K + 1,) + n2, dtype=torch.float32, device=dev),
+                F=torch.empty((K + 1,) + n2, dtype=torch.float32, device=dev),
+                q=torch.empty((K,) + n2, dtype=torch.float32, device=dev),
+                idx=torch.empty((K + 1, self.n_total, self.width), dtype=torch.int32,
+                                device=dev),
+                sc=torch.empty((K, S, 5), dtype=torch.float32, device=dev))
+
+    def back(mode, k):
+      nat.check(lib.jmd_fire_ensemble_adjoint_2d(
+          nat.ptr(self._systems), S, self.n_total, mode, k, nat.ptr(self.position),
+          nat.ptr(self.force), nat.ptr(self.idx), self.width, ctypes.byref(self._pot),
+          nat.ptr(dsig), nat.ptr(tape['x']), nat.ptr(tape['F']), nat.ptr(tape['q']),
+          nat.ptr(tape['idx']), nat.ptr(tape['sc']), nat.ptr(X), nat.ptr(P), nat.ptr(W),
+          nat.ptr(V), nat.ptr(theta), float(scale), nat.stream_handle(dev)),
+          'jmd_fire_ensemble_adjoint_2d')
+
+    back(0, 0)
+    for k, snap in reversed(snaps):
+      self._restore(snap)
+      self._launch(_RUN, k, tape)
+      back(1, k)
+    if snaps:
+      back(2, 0)
+    self._restore(final)
+    self.builds.copy_(final_builds)
+    self.energy.copy_(E_T)
+    return E_T, theta, X

@@ -744,11 +744,13 @@ def pair_d2(model, r2, sig, eps, alp=2.0):
   return tuple(np.where(inside, x, 0.0) for x in (g, gp, gs, ge, us, ue))
 
 
-def force_vjp(model, space, R, v, i, j):
+def force_vjp(model, space, R, v, i, j, dsigma=None):
   """FP64 VJP of F = -dE/dR over the ordered pairs (i, j) of a list holding
   both orders: (J v)_i = sum_j g w + 2 g' (d.w) d with d = disp(R_j, R_i),
   w = v_j - v_i; v^T dF/dsigma, v^T dF/depsilon (scalar parameters); and
-  dE/dsigma, dE/depsilon.  Returns (Jv, (ts, te), (Es, Ee))."""
+  dE/dsigma, dE/depsilon.  With ``dsigma`` (a species matrix of coefficients
+  d sigma_ab / d theta) the sigma terms are taken with respect to theta.
+  Returns (Jv, (ts, te), (Es, Ee))."""
   R = np.asarray(R, F64)
   v = np.asarray(v, F64)
   d = space.displacement(R[j], R[i]).astype(F64)
@@ -763,6 +765,10 @@ def force_vjp(model, space, R, v, i, j):
   Jv = np.zeros_like(R)
   np.add.at(Jv, i, h)
   lam = np.sum(v[i] * d, -1)
+  if dsigma is not None:
+    sp = np.asarray(model.species)
+    w_ab = np.asarray(dsigma, F64)[sp[i], sp[j]]
+    gs, us = gs * w_ab, us * w_ab
   return Jv, (float(np.sum(gs * lam)), float(np.sum(ge * lam))), \
       (0.5 * float(np.sum(us)), 0.5 * float(np.sum(ue)))
 
@@ -840,9 +846,10 @@ def fire_init(R, force_fn, params, mass=1.0):
   return FireState(R, np.zeros_like(R), force_fn(R), mass, dt_start, alpha_start, 0)
 
 
-def fire_step(state, force_fn, space, params):
+def fire_step(state, force_fn, space, params, tape=None):
   """FIRE (minimize.py:184-224): NVE step at dt=state.dt, then the mixing and
-  the float16 dt/alpha bookkeeping."""
+  the float16 dt/alpha bookkeeping.  ``tape`` (a list) receives what the
+  backward recursion of ``fire_energy_gradient`` needs."""
   dt_start, dt_max, n_min, f_inc, f_dec, alpha_start, f_alpha = params.cast()
   s = nve_step(NVEState(state.position, state.momentum, state.force, state.mass),
                force_fn, space, F32(state.dt))
@@ -851,6 +858,9 @@ def fire_step(state, force_fn, space, params):
   P_norm = np.sqrt(np.sum(P * P))
   F_dot_P = np.sum(F * P)
   alpha = state.alpha
+  if tape is not None:
+    tape.append(dict(x=R, q=P, F=F, dt=F64(F32(state.dt)), alpha=F64(alpha),
+                     keep=1.0 if F_dot_P >= 0 else 0.0, pn=P_norm, fn=F_norm))
   P = P + F64(alpha) * (F * P_norm / F_norm - P)
   n_pos = state.n_pos + 1 if F_dot_P >= 0 else 0
   dt = state.dt
@@ -864,6 +874,68 @@ def fire_step(state, force_fn, space, params):
     alpha = alpha_start
   P = P * (1.0 if F_dot_P >= 0 else 0.0)
   return FireState(R, P, F, state.mass, np.float16(dt), np.float16(alpha), n_pos)
+
+
+def fire_energy_after(model, space, R, n_steps, params):
+  """E(x_T) after n_steps of FP64 FIRE from rest with brute-force forces: the
+  function whose finite differences pin ``fire_energy_gradient``."""
+  def force(X):
+    return energy_force_brute(model, space, X)[2]
+  st = fire_init(R, force, params)
+  for _ in range(n_steps):
+    st = fire_step(st, force, space, params)
+  return energy_force_brute(model, space, st.position)[0]
+
+
+def fire_energy_gradient(model, space, R, n_steps, params, dsigma):
+  """(E(x_T), dE(x_T)/dtheta, dE(x_T)/dx_0) through n_steps of FP64 FIRE from
+  rest, where the species matrix sigma depends on theta with coefficients
+  ``dsigma``.  The reference would take this with jax.grad through the
+  minimiser's loop (minimize.py:184-224); the step-size / alpha decisions and
+  the F.P >= 0 reset are piecewise constant and carry no gradient.
+
+  Backward recursion per step k (h = dt_k / 2, q = momentum before mixing,
+  G = keep * P, s1 = F.G):
+    Q  = (1 - a) G + a s1 q / (|q| |F|~)
+    Fb = a (|q| / |F|~) G - a s1 |q| F / |F|~^3 + h Q + W
+    X += J(x_{k+1}) Fb;  theta += Fb . dF/dtheta (x_{k+1})
+    P  = Q + dt X;  W = h P
+  and finally X += J(x_0) W, theta += W . dF/dtheta (x_0)."""
+  cutoff = float(np.max(np.asarray(model.sigma, F64)))
+
+  def force(X):
+    return energy_force_brute(model, space, X)[2]
+
+  def jvp(X, v):
+    i, j = all_pairs(space, X, cutoff)
+    Jv, (ts, _), (es, _) = force_vjp(model, space, X, v, i, j, dsigma)
+    return Jv, ts, es
+
+  st = fire_init(R, force, params)
+  x0 = st.position
+  tape = []
+  for _ in range(n_steps):
+    st = fire_step(st, force, space, params, tape)
+  E_T, _, F_T = energy_force_brute(model, space, st.position)
+  _, _, theta = jvp(st.position, np.zeros_like(F_T))
+  X = -F_T
+  P = np.zeros_like(X)
+  W = np.zeros_like(X)
+  for rec in reversed(tape):
+    a, h, dt = rec['alpha'], 0.5 * rec['dt'], rec['dt']
+    G = rec['keep'] * P
+    s1 = float(np.sum(rec['F'] * G))
+    pn, fn = rec['pn'], rec['fn']
+    cq = a * s1 / (pn * fn) if pn > 0 else 0.0
+    Q = (1 - a) * G + cq * rec['q']
+    Fb = a * (pn / fn) * G - a * s1 * pn / fn ** 3 * rec['F'] + h * Q + W
+    Jv, ts, _ = jvp(rec['x'], Fb)
+    X = X + Jv
+    theta += ts
+    P = Q + dt * X
+    W = h * P
+  Jv, ts, _ = jvp(x0, W)
+  return E_T, theta + ts, X + Jv
 
 
 def kinetic_energy(momentum, mass=1.0):

@@ -113,3 +113,70 @@ def test_ensemble_rebuilds_and_overflow_flag():
   b = ens.builds.cpu().numpy()
   assert np.all(b >= 2), b
   assert int(ens.did_buffer_overflow.max()) == 0
+
+
+def _sigma_r(r):
+  return np.array([[1.0, (1 + r) / 2], [(1 + r) / 2, r]], F32)
+
+
+DSIGMA = np.array([[0.0, 0.5], [0.5, 1.0]], F32)
+
+
+def test_ensemble_gradient_matches_fp64_fire_adjoint():
+  """d E_s(x_T) / d r_s through T FIRE steps, per system with its own
+  diameter ratio r_s (sigma_ab = [[1, (1+r)/2], [(1+r)/2, r]], box fixed),
+  against the FP64 oracle's adjoint (itself pinned to central differences).
+  The GPU trajectory is float32, so the bar is 1e-5 relative on dE/dr and on
+  the projection of dE/dx_0; segment lengths change nothing (bit for bit)."""
+  jmd = _jmd()
+  sizes, ratios = [64, 150, 97, 256], [1.2, 1.3, 1.45, 1.6]
+  syst = [O.bidisperse_2d(n, d_large=r, seed=40 + k) for k, (n, r) in
+          enumerate(zip(sizes, ratios))]
+  sig = np.stack([_sigma_r(r) for r in ratios])
+  T = 30
+  print('case ensemble gradient', flush=True)
+
+  def make():
+    return jmd.ensemble.SoftSphereFireEnsemble(
+        [s[0] for s in syst], [s[2] for s in syst], [s[1] for s in syst], sigma=sig,
+        **FIRE).allocate()
+
+  ens = make()
+  E, th, X = ens.energy_gradient(T, DSIGMA, checkpoint_every=7, scale=1.0)
+  assert int(ens.did_buffer_overflow.max()) == 0
+  # the ensemble is left at its final state
+  ref = make().run(T)
+  assert torch.equal(ens.position, ref.position)
+  assert torch.equal(ens.energy, ref.energy)
+  # segment length does not change the result
+  E2, th2, X2 = make().energy_gradient(T, DSIGMA, checkpoint_every=T, scale=1.0)
+  assert torch.equal(th, th2) and torch.equal(X, X2)
+  th, X = th.cpu().numpy(), X.cpu().numpy()
+  par = O.FireParams(dt_start=0.01, dt_max=0.1)
+  v = np.random.default_rng(9).normal(size=X.shape)
+  for s, ((R, sp, L), r) in enumerate(zip(syst, ratios)):
+    model = O.PairModel('soft_sphere', sigma=_sigma_r(r).astype(np.float64), species=sp)
+    Eo, tho, Xo = O.fire_energy_gradient(model, O.Space(np.float64(L)),
+                                         R.astype(np.float64), T, par, DSIGMA)
+    sl = ens.system_slice(s)
+    print(s, th[s], tho, float(E[s]), Eo, flush=True)
+    assert abs(th[s] - tho) <= 1e-5 * abs(tho), (s, th[s], tho)
+    px, po = float(np.sum(X[sl] * v[sl])), float(np.sum(Xo * v[sl]))
+    assert abs(px - po) <= 1e-5 * np.linalg.norm(Xo) * np.linalg.norm(v[sl]), (s, px, po)
+
+
+def test_ensemble_gradient_mean_energy_scale():
+  """scale defaults to 1 / S (the mean final energy)."""
+  jmd = _jmd()
+  syst = [O.bidisperse_2d(n, d_large=1.4, seed=60 + k) for k, n in enumerate([80, 120])]
+  sig = np.stack([_sigma_r(1.4)] * 2)
+
+  def make():
+    return jmd.ensemble.SoftSphereFireEnsemble(
+        [s[0] for s in syst], [s[2] for s in syst], [s[1] for s in syst], sigma=sig,
+        **FIRE).allocate()
+
+  _, th_mean, X_mean = make().energy_gradient(12, DSIGMA, checkpoint_every=5)
+  _, th_sum, X_sum = make().energy_gradient(12, DSIGMA, checkpoint_every=5, scale=1.0)
+  assert torch.allclose(th_mean * 2, th_sum, rtol=1e-12, atol=0)
+  assert torch.allclose(X_mean * 2, X_sum, rtol=1e-6, atol=1e-12)

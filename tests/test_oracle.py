@@ -24,6 +24,10 @@ Which check pins which oracle function:
   nve_step                         energy conservation, time reversibility
   fire_step                        static_cast float16 values (F4), monotone
                                    descent on a quadratic (minimize_test.py:111-150)
+  force_vjp                        central differences of the force in R,
+                                   sigma, epsilon; a dropped term fails
+  fire_energy_gradient             central differences of fire_energy_after
+                                   in r and x_0, also across momentum resets
 """
 
 import math
@@ -452,3 +456,54 @@ def test_force_vjp_dropped_term_fails():
   partial = np.zeros_like(R)
   np.add.at(partial, i, g[:, None] * (v[j] - v[i]))
   assert np.linalg.norm(partial - Jv) > 1e-2 * np.linalg.norm(Jv)
+
+
+def _fire_system(r, n=40, seed=3):
+  R, species, L = O.bidisperse_2d(n, d_large=r, seed=seed)
+  sig = np.array([[1.0, (1 + r) / 2], [(1 + r) / 2, r]], np.float64)
+  return R.astype(np.float64), L, O.PairModel('soft_sphere', sigma=sig, species=species)
+
+
+def test_fire_adjoint_matches_central_differences():
+  """dE(x_T)/d r and dE(x_T)/dx_0 through FP64 FIRE, with sigma_ab(r) =
+  [[1, (1+r)/2], [(1+r)/2, r]], against central differences of
+  fire_energy_after (the dt / alpha decisions are fixed for |h| this small)."""
+  r, T, h = 1.4, 24, 1e-6
+  R, L, model = _fire_system(r)
+  sp = O.Space(np.float64(L))
+  par = O.FireParams(dt_start=0.01, dt_max=0.1)
+  ds = np.array([[0.0, 0.5], [0.5, 1.0]])
+  E, th, X = O.fire_energy_gradient(model, sp, R, T, par, ds)
+  assert abs(E - O.fire_energy_after(model, sp, R, T, par)) <= 1e-12 * abs(E)
+
+  def E_r(rr):
+    return O.fire_energy_after(_fire_system(rr)[2], sp, R, T, par)
+
+  fd = (E_r(r + h) - E_r(r - h)) / (2 * h)
+  assert abs(th - fd) <= 1e-7 * abs(fd), (th, fd)
+  v = np.random.default_rng(7).normal(size=R.shape)
+  fdx = (O.fire_energy_after(model, sp, R + h * v, T, par)
+         - O.fire_energy_after(model, sp, R - h * v, T, par)) / (2 * h)
+  assert abs(np.sum(X * v) - fdx) <= 1e-7 * abs(fdx), (np.sum(X * v), fdx)
+
+
+def test_fire_adjoint_across_momentum_resets():
+  """With dt_start 0.1 the run passes F.P < 0 twice (momentum reset to 0);
+  the gradient still matches central differences.  The bar of 1e-7 is sharp:
+  dropping any one term of the recursion (the q term of Q, the F term or the
+  |q|/|F| term of Fb, h Q, or the reset mask) moves dE/dr by 2e-3 to 1e-2
+  relative on these systems."""
+  r, T, h = 1.4, 40, 1e-6
+  R, L, model = _fire_system(r)
+  sp = O.Space(np.float64(L))
+  par = O.FireParams(dt_start=0.1, dt_max=0.4)
+  force = lambda X: O.energy_force_brute(model, sp, X)[2]
+  st, tape = O.fire_init(R, force, par), []
+  for _ in range(T):
+    st = O.fire_step(st, force, sp, par, tape)
+  assert sum(rec['keep'] == 0.0 for rec in tape) >= 2
+  ds = np.array([[0.0, 0.5], [0.5, 1.0]])
+  _, th, _ = O.fire_energy_gradient(model, sp, R, T, par, ds)
+  fd = (O.fire_energy_after(_fire_system(r + h)[2], sp, R, T, par)
+        - O.fire_energy_after(_fire_system(r - h)[2], sp, R, T, par)) / (2 * h)
+  assert abs(th - fd) <= 1e-7 * abs(fd), (th, fd)
