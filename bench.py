@@ -110,7 +110,10 @@ def adjoint_section(jmd, O, check=True):
   if check:
     try:
       from oracle import c_oracle
-      h = 1e-4
+      # E(x_T) is C1 in the parameters (the smoothing is C1 at r_on, r_c), so
+      # central differences converge at first order: on this run h = 1e-4 is
+      # 7% off the limit, h = 1e-7 within 3e-5
+      h = 1e-7
       rng = np.random.default_rng(5)
       u = rng.normal(size=R.shape)
       u /= np.linalg.norm(u)
@@ -212,6 +215,62 @@ def batched_section(jmd, O, n_systems=512, n_steps=500, plain_sample=8, plain_st
           'energy_rel_err_vs_fp64_max': float(max(errs)) if errs else None}
 
 
+def ensemble_gradient_section(jmd, O, n_rep=64, n=8192, n_steps=300, every=25):
+  """Ensemble-gradient workload: 64 replicas of a 2-D bidisperse soft-sphere
+  packing (N = 8192 each, phi = 0.84), replica k with diameter ratio
+  r_k = 1.2 + 0.4 k / 63 (sigma_ab = [[1, (1+r)/2], [(1+r)/2, r]]), 300 FIRE
+  steps in one batched launch, then d mean_k E_k(x_T) / d r_k for every k
+  through the checkpointed FIRE adjoint (checkpoints every 25 steps).  Each
+  box is set from r_k at construction and held fixed: the gradient is taken
+  at fixed box."""
+  import numpy as np
+  import torch
+  r = [float(np.float32(1.2 + 0.4 * k / (n_rep - 1))) for k in range(n_rep)]
+  systems = [O.bidisperse_2d(n, d_large=r[k], seed=2000 + k) for k in range(n_rep)]
+  fire = dict(dt_start=0.01, dt_max=0.1)
+  dsig = np.array([[0.0, 0.5], [0.5, 1.0]], np.float32)
+
+  def make():
+    sig = np.stack([np.array([[1.0, (1 + x) / 2], [(1 + x) / 2, x]], np.float32) for x in r])
+    return jmd.ensemble.SoftSphereFireEnsemble([s[0] for s in systems], [s[2] for s in systems],
+                                               [s[1] for s in systems], sigma=sig,
+                                               **fire).allocate()
+
+  make().run(5)  # warm
+  ens = make()
+  torch.cuda.synchronize()
+  t = time.perf_counter()
+  ens.run(n_steps)
+  torch.cuda.synchronize()
+  t_fwd = time.perf_counter() - t
+  ens = make()
+  ens.energy_gradient(5, dsig, checkpoint_every=every)  # warm
+  ens = make()
+  torch.cuda.synchronize()
+  t = time.perf_counter()
+  E, th, _ = ens.energy_gradient(n_steps, dsig, checkpoint_every=every)
+  torch.cuda.synchronize()
+  t_grad = time.perf_counter() - t
+  overflow = int((ens.did_buffer_overflow != 0).sum().item())
+  th = th.cpu().numpy()
+  out = {'workload': f'{n_rep} replicas of a 2-D bidisperse soft-sphere packing (N={n} each, '
+                     f'phi 0.84), diameter ratio r_k = 1.2 + 0.4 k/{n_rep - 1}, {n_steps} FIRE '
+                     f'steps batched, d mean E(x_T) / d r_k by the checkpointed adjoint '
+                     f'(checkpoints every {every}); boxes fixed at their initial r_k',
+         'forward_s': t_fwd, 'gradient_s': t_grad, 'ratio_vs_forward': t_grad / t_fwd,
+         'mean_energy_T': float(E.mean().item()), 'systems_overflowed': overflow,
+         'dmeanE_dr': [float(x) for x in th]}
+  # FIRE's dt / alpha / reset decisions make E(x_T) piecewise smooth in r: in
+  # FP64, central differences over 300 steps reach the derivative only for
+  # h <~ 1e-7, out of float32's reach, so parity is checked on small replicas
+  # against the FP64 adjoint (itself pinned by such differences) in the tests
+  out['parity'] = ('tests/test_gpu_ensemble.py: FP64 FIRE adjoint on four replicas '
+                   '(N 64..256, 30 steps), 1e-5 relative')
+  log(f'ensemble gradient: forward {t_fwd * 1e3:.1f} ms, gradient {t_grad * 1e3:.1f} ms, '
+      f'ratio {t_grad / t_fwd:.2f}')
+  return out
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument('--gpus', type=int, default=1)
@@ -228,6 +287,8 @@ def main():
   ap.add_argument('--copy-calibration', action='store_true',
                   help='only run a plain 256 MiB device copy (the reference kernel that '
                        'scales the FETCH_SIZE / WRITE_SIZE counters) and exit')
+  ap.add_argument('--section', choices=('batched', 'adjoint', 'ensemble_gradient'),
+                  help='only run this one extra measurement and print its JSON')
   args = ap.parse_args()
   if args.gpus != 1:
     print(f'bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU '
@@ -252,6 +313,12 @@ def main():
     torch.cuda.synchronize()
     print(json.dumps({'copy_calibration_bytes_read': nbytes, 'bytes_written': nbytes,
                       'dispatches': 5}), flush=True)
+    return
+  if args.section:
+    fn = {'batched': lambda: batched_section(jmd, O),
+          'adjoint': lambda: adjoint_section(jmd, O, check=not args.no_cpu),
+          'ensemble_gradient': lambda: ensemble_gradient_section(jmd, O)}[args.section]
+    print(json.dumps({args.section: fn()}), flush=True)
     return
   if args.tpa:
     nat.check(nat.load().jmd_set_threads_per_atom(args.tpa), 'jmd_set_threads_per_atom')
@@ -376,7 +443,8 @@ def main():
   }
   if not args.no_extra:
     for key, fn in (('batched', lambda: batched_section(jmd, O)),
-                    ('adjoint', lambda: adjoint_section(jmd, O, check=not args.no_cpu))):
+                    ('adjoint', lambda: adjoint_section(jmd, O, check=not args.no_cpu)),
+                    ('ensemble_gradient', lambda: ensemble_gradient_section(jmd, O))):
       try:
         out[key] = fn()
       except Exception as e:  # reported, never silently dropped

@@ -21,7 +21,7 @@
 namespace jmd {
 
 constexpr int kEnsThreads = 512;
-constexpr int kEnsMaxCells = 4096;
+constexpr int kEnsMaxCells = 8192;
 
 struct EnsSys {
   int offset, n;
@@ -101,12 +101,14 @@ __device__ __forceinline__ int wg_max(int v, int* s) {
 
 // Bins the system on grid g: LDS counts, exclusive starts, arrival ranks,
 // scatter into sorted[] (system-local), then each cell's ids in ascending
-// order (the stable argsort of partition.py:432).  Returns the max filling.
+// order (the stable argsort of partition.py:432).  s_cs holds the counts,
+// then, scanned in place, the starts (cell c spans s_cs[c] .. s_cs[c + 1]).
+// Returns the max filling.
 __device__ int ens_bin(const EnsSys& S, const Grid& g, const float* __restrict__ x,
                        int* __restrict__ cellp, int* __restrict__ rankp,
-                       int* __restrict__ sorted, int* s_cnt, int* s_start, int* s_red) {
+                       int* __restrict__ sorted, int* s_cs, int* s_red) {
   const int nt = blockDim.x, t = threadIdx.x;
-  for (int c = t; c < g.ncell; c += nt) s_cnt[c] = 0;
+  for (int c = t; c < g.ncell; c += nt) s_cs[c] = 0;
   __syncthreads();
   for (int a = t; a < S.n; a += nt) {
     float xi[2];
@@ -114,7 +116,7 @@ __device__ int ens_bin(const EnsSys& S, const Grid& g, const float* __restrict__
     int ci[2];
     const int h = cell_of<2>(xi, g, ci);
     cellp[a] = h;
-    rankp[a] = atomicAdd(&s_cnt[h], 1);
+    rankp[a] = atomicAdd(&s_cs[h], 1);
   }
   __syncthreads();
   // exclusive scan of the counts: per-thread chunks, a wavefront scan of the
@@ -123,8 +125,8 @@ __device__ int ens_bin(const EnsSys& S, const Grid& g, const float* __restrict__
   const int beg = min(t * per, g.ncell), end = min(beg + per, g.ncell);
   int local = 0, fill = 0;
   for (int c = beg; c < end; ++c) {
-    local += s_cnt[c];
-    fill = max(fill, s_cnt[c]);
+    local += s_cs[c];
+    fill = max(fill, s_cs[c]);
   }
   const int lane = lane_id(), wv = t / kWave, nw = nt / kWave;
   int incl = local;
@@ -148,17 +150,18 @@ __device__ int ens_bin(const EnsSys& S, const Grid& g, const float* __restrict__
   __syncthreads();
   int run = s_wave[wv] + incl - local;
   for (int c = beg; c < end; ++c) {
-    s_start[c] = run;
-    run += s_cnt[c];
+    const int cnt = s_cs[c];  // this thread's own chunk: read, then overwrite
+    s_cs[c] = run;
+    run += cnt;
   }
-  if (t == 0) s_start[g.ncell] = s_wave[nw];
+  if (t == 0) s_cs[g.ncell] = s_wave[nw];
   __syncthreads();
-  for (int a = t; a < S.n; a += nt) sorted[s_start[cellp[a]] + rankp[a]] = a;
+  for (int a = t; a < S.n; a += nt) sorted[s_cs[cellp[a]] + rankp[a]] = a;
   __syncthreads();
   // order each cell by id (thread per cell, insertion sort of a few ids)
   for (int c = t; c < g.ncell; c += nt) {
-    int* m = sorted + s_start[c];
-    const int cnt = s_cnt[c];
+    int* m = sorted + s_cs[c];
+    const int cnt = s_cs[c + 1] - s_cs[c];
     for (int u = 1; u < cnt; ++u) {
       const int v = m[u];
       int w = u - 1;
@@ -177,7 +180,7 @@ __device__ int ens_bin(const EnsSys& S, const Grid& g, const float* __restrict__
 // candidate order; returns the largest row count.
 __device__ int ens_rows(const EnsSys& S, const Grid& g, const Space& sp, float cutoff_sq,
                         const float* __restrict__ x, const int* __restrict__ cellp,
-                        const int* __restrict__ sorted, const int* s_cnt, const int* s_start,
+                        const int* __restrict__ sorted, const int* s_cs,
                         int* __restrict__ idx, int width, int pad, int* s_red) {
   const int nt = blockDim.x, t = threadIdx.x;
   int occ = 0;
@@ -191,8 +194,8 @@ __device__ int ens_rows(const EnsSys& S, const Grid& g, const Space& sp, float c
     for (int st = 0; st < 9; ++st) {  // iota order, x offset slowest
       const int ox = st / 3 - 1, oy = st % 3 - 1;
       const int c = floor_mod_int(hx + ox, g.cps[0]) + g.cps[0] * floor_mod_int(hy + oy, g.cps[1]);
-      const int start = s_start[c];
-      const int m = min(s_cnt[c], g.cap);
+      const int start = s_cs[c];
+      const int m = min(s_cs[c + 1] - start, g.cap);
       const int w = g.cap - (start % g.cap);
       for (int q = 0; q < m; ++q) {
         int k = q;  // slot order: rank (start + k) mod cap
@@ -264,8 +267,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     float* __restrict__ fire_state, int* __restrict__ fire_npos, uint8_t* __restrict__ error,
     int* __restrict__ out_stat, double* __restrict__ energy, int* __restrict__ builds,
     EnsTape tape) {
-  __shared__ int s_cnt[kEnsMaxCells];
-  __shared__ int s_start[kEnsMaxCells + 1];
+  __shared__ int s_cs[kEnsMaxCells + 1];
   __shared__ int s_red[kEnsThreads / kWave];
   __shared__ double s_d[3 * (kEnsThreads / kWave)];
   __shared__ float s_scal[4];  // dt, alpha, ratio, keep
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     g2.cell_size[0] = S.fill_cell_size[0];
     g2.cell_size[1] = S.fill_cell_size[1];
     g2.ncell = g2.cps[0] * g2.cps[1];
-    const int fill = ens_bin(S, g2, x, cellp, rankp, sorted, s_cnt, s_start, s_red);
+    const int fill = ens_bin(S, g2, x, cellp, rankp, sorted, s_cs, s_red);
     if (t == 0) out_stat[sid] = fill;
     return;
   }
@@ -316,8 +318,8 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   int nbuilds = 0;
   // list build at the current positions (allocate / update)
   auto rebuild = [&](bool store) {
-    const int fill = ens_bin(S, g, x, cellp, rankp, sorted, s_cnt, s_start, s_red);
-    const int occ = ens_rows(S, g, sp, cutoff_sq, x, cellp, sorted, s_cnt, s_start, idx,
+    const int fill = ens_bin(S, g, x, cellp, rankp, sorted, s_cs, s_red);
+    const int occ = ens_rows(S, g, sp, cutoff_sq, x, cellp, sorted, s_cs, idx,
                              store ? width : 0, pad, s_red);
     if (fill > g.cap) err |= JMD_CELL_LIST_OVERFLOW;
     if (occ > S.max_occ) err |= JMD_NEIGHBOR_LIST_OVERFLOW;

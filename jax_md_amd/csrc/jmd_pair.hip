@@ -26,7 +26,8 @@ namespace jmd {
 // difference x_j - x_i is already the minimum image and no wrap is needed
 // (pairs whose plain difference is not the minimum image are then beyond
 // range and contribute zero either way).
-template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, bool ENERGY, bool INTERIOR>
+template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, bool ENERGY, bool INTERIOR,
+          int SMOOTH>
 __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned n,
                                         const Space& s, const PairArgs& a,
                                         const int* __restrict__ row, int width, int sub,
@@ -40,60 +41,73 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
 #define JMD_PAIR_UNROLL 4
 #endif
   constexpr int UNR = JMD_PAIR_UNROLL;
-  if (width <= 0) return;
-  // Entries are read UNR at a time, and the next block's entries are loaded
-  // before this block's pair terms are computed (software pipeline), so a
-  // block costs one dependent memory round trip (the position gather).
-  // Loads are clamped to the row (always in bounds); entries past the row
-  // end count as padding.
-  unsigned nxt[UNR];
+  // One pair term; padding entries (j = n) read particle i and add nothing.
+  auto term = [&](unsigned jraw, const float* xj) {
+    const bool ok = jraw < n;
+    float d[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int u = 0; u < UNR; ++u) nxt[u] = (unsigned)row[min(sub + u * TPA, width - 1)];
-  for (int k0 = sub; k0 < width; k0 += UNR * TPA) {
-    unsigned jj[UNR];
-    unsigned jmin = n;
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      jj[u] = k0 + u * TPA < width ? nxt[u] : n;
-      jmin = min(jmin, jj[u]);
+    for (int c = 0; c < DIM; ++c)
+      d[c] = (INTERIOR || !s.periodic) ? xj[c] - xi[c]
+                                       : min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c]);
+    float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                        : fmaf(d[1], d[1], d[0] * d[0]);
+    if (GENERAL) {
+      unsigned j = ok ? jraw : i;
+      int sj = a.species ? a.species[j] : 0;
+      sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
+      eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
+      if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
     }
-    // rows are left-compacted with padding n: the block is past the row end
-    // iff its first entry is padding, iff all its entries are
-    if (jmin >= n) break;
-    float xj[UNR][3];
+    float dEr;
+    float uu = pair_eval<KIND, SMOOTH>(r2, sig, eps, alp, a, &dEr);
+    dEr = ok ? dEr : 0.0f;
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      xj[u][0] = xj[u][1] = xj[u][2] = 0.0f;
-      load_pos<DIM, STRIDE>(pos, jj[u] < n ? jj[u] : i, xj[u]);
-    }
+    for (int c = 0; c < DIM; ++c) f[c] = fmaf(dEr, d[c], f[c]);
+    if (ENERGY) *e += ok ? uu : 0.0f;
+  };
+  // This lane's entries are sub, sub + TPA, ... < width: nfull blocks of UNR
+  // entries, then fewer than UNR in a tail.  In the blocks every entry lies
+  // in the row, so they are read through one pointer with constant offsets;
+  // the next block's entries are loaded before this block's terms are
+  // computed (software pipeline: one dependent round trip per block, the
+  // position gather).  Rows are left-compacted with padding n, so the row
+  // ends at the first block whose entries are all padding.
+  const int nent = width > sub ? (width - sub + TPA - 1) / TPA : 0;
+  const int nfull = nent / UNR;
+  const int* p = row + sub;
+  if (nfull > 0) {
+    unsigned nxt[UNR];
 #pragma unroll
-    for (int u = 0; u < UNR; ++u)
-      nxt[u] = (unsigned)row[min(k0 + (UNR + u) * TPA, width - 1)];
+    for (int u = 0; u < UNR; ++u) nxt[u] = (unsigned)p[u * TPA];
+    for (int b = 0; b < nfull; ++b) {
+      unsigned jj[UNR];
+      unsigned jmin = n;
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      float d[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int c = 0; c < DIM; ++c)
-        d[c] = (INTERIOR || !s.periodic)
-                   ? xj[u][c] - xi[c]
-                   : min_image_disp(xj[u][c], xi[c], xim[c], s.L[c], s.half[c]);
-      float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
-                          : fmaf(d[1], d[1], d[0] * d[0]);
-      if (GENERAL) {
-        unsigned j = jj[u] < n ? jj[u] : i;
-        int sj = a.species ? a.species[j] : 0;
-        sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
-        eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
-        if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
+      for (int u = 0; u < UNR; ++u) {
+        jj[u] = nxt[u];
+        jmin = min(jmin, jj[u]);
       }
-      float dEr;
-      float uu = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr);
-      const bool ok = jj[u] < n;
-      dEr = ok ? dEr : 0.0f;
+      if (jmin >= n) return;
+      float xj[UNR][3];
 #pragma unroll
-      for (int c = 0; c < DIM; ++c) f[c] = fmaf(dEr, d[c], f[c]);
-      if (ENERGY) *e += ok ? uu : 0.0f;
+      for (int u = 0; u < UNR; ++u) {
+        xj[u][0] = xj[u][1] = xj[u][2] = 0.0f;
+        load_pos<DIM, STRIDE>(pos, jj[u] < n ? jj[u] : i, xj[u]);
+      }
+      // next block, clamped to the last full one (re-read, unused, at the end)
+      const int* q = p + min(b + 1, nfull - 1) * (UNR * TPA);
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) nxt[u] = (unsigned)q[u * TPA];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) term(jj[u], xj[u]);
     }
+  }
+  for (int t = nfull * UNR; t < nent; ++t) {
+    const unsigned j = (unsigned)p[t * TPA];
+    if (j >= n) return;
+    float xj[3] = {0.0f, 0.0f, 0.0f};
+    load_pos<DIM, STRIDE>(pos, j, xj);
+    term(j, xj);
   }
 }
 
@@ -124,12 +138,22 @@ __global__ __launch_bounds__(256) void k_pair_dense(
   const bool wave_interior = __all(interior || !active);
   if (active) {
     const int* row = idx + (long long)i * width;
-    if (wave_interior)
-      row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, true>(pos, n, s, a, row, width, sub, i,
-                                                             xi, xim, f, &e);
-    else
-      row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false>(pos, n, s, a, row, width, sub, i,
-                                                              xi, xim, f, &e);
+    // the smoothing switch is uniform: taken once here, not per pair
+    if (KIND == JMD_LENNARD_JONES && a.smooth) {
+      if (wave_interior)
+        row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, true, 1>(pos, n, s, a, row, width, sub,
+                                                                  i, xi, xim, f, &e);
+      else
+        row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false, 1>(pos, n, s, a, row, width, sub,
+                                                                   i, xi, xim, f, &e);
+    } else {
+      if (wave_interior)
+        row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, true, 0>(pos, n, s, a, row, width, sub,
+                                                                  i, xi, xim, f, &e);
+      else
+        row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false, 0>(pos, n, s, a, row, width, sub,
+                                                                   i, xi, xim, f, &e);
+    }
   }
   double fd[3];
 #pragma unroll

@@ -48,7 +48,7 @@ FIRE = dict(dt_start=0.01, dt_max=0.1)
 
 def test_ensemble_allocate_matches_plain_lists_and_forces():
   jmd = _jmd()
-  sizes = [64, 300, 97, 1024, 150]
+  sizes = [64, 300, 97, 1024, 150, 8192]  # 8192: 4356 cells
   syst = _systems(sizes, seed=3)
   print('case ensemble allocate', flush=True)
   ens = jmd.ensemble.SoftSphereFireEnsemble([s[0] for s in syst], [s[2] for s in syst],
