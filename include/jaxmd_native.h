@@ -377,6 +377,33 @@ int jmd_nve_kick_drift(const double* position_hi, const float* momentum,
                        const float* reference_position, float threshold_sq,
                        int32_t* gate, void* stream);
 
+/* One step's neighbour-list update fused with the opening half of the next
+ * velocity-Verlet step: the loop body
+ *   nbrs = nbrs.update(R); state = apply_fn(state, neighbor=nbrs)
+ * of the reference's NVE driver (examples/nve_neighbor_list.py:186-201,
+ * partition.py:1141-1164, simulate.py:227-243) up to the force evaluation.
+ * The list part is jmd_neighbor_list_build with `gate` (rebuild only when the
+ * gate is raised; the gate is cleared by the rebuild).  In the same pass over
+ * the particles as the binning, p += half_dt F and R' = shift(R, dt p / m) as
+ * in jmd_nve_kick_drift (position_hi and momentum updated in place, shift in
+ * the space given by shift_periodic / shift_side), with R' written to
+ * position_out, a second [n, dim] buffer of the same layout that must not
+ * alias position (the list stages still read R), and the skin criterion of
+ * R' against the list's reference positions (R itself after a rebuild)
+ * raising gate_next (a second word, distinct from gate).  The caller then
+ * evaluates the force at position_out.  No counterpart in the reference as a
+ * single call: it saves one kernel launch per step. */
+int jmd_nve_list_update_kick_drift(
+    const float* position, int64_t n, int32_t dim, int32_t position_stride,
+    int32_t periodic, const float* side, float cutoff_sq, int32_t use_cell_list,
+    const int32_t* cells_per_side, const float* cell_size, int32_t cell_capacity,
+    int32_t format, int32_t mask_self, int32_t* idx, int64_t max_occupancy,
+    int32_t* occupancy, uint8_t* error, float* reference_position, int32_t* gate,
+    void* workspace, double* position_hi, float* momentum, const float* force,
+    const float* mass, float mass_scalar, int32_t shift_periodic,
+    const float* shift_side, float dt, float half_dt, const float* dt_device,
+    float* position_out, float threshold_sq, int32_t* gate_next, void* stream);
+
 /* FIRE bookkeeping after the NVE step of minimize.fire_descent.apply_fn
  * (jax_md/minimize.py:184-224).  scalars (device float64 [3]) receives
  * sum F^2, sum P^2 and sum F.P (float64 sums).  Then the one-block update

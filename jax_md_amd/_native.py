@@ -67,6 +67,12 @@ _SIGNATURES = {
                                      c_int32, c_int32, POINTER(c_float), c_float, c_float,
                                      c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
                                      c_void_p, c_float, c_void_p, c_void_p]),
+    'jmd_nve_list_update_kick_drift': (
+        c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32, POINTER(c_float), c_float,
+                  c_int32, POINTER(c_int32), POINTER(c_float), c_int32, c_int32, c_int32,
+                  c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                  c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, POINTER(c_float),
+                  c_float, c_float, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
     'jmd_cell_order': (c_int32, [c_void_p, c_int64, c_int32, c_int32, POINTER(c_int32),
                                  POINTER(c_float), c_void_p, c_void_p, c_void_p, c_void_p]),
     'jmd_permute_rows': (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_int32,

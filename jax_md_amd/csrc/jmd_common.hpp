@@ -114,6 +114,45 @@ __device__ __forceinline__ double ref_mod64(double x, double L) {
   return r;
 }
 
+// The opening half of simulate.velocity_verlet for one particle
+// (simulate.py:168-188,227-239): p' = p + half_dt F, R' = mod(R + dt p' / m, L)
+// with R accumulated in float64; writes p', R' (float64) and returns the
+// float32 copy of R' in xn.
+struct KickDrift {
+  int active;
+  const double* xhi_in;
+  double* xhi_out;
+  const float* p_in;
+  float* p_out;
+  const float* F;
+  const float* mass;  // NULL: mass_scalar
+  float mass_scalar;
+  float dt, half_dt;
+  const float* dt_dev;  // device dt overriding dt (half_dt = f32(dt / 2)), or NULL
+  float* x_out;         // float32 R', row layout of the kernel's positions
+  const float* ref;     // [n, dim] reference positions of the list
+  float thr_sq;         // skin criterion (squared)
+  int* gate_next;       // raised when any particle moved beyond the skin
+  Space s;
+};
+
+template <int DIM>
+__device__ __forceinline__ void kick_drift_atom(const KickDrift& kd, long long i, float dt,
+                                                float half_dt, float* xn) {
+#pragma clang fp contract(off)
+  const double m = kd.mass ? (double)kd.mass[i] : (double)kd.mass_scalar;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const long long k = i * DIM + d;
+    const float pk = kd.p_in[k] + half_dt * kd.F[k];
+    double x = kd.xhi_in[k] + (double)dt * (double)pk / m;
+    if (kd.s.periodic) x = ref_mod64(x, (double)kd.s.L[d]);
+    kd.p_out[k] = pk;
+    kd.xhi_out[k] = x;
+    xn[d] = (float)x;
+  }
+}
+
 __device__ __forceinline__ int floor_mod_int(int a, int m) {
   int r = a % m;
   return r < 0 ? r + m : r;

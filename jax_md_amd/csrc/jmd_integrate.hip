@@ -17,41 +17,27 @@ struct FireParamsDev {
 };
 
 template <int DIM, int STRIDE>
-__global__ __launch_bounds__(256) void k_kick_drift(
-    const double* xhi_in, const float* p_in, const float* __restrict__ F,
-    const float* __restrict__ mass, float mass_scalar, int n, Space s, float dt,
-    float half_dt, const float* __restrict__ dt_dev, double* xhi_out,
-    float* __restrict__ x32, float* p_out, const float* __restrict__ xref, float thr_sq,
-    int* __restrict__ gate) {
-  if (dt_dev) {
-    dt = dt_dev[0];
+__global__ __launch_bounds__(256) void k_kick_drift(KickDrift kd, int n) {
+  float dt = kd.dt, half_dt = kd.half_dt;
+  if (kd.dt_dev) {
+    dt = kd.dt_dev[0];
     half_dt = dt / 2.0f;  // f32(dt / 2) of velocity_verlet (simulate.py:236)
   }
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool moved = false;
   if (i < n) {
-    double m = mass ? (double)mass[i] : (double)mass_scalar;
     float xn[3];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      long long k = (long long)i * DIM + d;
-      float pk = p_in[k] + half_dt * F[k];  // momentum_step (simulate.py:168-173)
-      double x = xhi_in[k] + (double)dt * (double)pk / m;  // position_step (:176-188)
-      if (s.periodic) x = ref_mod64(x, (double)s.L[d]);
-      p_out[k] = pk;
-      xhi_out[k] = x;
-      xn[d] = (float)x;
-    }
-    store_pos<DIM, STRIDE>(x32, i, xn);
-    if (gate && xref) {
+    kick_drift_atom<DIM>(kd, i, dt, half_dt, xn);
+    store_pos<DIM, STRIDE>(kd.x_out, i, xn);
+    if (kd.gate_next) {
       float r[3];
-      load_pos<DIM>(xref, i, r);
-      moved = ref_metric_sq<DIM>(xn, r, s) > thr_sq;
+      load_pos<DIM>(kd.ref, i, r);
+      moved = ref_metric_sq<DIM>(xn, r, kd.s) > kd.thr_sq;
     }
   }
-  if (gate) {
-    unsigned long long bal = __ballot(moved);
-    if (bal != 0ull && lane_id() == 0) atomicOr(gate, 1);
+  if (kd.gate_next) {
+    const unsigned long long bal = __ballot(moved);
+    if (bal != 0ull && lane_id() == 0) atomicOr(kd.gate_next, 1);
   }
 }
 
@@ -155,13 +141,25 @@ int jmd_nve_kick_drift(const double* position_hi, const float* momentum, const f
   Space s = make_space(dim, periodic, side);
   hipStream_t st = (hipStream_t)stream;
   int nn = (int)n;
-  const float* xref = gate ? reference_position : nullptr;
-  int* g = xref ? gate : nullptr;
+  KickDrift kd{};
+  kd.active = 1;
+  kd.xhi_in = position_hi;
+  kd.xhi_out = position_hi_out;
+  kd.p_in = momentum;
+  kd.p_out = momentum_out;
+  kd.F = force;
+  kd.mass = mass;
+  kd.mass_scalar = mass_scalar;
+  kd.dt = dt;
+  kd.half_dt = half_dt;
+  kd.dt_dev = dt_device;
+  kd.x_out = position_out;
+  kd.ref = gate ? reference_position : nullptr;
+  kd.thr_sq = threshold_sq;
+  kd.gate_next = kd.ref ? gate : nullptr;
+  kd.s = s;
   dim3 grid(ceil_div(nn, 256)), block(256);
-#define JMD_KD(D, S)                                                                        \
-  hipLaunchKernelGGL((k_kick_drift<D, S>), grid, block, 0, st, position_hi, momentum, force, \
-                     mass, mass_scalar, nn, s, dt, half_dt, dt_device, position_hi_out,     \
-                     position_out, momentum_out, xref, threshold_sq, g)
+#define JMD_KD(D, S) hipLaunchKernelGGL((k_kick_drift<D, S>), grid, block, 0, st, kd, nn)
   if (dim == 3 && position_stride == 4) JMD_KD(3, 4);
   else if (dim == 3) JMD_KD(3, 3);
   else JMD_KD(2, 2);

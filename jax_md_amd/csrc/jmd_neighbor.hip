@@ -52,7 +52,14 @@ struct Work {
   int* misc;         // [M_WORDS] (zero at rest except as documented)
 };
 
-constexpr int kCellBlocks = 2048;  // persistent grid of the cell build
+#ifndef JMD_CELL_BLOCKS
+#define JMD_CELL_BLOCKS 1024
+#endif
+#ifndef JMD_SORT_BLOCKS
+#define JMD_SORT_BLOCKS 1024
+#endif
+constexpr int kCellBlocks = JMD_CELL_BLOCKS;  // persistent grid of the cell build
+constexpr int kSortBlocks = JMD_SORT_BLOCKS;  // grid cap of the cell sort
 
 static inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -371,6 +378,7 @@ struct BuildArgs {
   uint8_t* error;
   int* gate;
   Work w;
+  KickDrift kd;  // fused opening half-step (k_bin_slots), inactive by default
 };
 
 // Candidates are staged as float4 (x, y, z, id bits) holding the periodic
@@ -481,12 +489,17 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
         for (int a0 = wave; a0 < gcount; a0 += kHomePass * nwave) {
           float xi[kHomePass][3];
           int ih[kHomePass], cnt[kHomePass];
+          // allowed candidates: j < lim and j != ex (OrderedSparse keeps
+          // j < i only; mask_self drops j = i)
+          int lim[kHomePass], ex[kHomePass];
           long long base[kHomePass];
 #pragma unroll
           for (int h = 0; h < kHomePass; ++h) {
             const int a = a0 + h * nwave;
             xi[h][0] = xi[h][1] = xi[h][2] = 0.0f;
             ih[h] = -1;
+            lim[h] = 0;
+            ex[h] = -1;
             cnt[h] = 0;
             base[h] = 0;
             if (a < gcount) {
@@ -496,19 +509,23 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
               xi[h][1] = xp.y;
               xi[h][2] = xp.z;
               ih[h] = A.w.sorted_id[hi];
+              lim[h] = A.ordered ? ih[h] : 0x7fffffff;
+              ex[h] = A.mask_self ? ih[h] : -1;
               cnt[h] = s_cnt[a];
               if (MODE == MODE_DENSE) base[h] = (long long)ih[h] * width;
               if (MODE == MODE_SPARSE_WRITE) base[h] = A.w.row_offset[ih[h]];
             }
           }
+          // homes of this pass that exist (wavefront-uniform, a scalar bound)
+          const int nh = min(kHomePass, (gcount - a0 + nwave - 1) / nwave);
           for (int q0 = 0; q0 < cpad; q0 += kWave) {
             const float4 c = s_pos[q0 + lane];
+            // staged padding sits 3e18 away: its d2 fails both tests below
             const int j = __float_as_int(c.w);
             const float cj[3] = {c.x, c.y, c.z};
-            const bool jok = j < n;
 #pragma unroll
             for (int h = 0; h < kHomePass; ++h) {
-              if (ih[h] < 0) continue;  // wavefront-uniform
+              if (h >= nh) break;
               const int i = ih[h];
               float d[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -518,23 +535,30 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
                               : xi[h][e] - cj[e];
               const float d2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
                                         : fmaf(d[1], d[1], d[0] * d[0]);
-              const bool ok = jok && (!A.mask_self || j != i) && (!A.ordered || j < i);
-              bool pass = ok && d2 < lo_thr;
-              const bool near = ok && d2 >= lo_thr && d2 <= hi_thr;
-              if (__any(near)) {  // rare: reference-exact test on original positions
-                if (near) {
+              const bool okj = j < lim[h] && j != ex[h];
+              const bool fast = d2 < lo_thr;
+              // within the margin of the cutoff: too close to call from the
+              // canonicalised positions
+              const bool near = !fast && d2 <= hi_thr;
+              bool exact = false;
+              if (__builtin_amdgcn_ballot_w64(near)) {  // rare: reference-exact test
+                // (an infinite margin also admits the staged padding, j = n)
+                if (near && okj && j < n) {
                   float xo[3] = {0.0f, 0.0f, 0.0f}, xj[3] = {0.0f, 0.0f, 0.0f};
                   load_pos<DIM, STRIDE>(A.pos, i, xo);
                   load_pos<DIM, STRIDE>(A.pos, j, xj);
-                  pass = pair_test<DIM>(xo, xj, s, A.cutoff_sq, 0.0f, A.test);
+                  exact = pair_test<DIM>(xo, xj, s, A.cutoff_sq, 0.0f, A.test);
                 }
               }
+              const bool pass = (fast && okj) || exact;
               const unsigned long long bal = __ballot(pass);
               if ((MODE == MODE_DENSE || MODE == MODE_SPARSE_WRITE) && pass) {
-                const long long slot = cnt[h] + wave_prefix_count(bal);
                 if (MODE == MODE_DENSE) {
-                  if (slot < width) A.idx[base[h] + slot] = j;
+                  // row slots stay below width < 2^31: 32-bit unsigned offsets
+                  const unsigned slot = (unsigned)cnt[h] + (unsigned)wave_prefix_count(bal);
+                  if (slot < (unsigned)width) A.idx[base[h] + slot] = j;
                 } else {
+                  const long long slot = cnt[h] + wave_prefix_count(bal);
                   const long long at = base[h] + slot;
                   if (at < width) {
                     A.idx[at] = j;          // receiver
@@ -622,8 +646,21 @@ __global__ __launch_bounds__(256) void k_bin_slots(const float* __restrict__ pos
                                                    int* __restrict__ cell_start,
                                                    int* __restrict__ slots,
                                                    float* __restrict__ ref_out,
-                                                   int* __restrict__ misc, const int* gate) {
-  if (gate_closed(gate)) return;
+                                                   int* __restrict__ misc, const int* gate,
+                                                   KickDrift kd) {
+  // kd.active: every step also applies the opening half of the next
+  // velocity-Verlet step to each particle it reads, R' into kd.x_out (a
+  // second position buffer, so the binning and the later list stages still
+  // read R), and raises kd.gate_next when R' moved beyond the skin from the
+  // list's reference positions (R itself when this call rebuilds).
+  const bool open = !gate_closed(gate);
+  if (!open && !kd.active) return;
+  float kdt = kd.dt, khalf = kd.half_dt;
+  if (kd.active && kd.dt_dev) {
+    kdt = kd.dt_dev[0];
+    khalf = kdt / 2.0f;
+  }
+  bool moved = false;
   const int lane = lane_id();
   for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const int i = base + threadIdx.x;
@@ -632,10 +669,27 @@ __global__ __launch_bounds__(256) void k_bin_slots(const float* __restrict__ pos
     if (valid) {
       float x[3];
       load_pos<DIM, STRIDE>(pos, i, x);
-      int ci[3];
-      h = cell_of<DIM>(x, g, ci);
-      if (ref_out) store_pos<DIM>(ref_out, i, x);
+      if (open) {
+        int ci[3];
+        h = cell_of<DIM>(x, g, ci);
+        if (ref_out) store_pos<DIM>(ref_out, i, x);
+      }
+      if (kd.active) {
+        float xn[3], r[3];
+        kick_drift_atom<DIM>(kd, i, kdt, khalf, xn);
+        store_pos<DIM, STRIDE>(kd.x_out, i, xn);
+        if (kd.gate_next) {
+          if (open) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) r[d] = x[d];
+          } else {
+            load_pos<DIM>(kd.ref, i, r);
+          }
+          moved = moved || ref_metric_sq<DIM>(xn, r, kd.s) > kd.thr_sq;
+        }
+      }
     }
+    if (!open) continue;  // wavefront-uniform
     // one atomic per distinct cell in the wavefront (few when particles are
     // stored in cell order), all issued by one instruction: first group the
     // lanes by cell (leader = lowest lane of the group), then the leaders add
@@ -661,6 +715,11 @@ __global__ __launch_bounds__(256) void k_bin_slots(const float* __restrict__ pos
       if (rank < g.cap) slots[(long long)h * g.cap + rank] = i;
     }
   }
+  if (kd.active && kd.gate_next) {
+    const unsigned long long bal = __ballot(moved);
+    if (bal != 0ull && lane == 0) atomicOr(kd.gate_next, 1);
+  }
+  if (!open) return;
   // every count was added by the device-scope atomics above
   if (!last_workgroup(&misc[M_TICKET_BIN], 0)) return;
   if (threadIdx.x == 0) misc[M_TICKET_BIN] = 0;
@@ -986,7 +1045,10 @@ static int cells_dispatch(const BuildArgs& A, int mode, int finish, hipStream_t 
   return JMD_OK;
 }
 
-constexpr int kBinBlocks = 1024;
+#ifndef JMD_BIN_BLOCKS
+#define JMD_BIN_BLOCKS 512
+#endif
+constexpr int kBinBlocks = JMD_BIN_BLOCKS;
 
 // Cell-list build: k_bin_slots, k_cell_sort, k_nl_cells (Sparse adds the
 // row scan, the padding and a second k_nl_cells pass).
@@ -996,9 +1058,11 @@ static int build_cells(const BuildArgs& A, bool sparse, bool count_only, hipStre
   int nb = ceil_div(A.n > 0 ? A.n : 1, 256);
   if (nb > kBinBlocks) nb = kBinBlocks;
   hipLaunchKernelGGL((k_bin_slots<DIM, STRIDE>), dim3(nb), dim3(256), 0, st, A.pos, A.n, A.g,
-                     w.cell_count, w.cell_start, w.slots, A.ref_out, w.misc, A.gate);
+                     w.cell_count, w.cell_start, w.slots, A.ref_out, w.misc, A.gate, A.kd);
   JMD_LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_cell_sort<DIM, STRIDE>), dim3(ceil_div((long long)A.g.ncell * kWave, 256)),
+  long long sb = ceil_div((long long)A.g.ncell * kWave, 256);
+  if (sb > kSortBlocks) sb = kSortBlocks;
+  hipLaunchKernelGGL((k_cell_sort<DIM, STRIDE>), dim3((unsigned)sb),
                      dim3(256), 0, st, A.pos, A.g, A.s, w.cell_start, w.slots, w.sorted_id,
                      w.sorted_pos, w.misc, A.gate);
   JMD_LAUNCH_CHECK();
@@ -1027,12 +1091,14 @@ template <int DIM, int STRIDE>
 static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
                       int use_cells, const Grid& g, int format, int mask_self,
                       int* idx, long long width, int* occupancy, uint8_t* error,
-                      float* ref_out, int* gate, const Work& w, hipStream_t st) {
+                      float* ref_out, int* gate, const Work& w, hipStream_t st,
+                      const KickDrift* kd = nullptr) {
   const bool sparse = format != JMD_DENSE;
   const int ordered = format == JMD_ORDERED_SPARSE;
   const bool count_only = error == nullptr;
   if (use_cells) {
-    BuildArgs A;
+    BuildArgs A{};
+    if (kd) A.kd = *kd;
     A.pos = pos;
     A.n = n;
     A.g = g;
@@ -1255,6 +1321,80 @@ int jmd_neighbor_list_build(const float* position, int64_t n, int32_t dim,
                         max_occupancy, occupancy, error, reference_position, gate, w, st)),
       (build_impl<2, 2>(position, nn, s, cutoff_sq, use_cell_list, g, format, mask_self, idx,
                         max_occupancy, occupancy, error, reference_position, gate, w, st)));
+}
+
+int jmd_nve_list_update_kick_drift(
+    const float* position, int64_t n, int32_t dim, int32_t position_stride, int32_t periodic,
+    const float* side, float cutoff_sq, int32_t use_cell_list, const int32_t* cells_per_side,
+    const float* cell_size, int32_t cell_capacity, int32_t format, int32_t mask_self,
+    int32_t* idx, int64_t max_occupancy, int32_t* occupancy, uint8_t* error,
+    float* reference_position, int32_t* gate, void* workspace, double* position_hi,
+    float* momentum, const float* force, const float* mass, float mass_scalar,
+    int32_t shift_periodic, const float* shift_side, float dt, float half_dt,
+    const float* dt_device, float* position_out, float threshold_sq, int32_t* gate_next,
+    void* stream) {
+  if (n <= 0) return n < 0 ? JMD_ERR_INVALID : JMD_OK;
+  if (!position_hi || !momentum || !force || !position_out || position_out == position ||
+      !error || !gate || !gate_next || gate_next == gate || !reference_position ||
+      (shift_periodic && !shift_side) || (!mass && !(mass_scalar > 0.0f)))
+    return JMD_ERR_INVALID;
+  if (!use_cell_list) {
+    // no binning pass to share: the two stages one after the other
+    int rc = jmd_neighbor_list_build(position, n, dim, position_stride, periodic, side,
+                                     cutoff_sq, use_cell_list, cells_per_side, cell_size,
+                                     cell_capacity, format, mask_self, idx, max_occupancy,
+                                     occupancy, error, reference_position, gate, workspace,
+                                     stream);
+    if (rc) return rc;
+    return jmd_nve_kick_drift(position_hi, momentum, force, mass, mass_scalar, n, dim,
+                              shift_periodic, shift_side, dt, half_dt, dt_device, position_hi,
+                              position_out, position_stride, momentum, reference_position,
+                              threshold_sq, gate_next, stream);
+  }
+  if ((dim != 2 && dim != 3) || n > 0x7fffffff || max_occupancy < 0 ||
+      (format != JMD_DENSE && format != JMD_SPARSE && format != JMD_ORDERED_SPARSE) ||
+      !workspace || (periodic && !side) || (max_occupancy > 0 && !idx))
+    return JMD_ERR_INVALID;
+  if (!stride_ok(dim, position_stride, position) ||
+      !stride_ok(dim, position_stride, position_out))
+    return JMD_ERR_INVALID;
+  if (format == JMD_DENSE && (long long)n * max_occupancy > 0x7fffffffffffLL)
+    return JMD_ERR_INVALID;
+  if (!cells_per_side || !cell_size || cell_capacity < 1) return JMD_ERR_INVALID;
+  Grid g{};
+  if (!make_grid(dim, cells_per_side, cell_size, cell_capacity, &g)) return JMD_ERR_INVALID;
+  for (int d = 0; d < dim; ++d)
+    if (cells_per_side[d] < 3) return JMD_ERR_INVALID;
+  Space s = make_space(dim, periodic, side);
+  KickDrift kd{};
+  kd.active = 1;
+  kd.xhi_in = position_hi;
+  kd.xhi_out = position_hi;
+  kd.p_in = momentum;
+  kd.p_out = momentum;
+  kd.F = force;
+  kd.mass = mass;
+  kd.mass_scalar = mass_scalar;
+  kd.dt = dt;
+  kd.half_dt = half_dt;
+  kd.dt_dev = dt_device;
+  kd.x_out = position_out;
+  kd.ref = reference_position;
+  kd.thr_sq = threshold_sq;
+  kd.gate_next = gate_next;
+  kd.s = make_space(dim, shift_periodic, shift_side);
+  Work w;
+  carve(workspace, n, g.ncell, cell_capacity, &w);
+  hipStream_t st = (hipStream_t)stream;
+  int nn = (int)n;
+  return JMD_DISPATCH_LAYOUT(
+      dim, position_stride,
+      (build_impl<3, 3>(position, nn, s, cutoff_sq, 1, g, format, mask_self, idx, max_occupancy,
+                        occupancy, error, reference_position, gate, w, st, &kd)),
+      (build_impl<3, 4>(position, nn, s, cutoff_sq, 1, g, format, mask_self, idx, max_occupancy,
+                        occupancy, error, reference_position, gate, w, st, &kd)),
+      (build_impl<2, 2>(position, nn, s, cutoff_sq, 1, g, format, mask_self, idx, max_occupancy,
+                        occupancy, error, reference_position, gate, w, st, &kd)));
 }
 
 int jmd_cell_order(const float* position, int64_t n, int32_t dim, int32_t position_stride,

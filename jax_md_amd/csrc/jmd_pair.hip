@@ -98,8 +98,18 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
       const int* q = p + min(b + 1, nfull - 1) * (UNR * TPA);
 #pragma unroll
       for (int u = 0; u < UNR; ++u) nxt[u] = (unsigned)q[u * TPA];
+      if (__builtin_amdgcn_ballot_w64(jj[UNR - 1] >= n) == 0ull) {
 #pragma unroll
-      for (int u = 0; u < UNR; ++u) term(jj[u], xj[u]);
+        for (int u = 0; u < UNR; ++u) term(jj[u], xj[u]);
+      } else {
+        // some row of the wavefront ends in this block: terms only while a
+        // lane still holds an entry (validity is monotone in u per lane)
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+          if (u > 0 && __builtin_amdgcn_ballot_w64(jj[u] < n) == 0ull) break;
+          term(jj[u], xj[u]);
+        }
+      }
     }
   }
   for (int t = nfull * UNR; t < nent; ++t) {

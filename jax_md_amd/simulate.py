@@ -161,6 +161,9 @@ class NVELoop:
   half kick.  The rebuild decisions are those of the reference's
   ``update`` before ``apply_fn``; overflow is reported in the list's error
   word and checked by the caller once per block, as the reference does.
+  With a cell list, (1) and (2) are one call (``fused``): the drift writes
+  the new positions into a second buffer while the list stages read the
+  current ones, the two buffers and two gate words alternating by step.
 
   With ``graph`` (default) a block is one HIP graph launch: the first call for
   a given set of buffers and block length runs eagerly and records the
@@ -170,13 +173,15 @@ class NVELoop:
   """
 
   def __init__(self, energy_or_force_fn, shift_fn, dt, reorder: bool = True,
-               graph: bool = True):
+               graph: bool = True, fused: bool = True):
     self.energy = quantity.native_energy_of(energy_or_force_fn)
     self.shift = native_shift(shift_fn)
     self.dt = f32(dt)
     self.half = f32(self.dt / f32(2))
     self.reorder = reorder
     self.graph = graph
+    self.fused = fused
+    self._alt = None   # second position buffer and gate word of the fused step
     self._bufs = None
     self._bufs_key = None
     self._graph = None
@@ -266,6 +271,20 @@ class NVELoop:
       self._bufs, self._bufs_key = b, key
     return self._bufs
 
+  def _alternates(self, R, gate):
+    """[R, R2] and [gate, gate2]: the second position buffer (R's layout) and
+    the second gate word (zero) of the fused step."""
+    key = (tuple(R.shape), str(R.device), nat.position_stride(R), R.data_ptr())
+    if self._alt is None or self._alt[0] != key:
+      N, dim = R.shape
+      if nat.position_stride(R) == 4:
+        R2 = nat.padded_positions(N, R.device)
+      else:
+        R2 = torch.empty_like(R)
+      g2 = torch.zeros_like(gate)
+      self._alt = (key, R2, g2)
+    return [R, self._alt[1]], [gate, self._alt[2]]
+
   def _run_reordered(self, state, nbrs, n_steps, events):
     lib = nat.load()
     plan = nbrs.plan
@@ -346,6 +365,34 @@ class NVELoop:
     width = nbrs.idx.shape[1]
     geo = self.energy.geometry
     esides = nat.f3(geo.side_values(dim))
+    if (self.fused and hook is None and gate is not None and plan.use_cell_list
+        and fmt == nat.DENSE):
+      xs, gs = self._alternates(R, plan.gate)
+      for k in range(n_steps):
+        xin, xout = xs[k % 2], xs[(k + 1) % 2]
+        nat.check(lib.jmd_nve_list_update_kick_drift(
+            nat.ptr(xin), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq), 1,
+            cps, cs, int(nbrs.cell_list_capacity or 0), fmt, int(spec.mask_self), idx,
+            int(nbrs.max_occupancy), occ, err, ref, nat.ptr(gs[k % 2]), ws, x_hi, p, F, mv,
+            mass_s if mass_v is None else 0.0, wraps, side, float(self.dt), float(self.half),
+            None, nat.ptr(xout), thr, nat.ptr(gs[(k + 1) % 2]), st),
+            'jmd_nve_list_update_kick_drift')
+        if events is not None:
+          e0 = torch.cuda.Event(enable_timing=True)
+          e0.record()
+        nat.check(lib.jmd_pair_energy_force(
+            nat.ptr(xout), N, dim, stride, int(geo.periodic), esides, nat.ctypes.byref(pot), idx,
+            width, fmt, F, None, None, p, float(self.half), None, pws, st),
+            'jmd_pair_energy_force')
+        if events is not None:
+          e1 = torch.cuda.Event(enable_timing=True)
+          e1.record()
+          events.append((e0, e1))
+      if n_steps % 2:  # the state's buffer and gate word hold step n's values
+        R.copy_(xs[1])
+        gs[0].copy_(gs[1])
+        gs[1].zero_()
+      return state
     for k in range(n_steps):
       nat.check(lib.jmd_neighbor_list_build(
           x, N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq), int(plan.use_cell_list),

@@ -169,6 +169,37 @@ def test_nve_block_loop_matches_drop_in(reorder):
   assert np.array_equal(a[:, :w], b[:, :w])
 
 
+@pytest.mark.parametrize('reorder', [False, True])
+def test_nve_fused_update_matches_separate_kernels(reorder):
+  """The fused list update + half kick + drift (alternating position buffers
+  and gate words) gives the bit-identical trajectory, lists, reference
+  positions and rebuild counts of the separate build and kick-drift kernels,
+  for odd and even block lengths."""
+  print(f'case NVELoop fused reorder={reorder}', flush=True)
+  jmd, R, P, L, shift, nfn, efn = _lj(12, seed=7)
+  Rt = torch.as_tensor(R, device=DEV)
+  Pt = torch.as_tensor(P, device=DEV)
+  init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
+  runs = []
+  for fused in (False, True):
+    nb = nfn.allocate(Rt)
+    s = init_fn(0, Rt, kT=1.0, momenta=Pt, neighbor=nb)
+    loop = jmd.simulate.NVELoop(efn, shift, 0.005, reorder=reorder, graph=False, fused=fused)
+    loop.prime(s, nb)
+    snaps = []
+    for n_steps in (15, 16, 9, 30):
+      loop.run(s, nb, n_steps)
+      snaps.append((s.position_hi.clone(), s.position.clone(), s.momentum.clone(),
+                    s.force.clone(), nb.idx.clone(), nb.reference_position.clone(),
+                    nb.plan.gate.clone(), nb.plan.builds))
+    runs.append(snaps)
+  for blk, (a, b) in enumerate(zip(*runs)):
+    for k in range(7):
+      assert torch.equal(a[k], b[k]), (reorder, blk, k)
+    assert a[7] == b[7], (reorder, blk)
+  assert runs[1][-1][7] > runs[1][0][7]  # rebuilds happened
+
+
 @pytest.mark.parametrize('reorder', [True, False])
 def test_nve_loop_graph_replays_match_eager(reorder):
   """The HIP-graph block (recorded once, replayed) gives bit-identical state,
