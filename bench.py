@@ -40,11 +40,12 @@ def log(*a):
 
 def force_kernel_us(jmd, loop, state, nbrs, efn, steps=100):
   """Average duration of the force + half-kick kernel inside the step loop:
-  one eager block of `steps` steps with HIP events around each force launch,
-  on the stream the kernels run on.  A spin kernel queued first keeps the
-  device behind the host, so no event interval contains a host gap; the
-  kernel runs where it runs in the timed region (right after the drift, in
-  cell order)."""
+  one eager block of `steps` steps (the kernels of the timed blocks, launched
+  from the host instead of a graph replay) with HIP events around each force
+  launch, on the stream the kernels run on, less the duration of an empty
+  event pair recorded right after each (what the markers themselves add).  A
+  spin kernel queued first keeps the device behind the host, so no interval
+  holds a host gap."""
   import torch
   ev = []
   loop.prime(state, nbrs)
@@ -54,7 +55,9 @@ def force_kernel_us(jmd, loop, state, nbrs, efn, steps=100):
     pass
   loop.run(state, nbrs, steps, ev)
   torch.cuda.synchronize()
-  return 1e3 * sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+  kern = sum(e[0].elapsed_time(e[1]) for e in ev) / len(ev)
+  mark = sum(e[2].elapsed_time(e[3]) for e in ev) / len(ev)
+  return 1e3 * (kern - mark)
 
 
 def counter_traffic(N):

@@ -148,6 +148,15 @@ def nve(energy_or_force_fn, shift_fn, dt=1e-3, **sim_kwargs):
   return init_fn, step_fn
 
 
+def _close_timing(e0):
+  """(e0, e1, e2, e3): e0..e1 brackets a kernel, e2..e3 (recorded right
+  after, nothing between) measures what an event pair itself adds."""
+  ev = [e0] + [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+  for e in ev[1:]:
+    e.record()
+  return tuple(ev)
+
+
 class NVELoop:
   """``n_steps`` of velocity Verlet with the neighbour list maintained on the
   device: the counterpart of the reference's
@@ -221,8 +230,9 @@ class NVELoop:
     cell order, so that a wavefront's neighbour gathers share cache lines;
     user order, and the list in user ids, are restored at the end (rows hold
     the reference's neighbour sets; inside a row the order follows the cell
-    order of the block).  ``events`` (a list) collects a (start, end) timing
-    event pair around every force kernel; it forces the eager path."""
+    order of the block).  ``events`` (a list) collects, per force kernel, a
+    (start, end) event pair around it and an empty pair recorded right after
+    (the markers' own cost); it forces the eager path."""
     if state.position_hi is None:
       state.position_hi = state.position.to(torch.float64)
     reorder = self._use_reorder(state, nbrs)
@@ -385,9 +395,7 @@ class NVELoop:
             width, fmt, F, None, None, p, float(self.half), None, pws, st),
             'jmd_pair_energy_force')
         if events is not None:
-          e1 = torch.cuda.Event(enable_timing=True)
-          e1.record()
-          events.append((e0, e1))
+          events.append(_close_timing(e0))
       if n_steps % 2:  # the state's buffer and gate word hold step n's values
         R.copy_(xs[1])
         gs[0].copy_(gs[1])
@@ -411,7 +419,5 @@ class NVELoop:
           x, N, dim, stride, int(geo.periodic), esides, nat.ctypes.byref(pot), idx, width, fmt, F,
           None, None, p, float(self.half), None, pws, st), 'jmd_pair_energy_force')
       if events is not None:
-        e1 = torch.cuda.Event(enable_timing=True)
-        e1.record()
-        events.append((e0, e1))
+        events.append(_close_timing(e0))
     return state
