@@ -109,6 +109,7 @@ __device__ __forceinline__ float min_image_disp(float xj, float xi, float xim, f
 
 // jnp.mod(x, L) for the position update (space.py:250-252), float64.
 __device__ __forceinline__ double ref_mod64(double x, double L) {
+  if (x >= 0.0 && x < L) return x;  // fmod's own result there, without its cost
   double r = fmod(x, L);
   if (r != 0.0 && r < 0.0) r = r + L;
   return r;
@@ -127,6 +128,8 @@ struct KickDrift {
   const float* F;
   const float* mass;  // NULL: mass_scalar
   float mass_scalar;
+  int mass_pow2;       // scalar mass a power of two: x / m == x * inv_mass exactly
+  double inv_mass;
   float dt, half_dt;
   const float* dt_dev;  // device dt overriding dt (half_dt = f32(dt / 2)), or NULL
   float* x_out;         // float32 R', row layout of the kernel's positions
@@ -135,6 +138,13 @@ struct KickDrift {
   int* gate_next;       // raised when any particle moved beyond the skin
   Space s;
 };
+
+inline void kick_drift_mass(KickDrift* kd) {
+  int e = 0;
+  const double mant = kd->mass ? 0.0 : frexp((double)kd->mass_scalar, &e);
+  kd->mass_pow2 = mant == 0.5;
+  kd->inv_mass = kd->mass_pow2 ? 1.0 / (double)kd->mass_scalar : 0.0;
+}
 
 template <int DIM>
 __device__ __forceinline__ void kick_drift_atom(const KickDrift& kd, long long i, float dt,
@@ -145,7 +155,8 @@ __device__ __forceinline__ void kick_drift_atom(const KickDrift& kd, long long i
   for (int d = 0; d < DIM; ++d) {
     const long long k = i * DIM + d;
     const float pk = kd.p_in[k] + half_dt * kd.F[k];
-    double x = kd.xhi_in[k] + (double)dt * (double)pk / m;
+    const double v = (double)dt * (double)pk;
+    double x = kd.xhi_in[k] + (kd.mass_pow2 ? v * kd.inv_mass : v / m);
     if (kd.s.periodic) x = ref_mod64(x, (double)kd.s.L[d]);
     kd.p_out[k] = pk;
     kd.xhi_out[k] = x;

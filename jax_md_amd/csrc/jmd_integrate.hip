@@ -150,6 +150,7 @@ int jmd_nve_kick_drift(const double* position_hi, const float* momentum, const f
   kd.F = force;
   kd.mass = mass;
   kd.mass_scalar = mass_scalar;
+  kick_drift_mass(&kd);
   kd.dt = dt;
   kd.half_dt = half_dt;
   kd.dt_dev = dt_device;

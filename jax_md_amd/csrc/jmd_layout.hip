@@ -49,14 +49,18 @@ __global__ __launch_bounds__(256) void k_remap_dense(const int* __restrict__ in,
                                                      int* __restrict__ out,
                                                      const int* __restrict__ perm, int n,
                                                      int width) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long total = (long long)n * width;
-  long long step = (long long)gridDim.x * blockDim.x;
-  for (; t < total; t += step) {
-    int r = (int)(t / width);
-    int k = (int)(t - (long long)r * width);
-    int j = in[t];
-    out[(long long)perm[r] * width + k] = j < n ? perm[j] : n;
+  // one wavefront per row: the row's destination is read once, lanes walk the
+  // entries (no per-entry division to find the row)
+  const int lane = lane_id();
+  const long long nw = (long long)gridDim.x * (blockDim.x / kWave);
+  for (long long r = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; r < n;
+       r += nw) {
+    const int* src = in + r * width;
+    int* dst = out + (long long)perm[r] * width;
+    for (int k = lane; k < width; k += kWave) {
+      const int j = src[k];
+      dst[k] = j < n ? perm[j] : n;
+    }
   }
 }
 
@@ -120,8 +124,9 @@ int jmd_remap_neighbor_ids(const int32_t* idx_in, int32_t* idx_out, const int32_
   hipStream_t st = (hipStream_t)stream;
   if (format == JMD_DENSE) {
     if (width > 0x7fffffff) return JMD_ERR_INVALID;
-    hipLaunchKernelGGL(k_remap_dense, dim3(grid_for(n * width)), dim3(256), 0, st, idx_in,
-                       idx_out, perm, (int)n, (int)width);
+    long long rb = (n + 3) / 4;  // four rows (wavefronts) per workgroup
+    hipLaunchKernelGGL(k_remap_dense, dim3((unsigned)(rb < 65536 ? (rb < 1 ? 1 : rb) : 65536)),
+                       dim3(256), 0, st, idx_in, idx_out, perm, (int)n, (int)width);
   } else {
     hipLaunchKernelGGL(k_remap_ids, dim3(grid_for(2 * width)), dim3(256), 0, st, idx_in,
                        idx_out, perm, (int)n, 2 * width);

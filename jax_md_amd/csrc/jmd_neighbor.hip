@@ -360,6 +360,19 @@ __device__ __forceinline__ bool pair_test(const float* xi, const float* xj,
   return pa && (b < cutoff_sq);
 }
 
+// pair_test on the original positions of particles i and j, kept out of line
+// for the cell build: it runs only for candidates within the margin of the
+// cutoff, and inlined its jnp.mod emulation would hold registers across the
+// whole candidate loop.
+template <int DIM, int STRIDE>
+__device__ __noinline__ bool exact_pair_test(const float* __restrict__ pos, int i, int j,
+                                             Space s, float cutoff_sq, int test) {
+  float xo[3] = {0.0f, 0.0f, 0.0f}, xj[3] = {0.0f, 0.0f, 0.0f};
+  load_pos<DIM, STRIDE>(pos, i, xo);
+  load_pos<DIM, STRIDE>(pos, j, xj);
+  return pair_test<DIM>(xo, xj, s, cutoff_sq, 0.0f, test);
+}
+
 // Arguments of the cell-grid build.
 struct BuildArgs {
   const float* pos;
@@ -398,6 +411,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
   __shared__ float4 s_pos[kStage];
   __shared__ int s_cnt[kHomeGroup];
   __shared__ int s_occ;
+  __shared__ int s_wc[16];  // kept candidates per wavefront of a staging chunk
 
   const Grid& g = A.g;
   const Space& s = A.s;
@@ -449,6 +463,24 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
     }
     __syncthreads();
     const int total = s_off[NST];
+    // The home cell's box in the staged frame, widened for the truncating
+    // cell index (x in (-cs, 0) belongs to cell 0) and its rounding: a
+    // candidate farther than sqrt(c^2 + margin) from it reaches no home
+    // particle by either test and is not staged.
+    const bool prune = !MINIMG && s.periodic && margin < 3.0e38f;
+    float blo[3] = {0.0f, 0.0f, 0.0f}, bhi[3] = {0.0f, 0.0f, 0.0f};
+    {
+      int h = home;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const int c = h % g.cps[d];
+        h /= g.cps[d];
+        const float cs = g.cell_size[d], dl = 1e-4f * cs;
+        blo[d] = (c == 0 ? -cs : (float)c * cs) - dl;
+        bhi[d] = (float)(c + 1) * cs + dl;
+      }
+    }
+    const float reach_sq = (A.cutoff_sq + margin) * 1.001f;
     const int hstart = s_cs[NST / 2];
     const int hcount = s_cc[NST / 2];
     const int cap = g.cap;
@@ -458,30 +490,58 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
       for (int a = tid; a < gcount; a += blockDim.x) s_cnt[a] = 0;
       for (int c0 = 0; c0 < total; c0 += kStage) {
         const int ccount = min(kStage, total - c0);
-        const int cpad = (ccount + kWave - 1) & ~(kWave - 1);
         __syncthreads();
-        // Stage candidates c0..c0+ccount in reference order, padded to a
+        // Stage candidates c0..c0+ccount in reference order, keeping those
+        // that can reach the home cell (compacted in order), padded to a
         // multiple of 64 with far-away entries.
-        for (int q = tid; q < cpad; q += blockDim.x) {
-          if (q >= ccount) {
-            s_pos[q] = make_float4(far, far, far, __int_as_float(n));
-            continue;
+        int kept = 0;  // workgroup-uniform
+        for (int q0 = 0; q0 < ccount; q0 += blockDim.x) {
+          const int q = q0 + tid;
+          bool keep = false;
+          float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          if (q < ccount) {
+            int p = c0 + q;
+            int t = 0;
+            while (t + 1 < NST && s_off[t + 1] <= p) ++t;
+            int m = p - s_off[t];
+            int cnt = s_cc[t];
+            int k = m;
+            if (cap > 0 && cnt <= cap) {
+              int w = cap - (s_cs[t] % cap);
+              if (cnt > w) k = (m < cnt - w) ? (w + m) : (m - (cnt - w));
+            }
+            int gi = s_cs[t] + k;
+            float4 xp = A.w.sorted_pos[gi];
+            v = make_float4(xp.x + s_shift[t][0], xp.y + s_shift[t][1], xp.z + s_shift[t][2],
+                            __int_as_float(A.w.sorted_id[gi]));
+            keep = true;
+            if (prune) {
+              const float cv[3] = {v.x, v.y, v.z};
+              float D2 = 0.0f;
+#pragma unroll
+              for (int d = 0; d < DIM; ++d) {
+                const float e = fmaxf(0.0f, fmaxf(blo[d] - cv[d], cv[d] - bhi[d]));
+                D2 = fmaf(e, e, D2);
+              }
+              keep = D2 <= reach_sq;
+            }
           }
-          int p = c0 + q;
-          int t = 0;
-          while (t + 1 < NST && s_off[t + 1] <= p) ++t;
-          int m = p - s_off[t];
-          int cnt = s_cc[t];
-          int k = m;
-          if (cap > 0 && cnt <= cap) {
-            int w = cap - (s_cs[t] % cap);
-            if (cnt > w) k = (m < cnt - w) ? (w + m) : (m - (cnt - w));
+          const unsigned long long bal = __ballot(keep);
+          if (lane == 0) s_wc[wave] = (int)__popcll(bal);
+          __syncthreads();
+          int before = 0, all = 0;
+          for (int w = 0; w < nwave; ++w) {
+            const int c = s_wc[w];
+            before += w < wave ? c : 0;
+            all += c;
           }
-          int gi = s_cs[t] + k;
-          float4 xp = A.w.sorted_pos[gi];
-          s_pos[q] = make_float4(xp.x + s_shift[t][0], xp.y + s_shift[t][1],
-                                 xp.z + s_shift[t][2], __int_as_float(A.w.sorted_id[gi]));
+          if (keep) s_pos[kept + before + wave_prefix_count(bal)] = v;
+          kept += all;
+          __syncthreads();  // s_wc is rewritten by the next chunk
         }
+        const int cpad = (kept + kWave - 1) & ~(kWave - 1);
+        for (int q = kept + tid; q < cpad; q += blockDim.x)
+          s_pos[q] = make_float4(far, far, far, __int_as_float(n));
         __syncthreads();
         // Each wavefront takes kHomePass home particles at a time: one LDS
         // read of 64 candidates serves all of them, and the loop overhead is
@@ -518,8 +578,10 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
           }
           // homes of this pass that exist (wavefront-uniform, a scalar bound)
           const int nh = min(kHomePass, (gcount - a0 + nwave - 1) / nwave);
+          float4 cnext = s_pos[lane];  // cpad >= 64 when this loop runs
           for (int q0 = 0; q0 < cpad; q0 += kWave) {
-            const float4 c = s_pos[q0 + lane];
+            const float4 c = cnext;
+            if (q0 + kWave < cpad) cnext = s_pos[q0 + kWave + lane];  // next chunk, ahead
             // staged padding sits 3e18 away: its d2 fails both tests below
             const int j = __float_as_int(c.w);
             const float cj[3] = {c.x, c.y, c.z};
@@ -543,12 +605,8 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
               bool exact = false;
               if (__builtin_amdgcn_ballot_w64(near)) {  // rare: reference-exact test
                 // (an infinite margin also admits the staged padding, j = n)
-                if (near && okj && j < n) {
-                  float xo[3] = {0.0f, 0.0f, 0.0f}, xj[3] = {0.0f, 0.0f, 0.0f};
-                  load_pos<DIM, STRIDE>(A.pos, i, xo);
-                  load_pos<DIM, STRIDE>(A.pos, j, xj);
-                  exact = pair_test<DIM>(xo, xj, s, A.cutoff_sq, 0.0f, A.test);
-                }
+                if (near && okj && j < n)
+                  exact = exact_pair_test<DIM, STRIDE>(A.pos, i, j, s, A.cutoff_sq, A.test);
               }
               const bool pass = (fast && okj) || exact;
               const unsigned long long bal = __ballot(pass);
@@ -1375,6 +1433,7 @@ int jmd_nve_list_update_kick_drift(
   kd.F = force;
   kd.mass = mass;
   kd.mass_scalar = mass_scalar;
+  kick_drift_mass(&kd);
   kd.dt = dt;
   kd.half_dt = half_dt;
   kd.dt_dev = dt_device;
