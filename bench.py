@@ -74,42 +74,51 @@ def counter_traffic(N):
 def adjoint_section(jmd, O, check=True):
   """Adjoint workload: LJ N = 32,768 (32^3 lattice), gradient of E(x_T) after
   200 NVE steps (dt = 0.005) with respect to x_0, sigma and epsilon, with
-  checkpoints every 20 steps; time against the same forward run."""
+  checkpoints every 20 steps; time against the same forward run.  Run in
+  float32 (the workload's precision) and in the x64 form; both are compared
+  with central differences of the C FP64 restatement."""
   import numpy as np
   import torch
   dev = 'cuda:0'
   R, P, L = O.lj_lattice(32, seed=1)
   disp, shift = jmd.space.periodic(L)
   nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense)
-  Rt = torch.as_tensor(R, device=dev)
-  Pt = torch.as_tensor(P, device=dev)
   T, K, dt = 200, 20, 0.005
-  # forward alone: the same kernels, no checkpoints
-  nb = nfn.allocate(Rt)
-  init_fn, _ = jmd.simulate.nve(efn, shift, dt)
-  st = init_fn(0, Rt, kT=0.0, momenta=Pt, neighbor=nb)
-  loop = jmd.simulate.NVELoop(efn, shift, dt, reorder=False, graph=False)
-  loop.prime(st, nb)
-  loop._run_steps(st, nb, 20)  # warm
-  st = init_fn(0, Rt, kT=0.0, momenta=Pt, neighbor=nb)
-  nb = nfn.allocate(Rt)
-  loop.prime(st, nb)
-  torch.cuda.synchronize()
-  t = time.perf_counter()
-  loop._run_steps(st, nb, T)
-  E_fwd, _ = efn.energy_and_force(st.position, nb)
-  torch.cuda.synchronize()
-  t_fwd = time.perf_counter() - t
-  jmd.adjoint.nve_energy_gradient(efn, shift, nfn, Rt, Pt, dt, 20, checkpoint_every=K)  # warm
-  torch.cuda.synchronize()
-  t = time.perf_counter()
-  g = jmd.adjoint.nve_energy_gradient(efn, shift, nfn, Rt, Pt, dt, T, checkpoint_every=K)
-  torch.cuda.synchronize()
-  t_grad = time.perf_counter() - t
+
+  def run(dtype):
+    Rt = torch.as_tensor(R, device=dev).to(dtype)
+    Pt = torch.as_tensor(P, device=dev).to(dtype)
+    R32 = Rt.to(torch.float32)
+    init_fn, _ = jmd.simulate.nve(efn, shift, dt)
+    loop = jmd.simulate.NVELoop(efn, shift, dt, reorder=False, graph=False)
+    nb = nfn.allocate(R32)
+    st = init_fn(0, Rt, kT=0.0, momenta=Pt, neighbor=nb)
+    loop.prime(st, nb)
+    loop._run_steps(st, nb, 20)  # warm
+    nb = nfn.allocate(R32)
+    st = init_fn(0, Rt, kT=0.0, momenta=Pt, neighbor=nb)
+    loop.prime(st, nb)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    loop._run_steps(st, nb, T)
+    efn.energy_and_force(st.position_hi if st.x64 else st.position, nb)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t
+    jmd.adjoint.nve_energy_gradient(efn, shift, nfn, Rt, Pt, dt, 20, checkpoint_every=K)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    g = jmd.adjoint.nve_energy_gradient(efn, shift, nfn, Rt, Pt, dt, T, checkpoint_every=K)
+    torch.cuda.synchronize()
+    return g, t_fwd, time.perf_counter() - t
+
+  g32, f32_fwd, f32_grad = run(torch.float32)
+  g64, f64_fwd, f64_grad = run(torch.float64)
   out = {'workload': 'LJ N=32768 (32^3 lattice, rho=0.8442, jitter 0.05, kT=1), 200 NVE '
                      'steps dt=0.005, dE(x_T)/d(x_0, sigma, epsilon), checkpoints every 20',
-         'forward_s': t_fwd, 'gradient_s': t_grad, 'ratio_vs_forward': t_grad / t_fwd,
-         'checkpoint_bytes': g.checkpoint_bytes, 'energy_T': g.energy}
+         'forward_s': f32_fwd, 'gradient_s': f32_grad, 'ratio_vs_forward': f32_grad / f32_fwd,
+         'checkpoint_bytes': g32.checkpoint_bytes, 'energy_T': g32.energy,
+         'x64': {'forward_s': f64_fwd, 'gradient_s': f64_grad,
+                 'ratio_vs_forward': f64_grad / f64_fwd, 'energy_T': g64.energy}}
   if check:
     try:
       from oracle import c_oracle
@@ -129,22 +138,28 @@ def adjoint_section(jmd, O, check=True):
 
       R64 = R.astype(np.float64)
       fd_x = (e_after(R64 + h * u) - e_after(R64 - h * u)) / (2 * h)
-      ad_x = float(np.sum(g.position.cpu().numpy() * u))
       fd_s = (e_after(R64, sigma=1 + h) - e_after(R64, sigma=1 - h)) / (2 * h)
       fd_e = (e_after(R64, eps=1 + h) - e_after(R64, eps=1 - h)) / (2 * h)
-      gn = float(np.linalg.norm(g.position.cpu().numpy()))
-      out['fd_check'] = {
-          'h': h, 'direction_x': {'adjoint': ad_x, 'fd': fd_x,
-                                  'rel_err': abs(ad_x - fd_x) / max(abs(fd_x), 1e-30),
-                                  'err_over_grad_norm': abs(ad_x - fd_x) / gn},
-          'sigma': {'adjoint': g.sigma, 'fd': fd_s, 'rel_err': abs(g.sigma - fd_s) / abs(fd_s)},
-          'epsilon': {'adjoint': g.epsilon, 'fd': fd_e,
-                      'rel_err': abs(g.epsilon - fd_e) / abs(fd_e)},
-          'reference': 'C FP64 restatement (oracle/md_oracle.c), central differences'}
+
+      def compare(g):
+        ad_x = float(np.sum(g.position.cpu().numpy() * u))
+        gn = float(np.linalg.norm(g.position.cpu().numpy()))
+        return {'direction_x': {'adjoint': ad_x, 'fd': fd_x,
+                                'rel_err': abs(ad_x - fd_x) / max(abs(fd_x), 1e-30),
+                                'err_over_grad_norm': abs(ad_x - fd_x) / gn},
+                'sigma': {'adjoint': g.sigma, 'fd': fd_s,
+                          'rel_err': abs(g.sigma - fd_s) / abs(fd_s)},
+                'epsilon': {'adjoint': g.epsilon, 'fd': fd_e,
+                            'rel_err': abs(g.epsilon - fd_e) / abs(fd_e)}}
+
+      out['fd_check'] = dict(compare(g32), h=h, reference='C FP64 restatement '
+                             '(oracle/md_oracle.c), central differences')
+      out['x64']['fd_check'] = compare(g64)
     except Exception as e:  # the checker needs gcc-built oracle/libmd_oracle.so
       out['fd_check'] = f'skipped: {e}'
-  log(f'adjoint: forward {t_fwd * 1e3:.1f} ms, gradient {t_grad * 1e3:.1f} ms, '
-      f'ratio {t_grad / t_fwd:.2f}')
+  log(f'adjoint: forward {f32_fwd * 1e3:.1f} ms, gradient {f32_grad * 1e3:.1f} ms, '
+      f'ratio {f32_grad / f32_fwd:.2f}; x64 forward {f64_fwd * 1e3:.1f} ms, gradient '
+      f'{f64_grad * 1e3:.1f} ms, ratio {f64_grad / f64_fwd:.2f}')
   return out
 
 

@@ -223,6 +223,21 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
                           float half_dt, const float* dt_device,
                           void* workspace, void* stream);
 
+/* The x64 form (the reference's JAX_ENABLE_X64 run): float64 positions,
+ * forces, momenta and pair arithmetic, parameters float32-valued as
+ * maybe_downcast leaves them (util.py:109-112).  Dense lists only
+ * (format = JMD_DENSE); same potential, list and outputs as
+ * jmd_pair_energy_force otherwise (smap.py:856-979, energy.py:246-272,
+ * 534-580).  side: host float64 [dim].  energy_total needs a workspace of
+ * jmd_pair_workspace_bytes_f64(n) bytes. */
+size_t jmd_pair_workspace_bytes_f64(int64_t n);
+int jmd_pair_energy_force_f64(const double* position, int64_t n, int32_t dim,
+                              int32_t periodic, const double* side,
+                              const jmd_pair_potential* potential,
+                              const int32_t* idx, int64_t idx_width, int32_t format,
+                              double* force, double* energy_total, double* momentum,
+                              double half_dt, void* workspace, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Adjoint of the pair force                                                 */
 /* ------------------------------------------------------------------------ */
@@ -265,6 +280,16 @@ int jmd_nve_adjoint_step(const float* position, int64_t n, int32_t dim,
                          float coef_param, void* stream);
 
 /* out[0..1] = fixed-order sums of the accumulator's two columns. */
+/* jmd_nve_adjoint_step for an x64 trajectory: positions float64 [n, dim]
+ * (each displacement formed in float64, rounded once; the product itself in
+ * float32 as in jmd_nve_adjoint_step). */
+int jmd_nve_adjoint_step_x64(const double* position, int64_t n, int32_t dim,
+                             int32_t periodic, const float* side,
+                             const jmd_pair_potential* potential, const int32_t* idx,
+                             int64_t idx_width, const float* ph_in, float* ph_out,
+                             double* x_adjoint, float coef_x, float coef_p,
+                             double* param_acc, float coef_param, void* stream);
+
 int jmd_adjoint_param_sum(const double* param_acc, int64_t n, int32_t dim, double* out,
                           void* stream);
 
@@ -376,6 +401,20 @@ int jmd_nve_kick_drift(const double* position_hi, const float* momentum,
                        int32_t position_stride, float* momentum_out,
                        const float* reference_position, float threshold_sq,
                        int32_t* gate, void* stream);
+
+/* jmd_nve_kick_drift in the x64 form (simulate.py:227-239 with float64
+ * state): position, momentum in place (float64 [n, dim]), force float64,
+ * mass NULL (mass_scalar) or device float64 [n]; dt and half_dt the float32
+ * values the reference casts them to.  position_f32 receives the float32
+ * mirror of the new positions (row stride position_stride) from which lists
+ * are built, and the skin criterion is evaluated on it as in
+ * jmd_nve_kick_drift. */
+int jmd_nve_kick_drift_f64(double* position, double* momentum, const double* force,
+                           const double* mass, double mass_scalar, int64_t n, int32_t dim,
+                           int32_t periodic, const float* side, double dt, double half_dt,
+                           float* position_f32, int32_t position_stride,
+                           const float* reference_position, float threshold_sq,
+                           int32_t* gate, void* stream);
 
 /* One step's neighbour-list update fused with the opening half of the next
  * velocity-Verlet step: the loop body

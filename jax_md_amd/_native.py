@@ -90,6 +90,19 @@ _SIGNATURES = {
                                        c_int64, c_void_p, c_void_p, c_void_p, c_float, c_float,
                                        c_void_p, c_float, c_void_p]),
     'jmd_adjoint_param_sum': (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    'jmd_nve_adjoint_step_x64': (c_int32, [c_void_p, c_int64, c_int32, c_int32,
+                                           POINTER(c_float), POINTER(JmdPairPotential), c_void_p,
+                                           c_int64, c_void_p, c_void_p, c_void_p, c_float,
+                                           c_float, c_void_p, c_float, c_void_p]),
+    'jmd_pair_workspace_bytes_f64': (c_size_t, [c_int64]),
+    'jmd_pair_energy_force_f64': (c_int32, [c_void_p, c_int64, c_int32, c_int32,
+                                            POINTER(c_double), POINTER(JmdPairPotential),
+                                            c_void_p, c_int64, c_int32, c_void_p, c_void_p,
+                                            c_void_p, c_double, c_void_p, c_void_p]),
+    'jmd_nve_kick_drift_f64': (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                                         c_int64, c_int32, c_int32, POINTER(c_float), c_double,
+                                         c_double, c_void_p, c_int32, c_void_p, c_float,
+                                         c_void_p, c_void_p]),
     'jmd_fire_ensemble_max_cells': (c_int32, []),
     'jmd_fire_ensemble_2d': (c_int32, [c_void_p, c_void_p, c_int32, c_int64, c_int32, c_int32,
                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -151,6 +164,13 @@ def stream_handle(device=None):
 
 def f3(vals):
   a = _F3()
+  for k, v in enumerate(vals):
+    a[k] = float(v)
+  return a
+
+
+def d3(vals):
+  a = (c_double * 3)()
   for k, v in enumerate(vals):
     a[k] = float(v)
   return a

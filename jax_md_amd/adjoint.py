@@ -65,8 +65,12 @@ def force_vjp(energy_fn, R, neighbor, cotangent, params: bool = True, **dynamic_
   pot, keep = energy._potential(R, dynamic_kwargs)
   geo = energy.geometry
   ws = _workspace(dev, N, dim)
+  # float64 positions (an x64 trajectory): row stride code 0
+  stride = 0 if R.dtype == torch.float64 else nat.position_stride(R, 'R')
+  if stride == 0:
+    nat.require_device_tensor(R, 'R', torch.float64)
   nat.check(lib.jmd_pair_force_vjp(
-      nat.ptr(R), N, dim, nat.position_stride(R, 'R'), int(geo.periodic),
+      nat.ptr(R), N, dim, stride, int(geo.periodic),
       nat.f3(geo.side_values(dim)), nat.ctypes.byref(pot), nat.ptr(neighbor.idx),
       neighbor.idx.shape[1], nat.DENSE, nat.ptr(v), nat.ptr(out), nat.ptr(pc), nat.ptr(eg),
       nat.ptr(ws), nat.stream_handle(dev)), 'jmd_pair_force_vjp')
@@ -145,7 +149,9 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
                         checkpoint_every: int = 20, mass=1.0,
                         neighbor=None) -> TrajectoryGradient:
   """Gradient of E(x_T) after ``n_steps`` velocity-Verlet steps from (R, P)
-  with respect to R, P, sigma and epsilon.
+  with respect to R, P, sigma and epsilon.  A float64 R runs the x64 form:
+  float64 trajectory (positions, momenta, forces), adjoint products formed
+  from float64 displacements.
 
   Backward recursion (one fused kernel per step): with adjoints X of x and PH
   of the half-step momentum, X_T = -F(x_T), PH = (dt/m) X_T, and for
@@ -161,9 +167,11 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   N, dim = R.shape
   dt32 = f32(dt)
   m = float(f32(mass))
-  nbrs = neighbor if neighbor is not None else neighbor_fn.allocate(R)
+  nbrs = neighbor if neighbor is not None else neighbor_fn.allocate(
+      R.to(torch.float32) if R.dtype == torch.float64 else R)
   if nbrs.format is not partition.Dense:
     raise NotImplementedError('the trajectory adjoint runs over Dense neighbour lists')
+  x64 = R.dtype == torch.float64
   init_fn, _ = simulate.nve(energy, shift_fn, dt)
   state = init_fn(0, R, kT=0.0, mass=mass, momenta=P, neighbor=nbrs)
   loop = simulate.NVELoop(energy, shift_fn, dt, reorder=False, graph=False)
@@ -182,9 +190,10 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   t1 = time.perf_counter()
 
   # final energy, X_T = -F(x_T), direct parameter gradient
-  E_T, F_T = energy.energy_and_force(state.position, nbrs)
+  xT = state.position_hi if x64 else state.position
+  E_T, F_T = energy.energy_and_force(xT, nbrs)
   zero = torch.zeros((N, dim), dtype=torch.float32, device=dev)
-  _, _, direct = force_vjp(energy, state.position, nbrs, zero, params=True)
+  _, _, direct = force_vjp(energy, xT, nbrs, zero, params=True)
   X = (-F_T).to(torch.float64).contiguous()
   PH = ((dt32 / m) * X).to(torch.float32).contiguous()
   PH2 = torch.empty_like(PH)
@@ -196,7 +205,10 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   st = nat.stream_handle(dev)
   if n_steps == 0:
     PH.zero_()  # dE/dp_0 = 0 when x_T = x_0
-  xs = [simulate.new_positions(N, dim, dev) for _ in range(K)]
+  if x64:
+    xs = [torch.empty((N, dim), dtype=torch.float64, device=dev) for _ in range(K)]
+  else:
+    xs = [simulate.new_positions(N, dim, dev) for _ in range(K)]
   idxs = [torch.empty_like(nbrs.idx) for _ in range(K)]
   width = nbrs.idx.shape[1]
   ckpt_bytes = sum(sum(t.numel() * t.element_size() for t in c[2].values()) for c in ckpts)
@@ -206,17 +218,23 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
     _restore(snap, state, nbrs)
 
     def hook(q, start=start):
-      xs[q].copy_(state.position)
+      xs[q].copy_(state.position_hi if x64 else state.position)
       idxs[q].copy_(nbrs.idx)
 
     loop._run_steps(state, nbrs, k, hook=hook)
     for q in range(k - 1, -1, -1):
       step = start + q
       c = float(dt32) if step >= 1 else float(f32(dt32 / f32(2)))
-      nat.check(lib.jmd_nve_adjoint_step(
-          nat.ptr(xs[q]), N, dim, nat.position_stride(xs[q]), int(geo.periodic), side,
-          nat.ctypes.byref(pot), nat.ptr(idxs[q]), width, nat.ptr(PH), nat.ptr(PH2),
-          nat.ptr(X), c, float(dt32) / m, nat.ptr(acc), c, st), 'jmd_nve_adjoint_step')
+      if x64:
+        nat.check(lib.jmd_nve_adjoint_step_x64(
+            nat.ptr(xs[q]), N, dim, int(geo.periodic), side, nat.ctypes.byref(pot),
+            nat.ptr(idxs[q]), width, nat.ptr(PH), nat.ptr(PH2), nat.ptr(X), c, float(dt32) / m,
+            nat.ptr(acc), c, st), 'jmd_nve_adjoint_step_x64')
+      else:
+        nat.check(lib.jmd_nve_adjoint_step(
+            nat.ptr(xs[q]), N, dim, nat.position_stride(xs[q]), int(geo.periodic), side,
+            nat.ctypes.byref(pot), nat.ptr(idxs[q]), width, nat.ptr(PH), nat.ptr(PH2),
+            nat.ptr(X), c, float(dt32) / m, nat.ptr(acc), c, st), 'jmd_nve_adjoint_step')
       if step >= 1:
         PH, PH2 = PH2, PH
   out = torch.empty((2,), dtype=torch.float64, device=dev)

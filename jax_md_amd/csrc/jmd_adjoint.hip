@@ -29,6 +29,8 @@ namespace jmd {
 
 enum { ADJ_VJP = 0, ADJ_STEP = 1 };
 
+// STRIDE 0: positions are float64 [n, DIM] (an x64 trajectory); each
+// displacement is formed in float64 and rounded once, the rest is float32.
 // One launch over all rows.  ADJ_VJP: hout = J v, part[4 b + k] = block sums
 // of (v^T dF/dsigma, v^T dF/deps, dE/dsigma, dE/deps).  ADJ_STEP:
 // X += c_x (J v) (float64), vout = v + c_p X, part[2 b + k] += c_t (v^T dF/dtheta).
@@ -47,8 +49,15 @@ __global__ __launch_bounds__(256) void k_pair_adjoint(
   float h[3] = {0.0f, 0.0f, 0.0f};
   float ts = 0.0f, te = 0.0f, us = 0.0f, ue = 0.0f;
   float xi[3] = {0, 0, 0}, xim[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+  double xi64[3] = {0.0, 0.0, 0.0};
+  const double* pos64 = reinterpret_cast<const double*>(pos);
   if (active) {
-    load_pos<DIM, STRIDE>(pos, i, xi);
+    if (STRIDE == 0) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) xi64[c] = pos64[(long long)i * DIM + c];
+    } else {
+      load_pos<DIM, STRIDE>(pos, i, xi);
+    }
     load_pos<DIM>(vin, i, vi);
 #pragma unroll
     for (int c = 0; c < DIM; ++c) xim[c] = xi[c] - s.L[c];
@@ -63,13 +72,18 @@ __global__ __launch_bounds__(256) void k_pair_adjoint(
       if (jr >= (unsigned)n) break;  // left-compacted rows, padding n
       const unsigned j = jr;
       float xj[3] = {0, 0, 0}, vj[3] = {0, 0, 0};
-      load_pos<DIM, STRIDE>(pos, j, xj);
+      if (STRIDE != 0) load_pos<DIM, STRIDE>(pos, j, xj);
       load_pos<DIM>(vin, j, vj);
       float d[3] = {0.0f, 0.0f, 0.0f}, w[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
-        d[c] = s.periodic ? min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c])
-                          : xj[c] - xi[c];
+        if (STRIDE == 0) {
+          const double v = pos64[(long long)j * DIM + c] - xi64[c];
+          d[c] = (float)(s.periodic ? wrap64(v, (double)s.L[c]) : v);
+        } else {
+          d[c] = s.periodic ? min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c])
+                            : xj[c] - xi[c];
+        }
         w[c] = vj[c] - vi[c];
       }
       const float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
@@ -209,14 +223,21 @@ static void dispatch_adj(int dim, int stride, const AdjLaunch& L) {
   if (dim == 3) {
     if (L.a.kind == JMD_LENNARD_JONES) {
       if (stride == 4) launch_adj<3, 4, JMD_LENNARD_JONES, kAdjTPA, MODE>(L);
+      else if (stride == 0) launch_adj<3, 0, JMD_LENNARD_JONES, kAdjTPA, MODE>(L);
       else launch_adj<3, 3, JMD_LENNARD_JONES, kAdjTPA, MODE>(L);
     } else {
       if (stride == 4) launch_adj<3, 4, JMD_SOFT_SPHERE, kAdjTPA, MODE>(L);
+      else if (stride == 0) launch_adj<3, 0, JMD_SOFT_SPHERE, kAdjTPA, MODE>(L);
       else launch_adj<3, 3, JMD_SOFT_SPHERE, kAdjTPA, MODE>(L);
     }
   } else {
-    if (L.a.kind == JMD_LENNARD_JONES) launch_adj<2, 2, JMD_LENNARD_JONES, 4, MODE>(L);
-    else launch_adj<2, 2, JMD_SOFT_SPHERE, 4, MODE>(L);
+    if (L.a.kind == JMD_LENNARD_JONES) {
+      if (stride == 0) launch_adj<2, 0, JMD_LENNARD_JONES, 4, MODE>(L);
+      else launch_adj<2, 2, JMD_LENNARD_JONES, 4, MODE>(L);
+    } else {
+      if (stride == 0) launch_adj<2, 0, JMD_SOFT_SPHERE, 4, MODE>(L);
+      else launch_adj<2, 2, JMD_SOFT_SPHERE, 4, MODE>(L);
+    }
   }
 }
 
@@ -227,7 +248,8 @@ static int adj_check(const float* position, int64_t n, int32_t dim, int32_t posi
       idx_width > 0x7fffffff || (n > 0 && !position) || (idx_width > 0 && n > 0 && !idx) ||
       (periodic && !side))
     return JMD_ERR_INVALID;
-  if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  if (n > 0 && position_stride != 0 && !stride_ok(dim, position_stride, position))
+    return JMD_ERR_INVALID;
   if ((long long)n * idx_width > 0x7fffffffffffLL) return JMD_ERR_INVALID;
   if (!make_pair_args(potential, a)) return JMD_ERR_INVALID;
   return JMD_OK;
@@ -312,6 +334,17 @@ int jmd_nve_adjoint_step(const float* position, int64_t n, int32_t dim, int32_t 
   dispatch_adj<ADJ_STEP>(dim, position_stride, L);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
+}
+
+int jmd_nve_adjoint_step_x64(const double* position, int64_t n, int32_t dim, int32_t periodic,
+                             const float* side, const jmd_pair_potential* potential,
+                             const int32_t* idx, int64_t idx_width, const float* ph_in,
+                             float* ph_out, double* x_adjoint, float coef_x, float coef_p,
+                             double* param_acc, float coef_param, void* stream) {
+  // stride code 0: float64 [n, dim] positions (k_pair_adjoint<.., 0, ..>)
+  return jmd_nve_adjoint_step(reinterpret_cast<const float*>(position), n, dim, 0, periodic,
+                              side, potential, idx, idx_width, ph_in, ph_out, x_adjoint, coef_x,
+                              coef_p, param_acc, coef_param, stream);
 }
 
 int jmd_adjoint_param_sum(const double* param_acc, int64_t n, int32_t dim, double* out,

@@ -107,6 +107,14 @@ __device__ __forceinline__ float min_image_disp(float xj, float xi, float xim, f
   return raw >= half ? hi : d;
 }
 
+// Minimum image of a float64 difference (the x64 form of the periodic
+// displacement, space.py:213-224); exact for |d| < L/2, one rounding beyond.
+__device__ __forceinline__ double wrap64(double d, double L) {
+  const double h = 0.5 * L;
+  if (d >= h || d < -h) d = fma(-L, rint(d / L), d);
+  return d;
+}
+
 // jnp.mod(x, L) for the position update (space.py:250-252), float64.
 __device__ __forceinline__ double ref_mod64(double x, double L) {
   if (x >= 0.0 && x < L) return x;  // fmod's own result there, without its cost

@@ -20,6 +20,7 @@ struct PairArgs {
   int ns;
   float ro2, rc2, inv_d;  // LJ smoothing: r_on^2, r_c^2, 1/(rc^2-ro^2)^3
   float c1;               // rc^2 - 3 ro^2
+  float ro;               // LJ r_on
   int smooth;             // r_on < r_c
   float rc;               // interaction range (LJ r_c, soft sphere max sigma)
 };
@@ -146,6 +147,7 @@ inline bool make_pair_args(const jmd_pair_potential* p, PairArgs* a) {
   a->rc2 = rc * rc;
   a->ro2 = ro * ro;
   a->rc = rc;
+  a->ro = ro;
   if (p->kind == JMD_SOFT_SPHERE) {
     // Interaction range (used only for the wrap-free interior test): a scalar
     // sigma, else the caller's r_cutoff bound on sigma (0: unknown, never

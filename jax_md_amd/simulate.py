@@ -26,12 +26,19 @@ f32 = onp.float32
 @dataclasses.dataclass
 class NVEState:
   """position, momentum, force [N, dim] float32; mass scalar or [N];
-  position_hi: the float64 master copy of the positions."""
+  position_hi: the float64 master copy of the positions.  In the x64 form
+  (float64 R given to ``init_fn``, the reference's JAX_ENABLE_X64 run)
+  momentum and force are float64 and ``position`` is the float32 mirror of
+  ``position_hi`` that the neighbour list is built from."""
   position: torch.Tensor
   momentum: torch.Tensor
   force: torch.Tensor
   mass: object
   position_hi: Optional[torch.Tensor] = None
+
+  @property
+  def x64(self) -> bool:
+    return self.momentum.dtype == torch.float64
 
   @property
   def velocity(self):
@@ -75,6 +82,8 @@ def velocity_verlet_native(energy, shift, state: NVEState, dt, neighbor, dt_devi
   R = state.position
   N, dim = R.shape
   dev = R.device
+  if state.x64:
+    return _velocity_verlet64(energy, shift, state, dt, neighbor, dt_device, **kwargs)
   dt32 = f32(dt)
   half = f32(dt32 / f32(2))
   mass_s, mass_v = canonicalize_mass(state.mass, dev)
@@ -92,6 +101,37 @@ def velocity_verlet_native(energy, shift, state: NVEState, dt, neighbor, dt_devi
   F_new = energy.force_and_kick(x_new, neighbor, p_new, half_dt=float(half),
                                 dt_device=dt_device, **kwargs)
   return NVEState(x_new, p_new, F_new, state.mass, x_hi_new)
+
+
+def _mass64(mass, device):
+  """The x64 mass: a float or a device float64 [N]."""
+  if isinstance(mass, (torch.Tensor, onp.ndarray)) and onp.ndim(mass) > 0:
+    return 0.0, torch.as_tensor(mass, device=device).to(torch.float64).reshape(-1).contiguous()
+  return float(mass), None
+
+
+def _velocity_verlet64(energy, shift, state: NVEState, dt, neighbor, dt_device=None,
+                       **kwargs) -> NVEState:
+  """velocity Verlet in float64 (the x64 run; dt stays the float32 value the
+  reference casts it to, simulate.py:236)."""
+  if dt_device is not None:
+    raise NotImplementedError('a device dt is float32-only')
+  lib = nat.load()
+  x_hi = state.position_hi.clone()
+  N, dim = x_hi.shape
+  dev = x_hi.device
+  dt32 = float(f32(dt))
+  half = float(f32(f32(dt) / f32(2)))
+  mass_s, mass_v = _mass64(state.mass, dev)
+  x_new = new_positions(N, dim, dev)
+  p_new = state.momentum.clone()
+  nat.check(lib.jmd_nve_kick_drift_f64(
+      nat.ptr(x_hi), nat.ptr(p_new), nat.ptr(state.force), nat.ptr(mass_v), mass_s, N, dim,
+      int(shift.wraps), nat.f3(_side(shift, dim)), dt32, half, nat.ptr(x_new),
+      nat.position_stride(x_new), None, 0.0, None, nat.stream_handle(dev)),
+      'jmd_nve_kick_drift_f64')
+  F_new = energy.force_and_kick(x_hi, neighbor, p_new, half_dt=half, **kwargs)
+  return NVEState(x_new, p_new, F_new, state.mass, x_hi)
 
 
 def new_positions(N, dim, device, like=None):
@@ -124,6 +164,8 @@ def nve(energy_or_force_fn, shift_fn, dt=1e-3, **sim_kwargs):
   shift = native_shift(shift_fn)
 
   def init_fn(key, R, kT, mass=1.0, momenta=None, **kwargs):
+    if isinstance(R, torch.Tensor) and R.dtype == torch.float64:
+      return _init64(key, R, kT, mass, momenta, **kwargs)
     nat.position_stride(R, 'R')
     R = new_positions(R.shape[0], R.shape[1], R.device, R)
     neighbor = kwargs.pop('neighbor', None)
@@ -136,6 +178,20 @@ def nve(energy_or_force_fn, shift_fn, dt=1e-3, **sim_kwargs):
     if isinstance(mass, torch.Tensor) and mass.ndim == 2:
       mass = mass.reshape(-1)
     return NVEState(R, P, F, mass, R.to(torch.float64))
+
+  def _init64(key, R, kT, mass, momenta, **kwargs):
+    nat.require_device_tensor(R, 'R', torch.float64)
+    x_hi = R.clone().contiguous()
+    neighbor = kwargs.pop('neighbor', None)
+    F = energy.force(x_hi, neighbor, **kwargs)
+    if momenta is None:
+      P = _momenta(key, R, kT, mass).to(torch.float64)
+    else:
+      P = momenta.to(device=R.device, dtype=torch.float64).clone().contiguous()
+    if isinstance(mass, torch.Tensor) and mass.ndim == 2:
+      mass = mass.reshape(-1)
+    x32 = new_positions(R.shape[0], R.shape[1], R.device, R.to(torch.float32))
+    return NVEState(x32, P, F, mass, x_hi)
 
   def step_fn(state: NVEState, **kwargs):
     _dt = kwargs.pop('dt', dt)
@@ -212,6 +268,8 @@ class NVELoop:
         nat.stream_handle(state.position.device)), 'jmd_neighbor_list_check')
 
   def _use_reorder(self, state, nbrs) -> bool:
+    if state.x64:
+      return False
     from . import partition
     return bool(self.reorder and nbrs.format is partition.Dense and nbrs.plan.use_cell_list
                 and state.position.shape[0] > 1)
@@ -280,6 +338,46 @@ class NVELoop:
           m32=(torch.empty((N,), dtype=torch.float32, device=dev) if key[-1] else None))
       self._bufs, self._bufs_key = b, key
     return self._bufs
+
+  def _run_steps64(self, state, nbrs, n_steps, events, hook, gate, thr, cps, cs, psides, side,
+                   wraps, pot, idx, width, err, ref, occ, ws):
+    """The x64 step: gated build on the float32 mirror, float64 half kick and
+    drift (mirror and skin test), float64 force with the closing half kick."""
+    lib = nat.load()
+    plan = nbrs.plan
+    spec = plan.spec
+    R = state.position
+    N, dim = R.shape
+    dev = R.device
+    st = nat.stream_handle(dev)
+    stride = nat.position_stride(R)
+    mass_s, mass_v = _mass64(state.mass, dev)
+    geo = self.energy.geometry
+    dsides = nat.d3(geo.side_values(dim))
+    dt32 = float(self.dt)
+    half = float(self.half)
+    for k in range(n_steps):
+      nat.check(lib.jmd_neighbor_list_build(
+          nat.ptr(R), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq),
+          int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0),
+          spec.format_code, int(spec.mask_self), idx, int(nbrs.max_occupancy), occ, err, ref,
+          gate, ws, st), 'jmd_neighbor_list_build')
+      if hook is not None:
+        hook(k)
+      nat.check(lib.jmd_nve_kick_drift_f64(
+          nat.ptr(state.position_hi), nat.ptr(state.momentum), nat.ptr(state.force),
+          nat.ptr(mass_v), mass_s, N, dim, wraps, side, dt32, half, nat.ptr(R), stride, ref,
+          thr, gate, st), 'jmd_nve_kick_drift_f64')
+      if events is not None:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+      nat.check(lib.jmd_pair_energy_force_f64(
+          nat.ptr(state.position_hi), N, dim, int(geo.periodic), dsides, nat.ctypes.byref(pot),
+          idx, width, nat.DENSE, nat.ptr(state.force), None, nat.ptr(state.momentum), half,
+          None, st), 'jmd_pair_energy_force_f64')
+      if events is not None:
+        events.append(_close_timing(e0))
+    return state
 
   def _alternates(self, R, gate):
     """[R, R2] and [gate, gate2]: the second position buffer (R's layout) and
@@ -375,6 +473,9 @@ class NVELoop:
     width = nbrs.idx.shape[1]
     geo = self.energy.geometry
     esides = nat.f3(geo.side_values(dim))
+    if state.x64:
+      return self._run_steps64(state, nbrs, n_steps, events, hook, gate, thr, cps, cs, psides,
+                               side, wraps, pot, idx, width, err, ref, occ, ws)
     if (self.fused and hook is None and gate is not None and plan.use_cell_list
         and fmt == nat.DENSE):
       xs, gs = self._alternates(R, plan.gate)

@@ -144,11 +144,50 @@ class PairEnergy:
     pot.species = ctypes.c_void_p(sp.data_ptr()) if sp is not None else None
     return pot, P
 
+  def _run64(self, R, neighbor, want_force, want_energy_total, dynamic_kwargs,
+             momentum=None, half_dt=0.0):
+    """The float64 path (the reference's x64 run): R, the force and the
+    momentum are float64; Dense lists only."""
+    nat.require_device_tensor(R, 'R', torch.float64)
+    if neighbor.format is not partition.Dense:
+      raise NotImplementedError('float64 positions run over Dense neighbour lists')
+    if momentum is not None:
+      nat.require_device_tensor(momentum, 'momentum', torch.float64)
+    lib = nat.load()
+    N, dim = R.shape
+    geo = self.geometry
+    pot, keep = self._potential(R, dynamic_kwargs)
+    dev = R.device
+    F = torch.empty((N, dim), dtype=torch.float64, device=dev) if want_force else None
+    e_tot = torch.empty((1,), dtype=torch.float64, device=dev) if want_energy_total else None
+    ws = self._workspace64(dev, N) if want_energy_total else None
+    idx = neighbor.idx
+    nat.check(lib.jmd_pair_energy_force_f64(
+        nat.ptr(R), N, dim, int(geo.periodic), nat.d3(geo.side_values(dim)), ctypes.byref(pot),
+        nat.ptr(idx), idx.shape[1], nat.DENSE, nat.ptr(F), nat.ptr(e_tot), nat.ptr(momentum),
+        float(half_dt), nat.ptr(ws), nat.stream_handle(dev)), 'jmd_pair_energy_force_f64')
+    del keep
+    return e_tot, None, F
+
+  def _workspace64(self, device, n):
+    key = ('f64', str(device), int(n))
+    ws = self._ws.get(key)
+    if ws is None:
+      lib = nat.load()
+      ws = torch.empty((lib.jmd_pair_workspace_bytes_f64(n),), dtype=torch.uint8, device=device)
+      self._ws[key] = ws
+    return ws
+
   def _run(self, R, neighbor, want_force, want_energy_total, want_energy_atom,
            dynamic_kwargs, momentum=None, half_dt=0.0, dt_device=None):
-    stride = nat.position_stride(R, 'R')
     if not isinstance(neighbor, partition.NeighborList):
       raise TypeError('energy_fn needs a partition.NeighborList as `neighbor`')
+    if isinstance(R, torch.Tensor) and R.dtype == torch.float64:
+      if want_energy_atom or dt_device is not None:
+        raise NotImplementedError('per-particle energies and device dt are float32-only')
+      return self._run64(R, neighbor, want_force, want_energy_total, dynamic_kwargs,
+                         momentum, half_dt)
+    stride = nat.position_stride(R, 'R')
     lib = nat.load()
     N, dim = R.shape
     geo = self.geometry
@@ -190,10 +229,10 @@ class PairEnergy:
     if R.requires_grad and torch.is_grad_enabled():
       return _EnergyWithForce.apply(R, self, neighbor, dynamic_kwargs)
     e_tot, _, _ = self._run(R, neighbor, False, True, False, dynamic_kwargs)
-    return e_tot.reshape(()).to(torch.float32)
+    return e_tot.reshape(()).to(R.dtype)
 
   def energy_and_force(self, R, neighbor=None, **dynamic_kwargs):
-    """(E float64 0-d, F [N, dim] float32) from one kernel pass."""
+    """(E float64 0-d, F [N, dim] of R's dtype) from one kernel pass."""
     e_tot, _, F = self._run(R, neighbor, True, True, False, dynamic_kwargs)
     return e_tot.reshape(()), F
 

@@ -153,3 +153,72 @@ def test_nve_trajectory_gradient_vs_finite_differences(n_steps, every):
     ad = getattr(res, name)
     print(f'  d/d{name}: adjoint {ad:.6e} fd {fd:.6e}')
     assert abs(ad - fd) <= 1e-4 * abs(fd)
+
+
+def test_x64_force_and_trajectory_match_fp64():
+  """The float64 path: forces against the FP64 oracle to 1e-12 (norm-wise)
+  and a 60-step trajectory against the oracle's float64 velocity Verlet to
+  1e-9 in positions."""
+  print('case x64 force and trajectory', flush=True)
+  jmd, R, P, L, shift, nfn, efn = _lj_system(6, seed=17)
+  R64, P64 = R.astype(np.float64), P.astype(np.float64)
+  Rt = torch.as_tensor(R64, device=DEV)
+  nb = nfn.allocate(Rt.to(torch.float32))
+  E, F = efn.energy_and_force(Rt, nb)
+  assert F.dtype == torch.float64
+  sp = O.Space(np.float64(L))
+  model = O.PairModel('lj', fixed_smoothing=(2.0, 2.5))
+  E64, _, F64 = O.energy_force_brute(model, sp, R64)
+  err = np.linalg.norm(F.cpu().numpy() - F64) / np.linalg.norm(F64)
+  print(f'  force rel err {err:.2e}, energy {float(E):.12f} vs {E64:.12f}')
+  assert err <= 1e-12
+  assert abs(float(E) - E64) <= 1e-12 * abs(E64)
+  init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
+  st = init_fn(0, Rt, kT=0.0, momenta=torch.as_tensor(P64, device=DEV), neighbor=nb)
+  assert st.x64
+  loop = jmd.simulate.NVELoop(efn, shift, 0.005)
+  loop.prime(st, nb)
+  loop.run(st, nb, 60)
+  o = O.NVEState(R64, P64, O.energy_force_brute(model, sp, R64)[2])
+  for _ in range(60):
+    o = O.nve_step(o, lambda X: O.energy_force_brute(model, sp, X)[2], sp, 0.005)
+  dx = np.abs(st.position_hi.cpu().numpy() - o.position)
+  dx = np.minimum(dx, L - dx).max()
+  dp = np.abs(st.momentum.cpu().numpy() - o.momentum).max()
+  print(f'  60 steps: max |dx| {dx:.2e}, max |dp| {dp:.2e}')
+  assert dx <= 1e-9 and dp <= 1e-8
+
+
+def test_x64_trajectory_gradient_long_horizon():
+  """Over 200 steps the float32 trajectory's rounding is amplified past the
+  1e-4 bar against FP64 finite differences (chaos); the x64 form keeps the
+  gradient within 1e-4 of converged central differences (h = 1e-7)."""
+  print('case x64 trajectory gradient T=200', flush=True)
+  jmd, R, P, L, shift, nfn, efn = _lj_system(8, seed=23)
+  R64, P64 = R.astype(np.float64), P.astype(np.float64)
+  dt, T = 0.005, 200
+  res = jmd.adjoint.nve_energy_gradient(efn, shift, nfn, torch.as_tensor(R64, device=DEV),
+                                        torch.as_tensor(P64, device=DEV), dt, T,
+                                        checkpoint_every=20)
+  sp = O.Space(np.float64(L))
+  model = O.PairModel('lj', fixed_smoothing=(2.0, 2.5))
+  E64 = O.nve_energy_after(model, sp, R64, P64, dt, T)
+  print(f'  E_T {res.energy:.10f} vs FP64 {E64:.10f}')
+  assert abs(res.energy - E64) <= 1e-9 * abs(E64)
+  h = 1e-7
+  for name in ('sigma', 'epsilon'):
+    mp = dataclasses.replace(model, **{name: 1.0 + h})
+    mm = dataclasses.replace(model, **{name: 1.0 - h})
+    fd = (O.nve_energy_after(mp, sp, R64, P64, dt, T)
+          - O.nve_energy_after(mm, sp, R64, P64, dt, T)) / (2 * h)
+    ad = getattr(res, name)
+    print(f'  d/d{name}: adjoint {ad:.8e} fd {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}')
+    assert abs(ad - fd) <= 1e-4 * abs(fd)
+  u = np.random.default_rng(29).normal(size=R.shape)
+  u /= np.linalg.norm(u)
+  fd = (O.nve_energy_after(model, sp, R64 + h * u, P64, dt, T)
+        - O.nve_energy_after(model, sp, R64 - h * u, P64, dt, T)) / (2 * h)
+  g = res.position.cpu().numpy()
+  ad = float(np.sum(g * u))
+  print(f'  d/dx0 . u: adjoint {ad:.8e} fd {fd:.8e} |grad| {np.linalg.norm(g):.3e}')
+  assert abs(ad - fd) <= 1e-4 * max(abs(fd), np.linalg.norm(g))
