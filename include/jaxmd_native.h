@@ -223,6 +223,20 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
                           float half_dt, const float* dt_device,
                           void* workspace, void* stream);
 
+/* The virial W_ab = dU/d eps_ab of the pair energy with every displacement
+ * perturbed as (I + eps) d, i.e. weight * sum over the list's ordered pairs
+ * of (U'(r)/r) d_a d_b (weight 1/2 for Dense and Sparse, 1 for
+ * OrderedSparse): what quantity.stress and quantity.pressure obtain with
+ * jax.grad through the `perturbation` argument of the displacement
+ * (quantity.py:202-282, space.py:299-300).  Pair terms float32, sums
+ * float64; virial: device float64 [dim, dim] (symmetric).  workspace:
+ * jmd_pair_virial_workspace_bytes() bytes. */
+size_t jmd_pair_virial_workspace_bytes(void);
+int jmd_pair_virial(const float* position, int64_t n, int32_t dim, int32_t position_stride,
+                    int32_t periodic, const float* side, const jmd_pair_potential* potential,
+                    const int32_t* idx, int64_t idx_width, int32_t format, double* virial,
+                    void* workspace, void* stream);
+
 /* The x64 form (the reference's JAX_ENABLE_X64 run): float64 positions,
  * forces, momenta and pair arithmetic, parameters float32-valued as
  * maybe_downcast leaves them (util.py:109-112).  Dense lists only

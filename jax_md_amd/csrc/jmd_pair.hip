@@ -314,6 +314,115 @@ static void dispatch_dense(int tpa, bool general, const DenseLaunch& L, bool ene
 static int g_tpa_override = 0;
 constexpr long long kSparseBlocks = 2048;
 
+// Virial W_ab = weight sum over the list's ordered pairs of (U'(r)/r) d_a d_b
+// (the box derivative quantity.stress / pressure take by autodiff,
+// quantity.py:202-282): float32 pair terms, float64 accumulation, block
+// partials [6 per block] (xx, yy, zz, xy, xz, yz; 2-D uses xx, yy, xy).
+// Dense: 8 lanes per row; Sparse / OrderedSparse: one thread per entry.
+constexpr int kVirBlocks = 2048;
+
+template <int DIM, int STRIDE, int KIND, bool GENERAL, bool DENSE>
+__global__ __launch_bounds__(256) void k_pair_virial(const float* __restrict__ pos, int n,
+                                                     Space s, PairArgs a,
+                                                     const int* __restrict__ idx,
+                                                     long long width, double weight,
+                                                     double* __restrict__ part) {
+#pragma clang fp contract(off)
+  double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  auto add = [&](unsigned ii, unsigned jj) {
+    float xi[3] = {0, 0, 0}, xj[3] = {0, 0, 0}, d[3] = {0, 0, 0};
+    load_pos<DIM, STRIDE>(pos, ii, xi);
+    load_pos<DIM, STRIDE>(pos, jj, xj);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c)
+      d[c] = s.periodic ? min_image_disp(xj[c], xi[c], xi[c] - s.L[c], s.L[c], s.half[c])
+                        : xj[c] - xi[c];
+    const float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                              : fmaf(d[1], d[1], d[0] * d[0]);
+    float sig = a.sig, eps = a.eps, alp = a.alp;
+    if (GENERAL) {
+      const int si = a.species ? a.species[ii] : 0, sj = a.species ? a.species[jj] : 0;
+      sig = pick(a.sig_mode, a.sig, a.sig_v, ii, jj, si, sj, a.ns);
+      eps = pick(a.eps_mode, a.eps, a.eps_v, ii, jj, si, sj, a.ns);
+      if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, ii, jj, si, sj, a.ns);
+    }
+    float dEr;
+    pair_eval<KIND>(r2, sig, eps, alp, a, &dEr);
+    const double g = (double)dEr;
+    if (DIM == 3) {
+      w[0] += g * ((double)d[0] * d[0]);
+      w[1] += g * ((double)d[1] * d[1]);
+      w[2] += g * ((double)d[2] * d[2]);
+      w[3] += g * ((double)d[0] * d[1]);
+      w[4] += g * ((double)d[0] * d[2]);
+      w[5] += g * ((double)d[1] * d[2]);
+    } else {
+      w[0] += g * ((double)d[0] * d[0]);
+      w[1] += g * ((double)d[1] * d[1]);
+      w[3] += g * ((double)d[0] * d[1]);
+    }
+  };
+  const long long nth = (long long)gridDim.x * blockDim.x;
+  const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (DENSE) {
+    for (long long t = t0; t < (long long)n * 8; t += nth) {
+      const unsigned i = (unsigned)(t / 8), sub = (unsigned)(t % 8);
+      const int* row = idx + (long long)i * width;
+      for (long long k = sub; k < width; k += 8) {
+        const unsigned j = (unsigned)row[k];
+        if (j >= (unsigned)n) break;  // left-compacted rows, padding n
+        add(i, j);
+      }
+    }
+  } else {
+    for (long long k = t0; k < width; k += nth) {
+      const unsigned ra = (unsigned)idx[k], sb = (unsigned)idx[width + k];
+      if (ra < (unsigned)n && sb < (unsigned)n) add(ra, sb);
+    }
+  }
+  __shared__ double s_w[6][256 / kWave];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double v = wave_reduce_sum(w[c]);
+    if (lane_id() == 0) s_w[c][threadIdx.x / kWave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x / kWave); ++q) t += s_w[threadIdx.x][q];
+    part[6LL * blockIdx.x + threadIdx.x] = weight * t;
+  }
+}
+
+// out[dim x dim] (row-major, symmetric) from the block partials, fixed order.
+__global__ __launch_bounds__(256) void k_virial_sum(const double* __restrict__ part, int nb,
+                                                    int dim, double* __restrict__ out) {
+  __shared__ double s[6][256];
+  for (int c = 0; c < 6; ++c) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) v += part[6LL * b + c];
+    s[c][threadIdx.x] = v;
+  }
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int c = 0; c < 6; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (dim == 3) {
+      const double m[9] = {s[0][0], s[3][0], s[4][0], s[3][0], s[1][0],
+                           s[5][0], s[4][0], s[5][0], s[2][0]};
+      for (int k = 0; k < 9; ++k) out[k] = m[k];
+    } else {
+      out[0] = s[0][0];
+      out[1] = s[3][0];
+      out[2] = s[3][0];
+      out[3] = s[1][0];
+    }
+  }
+}
+
 }  // namespace jmd
 
 using namespace jmd;
@@ -435,6 +544,62 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, (int)nb, energy_total);
     JMD_LAUNCH_CHECK();
   }
+  return JMD_OK;
+}
+
+size_t jmd_pair_virial_workspace_bytes(void) { return (size_t)kVirBlocks * 6 * sizeof(double) + 256; }
+
+int jmd_pair_virial(const float* position, int64_t n, int32_t dim, int32_t position_stride,
+                    int32_t periodic, const float* side, const jmd_pair_potential* potential,
+                    const int32_t* idx, int64_t idx_width, int32_t format, double* virial,
+                    void* workspace, void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || idx_width < 0 || !virial ||
+      !workspace || (n > 0 && !position) || (idx_width > 0 && !idx) || (periodic && !side))
+    return JMD_ERR_INVALID;
+  if (format != JMD_DENSE && format != JMD_SPARSE && format != JMD_ORDERED_SPARSE)
+    return JMD_ERR_INVALID;
+  if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  PairArgs a;
+  if (!make_pair_args(potential, &a)) return JMD_ERR_INVALID;
+  Space s = make_space(dim, periodic, side);
+  hipStream_t st = (hipStream_t)stream;
+  const bool dense = format == JMD_DENSE;
+  const long long work = dense ? (long long)n * 8 : idx_width;
+  long long nb = (work + 255) / 256;
+  if (nb > kVirBlocks) nb = kVirBlocks;
+  if (nb < 1) nb = 1;
+  const double weight = format == JMD_ORDERED_SPARSE ? 1.0 : 0.5;
+  double* part = (double*)workspace;
+  const int nn = (int)n;
+#define JMD_VIR(D, S, K)                                                                      \
+  do {                                                                                       \
+    if (a.general) {                                                                         \
+      if (dense) hipLaunchKernelGGL((k_pair_virial<D, S, K, true, true>), dim3(nb), dim3(256), \
+                                    0, st, position, nn, s, a, idx, idx_width, weight, part); \
+      else hipLaunchKernelGGL((k_pair_virial<D, S, K, true, false>), dim3(nb), dim3(256), 0,   \
+                              st, position, nn, s, a, idx, idx_width, weight, part);          \
+    } else {                                                                                 \
+      if (dense) hipLaunchKernelGGL((k_pair_virial<D, S, K, false, true>), dim3(nb),          \
+                                    dim3(256), 0, st, position, nn, s, a, idx, idx_width,     \
+                                    weight, part);                                            \
+      else hipLaunchKernelGGL((k_pair_virial<D, S, K, false, false>), dim3(nb), dim3(256), 0,  \
+                              st, position, nn, s, a, idx, idx_width, weight, part);          \
+    }                                                                                        \
+  } while (0)
+  if (dim == 3 && position_stride == 4) {
+    if (a.kind == JMD_LENNARD_JONES) JMD_VIR(3, 4, JMD_LENNARD_JONES);
+    else JMD_VIR(3, 4, JMD_SOFT_SPHERE);
+  } else if (dim == 3) {
+    if (a.kind == JMD_LENNARD_JONES) JMD_VIR(3, 3, JMD_LENNARD_JONES);
+    else JMD_VIR(3, 3, JMD_SOFT_SPHERE);
+  } else {
+    if (a.kind == JMD_LENNARD_JONES) JMD_VIR(2, 2, JMD_LENNARD_JONES);
+    else JMD_VIR(2, 2, JMD_SOFT_SPHERE);
+  }
+#undef JMD_VIR
+  JMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_virial_sum, dim3(1), dim3(256), 0, st, part, (int)nb, dim, virial);
+  JMD_LAUNCH_CHECK();
   return JMD_OK;
 }
 

@@ -89,3 +89,45 @@ def temperature(*unused_args, momentum=None, velocity=None, mass=1.0):
   q64 = q.to(torch.float64)
   t = q64 ** 2 * m if momentum is None else q64 ** 2 / m
   return (t.sum() / q.numel()).to(q.dtype)
+
+
+def _box_volume(box, dim):
+  """volume(dim, box) (quantity.py:285-300): a scalar side, a [dim] vector of
+  sides or a [dim, dim] box matrix."""
+  import numpy as onp
+  b = box.detach().cpu().numpy() if isinstance(box, torch.Tensor) else onp.asarray(box)
+  b = b.astype(onp.float64)
+  if b.ndim == 0:
+    return float(b) ** dim
+  if b.ndim == 1:
+    return float(onp.prod(b))
+  return float(abs(onp.linalg.det(b)))
+
+
+def pressure(energy_fn, position, box, kinetic_energy=0.0, **kwargs):
+  """(2 KE - dU/d eps) / (dim V) with every displacement scaled by (1 + eps)
+  (quantity.py:202-236): the trace of the native virial kernel's tensor.
+  ``neighbor=`` (and dynamic parameters) go in ``kwargs`` as for the energy
+  function.  Returns a float64 0-d tensor."""
+  energy = native_energy_of(energy_fn)
+  neighbor = kwargs.pop('neighbor', None)
+  dim = position.shape[1]
+  W = energy.virial(position, neighbor, **kwargs)
+  ke = kinetic_energy.to(torch.float64) if isinstance(kinetic_energy, torch.Tensor) \
+      else float(kinetic_energy)
+  return (2.0 * ke - torch.trace(W)) / (dim * _box_volume(box, dim))
+
+
+def stress(energy_fn, position, box, mass=1.0, velocity=None, **kwargs):
+  """(sum m v v^T - dU/d eps) / V with every displacement scaled by
+  (I + eps) (quantity.py:239-282).  Returns float64 [dim, dim]."""
+  energy = native_energy_of(energy_fn)
+  neighbor = kwargs.pop('neighbor', None)
+  dim = position.shape[1]
+  W = energy.virial(position, neighbor, **kwargs)
+  vv = torch.zeros_like(W)
+  if velocity is not None:
+    v = velocity.to(torch.float64)
+    m = _mass_col(mass, v)
+    vv = ((m * v)[:, :, None] * v[:, None, :]).sum(dim=0)
+  return (vv - W) / _box_volume(box, dim)

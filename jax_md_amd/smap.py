@@ -241,6 +241,31 @@ class PairEnergy:
     _, _, F = self._run(R, neighbor, True, False, False, dynamic_kwargs)
     return F
 
+  def virial(self, R, neighbor=None, **dynamic_kwargs):
+    """dU/d eps [dim, dim] float64 for every displacement perturbed as
+    (I + eps) d: weight * sum over the list's pairs of (U'(r)/r) d d^T."""
+    if not isinstance(neighbor, partition.NeighborList):
+      raise TypeError('virial needs a partition.NeighborList as `neighbor`')
+    stride = nat.position_stride(R, 'R')
+    lib = nat.load()
+    N, dim = R.shape
+    geo = self.geometry
+    pot, keep = self._potential(R, dynamic_kwargs)
+    W = torch.empty((dim, dim), dtype=torch.float64, device=R.device)
+    ws = self._ws.get(('virial', str(R.device)))
+    if ws is None:
+      ws = torch.empty((lib.jmd_pair_virial_workspace_bytes(),), dtype=torch.uint8,
+                       device=R.device)
+      self._ws[('virial', str(R.device))] = ws
+    idx = neighbor.idx
+    width = idx.shape[1]  # Dense: row width; Sparse: capacity of the [2, capacity] list
+    nat.check(lib.jmd_pair_virial(
+        nat.ptr(R), N, dim, stride, int(geo.periodic), nat.f3(geo.side_values(dim)),
+        ctypes.byref(pot), nat.ptr(idx), width, _FMT[neighbor.format], nat.ptr(W), nat.ptr(ws),
+        nat.stream_handle(R.device)), 'jmd_pair_virial')
+    del keep
+    return W
+
   def force_and_kick(self, R, neighbor, momentum, half_dt=0.0, dt_device=None, **dynamic_kwargs):
     """Force, plus the closing half kick p += half_dt F fused in the same pass."""
     _, _, F = self._run(R, neighbor, True, False, False, dynamic_kwargs, momentum=momentum,

@@ -631,6 +631,41 @@ def pair_energy_force(model, space, R, i, j, weight=0.5):
   return float(np.sum(E)), e_atom, F
 
 
+def pair_virial(model, space, R, i, j, weight=0.5):
+  """W_ab = weight sum over the ordered pairs (i, j) of (U'(r)/r) d_a d_b,
+  d = disp(R_j, R_i): the derivative dU/d eps of the energy with every
+  displacement perturbed as (I + eps) d, which quantity.stress / pressure
+  take with jax.grad (quantity.py:202-282, space.py:299-300).  FP64."""
+  R = np.asarray(R, F64)
+  dR = space.displacement(R[j], R[i]).astype(F64)
+  r2 = np.sum(dR * dR, axis=-1)
+  sig = model.pair_params(i, j, 'sigma')
+  eps = model.pair_params(i, j, 'epsilon')
+  if model.kind == 'lj':
+    r_on, r_c = model.smoothing()
+    dEr, _ = lj_dE_over_r(r2, sig, eps, r_on, r_c)
+  else:
+    dEr, _ = soft_sphere_dE_over_r(r2, sig, eps, model.pair_params(i, j, 'alpha'))
+  return weight * np.einsum('p,pa,pb->ab', dEr, dR, dR)
+
+
+def stress(W, volume, velocity=None, mass=1.0):
+  """(sum m v v^T - W) / V (quantity.py:239-282)."""
+  W = np.asarray(W, F64)
+  vv = np.zeros_like(W)
+  if velocity is not None:
+    v = np.asarray(velocity, F64)
+    m = mass if np.ndim(mass) == 0 else np.reshape(mass, (-1, 1, 1))
+    vv = np.sum(m * v[:, :, None] * v[:, None, :], axis=0)
+  return (vv - W) / volume
+
+
+def pressure(W, volume, kinetic_energy=0.0):
+  """(2 KE - tr W) / (dim V) (quantity.py:202-236)."""
+  W = np.asarray(W, F64)
+  return (2.0 * kinetic_energy - np.trace(W)) / (W.shape[0] * volume)
+
+
 def pair_force_scale(model, space, R):
   """Per-particle sum of pair-force magnitudes sum_j |f_ij| (FP64): the scale
   against which a net force is judged when it nearly cancels (a packing at

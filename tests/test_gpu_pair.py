@@ -165,3 +165,53 @@ def test_soft_sphere_per_particle_diameters_2d():
   assert abs(float(E) - E64) <= 1e-6 * E64
   rel, worst = _force_err(F.cpu().numpy(), F64)
   assert rel <= FORCE_RTOL and worst <= FORCE_WORST
+
+
+@pytest.mark.parametrize('fmt', ['Dense', 'Sparse', 'OrderedSparse'])
+def test_lammps_lj_stress(fmt):
+  """quantity.stress through the virial kernel against LAMMPS's stress
+  (quantity_test.py:436-461, tol 5e-5) and the FP64 oracle (1e-5)."""
+  print(f'case LAMMPS LJ stress {fmt}', flush=True)
+  jmd = _jmd()
+  d = fixtures.lammps_lj()
+  L = F32(d['box'])
+  R = (d['fractional_position'] * np.float64(L)).astype(F32)
+  disp, _ = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, r_onset=2.5, r_cutoff=2.5,
+                                                    dr_threshold=0.0,
+                                                    format=getattr(jmd.partition, fmt))
+  Rt = torch.as_tensor(R, device=DEV)
+  nbrs = nfn.allocate(Rt)
+  V = torch.as_tensor(d['velocity'], dtype=torch.float32, device=DEV)
+  C = jmd.quantity.stress(efn, Rt, L, velocity=V, neighbor=nbrs).cpu().numpy()
+  v = d['stress_voigt']
+  Cl = np.array([[v[0], v[3], v[4]], [v[3], v[1], v[5]], [v[4], v[5], v[2]]])
+  assert np.allclose(C, Cl, atol=5e-5, rtol=5e-5), (C, Cl)
+  m = O.PairModel('lj', r_onset=2.5, r_cutoff=2.5)
+  sp = O.Space(np.float64(L))
+  R64 = R.astype(np.float64)
+  i, j = O.all_pairs(sp, R64, 2.5)
+  Co = O.stress(O.pair_virial(m, sp, R64, i, j), float(L) ** 3, d['velocity'])
+  print(f'  max |C - oracle| {np.abs(C - Co).max():.2e}, |C - LAMMPS| {np.abs(C - Cl).max():.2e}')
+  assert np.allclose(C, Co, atol=1e-5 * np.abs(Co).max(), rtol=0)
+
+
+@pytest.mark.parametrize('fmt', ['Dense', 'Sparse'])
+def test_jammed_pressure(fmt):
+  """quantity.pressure of the jammed packing (quantity_test.py:134-146, float32
+  tolerance 2e-5)."""
+  print(f'case jammed pressure {fmt}', flush=True)
+  jmd = _jmd()
+  st = fixtures.jammed_state()
+  box = F32(st['box'][0, 0])
+  R = st['real_position'].astype(F32)
+  disp, _ = jmd.space.periodic(box)
+  sigma = torch.as_tensor(st['sigma'], dtype=torch.float32, device=DEV)
+  species = torch.as_tensor(st['species'], device=DEV)
+  nfn, efn = jmd.energy.soft_sphere_neighbor_list(disp, box, species=species, sigma=sigma,
+                                                  format=getattr(jmd.partition, fmt))
+  Rt = torch.as_tensor(R, device=DEV)
+  nbrs = nfn.allocate(Rt)
+  P = float(jmd.quantity.pressure(efn, Rt, box, neighbor=nbrs))
+  print(f'  P {P:.10f} vs {float(st["pressure"]):.10f}')
+  assert abs(P - float(st['pressure'])) <= 2e-5 * abs(float(st['pressure'])) + 2e-5

@@ -28,6 +28,10 @@ Which check pins which oracle function:
                                    sigma, epsilon; a dropped term fails
   fire_energy_gradient             central differences of fire_energy_after
                                    in r and x_0, also across momentum resets
+  pair_virial, stress, pressure    LAMMPS stress fixture (quantity_test.py:
+                                   436-461), jammed pressure fixture
+                                   (:134-146), central differences of an
+                                   affine strain
 """
 
 import math
@@ -250,6 +254,57 @@ def test_lammps_lj_energy():
   m = O.PairModel('lj', r_onset=2.5, r_cutoff=2.5)
   E, _, _ = O.energy_force_brute(m, O.Space(np.float64(L)), R)
   assert abs(E / len(R) - d['energy_per_atom']) < 5e-5
+
+
+def _lammps_stress(d):
+  v = d['stress_voigt']
+  return np.array([[v[0], v[3], v[4]], [v[3], v[1], v[5]], [v[4], v[5], v[2]]])
+
+
+def test_lammps_lj_stress():
+  """The virial stress against LAMMPS (quantity_test.py:436-461, tol 5e-5)."""
+  d = fixtures.lammps_lj()
+  L = d['box']
+  R = d['fractional_position'] * L
+  m = O.PairModel('lj', r_onset=2.5, r_cutoff=2.5)
+  sp = O.Space(np.float64(L))
+  i, j = O.all_pairs(sp, R, 2.5)
+  C = O.stress(O.pair_virial(m, sp, R, i, j), L ** 3, d['velocity'])
+  assert np.allclose(C, _lammps_stress(d), atol=5e-5, rtol=5e-5)
+
+
+def test_jammed_pressure():
+  """The virial pressure of the jammed packing (quantity_test.py:134-146)."""
+  st = fixtures.jammed_state()
+  L = np.diag(st['box']).astype(np.float64)
+  sp = O.Space(L)
+  m = O.PairModel('soft_sphere', sigma=st['sigma'], species=st['species'])
+  R = st['real_position']
+  i, j = O.all_pairs(sp, R, float(np.max(st['sigma'])))
+  P = O.pressure(O.pair_virial(m, sp, R, i, j), float(np.prod(L)))
+  assert abs(P - float(st['pressure'])) <= 1e-12 * abs(float(st['pressure']))
+
+
+def test_virial_is_the_box_derivative():
+  """W = dU/d eps with every displacement scaled by (I + eps): central
+  differences of the energy under an affine box strain (a sign or transpose
+  slip in W fails)."""
+  R, L, model, cutoff = _d2_system('lj', 5)
+  sp = O.Space(np.float64(L))
+  i, j = O.all_pairs(sp, R, cutoff + 0.3)
+  W = O.pair_virial(model, sp, R, i, j)
+  h = 1e-6
+  for a, b in ((0, 0), (0, 1), (2, 1)):
+    E = []
+    for sgn in (1, -1):
+      eps = np.zeros((3, 3))
+      eps[a, b] = sgn * h
+      dR = sp.displacement(R[j], R[i]) @ (np.eye(3) + eps).T
+      r2 = np.sum(dR * dR, -1)
+      _, e = O.lj_dE_over_r(r2, 1.0, 1.0, *model.smoothing())
+      E.append(0.5 * np.sum(e))
+    fd = (E[0] - E[1]) / (2 * h)
+    assert abs(fd - W[a, b]) <= 1e-6 * (abs(W[a, b]) + 1.0), (a, b, fd, W[a, b])
 
 
 def test_lj_closed_forms():
