@@ -430,6 +430,42 @@ int jmd_nve_kick_drift_f64(double* position, double* momentum, const double* for
                            const float* reference_position, float threshold_sq,
                            int32_t* gate, void* stream);
 
+/* Nose-Hoover chain thermostat of simulate.nvt_nose_hoover
+ * (simulate.py:328-347,384-672): the chain (chain = float64 [3, L]: positions,
+ * momenta, masses) is advanced on the device by one thread and hands the
+ * particles one momentum factor per half step (momentum_scale, device
+ * float64 [1]).  mode bits: 4 (init) kinetic = sum of ke_partials, chain at
+ * rest; 1 (finish a step) kinetic = sum of ke_partials (after the closing
+ * kick), half step; 2 (start a step) chain masses kT tau^2 (x dof for the
+ * first), half step.  dt, kT, tau are the float32 values the reference
+ * uses; dof = n * dim. */
+int jmd_nhc_step(int32_t mode, double* chain, int32_t chain_length, int32_t chain_steps,
+                 int32_t sy_steps, float dt, float kT, float tau, double dof,
+                 double* kinetic, const double* ke_partials, int32_t n_partials,
+                 double* momentum_scale, void* stream);
+
+/* p += half_dt F (force may be NULL: no kick) and per-block float64 partials
+ * of the kinetic energy sum p^2 / (2 m) (jmd_kick_kinetic_partials(n) of
+ * them) in a fixed order. */
+int jmd_kick_kinetic_partials(int64_t n);
+int jmd_kick_kinetic(float* momentum, const float* force, const float* mass, float mass_scalar,
+                     int64_t n, int32_t dim, float half_dt, double* ke_partials, void* stream);
+
+/* p *= momentum_scale[0] (the chain's closing factor at the end of a block). */
+int jmd_scale_momentum(float* momentum, int64_t n, int32_t dim, const double* scale,
+                       void* stream);
+
+/* jmd_nve_kick_drift with the thermostat factor applied to p before the
+ * kick (p' = s p + half_dt F), as nvt_nose_hoover's apply_fn scales the
+ * momenta before velocity_verlet (simulate.py:644-652). */
+int jmd_nvt_kick_drift(const double* position_hi, const float* momentum, const float* force,
+                       const float* mass, float mass_scalar, int64_t n, int32_t dim,
+                       int32_t periodic, const float* side, float dt, float half_dt,
+                       double* position_hi_out, float* position_out, int32_t position_stride,
+                       float* momentum_out, const float* reference_position,
+                       float threshold_sq, int32_t* gate, const double* momentum_scale,
+                       void* stream);
+
 /* One step's neighbour-list update fused with the opening half of the next
  * velocity-Verlet step: the loop body
  *   nbrs = nbrs.update(R); state = apply_fn(state, neighbor=nbrs)

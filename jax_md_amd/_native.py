@@ -95,6 +95,17 @@ _SIGNATURES = {
                                            c_int64, c_void_p, c_void_p, c_void_p, c_float,
                                            c_float, c_void_p, c_float, c_void_p]),
     'jmd_pair_workspace_bytes_f64': (c_size_t, [c_int64]),
+    'jmd_nvt_kick_drift': (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int64,
+                                     c_int32, c_int32, POINTER(c_float), c_float, c_float,
+                                     c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_float,
+                                     c_void_p, c_void_p, c_void_p]),
+    'jmd_kick_kinetic_partials': (c_int32, [c_int64]),
+    'jmd_kick_kinetic': (c_int32, [c_void_p, c_void_p, c_void_p, c_float, c_int64, c_int32,
+                                   c_float, c_void_p, c_void_p]),
+    'jmd_nhc_step': (c_int32, [c_int32, c_void_p, c_int32, c_int32, c_int32, c_float, c_float,
+                               c_float, c_double, c_void_p, c_void_p, c_int32, c_void_p,
+                               c_void_p]),
+    'jmd_scale_momentum': (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
     'jmd_pair_virial_workspace_bytes': (c_size_t, []),
     'jmd_pair_virial': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
                                   POINTER(c_float), POINTER(JmdPairPotential), c_void_p, c_int64,

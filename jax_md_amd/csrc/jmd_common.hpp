@@ -145,6 +145,7 @@ struct KickDrift {
   float thr_sq;         // skin criterion (squared)
   int* gate_next;       // raised when any particle moved beyond the skin
   Space s;
+  const double* p_scale;  // thermostat factor applied to p before the kick, or NULL
 };
 
 inline void kick_drift_mass(KickDrift* kd) {
@@ -162,7 +163,8 @@ __device__ __forceinline__ void kick_drift_atom(const KickDrift& kd, long long i
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     const long long k = i * DIM + d;
-    const float pk = kd.p_in[k] + half_dt * kd.F[k];
+    const float pin = kd.p_scale ? kd.p_in[k] * (float)kd.p_scale[0] : kd.p_in[k];
+    const float pk = pin + half_dt * kd.F[k];
     const double v = (double)dt * (double)pk;
     double x = kd.xhi_in[k] + (kd.mass_pow2 ? v * kd.inv_mass : v / m);
     if (kd.s.periodic) x = ref_mod64(x, (double)kd.s.L[d]);

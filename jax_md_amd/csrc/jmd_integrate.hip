@@ -118,11 +118,188 @@ __global__ __launch_bounds__(256) void k_fire_mix(float* __restrict__ p,
   p[k] = (float)(q * keep);
 }
 
+// ---------------------------------------------------------------------------
+// Nose-Hoover chain thermostat (simulate.py:328-347,384-672): the chain is a
+// few float64 scalars, updated by one thread; it hands the particles one
+// momentum factor per half step.  Chain layout: chain[0..L) positions xi,
+// chain[L..2L) momenta p_xi, chain[2L..3L) masses Q.
+// ---------------------------------------------------------------------------
+
+__device__ const double kSYWeights[4][7] = {
+    {1.0, 0, 0, 0, 0, 0, 0},
+    {0.828981543588751, -0.657963087177502, 0.828981543588751, 0, 0, 0, 0},
+    {0.2967324292201065, 0.2967324292201065, -0.186929716880426, 0.2967324292201065,
+     0.2967324292201065, 0, 0},
+    {0.784513610477560, 0.235573213359357, -1.17767998417887, 1.31518632068391,
+     -1.17767998417887, 0.235573213359357, 0.784513610477560}};
+
+constexpr int kMaxChain = 16;
+
+// exp(x) in float64: the chain's arguments are small (|x| << 0.05), where
+// eight Taylor terms are exact to rounding (x^9/9! < 2e-17 relative) and
+// several times cheaper than the library call on one thread.
+__device__ __forceinline__ double chain_exp(double x) {
+  if (fabs(x) > 0.03125) return exp(x);
+  double t = fma(x, 1.0 / 9.0, 1.0);
+  t = fma(x * (1.0 / 8.0), t, 1.0);
+  t = fma(x * (1.0 / 7.0), t, 1.0);
+  t = fma(x * (1.0 / 6.0), t, 1.0);
+  t = fma(x * (1.0 / 5.0), t, 1.0);
+  t = fma(x * (1.0 / 4.0), t, 1.0);
+  t = fma(x * (1.0 / 3.0), t, 1.0);
+  t = fma(x * 0.5, t, 1.0);
+  return fma(x, t, 1.0);
+}
+
+// One substep of length delta (Martyna et al.: backward sweep of the chain
+// momenta, rescaling of the particle momenta, chain positions, forward
+// sweep); returns the particle momentum factor and updates ke.
+__device__ double nhc_substep(double delta, double* xi, double* p, const double* Q,
+                              const double* iQ, int L, double* ke, double dof, double kT) {
+  const double d2 = delta / 2.0, d4 = d2 / 2.0, d8 = d4 / 2.0;
+  const int M = L - 1;
+  double old[kMaxChain];
+  for (int m = 0; m < L; ++m) old[m] = p[m];
+  p[M] += d4 * (old[M - 1] * old[M - 1] * iQ[M - 1] - kT);
+  double carry = p[M];
+  for (int m = M - 1; m >= 1; --m) {
+    const double G = old[m - 1] * old[m - 1] * iQ[m - 1] - kT;
+    const double sc = chain_exp(-d8 * carry * iQ[m + 1]);
+    carry = sc * (sc * old[m] + d4 * G);
+    p[m] = carry;
+  }
+  double sc = chain_exp(-d8 * p[1] * iQ[1]);
+  p[0] = sc * (sc * p[0] + d4 * (2.0 * *ke - dof * kT));
+  const double scale = chain_exp(-d2 * p[0] * iQ[0]);
+  *ke = *ke * scale * scale;
+  for (int m = 0; m < L; ++m) xi[m] += d2 * p[m] * iQ[m];
+  double G = 2.0 * *ke - dof * kT;
+  double q[kMaxChain];
+  for (int m = 0; m < L; ++m) q[m] = p[m];
+  for (int m = 0; m < M; ++m) {
+    sc = chain_exp(-d8 * p[m + 1] * iQ[m + 1]);
+    q[m] = sc * (sc * p[m] + d4 * G);
+    G = q[m] * q[m] * iQ[m] - kT;
+  }
+  q[M] += d4 * G;
+  for (int m = 0; m < L; ++m) p[m] = q[m];
+  return scale;
+}
+
+__device__ double nhc_half_step(double* chain, int L, int chain_steps, int sy_steps, float dt,
+                                double* ke, double dof, double kT) {
+  double* xi = chain;
+  double* p = chain + L;
+  const double* Q = chain + 2 * L;
+  double iQ[kMaxChain];
+  for (int m = 0; m < L; ++m) iQ[m] = 1.0 / Q[m];
+  double total = 1.0;
+  if (chain_steps == 1 && sy_steps == 1)
+    return nhc_substep((double)dt, xi, p, Q, iQ, L, ke, dof, kT);
+  const float delta = dt / (float)chain_steps;
+  const int row = sy_steps == 1 ? 0 : sy_steps == 3 ? 1 : sy_steps == 5 ? 2 : 3;
+  for (int i = 0; i < chain_steps * sy_steps; ++i) {
+    const float d = (float)((double)delta * kSYWeights[row][i % sy_steps]);
+    total *= nhc_substep((double)d, xi, p, Q, iQ, L, ke, dof, kT);
+  }
+  return total;
+}
+
+enum { NHC_FINISH = 1, NHC_START = 2, NHC_INIT = 4 };
+
+// mode NHC_INIT: ke = sum of the partials, chain at rest.  NHC_FINISH: ke =
+// sum of the partials (after the closing kick), half step.  NHC_START:
+// chain masses for kT, half step.  scale = the product of the factors.
+__global__ __launch_bounds__(256) void k_nhc(int mode, double* __restrict__ chain, int L,
+                                             int chain_steps, int sy_steps, float dt, float kT,
+                                             float tau, double dof, double* __restrict__ ke,
+                                             const double* __restrict__ part, int nparts,
+                                             double* __restrict__ scale) {
+  __shared__ double s[256];
+  double v = 0.0;
+  if (mode & (NHC_FINISH | NHC_INIT))
+    for (int b = threadIdx.x; b < nparts; b += 256) v += part[b];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  // the chain is serial: one lane runs it on an LDS copy
+  __shared__ double c[3 * kMaxChain];
+  if (threadIdx.x < 3 * L) c[threadIdx.x] = chain[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double kt = (double)kT;
+  double k = (mode & (NHC_FINISH | NHC_INIT)) ? s[0] : ke[0];
+  double total = 1.0;
+  if (mode & NHC_INIT) {
+    for (int m = 0; m < 2 * L; ++m) c[m] = 0.0;
+  }
+  if (mode & (NHC_INIT | NHC_START)) {
+    const double t = (double)tau;
+    for (int m = 0; m < L; ++m) c[2 * L + m] = (double)kT * t * t * (m == 0 ? dof : 1.0);
+  }
+  if (mode & NHC_FINISH) total *= nhc_half_step(c, L, chain_steps, sy_steps, dt, &k, dof, kt);
+  if (mode & NHC_START) total *= nhc_half_step(c, L, chain_steps, sy_steps, dt, &k, dof, kt);
+  for (int m = 0; m < 3 * L; ++m) chain[m] = c[m];
+  ke[0] = k;
+  scale[0] = total;
+}
+
+// p += half_dt F (optional), and per-block float64 partials of sum p^2 / m.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_kick_ke(float* __restrict__ p, const float* __restrict__ F,
+                                                 const float* __restrict__ mass, float mass_scalar,
+                                                 int n, float half_dt, double* __restrict__ part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double k = 0.0;
+  if (i < n) {
+    const double m = mass ? (double)mass[i] : (double)mass_scalar;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const long long q = (long long)i * DIM + d;
+      float v = p[q];
+      if (F) {
+        v += half_dt * F[q];
+        p[q] = v;
+      }
+      k += (double)v * (double)v;
+    }
+    k /= m;
+  }
+  __shared__ double s_k[256 / kWave];
+  k = wave_reduce_sum(k);
+  if (lane_id() == 0) s_k[threadIdx.x / kWave] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_k[w];
+    part[blockIdx.x] = 0.5 * t;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scale_momentum(float* __restrict__ p, long long m,
+                                                        const double* __restrict__ scale) {
+  const float sc = (float)scale[0];
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m;
+       k += (long long)gridDim.x * blockDim.x)
+    p[k] = p[k] * sc;
+}
+
 }  // namespace jmd
 
 using namespace jmd;
 
 extern "C" {
+
+static int kick_drift_impl(const double* position_hi, const float* momentum,
+                           const float* force, const float* mass, float mass_scalar, int64_t n,
+                           int32_t dim, int32_t periodic, const float* side, float dt,
+                           float half_dt, const float* dt_device, double* position_hi_out,
+                           float* position_out, int32_t position_stride, float* momentum_out,
+                           const float* reference_position, float threshold_sq, int32_t* gate,
+                           const double* momentum_scale, void* stream);
 
 int jmd_nve_kick_drift(const double* position_hi, const float* momentum, const float* force,
                        const float* mass, float mass_scalar, int64_t n, int32_t dim,
@@ -131,6 +308,32 @@ int jmd_nve_kick_drift(const double* position_hi, const float* momentum, const f
                        int32_t position_stride, float* momentum_out,
                        const float* reference_position, float threshold_sq, int32_t* gate,
                        void* stream) {
+  return kick_drift_impl(position_hi, momentum, force, mass, mass_scalar, n, dim, periodic,
+                         side, dt, half_dt, dt_device, position_hi_out, position_out,
+                         position_stride, momentum_out, reference_position, threshold_sq, gate,
+                         nullptr, stream);
+}
+
+int jmd_nvt_kick_drift(const double* position_hi, const float* momentum, const float* force,
+                       const float* mass, float mass_scalar, int64_t n, int32_t dim,
+                       int32_t periodic, const float* side, float dt, float half_dt,
+                       double* position_hi_out, float* position_out, int32_t position_stride,
+                       float* momentum_out, const float* reference_position, float threshold_sq,
+                       int32_t* gate, const double* momentum_scale, void* stream) {
+  if (!momentum_scale) return JMD_ERR_INVALID;
+  return kick_drift_impl(position_hi, momentum, force, mass, mass_scalar, n, dim, periodic,
+                         side, dt, half_dt, nullptr, position_hi_out, position_out,
+                         position_stride, momentum_out, reference_position, threshold_sq, gate,
+                         momentum_scale, stream);
+}
+
+static int kick_drift_impl(const double* position_hi, const float* momentum,
+                           const float* force, const float* mass, float mass_scalar, int64_t n,
+                           int32_t dim, int32_t periodic, const float* side, float dt,
+                           float half_dt, const float* dt_device, double* position_hi_out,
+                           float* position_out, int32_t position_stride, float* momentum_out,
+                           const float* reference_position, float threshold_sq, int32_t* gate,
+                           const double* momentum_scale, void* stream) {
   if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || (periodic && !side) ||
       (n > 0 && (!position_hi || !momentum || !force || !position_hi_out || !position_out ||
                  !momentum_out)))
@@ -159,6 +362,7 @@ int jmd_nve_kick_drift(const double* position_hi, const float* momentum, const f
   kd.thr_sq = threshold_sq;
   kd.gate_next = kd.ref ? gate : nullptr;
   kd.s = s;
+  kd.p_scale = momentum_scale;
   dim3 grid(ceil_div(nn, 256)), block(256);
 #define JMD_KD(D, S) hipLaunchKernelGGL((k_kick_drift<D, S>), grid, block, 0, st, kd, nn)
   if (dim == 3 && position_stride == 4) JMD_KD(3, 4);
@@ -214,6 +418,54 @@ int jmd_fire_update(float* momentum, const float* force, int64_t n, int32_t dim,
                          nn, scalars);
     JMD_LAUNCH_CHECK();
   }
+  return JMD_OK;
+}
+
+int jmd_kick_kinetic_partials(int64_t n) { return n <= 0 ? 1 : ceil_div(n, 256); }
+
+int jmd_kick_kinetic(float* momentum, const float* force, const float* mass, float mass_scalar,
+                     int64_t n, int32_t dim, float half_dt, double* ke_partials, void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || !ke_partials ||
+      (n > 0 && !momentum) || (!mass && !(mass_scalar > 0.0f)))
+    return JMD_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const int nn = (int)n;
+  dim3 grid(jmd_kick_kinetic_partials(n)), block(256);
+  if (dim == 3)
+    hipLaunchKernelGGL(k_kick_ke<3>, grid, block, 0, st, momentum, force, mass, mass_scalar, nn,
+                       half_dt, ke_partials);
+  else
+    hipLaunchKernelGGL(k_kick_ke<2>, grid, block, 0, st, momentum, force, mass, mass_scalar, nn,
+                       half_dt, ke_partials);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+int jmd_nhc_step(int32_t mode, double* chain, int32_t chain_length, int32_t chain_steps,
+                 int32_t sy_steps, float dt, float kT, float tau, double dof, double* kinetic,
+                 const double* ke_partials, int32_t n_partials, double* momentum_scale,
+                 void* stream) {
+  if (!chain || !kinetic || !momentum_scale || chain_length < 2 || chain_length > kMaxChain ||
+      chain_steps < 1 || (sy_steps != 1 && sy_steps != 3 && sy_steps != 5 && sy_steps != 7) ||
+      mode < 1 || mode > 7 || ((mode & (NHC_FINISH | NHC_INIT)) && (!ke_partials || n_partials < 1)))
+    return JMD_ERR_INVALID;
+  hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, (hipStream_t)stream, (int)mode, chain,
+                     (int)chain_length, (int)chain_steps, (int)sy_steps, dt, kT, tau, dof, kinetic,
+                     ke_partials, (int)n_partials, momentum_scale);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+int jmd_scale_momentum(float* momentum, int64_t n, int32_t dim, const double* scale,
+                       void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || !scale || (n > 0 && !momentum)) return JMD_ERR_INVALID;
+  const long long m = (long long)n * dim;
+  long long nb = (m + 255) / 256;
+  if (nb > 2048) nb = 2048;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_scale_momentum, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream,
+                     momentum, m, scale);
+  JMD_LAUNCH_CHECK();
   return JMD_OK;
 }
 

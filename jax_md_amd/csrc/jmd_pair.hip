@@ -194,19 +194,34 @@ __global__ __launch_bounds__(256) void k_pair_dense(
   }
 }
 
-// Sparse / OrderedSparse: one thread per stored pair, forces scattered with
-// atomics to both ends (smap.py:926-931,975-977).
+// Sparse / OrderedSparse: one lane per stored pair (smap.py:926-931,975-977).
+// The sender's share goes out with one atomic per component; the receiver's
+// is first summed over each run of equal receiver ids inside the wavefront
+// (a segmented inclusive scan; lists built here are receiver-major, so a run
+// is a row piece) and the run's last lane adds it.  Any order is correct;
+// ordered input only makes the runs longer.
 template <int DIM, int STRIDE, int KIND, bool GENERAL, bool ENERGY>
 __global__ __launch_bounds__(256) void k_pair_sparse(
     const float* __restrict__ pos, int n, Space s, PairArgs a, const int* __restrict__ idx,
     long long width, float inv_norm, float* __restrict__ force, float* __restrict__ e_atom,
     double* __restrict__ e_part) {
+  constexpr int NV = DIM + (ENERGY ? 1 : 0);
+  const int lane = lane_id();
   double e = 0.0;
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < width;
-       k += (long long)gridDim.x * blockDim.x) {
-    int ra = idx[k];           // receiver (idx[0])
-    int sb = idx[width + k];   // sender   (idx[1])
-    if (ra < n && sb < n) {
+  // the loop runs per wavefront, so every lane reaches the shuffles
+  for (long long k0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1));
+       k0 < width; k0 += (long long)gridDim.x * blockDim.x) {
+    const long long k = k0 + lane;
+    int ra = -1, sb = -1;
+    if (k < width) {
+      ra = idx[k];          // receiver (idx[0])
+      sb = idx[width + k];  // sender   (idx[1])
+    }
+    const bool ok = (unsigned)ra < (unsigned)n && (unsigned)sb < (unsigned)n;
+    float v[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) v[c] = 0.0f;
+    if (ok) {
       float xa[3] = {0, 0, 0}, xb[3] = {0, 0, 0}, d[3];  // disp(R[idx0], R[idx1]) = R_a - R_b
       load_pos<DIM, STRIDE>(pos, ra, xa);
       load_pos<DIM, STRIDE>(pos, sb, xb);
@@ -229,25 +244,49 @@ __global__ __launch_bounds__(256) void k_pair_sparse(
       if (force) {
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
-          float fc = dEr * d[c];  // dE/dR_a = dEr * (R_a - R_b)
-          atomicAdd(&force[(long long)ra * DIM + c], -fc);
+          const float fc = dEr * d[c];  // dE/dR_a = dEr * (R_a - R_b)
           atomicAdd(&force[(long long)sb * DIM + c], fc);
+          v[c] = -fc;
         }
       }
       if (ENERGY) {
         e += (double)u;
-        if (e_atom) atomicAdd(&e_atom[ra], u);
+        v[DIM] = u;
       }
+    }
+    if (!force && !(ENERGY && e_atom)) continue;
+    // segmented scan over runs of equal receiver ids (invalid lanes: key -1)
+    const int key = ok ? ra : -1;
+    const int prev = __shfl_up(key, 1, kWave);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    const int h = 63 - __clzll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      float t[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) t[c] = __shfl_up(v[c], off, kWave);
+      if (lane - off >= h) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) v[c] += t[c];
+      }
+    }
+    const bool tail = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
+    if (tail && ok) {
+      if (force) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) atomicAdd(&force[(long long)ra * DIM + c], v[c]);
+      }
+      if (ENERGY && e_atom) atomicAdd(&e_atom[ra], v[DIM]);
     }
   }
   if (ENERGY) {
     __shared__ double s_e[256 / kWave];
-    double v = wave_reduce_sum(e);
-    if (lane_id() == 0) s_e[threadIdx.x / kWave] = v;
+    double w = wave_reduce_sum(e);
+    if (lane_id() == 0) s_e[threadIdx.x / kWave] = w;
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_e[w];
+      for (int q = 0; q < (int)(blockDim.x / kWave); ++q) t += s_e[q];
       e_part[blockIdx.x] = t;
     }
   }

@@ -522,3 +522,257 @@ class NVELoop:
       if events is not None:
         events.append(_close_timing(e0))
     return state
+
+
+# ---------------------------------------------------------------------------
+# NVT with a Nose-Hoover chain (simulate.py:384-705)
+# ---------------------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class NVTNoseHooverState:
+  """position, momentum, force [N, dim] float32; mass; chain float64 [3, L]
+  (positions, momenta, masses of the chain); kinetic_energy float64 [1] (the
+  value the chain last saw); position_hi the float64 master positions."""
+  position: torch.Tensor
+  momentum: torch.Tensor
+  force: torch.Tensor
+  mass: object
+  chain: torch.Tensor
+  kinetic_energy: torch.Tensor
+  position_hi: Optional[torch.Tensor] = None
+
+  @property
+  def velocity(self):
+    m = self.mass.reshape(-1, 1) if isinstance(self.mass, torch.Tensor) and self.mass.ndim else self.mass
+    return self.momentum / m
+
+  @property
+  def x64(self) -> bool:
+    return False
+
+  def set(self, **kw):
+    return dataclasses.replace(self, **kw)
+
+
+class _Thermostat:
+  """The chain's constants and device scratch (partials, factor)."""
+
+  def __init__(self, dt, kT, chain_length, chain_steps, sy_steps, tau):
+    if sy_steps not in (1, 3, 5, 7):
+      raise ValueError('sy_steps must be 1, 3, 5 or 7')
+    if not 2 <= chain_length <= 16:
+      raise ValueError('chain_length must be between 2 and 16')
+    self.dt = f32(dt)
+    self.kT = f32(kT)
+    self.tau = f32(self.dt * 100) if tau is None else f32(tau)
+    self.L, self.chain_steps, self.sy_steps = int(chain_length), int(chain_steps), int(sy_steps)
+    self._bufs = {}
+
+  def buffers(self, N, device):
+    key = (int(N), str(device))
+    b = self._bufs.get(key)
+    if b is None:
+      lib = nat.load()
+      b = dict(part=torch.zeros((lib.jmd_kick_kinetic_partials(N),), dtype=torch.float64,
+                                device=device),
+               scale=torch.ones((1,), dtype=torch.float64, device=device))
+      self._bufs = {key: b}
+    return b
+
+  def step(self, mode, state, N, dim, kT=None, st=None):
+    lib = nat.load()
+    b = self.buffers(N, state.momentum.device)
+    kT = self.kT if kT is None else f32(kT)
+    nat.check(lib.jmd_nhc_step(
+        int(mode), nat.ptr(state.chain), self.L, self.chain_steps, self.sy_steps,
+        float(self.dt), float(kT), float(self.tau), float(N * dim), nat.ptr(state.kinetic_energy),
+        nat.ptr(b['part']), int(b['part'].numel()), nat.ptr(b['scale']),
+        st if st is not None else nat.stream_handle(state.momentum.device)), 'jmd_nhc_step')
+    return b['scale']
+
+  def kinetic(self, state, F, half, st=None):
+    """p += half F (F None: no kick) with the kinetic-energy partials."""
+    lib = nat.load()
+    N, dim = state.momentum.shape
+    dev = state.momentum.device
+    b = self.buffers(N, dev)
+    mass_s, mass_v = canonicalize_mass(state.mass, dev)
+    nat.check(lib.jmd_kick_kinetic(
+        nat.ptr(state.momentum), nat.ptr(F), nat.ptr(mass_v), mass_s if mass_v is None else 0.0,
+        N, dim, float(half), nat.ptr(b['part']),
+        st if st is not None else nat.stream_handle(dev)), 'jmd_kick_kinetic')
+
+
+_NHC_FINISH, _NHC_START, _NHC_INIT = 1, 2, 4
+
+
+def nvt_nose_hoover(energy_or_force_fn, shift_fn, dt, kT, chain_length=5, chain_steps=2,
+                    sy_steps=3, tau=None, **sim_kwargs):
+  """NVT with a Nose-Hoover chain thermostat (simulate.py:562-660):
+  ``(init_fn(key, R, mass=1.0, momenta=None, **kw), apply_fn(state, **kw))``.
+  Per step: chain masses and a chain half step (one momentum factor),
+  velocity Verlet, the kinetic energy, a second chain half step.  The chain
+  runs on the device in float64; ``kT=`` in ``apply_fn`` overrides the bath
+  temperature as in the reference."""
+  energy = quantity.native_energy_of(energy_or_force_fn)
+  shift = native_shift(shift_fn)
+  th = _Thermostat(dt, kT, chain_length, chain_steps, sy_steps, tau)
+
+  def init_fn(key, R, mass=1.0, momenta=None, **kwargs):
+    _kT = kwargs.pop('kT', kT)
+    nat.position_stride(R, 'R')
+    R = new_positions(R.shape[0], R.shape[1], R.device, R)
+    neighbor = kwargs.pop('neighbor', None)
+    F = energy.force(R, neighbor, **kwargs)
+    if momenta is None:
+      P = _momenta(key, R, float(f32(_kT)), mass)
+    else:
+      P = momenta.to(device=R.device, dtype=torch.float32).clone().contiguous()
+    if isinstance(mass, torch.Tensor) and mass.ndim == 2:
+      mass = mass.reshape(-1)
+    N, dim = R.shape
+    state = NVTNoseHooverState(R, P, F, mass,
+                               torch.zeros((3, th.L), dtype=torch.float64, device=R.device),
+                               torch.zeros((1,), dtype=torch.float64, device=R.device),
+                               R.to(torch.float64))
+    th.kinetic(state, None, 0.0)
+    th.step(_NHC_INIT, state, N, dim, kT=_kT)
+    return state
+
+  def apply_fn(state: NVTNoseHooverState, **kwargs):
+    _kT = kwargs.pop('kT', kT)
+    neighbor = kwargs.pop('neighbor', None)
+    lib = nat.load()
+    R = state.position
+    N, dim = R.shape
+    dev = R.device
+    st = nat.stream_handle(dev)
+    new = state.set(momentum=state.momentum.clone(), chain=state.chain.clone(),
+                    kinetic_energy=state.kinetic_energy.clone())
+    scale = th.step(_NHC_START, new, N, dim, kT=_kT)
+    mass_s, mass_v = canonicalize_mass(state.mass, dev)
+    x_hi = state.position_hi if state.position_hi is not None else R.to(torch.float64)
+    x_hi_new = torch.empty_like(x_hi)
+    x_new = new_positions(N, dim, dev)
+    half = f32(th.dt / f32(2))
+    nat.check(lib.jmd_nvt_kick_drift(
+        nat.ptr(x_hi), nat.ptr(new.momentum), nat.ptr(state.force), nat.ptr(mass_v),
+        mass_s if mass_v is None else 0.0, N, dim, int(shift.wraps), nat.f3(_side(shift, dim)),
+        float(th.dt), float(half), nat.ptr(x_hi_new), nat.ptr(x_new), nat.position_stride(x_new),
+        nat.ptr(new.momentum), None, 0.0, None, nat.ptr(scale), st), 'jmd_nvt_kick_drift')
+    F = energy.force(x_new, neighbor, **kwargs)
+    new = new.set(position=x_new, force=F, position_hi=x_hi_new)
+    th.kinetic(new, F, half)
+    scale = th.step(_NHC_FINISH, new, N, dim, kT=_kT)
+    nat.check(lib.jmd_scale_momentum(nat.ptr(new.momentum), N, dim, nat.ptr(scale), st),
+              'jmd_scale_momentum')
+    return new
+
+  init_fn.thermostat = th
+  return init_fn, apply_fn
+
+
+def nvt_nose_hoover_invariant(energy_fn, state: NVTNoseHooverState, kT, **kwargs) -> float:
+  """Extended-system energy of the NVT chain (simulate.py:675-705), float64."""
+  energy = quantity.native_energy_of(energy_fn)
+  neighbor = kwargs.pop('neighbor', None)
+  E, _ = energy.energy_and_force(state.position, neighbor, **kwargs)
+  KE = float(quantity.kinetic_energy(momentum=state.momentum.to(torch.float64),
+                                     mass=state.mass))
+  c = state.chain.cpu().numpy()
+  L = c.shape[1]
+  xi, p, Q = c[0], c[1], c[2]
+  dof = state.position.numel()
+  kT = float(f32(kT))
+  H = float(E) + KE + p[0] ** 2 / (2 * Q[0]) + dof * kT * xi[0]
+  for m in range(1, L):
+    H += p[m] ** 2 / (2 * Q[m]) + kT * xi[m]
+  return H
+
+
+class NVTLoop(NVELoop):
+  """``n_steps`` of nvt_nose_hoover's apply_fn with the neighbour list
+  maintained on the device (the reference's ``fori_loop`` of ``update`` and
+  ``apply_fn``, notebooks/lj_benchmark.ipynb), one HIP graph per block like
+  ``NVELoop``.  Per step: the gated list update, the half kick and drift
+  with the chain's factor, the force, the closing kick with the kinetic
+  energy partials, and the chain (closing half step of this step and
+  opening half step of the next in one launch)."""
+
+  def __init__(self, energy_or_force_fn, shift_fn, dt, kT, chain_length=5, chain_steps=2,
+               sy_steps=3, tau=None, reorder: bool = True, graph: bool = True):
+    super().__init__(energy_or_force_fn, shift_fn, dt, reorder=reorder, graph=graph,
+                     fused=False)
+    self.thermostat = _Thermostat(dt, kT, chain_length, chain_steps, sy_steps, tau)
+    self._nvt_state = None
+
+  def run(self, state: NVTNoseHooverState, nbrs, n_steps: int, events=None):
+    self._nvt_state = state
+    return super().run(state, nbrs, n_steps, events)
+
+  def _key(self, state, nbrs, n_steps, reorder):
+    return super()._key(state, nbrs, n_steps, reorder) + (state.chain.data_ptr(),
+                                                          state.kinetic_energy.data_ptr())
+
+  def _run_steps(self, state, nbrs, n_steps: int, events=None, hook=None):
+    if hook is not None:
+      raise NotImplementedError('NVTLoop runs without a step hook')
+    lib = nat.load()
+    th = self.thermostat
+    chain_state = self._nvt_state
+    plan = nbrs.plan
+    spec = plan.spec
+    R = state.position
+    N, dim = R.shape
+    dev = R.device
+    st = nat.stream_handle(dev)
+    stride = nat.position_stride(R)
+    if state.position_hi is None:
+      state.position_hi = R.to(torch.float64)
+    mass_s, mass_v = canonicalize_mass(state.mass, dev)
+    side = nat.f3(_side(self.shift, dim))
+    gate = None if spec.always_rebuild else nat.ptr(plan.gate)
+    thr = float(spec.threshold_sq)
+    cps, cs = nat.i3(plan.cells_per_side), nat.f3(plan.cell_size3)
+    psides = nat.f3(plan.side)
+    idx, err, ref = nat.ptr(nbrs.idx), nat.ptr(nbrs.error.code), nat.ptr(nbrs.reference_position)
+    occ, ws = nat.ptr(plan.occupancy), plan.workspace.ptr
+    pot, keep = self.energy._potential(R, {})
+    self._keep = (pot, keep)
+    pws = nat.ptr(self.energy._workspace(dev, N))
+    fmt = spec.format_code
+    width = nbrs.idx.shape[1]
+    geo = self.energy.geometry
+    esides = nat.f3(geo.side_values(dim))
+    b = th.buffers(N, dev)
+    kin = dataclasses.replace(chain_state, momentum=state.momentum, mass=state.mass)
+    if n_steps <= 0:
+      return state
+    th.step(_NHC_START, kin, N, dim, st=st)
+    for k in range(n_steps):
+      nat.check(lib.jmd_neighbor_list_build(
+          nat.ptr(R), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq),
+          int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0), fmt,
+          int(spec.mask_self), idx, int(nbrs.max_occupancy), occ, err, ref, gate, ws, st),
+          'jmd_neighbor_list_build')
+      nat.check(lib.jmd_nvt_kick_drift(
+          nat.ptr(state.position_hi), nat.ptr(state.momentum), nat.ptr(state.force),
+          nat.ptr(mass_v), mass_s if mass_v is None else 0.0, N, dim, int(self.shift.wraps),
+          side, float(self.dt), float(self.half), nat.ptr(state.position_hi), nat.ptr(R), stride,
+          nat.ptr(state.momentum), ref, thr, gate, nat.ptr(b['scale']), st),
+          'jmd_nvt_kick_drift')
+      if events is not None:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+      nat.check(lib.jmd_pair_energy_force(
+          nat.ptr(R), N, dim, stride, int(geo.periodic), esides, nat.ctypes.byref(pot), idx, width,
+          fmt, nat.ptr(state.force), None, None, None, 0.0, None, pws, st),
+          'jmd_pair_energy_force')
+      if events is not None:
+        events.append(_close_timing(e0))
+      th.kinetic(state, state.force, self.half, st=st)
+      th.step(_NHC_FINISH | _NHC_START if k + 1 < n_steps else _NHC_FINISH, kin, N, dim, st=st)
+    nat.check(lib.jmd_scale_momentum(nat.ptr(state.momentum), N, dim, nat.ptr(b['scale']), st),
+              'jmd_scale_momentum')
+    return state

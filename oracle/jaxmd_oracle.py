@@ -973,6 +973,118 @@ def fire_energy_gradient(model, space, R, n_steps, params, dsigma):
   return E_T, theta + ts, X + Jv
 
 
+# Nose-Hoover chain thermostat (simulate.py:328-347,384-672; Martyna et al.
+# 1992/1996 with Suzuki-Yoshida substeps).  The weights are the published
+# constants of the scheme.
+SY_WEIGHTS = {
+    1: [1.0],
+    3: [0.828981543588751, -0.657963087177502, 0.828981543588751],
+    5: [0.2967324292201065, 0.2967324292201065, -0.186929716880426, 0.2967324292201065,
+        0.2967324292201065],
+    7: [0.784513610477560, 0.235573213359357, -1.17767998417887, 1.31518632068391,
+        -1.17767998417887, 0.235573213359357, 0.784513610477560],
+}
+
+
+@dataclass
+class NHCState:
+  position: np.ndarray
+  momentum: np.ndarray
+  force: np.ndarray
+  mass: object
+  xi: np.ndarray        # chain positions [L]
+  p_xi: np.ndarray      # chain momenta [L]
+  Q: np.ndarray         # chain masses [L]
+  ke: float             # kinetic energy the chain last saw
+  dof: int
+
+
+def nhc_masses(L, kT, tau, dof):
+  Q = F64(F32(kT)) * F64(F32(tau)) ** 2 * np.ones(L)
+  Q[0] *= dof
+  return Q
+
+
+def nhc_substep(delta, xi, p_xi, Q, ke, dof, kT):
+  """One chain substep of length delta; returns (xi, p_xi, ke, momentum scale)."""
+  d2, d4, d8 = delta / 2, delta / 4, delta / 8
+  L = len(p_xi)
+  M = L - 1
+  p = p_xi.copy()
+  p[M] += d4 * (p_xi[M - 1] ** 2 / Q[M - 1] - kT)
+  carry = p[M]
+  for m in range(M - 1, 0, -1):   # backward sweep, old neighbours below
+    G = p_xi[m - 1] ** 2 / Q[m - 1] - kT
+    s = np.exp(-d8 * carry / Q[m + 1])
+    carry = s * (s * p_xi[m] + d4 * G)
+    p[m] = carry
+  s = np.exp(-d8 * p[1] / Q[1])
+  p[0] = s * (s * p[0] + d4 * (2.0 * ke - dof * kT))
+  scale = np.exp(-d2 * p[0] / Q[0])
+  ke = ke * scale ** 2
+  xi = xi + d2 * p / Q
+  G = 2.0 * ke - dof * kT
+  q = p.copy()
+  for m in range(M):               # forward sweep, neighbours above as left
+    s = np.exp(-d8 * p[m + 1] / Q[m + 1])
+    q[m] = s * (s * p[m] + d4 * G)
+    G = q[m] ** 2 / Q[m] - kT
+  q[M] += d4 * G
+  return xi, q, ke, scale
+
+
+def nhc_half_step(dt, chain_steps, sy_steps, xi, p_xi, Q, ke, dof, kT):
+  """Half a step of the chain (simulate.py:494-507); the momentum scale is
+  the product over the substeps."""
+  total = 1.0
+  if chain_steps == 1 and sy_steps == 1:
+    subs = [F64(F32(dt))]
+  else:
+    delta = F64(F32(dt)) / chain_steps
+    ws = SY_WEIGHTS[sy_steps]
+    subs = [F64(F32(delta * ws[i % sy_steps])) for i in range(chain_steps * sy_steps)]
+  for d in subs:
+    xi, p_xi, ke, sc = nhc_substep(d, xi, p_xi, Q, ke, dof, kT)
+    total *= sc
+  return xi, p_xi, ke, total
+
+
+def nvt_init(R, P, force_fn, kT, dt, mass=1.0, chain_length=5, tau=None):
+  R = np.asarray(R, F64)
+  P = np.asarray(P, F64)
+  dof = R.size
+  tau = F32(dt) * 100 if tau is None else tau
+  m = mass if np.ndim(mass) == 0 else np.reshape(mass, (-1, 1))
+  ke = 0.5 * float(np.sum(P ** 2 / m))
+  L = chain_length
+  return NHCState(R, P, force_fn(R), mass, np.zeros(L), np.zeros(L),
+                  nhc_masses(L, kT, tau, dof), ke, dof), F64(F32(tau))
+
+
+def nvt_step(st, force_fn, space, dt, kT, tau, chain_steps=2, sy_steps=3):
+  """NVT Nose-Hoover step (simulate.py:644-660): chain masses, chain half
+  step, velocity Verlet, kinetic energy, chain half step."""
+  Q = nhc_masses(len(st.Q), kT, tau, st.dof)
+  xi, p_xi, ke, s1 = nhc_half_step(dt, chain_steps, sy_steps, st.xi, st.p_xi, Q, st.ke,
+                                   st.dof, kT)
+  nv = nve_step(NVEState(st.position, st.momentum * s1, st.force, st.mass), force_fn, space,
+                dt)
+  m = st.mass if np.ndim(st.mass) == 0 else np.reshape(st.mass, (-1, 1))
+  ke = 0.5 * float(np.sum(nv.momentum ** 2 / m))
+  xi, p_xi, ke, s2 = nhc_half_step(dt, chain_steps, sy_steps, xi, p_xi, Q, ke, st.dof, kT)
+  return NHCState(nv.position, nv.momentum * s2, nv.force, st.mass, xi, p_xi, Q, ke, st.dof)
+
+
+def nvt_invariant(st, energy, kT):
+  """Extended-system energy (simulate.py:675-705)."""
+  m = st.mass if np.ndim(st.mass) == 0 else np.reshape(st.mass, (-1, 1))
+  E = energy + 0.5 * float(np.sum(st.momentum ** 2 / m))
+  E += st.p_xi[0] ** 2 / (2 * st.Q[0]) + st.dof * kT * st.xi[0]
+  for r, p, q in zip(st.xi[1:], st.p_xi[1:], st.Q[1:]):
+    E += p ** 2 / (2 * q) + kT * r
+  return E
+
+
 def kinetic_energy(momentum, mass=1.0):
   """0.5 sum p^2/m (quantity.py:124-159)."""
   m = mass if np.ndim(mass) == 0 else np.reshape(mass, (-1, 1))

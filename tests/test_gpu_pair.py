@@ -78,6 +78,45 @@ def test_lj_formats_agree(fmt):
   assert rel <= FORCE_RTOL and worst <= FORCE_WORST
 
 
+@pytest.mark.parametrize('fmt', ['Sparse', 'OrderedSparse'])
+def test_sparse_any_pair_order_and_per_particle(fmt):
+  """The sparse kernel sums receiver shares over runs of equal receiver ids;
+  a shuffled pair order (runs of one, the same id in several runs of one
+  wavefront) must give the same forces and per-particle energies."""
+  print(f'case sparse shuffled {fmt}', flush=True)
+  import dataclasses
+  jmd, R, L, Rt, nbrs, efn = _lj_setup(8, fmt, seed=3)
+  disp, _ = jmd.space.periodic(L)
+  _, efn_pp = jmd.energy.lennard_jones_neighbor_list(
+      disp, L, format=getattr(jmd.partition, fmt), per_particle=True)
+  g = torch.Generator().manual_seed(5)
+  perm = torch.randperm(nbrs.idx.shape[1], generator=g).to(DEV)
+  # half shuffled, half in receiver order with every other pair swapped
+  # with its neighbour, so runs break up inside a wavefront
+  idx = nbrs.idx.clone()
+  half = idx.shape[1] // 2
+  idx[:, :half] = nbrs.idx[:, perm[perm < half]]
+  sw = torch.arange(half, idx.shape[1] - 1, 2, device=DEV)
+  a, b = idx[:, sw].clone(), idx[:, sw + 1].clone()
+  idx[:, sw], idx[:, sw + 1] = b, a
+  shuffled = dataclasses.replace(nbrs, idx=idx)
+  m = O.PairModel('lj')
+  _, e64, F64 = O.energy_force_brute(m, O.Space(np.float64(L)), R.astype(np.float64))
+  for nb in (nbrs, shuffled):
+    E, F = efn.energy_and_force(Rt, nb)
+    rel, worst = _force_err(F.cpu().numpy(), F64)
+    print(f'  force rel={rel:.3e} worst={worst:.3e}')
+    assert rel <= FORCE_RTOL and worst <= FORCE_WORST
+    assert abs(float(E) - e64.sum()) <= 1e-6 * abs(e64.sum())
+    if fmt == 'OrderedSparse':  # per-particle values are refused (smap.py:967-972)
+      with pytest.raises(ValueError):
+        efn_pp(Rt, nb)
+      continue
+    e = efn_pp(Rt, nb).cpu().numpy()
+    print(f'  per-particle {np.max(np.abs(e - e64)) / np.max(np.abs(e64)):.2e}')
+    assert np.max(np.abs(e - e64)) <= 1e-5 * np.max(np.abs(e64))
+
+
 def test_lj_per_particle_and_autograd():
   print('case LJ per-particle + autograd', flush=True)
   jmd, R, L, Rt, nbrs, efn = _lj_setup(8)

@@ -28,6 +28,9 @@ Which check pins which oracle function:
                                    sigma, epsilon; a dropped term fails
   fire_energy_gradient             central differences of fire_energy_after
                                    in r and x_0, also across momentum resets
+  nvt_step (Nose-Hoover chain)     invariant conserved for sy = 1, 3, 5, 7 on
+                                   the jammed fixture (simulate_test.py:
+                                   317-350), second order in dt
   pair_virial, stress, pressure    LAMMPS stress fixture (quantity_test.py:
                                    436-461), jammed pressure fixture
                                    (:134-146), central differences of an
@@ -562,3 +565,52 @@ def test_fire_adjoint_across_momentum_resets():
   fd = (O.fire_energy_after(_fire_system(r + h)[2], sp, R, T, par)
         - O.fire_energy_after(_fire_system(r - h)[2], sp, R, T, par)) / (2 * h)
   assert abs(th - fd) <= 1e-7 * abs(fd), (th, fd)
+
+
+def _jammed_nvt_setup(kT=1e-3, seed=0):
+  st = fixtures.jammed_state()
+  L = np.diag(st['box']).astype(np.float64)
+  sp = O.Space(L)
+  m = O.PairModel('soft_sphere', sigma=st['sigma'], species=st['species'])
+  R = st['real_position'].astype(np.float64)
+  rng = np.random.default_rng(seed)
+  P = rng.normal(size=R.shape) * np.sqrt(2 * kT)
+  P -= P.mean(0)
+  return sp, m, R, P
+
+
+@pytest.mark.parametrize('sy_steps', [1, 3, 5, 7])
+def test_nvt_invariant_conserved(sy_steps):
+  """The extended-system energy of the Nose-Hoover chain is conserved
+  (simulate_test.py:317-350 on the jammed fixture, float64 bar 1e-7 there):
+  here within 2e-6 over 200 steps started at twice the bath temperature;
+  a wrong factor or index in the chain substep moves it by 1.5e-4 or more."""
+  kT, dt = 1e-3, 1e-3
+  sp, m, R, P = _jammed_nvt_setup(kT)
+  force = lambda X: O.energy_force_brute(m, sp, X)[2]
+  energy = lambda X: O.energy_force_brute(m, sp, X)[0]
+  s, tau = O.nvt_init(R, P, force, kT, dt)
+  H0 = O.nvt_invariant(s, energy(s.position), kT)
+  for _ in range(200):
+    s = O.nvt_step(s, force, sp, dt, kT, tau, sy_steps=sy_steps)
+  H = O.nvt_invariant(s, energy(s.position), kT)
+  assert abs(H - H0) <= 2e-6 * abs(H0), (H, H0)
+
+
+def test_nvt_second_order():
+  """Halving dt divides the invariant's drift over a fixed time by ~4."""
+  kT = 1e-3
+  sp, m, R, P = _jammed_nvt_setup(kT, seed=1)
+  force = lambda X: O.energy_force_brute(m, sp, X)[2]
+  energy = lambda X: O.energy_force_brute(m, sp, X)[0]
+  drifts = []
+  for dt, n in ((4e-3, 50), (2e-3, 100)):
+    s, tau = O.nvt_init(R, P, force, kT, dt, tau=0.4)
+    H0 = O.nvt_invariant(s, energy(s.position), kT)
+    worst = 0.0
+    for _ in range(n):
+      s = O.nvt_step(s, force, sp, dt, kT, tau)
+      worst = max(worst, abs(O.nvt_invariant(s, energy(s.position), kT) - H0))
+    drifts.append(worst)
+  ratio = drifts[0] / drifts[1]
+  assert 3.0 < ratio < 5.5, (drifts, ratio)
