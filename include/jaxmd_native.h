@@ -209,10 +209,11 @@ int jmd_set_threads_per_atom(int32_t threads_per_atom);
  *                                    Dense and Sparse only)
  *   energy_total   [1]      float64  total energy, summed in float64
  * Pair arithmetic is float32 with float64 per-particle accumulation.
- * When momentum is non-NULL (Dense only) the call also applies
+ * When momentum is non-NULL (force required) the call also applies
  * p += half_dt * F (the closing half kick of simulate.velocity_verlet,
- * simulate.py:240-241) in the same pass; dt_device (device float32 [1], may
- * be NULL) overrides half_dt with float32(dt / 2). */
+ * simulate.py:240-241): in the same pass for Dense lists, in a second
+ * launch for the Sparse formats; dt_device (device float32 [1], may be
+ * NULL) overrides half_dt with float32(dt / 2). */
 int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
                           int32_t position_stride, int32_t periodic,
                           const float* side,

@@ -153,15 +153,20 @@ __device__ __forceinline__ double chain_exp(double x) {
 
 // One substep of length delta (Martyna et al.: backward sweep of the chain
 // momenta, rescaling of the particle momenta, chain positions, forward
-// sweep); returns the particle momentum factor and updates ke.
-__device__ double nhc_substep(double delta, double* xi, double* p, const double* Q,
-                              const double* iQ, int L, double* ke, double dof, double kT) {
+// sweep); returns the particle momentum factor and updates ke.  The chain
+// length is a template parameter so the chain stays in registers.
+template <int L>
+__device__ __forceinline__ double nhc_substep(double delta, double (&xi)[L], double (&p)[L],
+                                              const double (&iQ)[L], double* ke, double dof,
+                                              double kT) {
   const double d2 = delta / 2.0, d4 = d2 / 2.0, d8 = d4 / 2.0;
-  const int M = L - 1;
-  double old[kMaxChain];
+  constexpr int M = L - 1;
+  double old[L];
+#pragma unroll
   for (int m = 0; m < L; ++m) old[m] = p[m];
   p[M] += d4 * (old[M - 1] * old[M - 1] * iQ[M - 1] - kT);
   double carry = p[M];
+#pragma unroll
   for (int m = M - 1; m >= 1; --m) {
     const double G = old[m - 1] * old[m - 1] * iQ[m - 1] - kT;
     const double sc = chain_exp(-d8 * carry * iQ[m + 1]);
@@ -172,35 +177,35 @@ __device__ double nhc_substep(double delta, double* xi, double* p, const double*
   p[0] = sc * (sc * p[0] + d4 * (2.0 * *ke - dof * kT));
   const double scale = chain_exp(-d2 * p[0] * iQ[0]);
   *ke = *ke * scale * scale;
+#pragma unroll
   for (int m = 0; m < L; ++m) xi[m] += d2 * p[m] * iQ[m];
   double G = 2.0 * *ke - dof * kT;
-  double q[kMaxChain];
+  double q[L];
+#pragma unroll
   for (int m = 0; m < L; ++m) q[m] = p[m];
+#pragma unroll
   for (int m = 0; m < M; ++m) {
     sc = chain_exp(-d8 * p[m + 1] * iQ[m + 1]);
     q[m] = sc * (sc * p[m] + d4 * G);
     G = q[m] * q[m] * iQ[m] - kT;
   }
   q[M] += d4 * G;
+#pragma unroll
   for (int m = 0; m < L; ++m) p[m] = q[m];
   return scale;
 }
 
-__device__ double nhc_half_step(double* chain, int L, int chain_steps, int sy_steps, float dt,
-                                double* ke, double dof, double kT) {
-  double* xi = chain;
-  double* p = chain + L;
-  const double* Q = chain + 2 * L;
-  double iQ[kMaxChain];
-  for (int m = 0; m < L; ++m) iQ[m] = 1.0 / Q[m];
+template <int L>
+__device__ double nhc_half_step(double (&xi)[L], double (&p)[L], const double (&iQ)[L],
+                                int chain_steps, int sy_steps, float dt, double* ke, double dof,
+                                double kT) {
+  if (chain_steps == 1 && sy_steps == 1) return nhc_substep<L>((double)dt, xi, p, iQ, ke, dof, kT);
   double total = 1.0;
-  if (chain_steps == 1 && sy_steps == 1)
-    return nhc_substep((double)dt, xi, p, Q, iQ, L, ke, dof, kT);
   const float delta = dt / (float)chain_steps;
   const int row = sy_steps == 1 ? 0 : sy_steps == 3 ? 1 : sy_steps == 5 ? 2 : 3;
   for (int i = 0; i < chain_steps * sy_steps; ++i) {
     const float d = (float)((double)delta * kSYWeights[row][i % sy_steps]);
-    total *= nhc_substep((double)d, xi, p, Q, iQ, L, ke, dof, kT);
+    total *= nhc_substep<L>((double)d, xi, p, iQ, ke, dof, kT);
   }
   return total;
 }
@@ -210,7 +215,9 @@ enum { NHC_FINISH = 1, NHC_START = 2, NHC_INIT = 4 };
 // mode NHC_INIT: ke = sum of the partials, chain at rest.  NHC_FINISH: ke =
 // sum of the partials (after the closing kick), half step.  NHC_START:
 // chain masses for kT, half step.  scale = the product of the factors.
-__global__ __launch_bounds__(256) void k_nhc(int mode, double* __restrict__ chain, int L,
+// chain = [xi (L), p_xi (L), Q (L)] in float64.
+template <int L>
+__global__ __launch_bounds__(256) void k_nhc(int mode, double* __restrict__ chain,
                                              int chain_steps, int sy_steps, float dt, float kT,
                                              float tau, double dof, double* __restrict__ ke,
                                              const double* __restrict__ part, int nparts,
@@ -225,24 +232,33 @@ __global__ __launch_bounds__(256) void k_nhc(int mode, double* __restrict__ chai
     if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
     __syncthreads();
   }
-  // the chain is serial: one lane runs it on an LDS copy
-  __shared__ double c[3 * kMaxChain];
-  if (threadIdx.x < 3 * L) c[threadIdx.x] = chain[threadIdx.x];
-  __syncthreads();
   if (threadIdx.x != 0) return;
   const double kt = (double)kT;
   double k = (mode & (NHC_FINISH | NHC_INIT)) ? s[0] : ke[0];
-  double total = 1.0;
-  if (mode & NHC_INIT) {
-    for (int m = 0; m < 2 * L; ++m) c[m] = 0.0;
+  double xi[L], p[L], Q[L], iQ[L];
+#pragma unroll
+  for (int m = 0; m < L; ++m) {
+    const bool init = (mode & NHC_INIT) != 0;
+    xi[m] = init ? 0.0 : chain[m];
+    p[m] = init ? 0.0 : chain[L + m];
+    Q[m] = chain[2 * L + m];
   }
   if (mode & (NHC_INIT | NHC_START)) {
     const double t = (double)tau;
-    for (int m = 0; m < L; ++m) c[2 * L + m] = (double)kT * t * t * (m == 0 ? dof : 1.0);
+#pragma unroll
+    for (int m = 0; m < L; ++m) Q[m] = (double)kT * t * t * (m == 0 ? dof : 1.0);
   }
-  if (mode & NHC_FINISH) total *= nhc_half_step(c, L, chain_steps, sy_steps, dt, &k, dof, kt);
-  if (mode & NHC_START) total *= nhc_half_step(c, L, chain_steps, sy_steps, dt, &k, dof, kt);
-  for (int m = 0; m < 3 * L; ++m) chain[m] = c[m];
+#pragma unroll
+  for (int m = 0; m < L; ++m) iQ[m] = 1.0 / Q[m];
+  double total = 1.0;
+  if (mode & NHC_FINISH) total *= nhc_half_step<L>(xi, p, iQ, chain_steps, sy_steps, dt, &k, dof, kt);
+  if (mode & NHC_START) total *= nhc_half_step<L>(xi, p, iQ, chain_steps, sy_steps, dt, &k, dof, kt);
+#pragma unroll
+  for (int m = 0; m < L; ++m) {
+    chain[m] = xi[m];
+    chain[L + m] = p[m];
+    chain[2 * L + m] = Q[m];
+  }
   ke[0] = k;
   scale[0] = total;
 }
@@ -449,9 +465,20 @@ int jmd_nhc_step(int32_t mode, double* chain, int32_t chain_length, int32_t chai
       chain_steps < 1 || (sy_steps != 1 && sy_steps != 3 && sy_steps != 5 && sy_steps != 7) ||
       mode < 1 || mode > 7 || ((mode & (NHC_FINISH | NHC_INIT)) && (!ke_partials || n_partials < 1)))
     return JMD_ERR_INVALID;
-  hipLaunchKernelGGL(k_nhc, dim3(1), dim3(256), 0, (hipStream_t)stream, (int)mode, chain,
-                     (int)chain_length, (int)chain_steps, (int)sy_steps, dt, kT, tau, dof, kinetic,
-                     ke_partials, (int)n_partials, momentum_scale);
+  hipStream_t st = (hipStream_t)stream;
+#define JMD_NHC(LEN)                                                                         \
+  case LEN:                                                                                  \
+    hipLaunchKernelGGL(k_nhc<LEN>, dim3(1), dim3(256), 0, st, (int)mode, chain,              \
+                       (int)chain_steps, (int)sy_steps, dt, kT, tau, dof, kinetic, ke_partials, \
+                       (int)n_partials, momentum_scale);                                     \
+    break;
+  switch (chain_length) {
+    JMD_NHC(2) JMD_NHC(3) JMD_NHC(4) JMD_NHC(5) JMD_NHC(6) JMD_NHC(7) JMD_NHC(8) JMD_NHC(9)
+    JMD_NHC(10) JMD_NHC(11) JMD_NHC(12) JMD_NHC(13) JMD_NHC(14) JMD_NHC(15) JMD_NHC(16)
+    default:
+      return JMD_ERR_INVALID;
+  }
+#undef JMD_NHC
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }

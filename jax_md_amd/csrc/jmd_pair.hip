@@ -37,6 +37,12 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
   float sig = a.sig, eps = a.eps, alp = a.alp;
   int si = 0;
   if (GENERAL && a.species) si = a.species[i];
+#ifndef JMD_SPARSE_CHUNK
+#define JMD_SPARSE_CHUNK 4096
+#endif
+#ifndef JMD_SPARSE_TABLE
+#define JMD_SPARSE_TABLE 2048
+#endif
 #ifndef JMD_PAIR_UNROLL
 #define JMD_PAIR_UNROLL 4
 #endif
@@ -195,94 +201,158 @@ __global__ __launch_bounds__(256) void k_pair_dense(
 }
 
 // Sparse / OrderedSparse: one lane per stored pair (smap.py:926-931,975-977).
-// The sender's share goes out with one atomic per component; the receiver's
-// is first summed over each run of equal receiver ids inside the wavefront
-// (a segmented inclusive scan; lists built here are receiver-major, so a run
-// is a row piece) and the run's last lane adds it.  Any order is correct;
-// ordered input only makes the runs longer.
+// Lists are sender-major (partition.py:990-1008: row i of the candidate
+// array becomes the pairs with idx[1] = i), so a wavefront holds one or two
+// runs of equal sender ids: the sender's share is summed over each run with a
+// segmented inclusive scan and added by the run's last lane (lanes adding to
+// one address would be serialised).  The receivers of a chunk of pairs are
+// neighbours of a few consecutive senders, so the chunk touches far fewer
+// distinct receivers than it holds pairs: each workgroup sums the receiver
+// shares (and per-particle energies, summed over idx[0]) of its chunk in an
+// LDS hash table and adds each entry to memory once.  A receiver that finds
+// no free slot within kProbe probes goes to memory directly.  Any pair order
+// is correct; ordered lists only make the sums shorter.
+constexpr int kSparseChunk = JMD_SPARSE_CHUNK;  // pairs per workgroup pass
+constexpr int kSparseTable = JMD_SPARSE_TABLE;  // LDS hash entries (power of two)
+constexpr int kProbe = 16;
+
 template <int DIM, int STRIDE, int KIND, bool GENERAL, bool ENERGY>
 __global__ __launch_bounds__(256) void k_pair_sparse(
     const float* __restrict__ pos, int n, Space s, PairArgs a, const int* __restrict__ idx,
     long long width, float inv_norm, float* __restrict__ force, float* __restrict__ e_atom,
     double* __restrict__ e_part) {
-  constexpr int NV = DIM + (ENERGY ? 1 : 0);
+  constexpr int NT = DIM + (ENERGY ? 1 : 0);  // table values per entry
+  constexpr int kEmpty = -1;
+  __shared__ int s_key[kSparseTable];
+  __shared__ float s_val[NT][kSparseTable];
   const int lane = lane_id();
+  const int tid = threadIdx.x;
+  const bool table_on = force || (ENERGY && e_atom);
+  for (int t = tid; t < kSparseTable; t += blockDim.x) {
+    s_key[t] = kEmpty;
+#pragma unroll
+    for (int c = 0; c < NT; ++c) s_val[c][t] = 0.0f;
+  }
+  __syncthreads();
   double e = 0.0;
-  // the loop runs per wavefront, so every lane reaches the shuffles
-  for (long long k0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1));
-       k0 < width; k0 += (long long)gridDim.x * blockDim.x) {
-    const long long k = k0 + lane;
-    int ra = -1, sb = -1;
-    if (k < width) {
-      ra = idx[k];          // receiver (idx[0])
-      sb = idx[width + k];  // sender   (idx[1])
-    }
-    const bool ok = (unsigned)ra < (unsigned)n && (unsigned)sb < (unsigned)n;
-    float v[NV];
-#pragma unroll
-    for (int c = 0; c < NV; ++c) v[c] = 0.0f;
-    if (ok) {
-      float xa[3] = {0, 0, 0}, xb[3] = {0, 0, 0}, d[3];  // disp(R[idx0], R[idx1]) = R_a - R_b
-      load_pos<DIM, STRIDE>(pos, ra, xa);
-      load_pos<DIM, STRIDE>(pos, sb, xb);
-#pragma unroll
-      for (int c = 0; c < DIM; ++c)
-        d[c] = s.periodic ? min_image_disp(xa[c], xb[c], xb[c] - s.L[c], s.L[c], s.half[c])
-                          : xa[c] - xb[c];
-      float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
-                          : fmaf(d[1], d[1], d[0] * d[0]);
-      float sig = a.sig, eps = a.eps, alp = a.alp;
-      if (GENERAL) {
-        int s0 = a.species ? a.species[ra] : 0, s1 = a.species ? a.species[sb] : 0;
-        sig = pick(a.sig_mode, a.sig, a.sig_v, ra, sb, s0, s1, a.ns);
-        eps = pick(a.eps_mode, a.eps, a.eps_v, ra, sb, s0, s1, a.ns);
-        if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, ra, sb, s0, s1, a.ns);
+  for (long long base = (long long)blockIdx.x * kSparseChunk; base < width;
+       base += (long long)gridDim.x * kSparseChunk) {
+    // every lane of a wavefront runs the same iterations (for the shuffles)
+    for (int r = 0; r < kSparseChunk; r += blockDim.x) {
+      const long long k = base + r + tid;
+      int ra = -1, sb = -1;
+      if (k < width) {
+        ra = idx[k];          // receiver (idx[0])
+        sb = idx[width + k];  // sender   (idx[1])
       }
-      float dEr;
-      float u = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr) * inv_norm;
-      dEr *= inv_norm;
-      if (force) {
+      const bool ok = (unsigned)ra < (unsigned)n && (unsigned)sb < (unsigned)n;
+      float v[DIM], w[NT];
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) {
-          const float fc = dEr * d[c];  // dE/dR_a = dEr * (R_a - R_b)
-          atomicAdd(&force[(long long)sb * DIM + c], fc);
-          v[c] = -fc;
+      for (int c = 0; c < DIM; ++c) v[c] = 0.0f;
+#pragma unroll
+      for (int c = 0; c < NT; ++c) w[c] = 0.0f;
+      if (ok) {
+        float xa[3] = {0, 0, 0}, xb[3] = {0, 0, 0}, d[3];  // disp(R[idx0], R[idx1]) = R_a - R_b
+        load_pos<DIM, STRIDE>(pos, ra, xa);
+        load_pos<DIM, STRIDE>(pos, sb, xb);
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+          d[c] = s.periodic ? min_image_disp(xa[c], xb[c], xb[c] - s.L[c], s.L[c], s.half[c])
+                            : xa[c] - xb[c];
+        float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                            : fmaf(d[1], d[1], d[0] * d[0]);
+        float sig = a.sig, eps = a.eps, alp = a.alp;
+        if (GENERAL) {
+          int s0 = a.species ? a.species[ra] : 0, s1 = a.species ? a.species[sb] : 0;
+          sig = pick(a.sig_mode, a.sig, a.sig_v, ra, sb, s0, s1, a.ns);
+          eps = pick(a.eps_mode, a.eps, a.eps_v, ra, sb, s0, s1, a.ns);
+          if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, ra, sb, s0, s1, a.ns);
+        }
+        float dEr;
+        float u = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr) * inv_norm;
+        dEr *= inv_norm;
+        if (force) {
+#pragma unroll
+          for (int c = 0; c < DIM; ++c) {
+            const float fc = dEr * d[c];  // dE/dR_a = dEr * (R_a - R_b)
+            v[c] = fc;
+            w[c] = -fc;
+          }
+        }
+        if (ENERGY) {
+          e += (double)u;
+          w[NT - 1] = u;
         }
       }
-      if (ENERGY) {
-        e += (double)u;
-        v[DIM] = u;
+      if (!table_on) continue;
+      // receiver share into the table
+      if (ok) {
+        unsigned slot = ((unsigned)ra * 2654435761u) >> (32 - __builtin_ctz(kSparseTable));
+        bool placed = false;
+        for (int q = 0; q < kProbe; ++q, slot = (slot + 1) & (kSparseTable - 1)) {
+          int cur = __hip_atomic_load(&s_key[slot], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (cur == kEmpty) cur = atomicCAS(&s_key[slot], kEmpty, ra);
+          if (cur == kEmpty || cur == ra) {
+            placed = true;
+            break;
+          }
+        }
+        if (placed) {
+#pragma unroll
+          for (int c = 0; c < NT; ++c) atomicAdd(&s_val[c][slot], w[c]);
+        } else {
+          if (force) {
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) atomicAdd(&force[(long long)ra * DIM + c], w[c]);
+          }
+          if (ENERGY && e_atom) atomicAdd(&e_atom[ra], w[NT - 1]);
+        }
+      }
+      if (!force) continue;
+      // sender share: segmented scan over runs of equal sender ids
+      const int key = ok ? sb : -1;
+      const int prev = __shfl_up(key, 1, kWave);
+      const unsigned long long heads = __ballot(lane == 0 || prev != key);
+      const int h = 63 - __clzll(heads & ((2ull << lane) - 1ull));
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        float t[DIM];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) t[c] = __shfl_up(v[c], off, kWave);
+        if (lane - off >= h) {
+#pragma unroll
+          for (int c = 0; c < DIM; ++c) v[c] += t[c];
+        }
+      }
+      const bool tail = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
+      if (tail && ok) {
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) atomicAdd(&force[(long long)sb * DIM + c], v[c]);
       }
     }
-    if (!force && !(ENERGY && e_atom)) continue;
-    // segmented scan over runs of equal receiver ids (invalid lanes: key -1)
-    const int key = ok ? ra : -1;
-    const int prev = __shfl_up(key, 1, kWave);
-    const unsigned long long heads = __ballot(lane == 0 || prev != key);
-    const int h = 63 - __clzll(heads & ((2ull << lane) - 1ull));
+    if (!table_on) continue;
+    // flush the table and clear it for the next chunk
+    __syncthreads();
+    for (int t = tid; t < kSparseTable; t += blockDim.x) {
+      const int key = s_key[t];
+      if (key != kEmpty) {
+        if (force) {
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      float t[NV];
+          for (int c = 0; c < DIM; ++c) atomicAdd(&force[(long long)key * DIM + c], s_val[c][t]);
+        }
+        if (ENERGY && e_atom) atomicAdd(&e_atom[key], s_val[NT - 1][t]);
+        s_key[t] = kEmpty;
 #pragma unroll
-      for (int c = 0; c < NV; ++c) t[c] = __shfl_up(v[c], off, kWave);
-      if (lane - off >= h) {
-#pragma unroll
-        for (int c = 0; c < NV; ++c) v[c] += t[c];
+        for (int c = 0; c < NT; ++c) s_val[c][t] = 0.0f;
       }
     }
-    const bool tail = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
-    if (tail && ok) {
-      if (force) {
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) atomicAdd(&force[(long long)ra * DIM + c], v[c]);
-      }
-      if (ENERGY && e_atom) atomicAdd(&e_atom[ra], v[DIM]);
-    }
+    __syncthreads();
   }
   if (ENERGY) {
     __shared__ double s_e[256 / kWave];
-    double w = wave_reduce_sum(e);
-    if (lane_id() == 0) s_e[threadIdx.x / kWave] = w;
+    double wsum = wave_reduce_sum(e);
+    if (lane_id() == 0) s_e[threadIdx.x / kWave] = wsum;
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
@@ -290,6 +360,18 @@ __global__ __launch_bounds__(256) void k_pair_sparse(
       e_part[blockIdx.x] = t;
     }
   }
+}
+
+// Closing half kick after a Sparse-format force: p += half_dt F (the same
+// expression the Dense kernel fuses into its epilogue).
+__global__ __launch_bounds__(256) void k_kick_after(float* __restrict__ mom,
+                                                    const float* __restrict__ force, long long m,
+                                                    float half_dt,
+                                                    const float* __restrict__ dt_dev) {
+  if (dt_dev) half_dt = dt_dev[0] / 2.0f;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < m;
+       q += (long long)gridDim.x * blockDim.x)
+    mom[q] += half_dt * force[q];
 }
 
 // Fixed-order sum of block partials (one workgroup).
@@ -498,7 +580,7 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
   const bool energy = energy_total != nullptr || energy_atom != nullptr;
   if (energy && !workspace) return JMD_ERR_INVALID;
   if (format == JMD_ORDERED_SPARSE && energy_atom) return JMD_ERR_UNSUPPORTED;
-  if (format != JMD_DENSE && momentum) return JMD_ERR_UNSUPPORTED;
+  if (momentum && !force) return JMD_ERR_INVALID;
   PairArgs a;
   if (!make_pair_args(potential, &a)) return JMD_ERR_INVALID;
   Space s = make_space(dim, periodic, side);
@@ -553,7 +635,7 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
     int rc = launch_fill<float>(energy_atom, (long long)nn, 0.0f, st);
     if (rc) return rc;
   }
-  long long nb = (idx_width + 255) / 256;
+  long long nb = (idx_width + kSparseChunk - 1) / kSparseChunk;
   if (nb == 0) nb = 1;
   if (nb > kSparseBlocks) nb = kSparseBlocks;
 #define JMD_PS(D, S, K)                                                                        \
@@ -579,6 +661,14 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
   }
 #undef JMD_PS
   JMD_LAUNCH_CHECK();
+  if (momentum && nn > 0) {
+    const long long m = (long long)nn * dim;
+    long long kb = (m + 255) / 256;
+    if (kb > 4096) kb = 4096;
+    hipLaunchKernelGGL(k_kick_after, dim3((unsigned)kb), dim3(256), 0, st, momentum, force, m,
+                       half_dt, dt_device);
+    JMD_LAUNCH_CHECK();
+  }
   if (energy_total) {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, part, (int)nb, energy_total);
     JMD_LAUNCH_CHECK();

@@ -271,8 +271,9 @@ class NVELoop:
     if state.x64:
       return False
     from . import partition
+    # per-particle parameter arrays are indexed by the caller's ids
     return bool(self.reorder and nbrs.format is partition.Dense and nbrs.plan.use_cell_list
-                and state.position.shape[0] > 1)
+                and state.position.shape[0] > 1 and not self.energy.reads_particle_data())
 
   @staticmethod
   def _key(state, nbrs, n_steps, reorder):

@@ -105,6 +105,18 @@ class PairEnergy:
     return 0.0
 
 
+  def reads_particle_data(self) -> bool:
+    """True when the potential reads arrays indexed by particle id (species
+    ids or per-particle [N] parameters): such arrays follow the caller's
+    particle order, so a loop may not renumber the particles."""
+    if self.species is not None:
+      return True
+    for k in ('sigma', 'epsilon', 'alpha'):
+      v = self.params.get(k)
+      if v is not None and hasattr(v, 'ndim') and v.ndim == 1:
+        return True
+    return False
+
   def _workspace(self, device, n):
     key = (str(device), int(n))
     ws = self._ws.get(key)

@@ -169,6 +169,67 @@ def test_nve_block_loop_matches_drop_in(reorder):
   assert np.array_equal(a[:, :w], b[:, :w])
 
 
+@pytest.mark.parametrize('fmt', ['Sparse', 'OrderedSparse'])
+def test_nve_sparse_formats_follow_dense(fmt):
+  """simulate.nve over Sparse-format lists (the reference's default format)
+  follows the Dense trajectory: 50 steps, N = 1728."""
+  print(f'case NVE {fmt} vs Dense', flush=True)
+  jmd = _jmd()
+  R, P, L = O.lj_lattice(12, seed=9)
+  disp, shift = jmd.space.periodic(L)
+  Rt = torch.as_tensor(R, device=DEV)
+  Pt = torch.as_tensor(P, device=DEV)
+  out = []
+  for f in ('Dense', fmt):
+    nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=getattr(jmd.partition, f))
+    init_fn, apply_fn = jmd.simulate.nve(efn, shift, 0.005)
+    nb = nfn.allocate(Rt)
+    st = init_fn(0, Rt, 0.0, momenta=Pt, neighbor=nb)
+    for _ in range(50):
+      nb = nb.update(st.position)
+      st = apply_fn(st, neighbor=nb)
+    assert not bool(nb.did_buffer_overflow)
+    out.append(st)
+  dR = (out[0].position_hi - out[1].position_hi).abs()
+  dR = torch.minimum(dR, float(L) - dR).max().item()
+  dP = (out[0].momentum - out[1].momentum).abs().max().item()
+  print(f'  max |dR|={dR:.3e} max |dP|={dP:.3e}')
+  assert dR < 1e-5 and dP < 1e-4
+
+
+def test_nve_loop_with_species_matches_drop_in():
+  """A species-dependent potential through NVELoop (cell-order execution
+  requested) follows the per-step path: arrays indexed by particle id keep the
+  caller's numbering."""
+  print('case NVELoop species', flush=True)
+  from tests.golden import fixtures
+  jmd = _jmd()
+  stj = fixtures.jammed_state()
+  box = F32(stj['box'][0, 0])
+  R = torch.as_tensor(stj['real_position'].astype(F32), device=DEV)
+  disp, shift = jmd.space.periodic(box)
+  nfn, efn = jmd.energy.soft_sphere_neighbor_list(
+      disp, box, species=torch.as_tensor(stj['species'], device=DEV),
+      sigma=torch.as_tensor(stj['sigma'], dtype=torch.float32, device=DEV))
+  P = torch.as_tensor(np.random.default_rng(4).normal(size=R.shape).astype(F32) * 0.05,
+                      device=DEV)
+  init_fn, apply_fn = jmd.simulate.nve(efn, shift, 1e-2)
+  nb1 = nfn.allocate(R)
+  s1 = init_fn(0, R, 0.0, momenta=P, neighbor=nb1)
+  for _ in range(40):
+    nb1 = nb1.update(s1.position)
+    s1 = apply_fn(s1, neighbor=nb1)
+  nb2 = nfn.allocate(R)
+  s2 = init_fn(0, R, 0.0, momenta=P, neighbor=nb2)
+  loop = jmd.simulate.NVELoop(efn, shift, 1e-2, reorder=True)
+  loop.prime(s2, nb2)
+  loop.run(s2, nb2, 40)
+  dR = (s1.position_hi - s2.position_hi).abs().max().item()
+  dP = (s1.momentum - s2.momentum).abs().max().item()
+  print(f'  max |dR|={dR:.3e} max |dP|={dP:.3e}')
+  assert dR < 1e-6 and dP < 1e-5
+
+
 @pytest.mark.parametrize('reorder', [False, True])
 def test_nve_fused_update_matches_separate_kernels(reorder):
   """The fused list update + half kick + drift (alternating position buffers

@@ -80,7 +80,7 @@ def test_lj_formats_agree(fmt):
 
 @pytest.mark.parametrize('fmt', ['Sparse', 'OrderedSparse'])
 def test_sparse_any_pair_order_and_per_particle(fmt):
-  """The sparse kernel sums receiver shares over runs of equal receiver ids;
+  """The sparse kernel sums sender shares over runs of equal sender ids;
   a shuffled pair order (runs of one, the same id in several runs of one
   wavefront) must give the same forces and per-particle energies."""
   print(f'case sparse shuffled {fmt}', flush=True)
