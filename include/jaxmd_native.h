@@ -182,7 +182,9 @@ int jmd_permute_rows(const void* in, void* out, const int32_t* perm, int64_t n,
 
 /* Relabels a neighbour list under a particle permutation: Dense
  * out[perm[r], k] = perm[in[r, k]]; Sparse / OrderedSparse entries keep their
- * place and each id j becomes perm[j].  Padding n stays n. */
+ * place and each id j becomes perm[j] (OrderedSparse pairs are stored with
+ * receiver < sender, swapped where the relabelling reversed them,
+ * partition.py:998).  Padding n stays n. */
 int jmd_remap_neighbor_ids(const int32_t* idx_in, int32_t* idx_out,
                            const int32_t* perm, int64_t n, int32_t format,
                            int64_t width, void* stream);

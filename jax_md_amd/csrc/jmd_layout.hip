@@ -64,16 +64,26 @@ __global__ __launch_bounds__(256) void k_remap_dense(const int* __restrict__ in,
   }
 }
 
-// Sparse list in permuted ids -> user ids (entries keep their order).
+// Sparse list [2, width] through perm (entries keep their slots).
+// OrderedSparse keeps receiver < sender (partition.py:998): a pair whose
+// order the renumbering reversed is stored swapped.
 __global__ __launch_bounds__(256) void k_remap_ids(const int* __restrict__ in,
                                                    int* __restrict__ out,
                                                    const int* __restrict__ perm, int n,
-                                                   long long count) {
+                                                   long long width, int ordered) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long step = (long long)gridDim.x * blockDim.x;
-  for (; t < count; t += step) {
-    int j = in[t];
-    out[t] = j < n ? perm[j] : n;
+  for (; t < width; t += step) {
+    const int a = in[t], b = in[width + t];
+    int ra = a < n ? perm[a] : n;
+    int sb = b < n ? perm[b] : n;
+    if (ordered && ra < n && sb < n && ra > sb) {
+      const int x = ra;
+      ra = sb;
+      sb = x;
+    }
+    out[t] = ra;
+    out[width + t] = sb;
   }
 }
 
@@ -128,8 +138,8 @@ int jmd_remap_neighbor_ids(const int32_t* idx_in, int32_t* idx_out, const int32_
     hipLaunchKernelGGL(k_remap_dense, dim3((unsigned)(rb < 65536 ? (rb < 1 ? 1 : rb) : 65536)),
                        dim3(256), 0, st, idx_in, idx_out, perm, (int)n, (int)width);
   } else {
-    hipLaunchKernelGGL(k_remap_ids, dim3(grid_for(2 * width)), dim3(256), 0, st, idx_in,
-                       idx_out, perm, (int)n, 2 * width);
+    hipLaunchKernelGGL(k_remap_ids, dim3(grid_for(width)), dim3(256), 0, st, idx_in, idx_out,
+                       perm, (int)n, width, format == JMD_ORDERED_SPARSE ? 1 : 0);
   }
   JMD_LAUNCH_CHECK();
   return JMD_OK;

@@ -270,10 +270,9 @@ class NVELoop:
   def _use_reorder(self, state, nbrs) -> bool:
     if state.x64:
       return False
-    from . import partition
     # per-particle parameter arrays are indexed by the caller's ids
-    return bool(self.reorder and nbrs.format is partition.Dense and nbrs.plan.use_cell_list
-                and state.position.shape[0] > 1 and not self.energy.reads_particle_data())
+    return bool(self.reorder and nbrs.plan.use_cell_list and state.position.shape[0] > 1
+                and not self.energy.reads_particle_data())
 
   @staticmethod
   def _key(state, nbrs, n_steps, reorder):
@@ -285,11 +284,11 @@ class NVELoop:
 
   def run(self, state: NVEState, nbrs, n_steps: int, events=None) -> NVEState:
     """Advances ``state`` and ``nbrs`` in place by ``n_steps``.  With
-    ``reorder`` (Dense lists on a cell grid) the block runs with particles in
+    ``reorder`` (lists on a cell grid) the block runs with particles in
     cell order, so that a wavefront's neighbour gathers share cache lines;
-    user order, and the list in user ids, are restored at the end (rows hold
-    the reference's neighbour sets; inside a row the order follows the cell
-    order of the block).  ``events`` (a list) collects, per force kernel, a
+    user order, and the list in user ids, are restored at the end (Dense rows
+    hold the reference's neighbour sets and Sparse lists its pairs, in the
+    cell order of the block; OrderedSparse pairs keep receiver < sender).  ``events`` (a list) collects, per force kernel, a
     (start, end) event pair around it and an empty pair recorded right after
     (the markers' own cost); it forces the eager path."""
     if state.position_hi is None:
@@ -426,8 +425,9 @@ class NVELoop:
       mass = b['mass']
     permute(nbrs.reference_position, ref, rb, rb, rb, 0)
     width = nbrs.idx.shape[1]
+    fmt = plan.spec.format_code
     nat.check(lib.jmd_remap_neighbor_ids(nat.ptr(nbrs.idx), nat.ptr(idx), nat.ptr(inv), N,
-                                         nat.DENSE, width, st), 'jmd_remap_neighbor_ids')
+                                         fmt, width, st), 'jmd_remap_neighbor_ids')
     nb = dataclasses.replace(nbrs, idx=idx, reference_position=ref)
     sp = NVEState(x, p, F, mass, xh)
     self._run_steps(sp, nb, n_steps, events)
@@ -438,7 +438,7 @@ class NVELoop:
     permute(F, state.force, rb, rb, rb, 1)
     permute(ref, nbrs.reference_position, rb, rb, rb, 1)
     nat.check(lib.jmd_remap_neighbor_ids(nat.ptr(idx), nat.ptr(nbrs.idx), nat.ptr(order), N,
-                                         nat.DENSE, width, st), 'jmd_remap_neighbor_ids')
+                                         fmt, width, st), 'jmd_remap_neighbor_ids')
     return state
 
   def _run_steps(self, state: NVEState, nbrs, n_steps: int, events=None,

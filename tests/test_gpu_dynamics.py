@@ -197,6 +197,52 @@ def test_nve_sparse_formats_follow_dense(fmt):
   assert dR < 1e-5 and dP < 1e-4
 
 
+@pytest.mark.parametrize('fmt', ['Sparse', 'OrderedSparse'])
+def test_nve_loop_sparse_reordered(fmt):
+  """NVELoop in cell order over a Sparse-format list: the trajectory follows
+  the per-step path, and the list handed back holds, in user ids, exactly
+  the pairs of a fresh build at its reference positions (OrderedSparse
+  pairs with receiver < sender)."""
+  print(f'case NVELoop reordered {fmt}', flush=True)
+  jmd = _jmd()
+  R, P, L = O.lj_lattice(12, seed=11)
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=getattr(jmd.partition, fmt))
+  Rt = torch.as_tensor(R, device=DEV)
+  Pt = torch.as_tensor(P, device=DEV)
+  init_fn, apply_fn = jmd.simulate.nve(efn, shift, 0.005)
+  nb1 = nfn.allocate(Rt)
+  s1 = init_fn(0, Rt, 0.0, momenta=Pt, neighbor=nb1)
+  for _ in range(60):
+    nb1 = nb1.update(s1.position)
+    s1 = apply_fn(s1, neighbor=nb1)
+  nb2 = nfn.allocate(Rt)
+  s2 = init_fn(0, Rt, 0.0, momenta=Pt, neighbor=nb2)
+  loop = jmd.simulate.NVELoop(efn, shift, 0.005, reorder=True)
+  loop.prime(s2, nb2)
+  loop.run(s2, nb2, 30)
+  loop.run(s2, nb2, 30)
+  assert not bool(nb2.did_buffer_overflow)
+  dR = (s1.position_hi - s2.position_hi).abs()
+  dR = torch.minimum(dR, float(L) - dR).max().item()
+  dP = (s1.momentum - s2.momentum).abs().max().item()
+  print(f'  max |dR|={dR:.3e} max |dP|={dP:.3e}')
+  assert dR < 1e-4 and dP < 1e-3
+  N = R.shape[0]
+
+  def pairs(idx):
+    a, b = idx[0].cpu().numpy(), idx[1].cpu().numpy()
+    m = (a < N) & (b < N)
+    return a[m], b[m]
+
+  a, b = pairs(nb2.idx)
+  if fmt == 'OrderedSparse':
+    assert np.all(a < b)
+  fresh = nfn.allocate(nb2.reference_position.clone())
+  fa, fb = pairs(fresh.idx)
+  assert sorted(zip(a.tolist(), b.tolist())) == sorted(zip(fa.tolist(), fb.tolist()))
+
+
 def test_nve_loop_with_species_matches_drop_in():
   """A species-dependent potential through NVELoop (cell-order execution
   requested) follows the per-step path: arrays indexed by particle id keep the
