@@ -226,7 +226,8 @@ __device__ __forceinline__ void dev_bin_scatter(int n, const int* __restrict__ a
 // partition.py:432 keeps ids ascending inside a cell), gather positions.
 // sorted_pos holds each position moved by the whole box periods implied by
 // its cell assignment (x - L * floor(int(x / cs) / cps)), so that positions
-// of a cell lie within that cell's range; the raw input stays untouched.
+// of a cell lie within that cell's range, and the id's bits in .w; the raw
+// input stays untouched.
 template <int DIM, int STRIDE>
 __device__ __forceinline__ void dev_cell_order(const float* __restrict__ pos, const Grid& g,
                                                const Space& s,
@@ -258,7 +259,7 @@ __device__ __forceinline__ void dev_cell_order(const float* __restrict__ pos, co
           if (wraps > 1 || wraps < -1) far = true;
         }
       }
-      sorted_pos[beg + rank] = make_float4(x[0], x[1], x[2], 0.0f);
+      sorted_pos[beg + rank] = make_float4(x[0], x[1], x[2], __int_as_float(id));
     }
   }
   if (__ballot(far) != 0ull && lane == 0) atomicOr(&misc[M_FAR], 1);
@@ -406,7 +407,7 @@ struct BuildArgs {
 template <int DIM, int STRIDE, int MODE, bool MINIMG>
 __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
   constexpr int NST = DIM == 3 ? 27 : 9;
-  __shared__ int s_cs[NST], s_cc[NST], s_off[NST + 1];
+  __shared__ int s_cs[NST], s_cc[NST], s_off[NST + 1], s_rot[NST];
   __shared__ float s_shift[NST][3];
   __shared__ float4 s_pos[kStage];
   __shared__ int s_cnt[kHomeGroup];
@@ -450,10 +451,20 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
         }
         s_shift[tid][d] = sh;
       }
-      s_cs[tid] = A.w.cell_start[c];
-      s_cc[tid] = A.w.cell_start[c + 1] - s_cs[tid];
+      const int cs0 = A.w.cell_start[c];
+      int cnt = A.w.cell_start[c + 1] - cs0;
       // k_cell_sort orders at most cap members of a cell (more: overflow)
-      if (g.cap > 0) s_cc[tid] = min(s_cc[tid], g.cap);
+      if (g.cap > 0) cnt = min(cnt, g.cap);
+      // the slot rotation of a cell's members (start mod cap, the cell
+      // slots of partition.py): zero when the members do not wrap
+      int rot = 0;
+      if (g.cap > 0) {
+        const int w = g.cap - (cs0 % g.cap);
+        rot = cnt > w ? w : 0;
+      }
+      s_cs[tid] = cs0;
+      s_cc[tid] = cnt;
+      s_rot[tid] = rot;
     }
     __syncthreads();
     if (tid == 0) {
@@ -483,7 +494,6 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
     const float reach_sq = (A.cutoff_sq + margin) * 1.001f;
     const int hstart = s_cs[NST / 2];
     const int hcount = s_cc[NST / 2];
-    const int cap = g.cap;
 
     for (int g0 = 0; g0 < hcount; g0 += kHomeGroup) {
       const int gcount = min(kHomeGroup, hcount - g0);
@@ -501,19 +511,20 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
           float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
           if (q < ccount) {
             int p = c0 + q;
+            // the stencil cell of candidate p: the last t with s_off[t] <= p
+            // (empty cells have s_off[t] == s_off[t + 1] and are passed over)
             int t = 0;
-            while (t + 1 < NST && s_off[t + 1] <= p) ++t;
+#pragma unroll
+            for (int step = 16; step > 0; step >>= 1)
+              if (t + step < NST && s_off[t + step] <= p) t += step;
             int m = p - s_off[t];
-            int cnt = s_cc[t];
             int k = m;
-            if (cap > 0 && cnt <= cap) {
-              int w = cap - (s_cs[t] % cap);
-              if (cnt > w) k = (m < cnt - w) ? (w + m) : (m - (cnt - w));
-            }
+            const int w = s_rot[t];
+            if (w > 0) k = (m < s_cc[t] - w) ? (w + m) : (m - (s_cc[t] - w));
             int gi = s_cs[t] + k;
-            float4 xp = A.w.sorted_pos[gi];
+            float4 xp = A.w.sorted_pos[gi];  // w: the id bits
             v = make_float4(xp.x + s_shift[t][0], xp.y + s_shift[t][1], xp.z + s_shift[t][2],
-                            __int_as_float(A.w.sorted_id[gi]));
+                            xp.w);
             keep = true;
             if (prune) {
               const float cv[3] = {v.x, v.y, v.z};
@@ -568,7 +579,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
               xi[h][0] = xp.x;
               xi[h][1] = xp.y;
               xi[h][2] = xp.z;
-              ih[h] = A.w.sorted_id[hi];
+              ih[h] = __float_as_int(xp.w);
               lim[h] = A.ordered ? ih[h] : 0x7fffffff;
               ex[h] = A.mask_self ? ih[h] : -1;
               cnt[h] = s_cnt[a];
@@ -826,7 +837,7 @@ __global__ __launch_bounds__(256) void k_cell_sort(const float* __restrict__ pos
           }
         }
         sorted_id[start + rank] = id;
-        sorted_pos[start + rank] = make_float4(x[0], x[1], x[2], 0.0f);
+        sorted_pos[start + rank] = make_float4(x[0], x[1], x[2], __int_as_float(id));
       }
     }
   }
