@@ -425,7 +425,10 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
   // Positions canonicalised with a rounding beyond the margin's bound: every
   // candidate takes the reference-exact test.
   const float margin = A.w.misc[M_FAR] ? __builtin_inff() : A.margin;
-  const float lo_thr = A.cutoff_sq - margin, hi_thr = A.cutoff_sq + margin;
+  // with an infinite margin every real candidate is "near"; the finite upper
+  // bound keeps the staged padding (d^2 > 2e37) out of the exact test
+  const float lo_thr = A.cutoff_sq - margin;
+  const float hi_thr = fminf(A.cutoff_sq + margin, 1.0e36f);
   const float far = 3.0e18f;  // staged padding: never within the cutoff
   int occ_local = 0;          // largest row count seen by this wavefront
 
@@ -608,19 +611,24 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
                               : xi[h][e] - cj[e];
               const float d2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
                                         : fmaf(d[1], d[1], d[0] * d[0]);
-              const bool okj = j < lim[h] && j != ex[h];
+              // Dense rows never drop j > i: only the Sparse modes read lim
+              constexpr bool kLim = MODE != MODE_DENSE && MODE != MODE_COUNT_MAX;
+              const bool okj = (!kLim || j < lim[h]) && j != ex[h];
               const bool fast = d2 < lo_thr;
+              bool pass = fast && okj;
+              // ballots of comparisons stay lane masks (no VGPR round trip)
+              const unsigned long long m_fast = __builtin_amdgcn_ballot_w64(d2 < lo_thr);
+              unsigned long long bal = m_fast & __builtin_amdgcn_ballot_w64(j != ex[h]);
+              if (kLim) bal &= __builtin_amdgcn_ballot_w64(j < lim[h]);
               // within the margin of the cutoff: too close to call from the
-              // canonicalised positions
-              const bool near = !fast && d2 <= hi_thr;
-              bool exact = false;
-              if (__builtin_amdgcn_ballot_w64(near)) {  // rare: reference-exact test
-                // (an infinite margin also admits the staged padding, j = n)
-                if (near && okj && j < n)
-                  exact = exact_pair_test<DIM, STRIDE>(A.pos, i, j, s, A.cutoff_sq, A.test);
+              // canonicalised positions (rare: reference-exact test)
+              const unsigned long long near_any =
+                  __builtin_amdgcn_ballot_w64(d2 <= hi_thr) & ~m_fast;
+              if (near_any) {
+                if (!fast && d2 <= hi_thr && okj)
+                  pass = exact_pair_test<DIM, STRIDE>(A.pos, i, j, s, A.cutoff_sq, A.test);
+                bal = __builtin_amdgcn_ballot_w64(pass);
               }
-              const bool pass = (fast && okj) || exact;
-              const unsigned long long bal = __ballot(pass);
               if ((MODE == MODE_DENSE || MODE == MODE_SPARSE_WRITE) && pass) {
                 if (MODE == MODE_DENSE) {
                   // row slots stay below width < 2^31: 32-bit unsigned offsets
