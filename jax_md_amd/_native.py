@@ -178,7 +178,9 @@ def stream_handle(device=None):
 
 
 def f3(vals):
-  a = _F3()
+  """float32 array of at least 3 entries (a box matrix passes 4 or 9)."""
+  vals = list(vals)
+  a = _F3() if len(vals) <= 3 else (c_float * len(vals))()
   for k, v in enumerate(vals):
     a[k] = float(v)
   return a

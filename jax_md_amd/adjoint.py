@@ -64,6 +64,9 @@ def force_vjp(energy_fn, R, neighbor, cotangent, params: bool = True, **dynamic_
   eg = torch.empty((2,), dtype=torch.float64, device=dev) if params else None
   pot, keep = energy._potential(R, dynamic_kwargs)
   geo = energy.geometry
+  if int(geo.periodic) == 2:
+    raise NotImplementedError('the adjoint kernels take periodic boxes and free space; '
+                              'periodic_general boxes are not supported')
   ws = _workspace(dev, N, dim)
   # float64 positions (an x64 trajectory): row stride code 0
   stride = 0 if R.dtype == torch.float64 else nat.position_stride(R, 'R')
@@ -201,6 +204,9 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
                     device=dev)
   pot, keep = energy._potential(state.position, {})
   geo = energy.geometry
+  if int(geo.periodic) == 2:
+    raise NotImplementedError('the adjoint kernels take periodic boxes and free space; '
+                              'periodic_general boxes are not supported')
   side = nat.f3(geo.side_values(dim))
   st = nat.stream_handle(dev)
   if n_steps == 0:

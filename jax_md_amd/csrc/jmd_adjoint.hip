@@ -295,6 +295,10 @@ int jmd_pair_force_vjp(const float* position, int64_t n, int32_t dim, int32_t po
   }
   float* hout = position_cotangent;
   if (!hout) return JMD_ERR_INVALID;  // the product is the kernel's main output
+  {
+    const int grc = space_check(dim, periodic, side, false);
+    if (grc) return grc;
+  }
   Space s = make_space(dim, periodic, side);
   AdjLaunch L{dim3(nb), st, position, (int)n, s, a, idx, (int)idx_width, cotangent, hout,
               part, params ? 1 : 0, nullptr, nullptr, 0.0f, 0.0f, 0.0f};
@@ -327,6 +331,10 @@ int jmd_nve_adjoint_step(const float* position, int64_t n, int32_t dim, int32_t 
   if (!ph_in || !ph_out || !x_adjoint || ph_in == ph_out) return JMD_ERR_INVALID;
   if (param_acc && a.general) return JMD_ERR_UNSUPPORTED;
   hipStream_t st = (hipStream_t)stream;
+  {
+    const int grc = space_check(dim, periodic, side, false);
+    if (grc) return grc;
+  }
   Space s = make_space(dim, periodic, side);
   AdjLaunch L{dim3(adj_blocks(n, dim)), st, position, (int)n, s, a, idx, (int)idx_width, ph_in,
               nullptr, param_acc, param_acc ? 1 : 0, x_adjoint, ph_out, coef_x, coef_p,

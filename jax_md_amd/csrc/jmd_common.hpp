@@ -26,24 +26,119 @@ namespace jmd {
 
 constexpr int kWave = 64;
 
-// Geometry passed by value to kernels.
+// Geometry passed by value to kernels.  periodic_general with fractional
+// coordinates (space.py:332-472) is the unit cube (L = 1) plus the box
+// matrix T: displacements are T wrap(u_a - u_b), shifts mod(u + T^-1 dR, 1).
 struct Space {
   int dim;
   int periodic;
   float L[3];
   float half[3];  // float32(L) * 0.5, exact
+  int general;    // 1: positions are fractional, d = T d_unit
+  float T[3][3];  // box matrix, row-major (general only)
+  float Ti[3][3]; // its inverse (float64 inverse rounded once)
 };
 
+// periodic: 0 free, 1 periodic box of sides side[0..dim), 2 periodic_general
+// with fractional coordinates, side = the [dim, dim] box matrix (row-major).
 inline Space make_space(int dim, int periodic, const float* side) {
   Space s;
   s.dim = dim;
-  s.periodic = periodic;
+  s.periodic = periodic != 0;
+  s.general = periodic == 2;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      s.T[a][b] = a == b ? 1.0f : 0.0f;
+      s.Ti[a][b] = a == b ? 1.0f : 0.0f;
+    }
   for (int d = 0; d < 3; ++d) {
-    float L = (side && d < dim) ? side[d] : 1.0f;
+    float L = (!s.general && side && d < dim) ? side[d] : 1.0f;
     s.L[d] = L;
     s.half[d] = L * 0.5f;
   }
+  if (s.general && side) {
+    double m[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int a = 0; a < dim; ++a)
+      for (int b = 0; b < dim; ++b) {
+        s.T[a][b] = side[a * dim + b];
+        m[a][b] = (double)side[a * dim + b];
+      }
+    // inverse by cofactors (dim <= 3; the 2-D matrix sits in the upper block)
+    const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                       m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                       m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    const double inv[3][3] = {
+        {(m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det, (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det,
+         (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det},
+        {(m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det, (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det,
+         (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det},
+        {(m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det, (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det,
+         (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) s.Ti[a][b] = (float)inv[a][b];
+  }
   return s;
+}
+
+// True when a periodic_general box matrix is usable: finite and invertible.
+inline bool box_matrix_ok(int dim, const float* T) {
+  if (!T) return false;
+  double m[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int a = 0; a < dim; ++a)
+    for (int b = 0; b < dim; ++b) {
+      const float v = T[a * dim + b];
+      if (!(v == v) || v > 3.0e38f || v < -3.0e38f) return false;
+      m[a][b] = v;
+    }
+  const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                     m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                     m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  return det != 0.0;
+}
+
+// Geometry arguments of an entry point: periodic 0 / 1 (side[dim] when 1), or
+// 2 (periodic_general, side = an invertible [dim, dim] box matrix) where the
+// entry point supports it.
+inline int space_check(int dim, int periodic, const float* side, bool allow_general) {
+  if (periodic < 0 || periodic > 2) return JMD_ERR_INVALID;
+  if (periodic && !side) return JMD_ERR_INVALID;
+  if (periodic == 2) {
+    if (!allow_general) return JMD_ERR_UNSUPPORTED;
+    if (!box_matrix_ok(dim, side)) return JMD_ERR_INVALID;
+  }
+  return JMD_OK;
+}
+
+// d <- T d for the force kernels (tolerance parity: contraction allowed).
+template <int DIM>
+__device__ __forceinline__ void box_apply(const Space& s, float* d) {
+  float r[3];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    float acc = s.T[a][0] * d[0];
+#pragma unroll
+    for (int b = 1; b < DIM; ++b) acc = fmaf(s.T[a][b], d[b], acc);
+    r[a] = acc;
+  }
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) d[a] = r[a];
+}
+
+// d <- T d as the reference's transform (einsum 'ij,j->i', space.py:128-153):
+// products rounded, summed left to right.
+template <int DIM>
+__device__ __forceinline__ void box_apply_ref(const Space& s, float* d) {
+#pragma clang fp contract(off)
+  float r[3];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    float acc = s.T[a][0] * d[0];
+#pragma unroll
+    for (int b = 1; b < DIM; ++b) acc = acc + s.T[a][b] * d[b];
+    r[a] = acc;
+  }
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) d[a] = r[a];
 }
 
 // Cell grid (jax_md/partition.py:146-188,211-223).
@@ -80,12 +175,17 @@ template <int DIM>
 __device__ __forceinline__ float ref_metric_sq(const float* a, const float* b,
                                                const Space& s) {
 #pragma clang fp contract(off)
+  float v[3];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    v[d] = a[d] - b[d];
+    if (s.periodic) v[d] = ref_wrap1(v[d], s.L[d], s.half[d]);
+  }
+  if (s.general) box_apply_ref<DIM>(s, v);
   float acc = 0.0f;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    float v = a[d] - b[d];
-    if (s.periodic) v = ref_wrap1(v, s.L[d], s.half[d]);
-    float sq = v * v;
+    float sq = v[d] * v[d];
     acc = (d == 0) ? sq : acc + sq;
   }
   return acc;
@@ -160,15 +260,33 @@ __device__ __forceinline__ void kick_drift_atom(const KickDrift& kd, long long i
                                                 float half_dt, float* xn) {
 #pragma clang fp contract(off)
   const double m = kd.mass ? (double)kd.mass[i] : (double)kd.mass_scalar;
+  double dx[3];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     const long long k = i * DIM + d;
     const float pin = kd.p_scale ? kd.p_in[k] * (float)kd.p_scale[0] : kd.p_in[k];
     const float pk = pin + half_dt * kd.F[k];
     const double v = (double)dt * (double)pk;
-    double x = kd.xhi_in[k] + (kd.mass_pow2 ? v * kd.inv_mass : v / m);
-    if (kd.s.periodic) x = ref_mod64(x, (double)kd.s.L[d]);
+    dx[d] = kd.mass_pow2 ? v * kd.inv_mass : v / m;
     kd.p_out[k] = pk;
+  }
+  if (kd.s.general) {  // fractional positions: u += T^-1 dR (space.py:450-470)
+    double w[3];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      double acc = 0.0;
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) acc += (double)kd.s.Ti[a][b] * dx[b];
+      w[a] = acc;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) dx[d] = w[d];
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const long long k = i * DIM + d;
+    double x = kd.xhi_in[k] + dx[d];
+    if (kd.s.periodic) x = ref_mod64(x, (double)kd.s.L[d]);
     kd.xhi_out[k] = x;
     xn[d] = (float)x;
   }

@@ -357,6 +357,10 @@ static int kick_drift_impl(const double* position_hi, const float* momentum,
   if (n == 0) return JMD_OK;
   if (!stride_ok(dim, position_stride, position_out)) return JMD_ERR_INVALID;
   if (!mass && !(mass_scalar > 0.0f)) return JMD_ERR_INVALID;
+  {
+    const int grc = space_check(dim, periodic, side, true);
+    if (grc) return grc;
+  }
   Space s = make_space(dim, periodic, side);
   hipStream_t st = (hipStream_t)stream;
   int nn = (int)n;

@@ -481,7 +481,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
     // cell index (x in (-cs, 0) belongs to cell 0) and its rounding: a
     // candidate farther than sqrt(c^2 + margin) from it reaches no home
     // particle by either test and is not staged.
-    const bool prune = !MINIMG && s.periodic && margin < 3.0e38f;
+    const bool prune = !MINIMG && s.periodic && margin < 3.0e38f;  // never for general
     float blo[3] = {0.0f, 0.0f, 0.0f}, bhi[3] = {0.0f, 0.0f, 0.0f};
     {
       int h = home;
@@ -609,6 +609,8 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
                 d[e] = MINIMG ? -min_image_disp(cj[e], xi[h][e], xi[h][e] - s.L[e], s.L[e],
                                                 s.half[e])
                               : xi[h][e] - cj[e];
+              // periodic_general: the fold is in fractional units, then T
+              if (MINIMG && s.general) box_apply<DIM>(s, d);
               const float d2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
                                         : fmaf(d[1], d[1], d[0] * d[0]);
               // Dense rows never drop j > i: only the Sparse modes read lim
@@ -1113,7 +1115,8 @@ template <int DIM, int STRIDE>
 static int cells_dispatch(const BuildArgs& A, int mode, int finish, hipStream_t st) {
   // The staged image of a stencil cell is the nearest one when every
   // periodic dimension has at least 5 cells (|d| < 3 cell sizes < L - c).
-  bool minimg = false;
+  // periodic_general always folds (in fractional units) and applies T.
+  bool minimg = A.s.general != 0;
   if (A.s.periodic)
     for (int d = 0; d < DIM; ++d) minimg = minimg || A.g.cps[d] < 5;
   if (minimg) launch_cells<DIM, STRIDE, true>(A, mode, finish, st);
@@ -1185,6 +1188,14 @@ static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
     // at the scale of 2L on each of the components of d.
     float Lmax = 0.0f;
     for (int d = 0; d < DIM; ++d) Lmax = fmaxf(Lmax, s.periodic ? s.L[d] : 0.0f);
+    if (s.general) {  // real-space extent of the cell: largest row sum of |T|
+      Lmax = 0.0f;
+      for (int a = 0; a < DIM; ++a) {
+        float r = 0.0f;
+        for (int b = 0; b < DIM; ++b) r += fabsf(s.T[a][b]);
+        Lmax = fmaxf(Lmax, r);
+      }
+    }
     float c = sqrtf(cutoff_sq);
     float margin = 128.0f * (c + 1.0f) * (Lmax + c + 1.0f) * 1.1920929e-7f;
     A.margin = fmaxf(margin, 1e-4f * cutoff_sq);
@@ -1385,6 +1396,10 @@ int jmd_neighbor_list_build(const float* position, int64_t n, int32_t dim,
     for (int d = 0; d < dim; ++d)
       if (cells_per_side[d] < 3) return JMD_ERR_INVALID;
   }
+  {
+    const int grc = space_check(dim, periodic, side, true);
+    if (grc) return grc;
+  }
   Space s = make_space(dim, periodic, side);
   Work w;
   carve(workspace, n, use_cell_list ? g.ncell : 1, use_cell_list ? cell_capacity : 0, &w);
@@ -1442,6 +1457,10 @@ int jmd_nve_list_update_kick_drift(
   if (!make_grid(dim, cells_per_side, cell_size, cell_capacity, &g)) return JMD_ERR_INVALID;
   for (int d = 0; d < dim; ++d)
     if (cells_per_side[d] < 3) return JMD_ERR_INVALID;
+  {
+    const int grc = space_check(dim, periodic, side, true);
+    if (grc) return grc;
+  }
   Space s = make_space(dim, periodic, side);
   KickDrift kd{};
   kd.active = 1;
@@ -1460,6 +1479,10 @@ int jmd_nve_list_update_kick_drift(
   kd.ref = reference_position;
   kd.thr_sq = threshold_sq;
   kd.gate_next = gate_next;
+  {
+    const int grc = space_check(dim, shift_periodic, shift_side, true);
+    if (grc) return grc;
+  }
   kd.s = make_space(dim, shift_periodic, shift_side);
   Work w;
   carve(workspace, n, g.ncell, cell_capacity, &w);
@@ -1515,6 +1538,10 @@ int jmd_neighbor_list_check(const float* position, int32_t position_stride,
     return JMD_ERR_INVALID;
   if (n == 0) return JMD_OK;
   if (!stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  {
+    const int grc = space_check(dim, periodic, side, true);
+    if (grc) return grc;
+  }
   Space s = make_space(dim, periodic, side);
   hipStream_t st = (hipStream_t)stream;
   int nn = (int)n;

@@ -27,7 +27,7 @@ namespace jmd {
 // (pairs whose plain difference is not the minimum image are then beyond
 // range and contribute zero either way).
 template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, bool ENERGY, bool INTERIOR,
-          int SMOOTH>
+          int SMOOTH, bool BOX = false>
 __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned n,
                                         const Space& s, const PairArgs& a,
                                         const int* __restrict__ row, int width, int sub,
@@ -55,6 +55,7 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
     for (int c = 0; c < DIM; ++c)
       d[c] = (INTERIOR || !s.periodic) ? xj[c] - xi[c]
                                        : min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c]);
+    if (BOX) box_apply<DIM>(s, d);  // periodic_general: fractional fold, then T
     float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
                         : fmaf(d[1], d[1], d[0] * d[0]);
     if (GENERAL) {
@@ -151,8 +152,17 @@ __global__ __launch_bounds__(256) void k_pair_dense(
       interior = interior && xi[c] >= a.rc && xi[c] <= s.L[c] - a.rc;
     }
   }
-  const bool wave_interior = __all(interior || !active);
-  if (active) {
+  // (fractional coordinates have no real-space faces to test against)
+  const bool wave_interior = !s.general && __all(interior || !active);
+  if (active && s.general) {
+    const int* row = idx + (long long)i * width;
+    if (KIND == JMD_LENNARD_JONES && a.smooth)
+      row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false, 1, true>(pos, n, s, a, row, width,
+                                                                       sub, i, xi, xim, f, &e);
+    else
+      row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false, 0, true>(pos, n, s, a, row, width,
+                                                                       sub, i, xi, xim, f, &e);
+  } else if (active) {
     const int* row = idx + (long long)i * width;
     // the smoothing switch is uniform: taken once here, not per pair
     if (KIND == JMD_LENNARD_JONES && a.smooth) {
@@ -259,6 +269,7 @@ __global__ __launch_bounds__(256) void k_pair_sparse(
         for (int c = 0; c < DIM; ++c)
           d[c] = s.periodic ? min_image_disp(xa[c], xb[c], xb[c] - s.L[c], s.L[c], s.half[c])
                             : xa[c] - xb[c];
+        if (s.general) box_apply<DIM>(s, d);
         float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
                             : fmaf(d[1], d[1], d[0] * d[0]);
         float sig = a.sig, eps = a.eps, alp = a.alp;
@@ -458,6 +469,7 @@ __global__ __launch_bounds__(256) void k_pair_virial(const float* __restrict__ p
     for (int c = 0; c < DIM; ++c)
       d[c] = s.periodic ? min_image_disp(xj[c], xi[c], xi[c] - s.L[c], s.L[c], s.half[c])
                         : xj[c] - xi[c];
+    if (s.general) box_apply<DIM>(s, d);
     const float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
                               : fmaf(d[1], d[1], d[0] * d[0]);
     float sig = a.sig, eps = a.eps, alp = a.alp;
@@ -583,6 +595,10 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
   if (momentum && !force) return JMD_ERR_INVALID;
   PairArgs a;
   if (!make_pair_args(potential, &a)) return JMD_ERR_INVALID;
+  {
+    const int grc = space_check(dim, periodic, side, true);
+    if (grc) return grc;
+  }
   Space s = make_space(dim, periodic, side);
   hipStream_t st = (hipStream_t)stream;
   int nn = (int)n;
@@ -690,6 +706,10 @@ int jmd_pair_virial(const float* position, int64_t n, int32_t dim, int32_t posit
   if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
   PairArgs a;
   if (!make_pair_args(potential, &a)) return JMD_ERR_INVALID;
+  {
+    const int grc = space_check(dim, periodic, side, true);
+    if (grc) return grc;
+  }
   Space s = make_space(dim, periodic, side);
   hipStream_t st = (hipStream_t)stream;
   const bool dense = format == JMD_DENSE;

@@ -205,6 +205,7 @@ int jmd_pair_energy_force_f64(const double* position, int64_t n, int32_t dim, in
                               const int32_t* idx, int64_t idx_width, int32_t format,
                               double* force, double* energy_total, double* momentum,
                               double half_dt, void* workspace, void* stream) {
+  if (periodic == 2) return JMD_ERR_UNSUPPORTED;  // periodic_general: float32 path only
   if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || idx_width < 0 ||
       idx_width > 0x7fffffff || (n > 0 && !position) || (idx_width > 0 && !idx) ||
       (periodic && !side))
@@ -258,6 +259,10 @@ int jmd_nve_kick_drift_f64(double* position, double* momentum, const double* for
   if (n == 0) return JMD_OK;
   if (!stride_ok(dim, position_stride, position_f32)) return JMD_ERR_INVALID;
   if (!mass && !(mass_scalar > 0.0)) return JMD_ERR_INVALID;
+  {
+    const int grc = space_check(dim, periodic, side, false);
+    if (grc) return grc;
+  }
   Space s = make_space(dim, periodic, side);
   hipStream_t st = (hipStream_t)stream;
   const int nn = (int)n;

@@ -61,15 +61,6 @@ def is_format_valid(fmt: NeighborListFormat):
         f'Neighbor list format must be a member of NeighborListFormat found {fmt}.')
 
 
-def is_box_valid(box) -> bool:
-  box = onp.asarray(box)
-  if box.ndim <= 1:
-    return True
-  if box.ndim == 2:
-    return bool(onp.all(onp.triu(box) == box))
-  return False
-
-
 class PartitionErrorCode(IntEnum):
   NONE = 0
   NEIGHBOR_LIST_OVERFLOW = 1 << 0
@@ -121,6 +112,54 @@ def _box_f32(box) -> onp.ndarray:
         'Box matrices (periodic_general / fractional coordinates) are outside '
         'this package.')
   return b
+
+
+def _fractional_cell_size(box, cutoff) -> f32:
+  """Cell size on the unit cube for a box (partition.py:595-633), float32:
+  the cutoff over the smallest perpendicular width of the cell."""
+  if isinstance(box, torch.Tensor):
+    box = box.detach().cpu().numpy()
+  b = onp.asarray(box, f32)
+  c = f32(cutoff)
+  if b.ndim == 0:
+    return f32(c / b)
+  if b.ndim == 1:
+    return f32(c / onp.min(b))
+  one = f32(1)
+  if b.shape[0] == 1:
+    return f32(one / onp.floor(f32(b[0, 0] / c)))
+  if b.shape[0] == 2:
+    xx, yy = b[0, 0], b[1, 1]
+    xy = f32(b[0, 1] / yy)
+    nx = f32(xx / onp.sqrt(f32(one + xy * xy)))
+    ny = yy
+  elif b.shape[0] == 3:
+    xx, yy, zz = b[0, 0], b[1, 1], b[2, 2]
+    xy = f32(b[0, 1] / yy)
+    xz = f32(b[0, 2] / zz)
+    yz = f32(b[1, 2] / zz)
+    t = f32(xy * yz - xz)
+    nx = f32(xx / onp.sqrt(f32(f32(one + f32(xy * xy)) + f32(t * t))))
+    ny = f32(yy / onp.sqrt(f32(one + f32(yz * yz))))
+    nz = zz
+  else:
+    raise ValueError(f'Expected box to be either 1-, 2-, or 3-dimensional found {b.shape[0]}')
+  widths = [nx, ny] if b.shape[0] == 2 else [nx, ny, nz]
+  nmin = onp.floor(f32(f32(min(widths)) / c))
+  nmin = f32(1) if nmin == 0 else f32(nmin)
+  return f32(one / nmin)
+
+
+def is_box_valid(box) -> bool:
+  """Scalar / vector boxes, or upper-triangular matrices (partition.py:676-681)."""
+  if isinstance(box, torch.Tensor):
+    box = box.detach().cpu().numpy()
+  b = onp.asarray(box)
+  if b.ndim <= 1:
+    return True
+  if b.ndim == 2:
+    return bool(onp.all(onp.triu(b) == b))
+  return False
 
 
 def cell_dimensions(dim: int, box, minimum_cell_size):
@@ -283,7 +322,7 @@ class _Plan:
   cells_per_side: list
   cell_size3: list
   side: list
-  periodic: bool
+  periodic: int               # 0 free, 1 periodic box, 2 box matrix (fractional)
   dim: int
   spec: Any = None
 
@@ -342,12 +381,21 @@ class NeighborListSpec:
   derived constants (partition.py:887-895)."""
 
   def __init__(self, displacement_or_metric, box, r_cutoff, dr_threshold,
-               capacity_multiplier, disable_cell_list, mask_self, fmt):
+               capacity_multiplier, disable_cell_list, mask_self, fmt,
+               fractional_coordinates=False):
     self.geometry = space_mod.geometry_of(displacement_or_metric)
-    self.box = _box_f32(box)
+    self.fractional = bool(fractional_coordinates)
     rc = f32(onp.asarray(r_cutoff, f32))
     dr = f32(onp.asarray(dr_threshold, f32))
     self.cutoff = f32(rc + dr)
+    if self.fractional:
+      # cells on the unit cube, sized from the box (partition.py:1045-1050)
+      self.user_box = box
+      self.box = onp.asarray(f32(1.0))
+      self.min_cell_size = _fractional_cell_size(box, self.cutoff)
+    else:
+      self.box = _box_f32(box)
+      self.min_cell_size = self.cutoff
     self.cutoff_sq = f32(self.cutoff * self.cutoff)
     self.threshold_sq = f32((dr / f32(2)) * (dr / f32(2)))
     self.always_rebuild = bool(dr == 0)
@@ -359,7 +407,7 @@ class NeighborListSpec:
   def use_cell_list(self) -> bool:
     if self.disable_cell_list:
       return False
-    return bool(onp.all(self.cutoff < (self.box / f32(3.0)).astype(f32)))
+    return bool(onp.all(self.min_cell_size < (self.box / f32(3.0)).astype(f32)))
 
 
 _FORMAT_CODE = {Dense: nat.DENSE, Sparse: nat.SPARSE, OrderedSparse: nat.ORDERED_SPARSE}
@@ -373,18 +421,27 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
                   **static_kwargs) -> NeighborListFns:
   """Neighbour lists for a periodic or free space (partition.py:801-1164).
 
-  Supported: ``space.periodic`` (scalar or per-dimension side) and
-  ``space.free`` displacements, Dense / Sparse / OrderedSparse, with or
-  without the cell list, ``mask_self``.  ``custom_mask_function`` and
-  fractional coordinates raise ``NotImplementedError``.
+  Supported: ``space.periodic`` (scalar or per-dimension side),
+  ``space.periodic_general`` with ``fractional_coordinates=True`` (cells on
+  the unit cube, distances through the box matrix) and ``space.free``
+  displacements, Dense / Sparse / OrderedSparse, with or without the cell
+  list, ``mask_self``.  ``custom_mask_function`` raises
+  ``NotImplementedError``.
   """
   is_format_valid(format)
   if custom_mask_function is not None:
     raise NotImplementedError('custom_mask_function is not supported by the native build')
-  if fractional_coordinates:
-    raise NotImplementedError('fractional coordinates (periodic_general) are out of scope')
+  general = isinstance(space_mod.geometry_of(displacement_or_metric),
+                       space_mod.GeneralDisplacement)
+  if fractional_coordinates and not general:
+    raise NotImplementedError('fractional coordinates need a space.periodic_general '
+                              'displacement')
+  if general and not fractional_coordinates:
+    raise NotImplementedError('a space.periodic_general displacement stores fractional '
+                              'coordinates: pass fractional_coordinates=True')
   spec = NeighborListSpec(displacement_or_metric, box, r_cutoff, dr_threshold,
-                          capacity_multiplier, disable_cell_list, mask_self, format)
+                          capacity_multiplier, disable_cell_list, mask_self, format,
+                          fractional_coordinates)
   fcode = _FORMAT_CODE[format]
 
   def _build(position, nbrs: Optional[NeighborList], plan: _Plan, gate):
@@ -404,6 +461,9 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
 
   def allocate_fn(position, extra_capacity: int = 0, **kwargs) -> NeighborList:
     if 'box' in kwargs:
+      if spec.fractional:
+        raise NotImplementedError('a changing box (box= keyword) is not supported by the '
+                                  'native build; the box matrix is fixed by the space')
       raise ValueError('Neighbor list cannot accept a box keyword argument '
                        'if fractional_coordinates is not enabled.')
     stride = _check_position(position)
@@ -416,17 +476,17 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
     cps3, cs3, ncell, cap = [1] * dim, [1.0] * dim, 1, None
     cell_size = None
     if use_cells:
-      _, cell_size, cps, ncell = cell_dimensions(dim, spec.box, spec.cutoff)
+      _, cell_size, cps, ncell = cell_dimensions(dim, spec.box, spec.min_cell_size)
       cps3, cs3 = _per_dim(cps, dim), _per_dim(cell_size, dim)
       _, cs2, cps2, _ = cell_dimensions(dim, spec.box, cell_size)
       fill = _filling_max(position, _per_dim(cps2, dim), _per_dim(cs2, dim))
       cap = int(onp.int32(fill) * capacity_multiplier) + extra_capacity
-      cell_size = spec.cutoff
+      cell_size = spec.min_cell_size
     ws = _Workspace(dev, lib.jmd_neighbor_workspace_bytes(N, dim, ncell if use_cells else 0,
                                                           int(cap or 0), fcode))
     plan = _Plan(ws, torch.zeros((1,), dtype=torch.int32, device=dev),
                  torch.zeros((1,), dtype=torch.int32, device=dev), use_cells, cps3, cs3, side,
-                 bool(geo.periodic), dim, spec)
+                 int(geo.periodic), dim, spec)
     st = nat.stream_handle(dev)
     # 1) measure the occupancy (allocate is not jittable in the reference either)
     nat.check(lib.jmd_neighbor_list_build(
@@ -458,10 +518,17 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
         cell_list_capacity=cap, max_occupancy=max_occ, format=format, cell_size=cell_size,
         cell_list_fn=None, update_fn=update_fn, plan=plan)
     _build(position, nbrs, plan, None)
+    if spec.fractional and not spec.disable_cell_list and is_box_valid(spec.user_box):
+      # the reference raises this bit when is_box_valid holds (its predicate,
+      # partition.py:1049, as written)
+      nbrs.error.code.bitwise_or_(int(PEC.MALFORMED_BOX))
     return nbrs
 
   def update_fn(position, neighbors: NeighborList, **kwargs) -> NeighborList:
     if 'box' in kwargs:
+      if spec.fractional:
+        raise NotImplementedError('a changing box (box= keyword) is not supported by the '
+                                  'native build; the box matrix is fixed by the space')
       raise ValueError('Neighbor list cannot accept a box keyword argument '
                        'if fractional_coordinates is not enabled.')
     stride = _check_position(position)

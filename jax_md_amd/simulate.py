@@ -63,11 +63,13 @@ def native_shift(shift_fn) -> space_mod.Shift:
   if isinstance(shift_fn, space_mod.Shift):
     return shift_fn
   raise NotImplementedError(
-      'The native integrators need a shift_fn from jax_md_amd.space.periodic or '
-      f'jax_md_amd.space.free; got {type(shift_fn).__name__}.')
+      'The native integrators need a shift_fn from jax_md_amd.space.periodic, '
+      f'periodic_general or free; got {type(shift_fn).__name__}.')
 
 
 def _side(shift: space_mod.Shift, dim):
+  if isinstance(shift, space_mod.GeneralShift):
+    return shift.side_values(dim)  # the box matrix, row-major
   if not shift.periodic:
     return [1.0] * dim
   s = onp.asarray(shift.side, f32)

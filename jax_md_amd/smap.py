@@ -161,6 +161,9 @@ class PairEnergy:
     """The float64 path (the reference's x64 run): R, the force and the
     momentum are float64; Dense lists only."""
     nat.require_device_tensor(R, 'R', torch.float64)
+    if int(self.geometry.periodic) == 2:
+      raise NotImplementedError('float64 positions with a periodic_general box are not '
+                                'supported natively')
     if neighbor.format is not partition.Dense:
       raise NotImplementedError('float64 positions run over Dense neighbour lists')
     if momentum is not None:
@@ -224,9 +227,13 @@ class PairEnergy:
   # -- public ----------------------------------------------------------------
   def __call__(self, R, neighbor=None, **dynamic_kwargs):
     if 'fractional_coordinates' in dynamic_kwargs:
+      # an unused pair-function argument in the reference as well; the
+      # geometry comes from the displacement
       dynamic_kwargs = dict(dynamic_kwargs)
-      if dynamic_kwargs.pop('fractional_coordinates'):
-        raise NotImplementedError('fractional coordinates are out of scope')
+      dynamic_kwargs.pop('fractional_coordinates')
+    if 'box' in dynamic_kwargs or 'new_box' in dynamic_kwargs:
+      raise NotImplementedError('a changing box (box= / new_box= keyword) is not supported '
+                                'by the native build')
     if self.reduce_axis is not None and len(self.reduce_axis) == 0:
       raise NotImplementedError('reduce_axis=() (edge-wise output) is not supported natively')
     per_particle = self.reduce_axis is not None
@@ -307,8 +314,8 @@ def pair_neighbor_list(fn, displacement_or_metric, species=None,
   (jax_md/smap.py:856-979)."""
   kind, r_on, r_c = _native_kind(fn)
   kwargs = dict(kwargs)
-  if kwargs.pop('fractional_coordinates', False):
-    raise NotImplementedError('fractional coordinates are out of scope')
+  # the reference passes it on to the pair function, which ignores it
+  kwargs.pop('fractional_coordinates', None)
   geo = space_mod.geometry_of(displacement_or_metric)
   return PairEnergy(kind, r_on, r_c, geo, species, reduce_axis, ignore_unused_parameters,
                     kwargs)

@@ -1,11 +1,11 @@
 """Spaces (jax_md/space.py), carrying the geometry the native kernels need.
 
-``periodic(side)`` and ``free()`` return ``(displacement_fn, shift_fn)``
-callables with the reference semantics on torch tensors.  The callables also
-describe their geometry (``side`` as float32, ``periodic``), which is how
-``partition.neighbor_list``, ``smap.pair_neighbor_list`` and the integrators
-route to the gfx950 kernels.  Other spaces (``periodic_general``) are outside
-this package's scope.
+``periodic(side)``, ``periodic_general(box)`` (fractional coordinates) and
+``free()`` return ``(displacement_fn, shift_fn)`` callables with the
+reference semantics on torch tensors.  The callables also describe their
+geometry (``side`` as float32, ``periodic``: 0 free, 1 periodic box, 2 box
+matrix), which is how ``partition.neighbor_list``, ``smap.pair_neighbor_list``
+and the integrators route to the gfx950 kernels.
 """
 
 from __future__ import annotations
@@ -118,6 +118,100 @@ class Metric:
 
   def __call__(self, Ra, Rb, **kwargs):
     return distance(self.displacement(Ra, Rb, **kwargs))
+
+
+def _box_as_matrix(box, dim: int) -> onp.ndarray:
+  """[dim, dim] float32 matrix of a scalar, vector or matrix box."""
+  b = onp.asarray(box.detach().cpu().numpy() if isinstance(box, torch.Tensor) else box,
+                  onp.float32)
+  if b.ndim == 0:
+    return onp.eye(dim, dtype=onp.float32) * b
+  if b.ndim == 1:
+    if b.shape[0] != dim:
+      raise ValueError(f'box has {b.shape[0]} entries for dimension {dim}')
+    return onp.diag(b).astype(onp.float32)
+  if b.shape != (dim, dim):
+    raise ValueError(f'box of shape {b.shape} for dimension {dim}')
+  return b
+
+
+def transform(box, R: torch.Tensor) -> torch.Tensor:
+  """box applied to R (jax_md/space.py:128-169): R * box, box * R per
+  component, or the matrix product sum_j box[i, j] R[..., j]."""
+  b = torch.as_tensor(onp.asarray(box, onp.float32) if not isinstance(box, torch.Tensor)
+                      else box, dtype=R.dtype, device=R.device)
+  if b.ndim <= 1:
+    return R * b
+  return torch.einsum('ij,...j->...i', b, R)
+
+
+def inverse(box):
+  """Inverse of an affine box (jax_md/space.py:110-125)."""
+  b = onp.asarray(box, onp.float32)
+  if b.ndim <= 1:
+    return (onp.float32(1) / b).astype(onp.float32)
+  return onp.linalg.inv(b.astype(onp.float64)).astype(onp.float32)
+
+
+class GeneralDisplacement(Displacement):
+  """periodic_general with fractional coordinates: d = T wrap(u_a - u_b),
+  the wrap on the unit cube (jax_md/space.py:420-443)."""
+
+  def __init__(self, box):
+    super().__init__(onp.float32(1.0), True)
+    self.box = box
+    self.periodic = 2
+
+  def __call__(self, Ra, Rb, perturbation=None, **kwargs):
+    box = kwargs.get('new_box', kwargs.get('box', self.box))
+    dR = periodic_displacement(1.0, Ra - Rb)
+    dR = transform(box, dR)
+    if perturbation is not None:
+      dR = dR @ torch.as_tensor(perturbation, dtype=dR.dtype, device=dR.device).T
+    return dR
+
+  def side_values(self, dim: int):
+    return [float(v) for v in _box_as_matrix(self.box, dim).reshape(-1)]
+
+
+class GeneralShift(Shift):
+  """shift_fn(u, dR) = mod(u + T^-1 dR, 1) (jax_md/space.py:450-470)."""
+
+  def __init__(self, box, wrapped: bool = True):
+    super().__init__(onp.float32(1.0), True, wrapped)
+    self.box = box
+
+  @property
+  def wraps(self):
+    if not self.wrapped:
+      raise NotImplementedError('periodic_general(wrapped=False) is not supported by the '
+                                'native integrators')
+    return 2
+
+  def side_values(self, dim: int):
+    return [float(v) for v in _box_as_matrix(self.box, dim).reshape(-1)]
+
+  def __call__(self, R, dR, **kwargs):
+    box = kwargs.get('new_box', kwargs.get('box', self.box))
+    dR = transform(inverse(box), dR)
+    if self.wrapped:
+      return periodic_shift(1.0, R, dR)
+    return R + dR
+
+
+def periodic_general(box, fractional_coordinates: bool = True, wrapped: bool = True):
+  """Periodic parallelepiped X = T [0, 1]^d (jax_md/space.py:332-472), with
+  positions stored as fractional coordinates u (x = T u).  Displacements and
+  forces live in X.  ``fractional_coordinates=False`` is not supported."""
+  if not fractional_coordinates:
+    raise NotImplementedError('periodic_general with real-space positions '
+                              '(fractional_coordinates=False) is not supported; store '
+                              'fractional coordinates')
+  b = onp.asarray(box.detach().cpu().numpy() if isinstance(box, torch.Tensor) else box,
+                  onp.float32)
+  if b.ndim > 2:
+    raise ValueError(f'Box must be either: a scalar, a vector, or a matrix. Found {box}.')
+  return GeneralDisplacement(b), GeneralShift(b, wrapped)
 
 
 def periodic(side, wrapped: bool = True):

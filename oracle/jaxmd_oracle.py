@@ -7,7 +7,9 @@ only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 It restates, in NumPy, the reference algorithm of jax-md 0.2.29 for:
 
 * periodic / free displacement, square distance and shift
-  (jax_md/space.py:213-252,258-329,494-502);
+  (jax_md/space.py:213-252,258-329,494-502), and periodic_general with
+  fractional coordinates (jax_md/space.py:110-169,332-472) with the
+  fractional cell size of its neighbour lists (jax_md/partition.py:595-633);
 * cell-list binning with the reference's slot layout
   (jax_md/partition.py:146-293,349-471);
 * the Dense / Sparse / OrderedSparse neighbour-list build, capacities, error
@@ -131,6 +133,87 @@ class Space:
     if not self.periodic:
       return R + dR
     return periodic_shift(np.asarray(self.side, R.dtype), R, dR)
+
+
+def box_matrix(box, dim, dtype=F32):
+  """[dim, dim] matrix of a scalar, vector or matrix box."""
+  b = np.asarray(box, dtype)
+  if b.ndim == 0:
+    return np.eye(dim, dtype=dtype) * b
+  if b.ndim == 1:
+    return np.diag(b).astype(dtype)
+  return b
+
+
+def transform(T, dR):
+  """T applied to the last axis of dR (raw_transform, space.py:128-153): each
+  component sum_j T[i, j] dR[..., j], products rounded, summed left to right
+  in the arrays' dtype (NumPy never fuses a multiply-add)."""
+  dR = np.asarray(dR)
+  T = np.asarray(T, dR.dtype)
+  out = np.empty_like(dR)
+  d = dR.shape[-1]
+  for i in range(d):
+    acc = T[i, 0] * dR[..., 0]
+    for j in range(1, d):
+      acc = acc + T[i, j] * dR[..., j]
+    out[..., i] = acc
+  return out
+
+
+@dataclass
+class GeneralSpace:
+  """periodic_general with fractional coordinates (jax_md/space.py:332-472):
+  positions u in the unit cube, x = T u.  displacement = T wrap_1(u_a - u_b),
+  shift = mod(u + T^-1 dR, 1).  ``box`` is a scalar, vector or matrix, kept
+  float32-valued as the reference's box; the inverse is inv(T) in float64
+  rounded to the arrays' dtype."""
+  box: np.ndarray
+  periodic: bool = True
+
+  def matrix(self, dim, dtype):
+    return box_matrix(np.asarray(self.box, F32), dim).astype(dtype)
+
+  def displacement(self, Ra, Rb):
+    d = np.asarray(Ra) - np.asarray(Rb)
+    d = periodic_displacement(np.asarray(1.0, d.dtype), d)
+    return transform(self.matrix(d.shape[-1], d.dtype), d)
+
+  def metric_sq(self, Ra, Rb):
+    return square_distance(self.displacement(Ra, Rb))
+
+  def shift(self, R, dR):
+    R = np.asarray(R)
+    T = self.matrix(R.shape[-1], F64)
+    Ti = np.linalg.inv(T).astype(R.dtype)
+    return periodic_shift(np.asarray(1.0, R.dtype), R, transform(Ti, np.asarray(dR, R.dtype)))
+
+
+def fractional_cell_size(box, cutoff):
+  """Cell size on the unit cube (jax_md/partition.py:595-633), float32: the
+  cutoff over the smallest perpendicular width of the box."""
+  b = np.asarray(box, F32)
+  c = F32(cutoff)
+  if b.ndim == 0:
+    return F32(c / b)
+  if b.ndim == 1:
+    return F32(c / np.min(b))
+  one = F32(1)
+  if b.shape[0] == 1:
+    return F32(one / np.floor(F32(b[0, 0] / c)))
+  if b.shape[0] == 2:
+    xy = F32(b[0, 1] / b[1, 1])
+    widths = [F32(b[0, 0] / np.sqrt(F32(one + F32(xy * xy)))), b[1, 1]]
+  else:
+    xy = F32(b[0, 1] / b[1, 1])
+    xz = F32(b[0, 2] / b[2, 2])
+    yz = F32(b[1, 2] / b[2, 2])
+    t = F32(F32(xy * yz) - xz)
+    widths = [F32(b[0, 0] / np.sqrt(F32(F32(one + F32(xy * xy)) + F32(t * t)))),
+              F32(b[1, 1] / np.sqrt(F32(one + F32(yz * yz)))), b[2, 2]]
+  nmin = np.floor(F32(F32(min(widths)) / c))
+  nmin = F32(1) if nmin == 0 else F32(nmin)
+  return F32(one / nmin)
 
 
 # ---------------------------------------------------------------------------
@@ -285,9 +368,12 @@ class NeighborListSpec:
   mask_self: bool = True
   format: int = DENSE
   custom_mask_function: Optional[object] = None
+  fractional_coordinates: bool = False
   cutoff: np.ndarray = field(init=False)
   cutoff_sq: np.ndarray = field(init=False)
   threshold_sq: np.ndarray = field(init=False)
+  cell_box: np.ndarray = field(init=False)
+  min_cell_size: np.ndarray = field(init=False)
 
   def __post_init__(self):
     # r_cutoff and dr_threshold arrive as float32 (maybe_downcast in the
@@ -298,6 +384,13 @@ class NeighborListSpec:
     self.cutoff = F32(rc + dr)
     self.cutoff_sq = F32(self.cutoff * self.cutoff)
     self.threshold_sq = F32((dr / F32(2)) * (dr / F32(2)))
+    if self.fractional_coordinates:
+      # cells on the unit cube (partition.py:1045-1050)
+      self.cell_box = np.asarray(F32(1.0))
+      self.min_cell_size = fractional_cell_size(self.box, self.cutoff)
+    else:
+      self.cell_box = self.box
+      self.min_cell_size = self.cutoff
 
   @property
   def always_rebuild(self):
@@ -306,8 +399,8 @@ class NeighborListSpec:
   def uses_cell_list(self, dim):
     if self.disable_cell_list:
       return False
-    box3 = (self.box / F32(3.0)).astype(F32)
-    return bool(np.all(self.cutoff < box3))
+    box3 = (self.cell_box / F32(3.0)).astype(F32)
+    return bool(np.all(self.min_cell_size < box3))
 
 
 def _candidates_cell(spec, position, cl, rows):
@@ -392,11 +485,12 @@ def build_cell_list_for(spec, position, neighbors=None, extra_capacity=0):
   if not spec.uses_cell_list(position.shape[1]):
     return None
   if neighbors is None:
-    return cell_list(position, spec.box, spec.cutoff, spec.capacity_multiplier,
+    return cell_list(position, spec.cell_box, spec.min_cell_size, spec.capacity_multiplier,
                      extra_capacity=extra_capacity)
   if neighbors.cell_list_capacity is None:
     return None
-  return cell_list(position, spec.box, spec.cutoff, capacity=neighbors.cell_list_capacity)
+  return cell_list(position, spec.cell_box, spec.min_cell_size,
+                   capacity=neighbors.cell_list_capacity)
 
 
 def neighbor_list_build(spec, position, neighbors=None, extra_capacity=0):
@@ -405,6 +499,12 @@ def neighbor_list_build(spec, position, neighbors=None, extra_capacity=0):
   position = np.asarray(position)
   N, dim = position.shape
   err = 0 if neighbors is None else neighbors.error
+  if neighbors is None and spec.fractional_coordinates and not spec.disable_cell_list:
+    # raised when the box IS upper triangular: the reference's predicate as
+    # written (partition.py:1049 with is_box_valid, partition.py:676-681)
+    b = np.asarray(spec.box)
+    if b.ndim <= 1 or bool(np.all(np.triu(b) == b)):
+      err |= MALFORMED_BOX
   cl = build_cell_list_for(spec, position, neighbors, extra_capacity)
   if cl is not None and cl.did_buffer_overflow:
     err |= CELL_LIST_OVERFLOW
@@ -1112,6 +1212,40 @@ def lj_lattice(n_side, rho=0.8442, jitter=0.05, kT=1.0, seed=0, dim=3):
   P = rng.normal(0.0, np.sqrt(kT), R.shape)
   P = (P - P.mean(0, keepdims=True)).astype(F32)
   return R, P, L
+
+
+def fcc_cubic(n_cells, a=1.5496):
+  """FCC crystal, n_cells^3 conventional cubic cells of side a (4 atoms
+  each) in a periodic cube.  Returns (R f32, L f32)."""
+  basis = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]], F64)
+  g = np.stack(np.meshgrid(*([np.arange(n_cells)] * 3), indexing='ij'), -1).reshape(-1, 3)
+  R = ((g[:, None, :] + basis[None]) * a).reshape(-1, 3)
+  return R.astype(F32), F32(n_cells * a)
+
+
+def fcc_triclinic(n, a=1.5496, jitter=0.0, kT=0.0, seed=0):
+  """The same FCC crystal as an n^3 supercell of its primitive cell (one
+  atom each), rotated so that the box matrix is upper triangular: the
+  primitive vectors M = a/2 [(0,1,1), (1,0,1), (1,1,0)] (columns) factor as
+  M = Q R, and the box is T = n R with positive diagonal.  Optional Gaussian
+  jitter in real space.  Returns (fractional positions f32 in [0, 1), T f32
+  [3, 3], momenta f32 with the mean removed)."""
+  M = 0.5 * a * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]], F64)
+  Q, Rm = np.linalg.qr(M)
+  D = np.diag(np.sign(np.diag(Rm)))
+  Rm = D @ Rm
+  T = (n * Rm).astype(F32)
+  g = np.stack(np.meshgrid(*([np.arange(n)] * 3), indexing='ij'), -1).reshape(-1, 3)
+  u = g.astype(F64) / n
+  rng = np.random.default_rng(seed)
+  if jitter > 0:
+    x = u @ T.astype(F64).T + rng.normal(0.0, jitter, u.shape)
+    u = x @ np.linalg.inv(T.astype(F64)).T
+  u = np.mod(u, 1.0).astype(F32)
+  u = np.where(u >= F32(1), F32(0), u).astype(F32)
+  P = rng.normal(0.0, np.sqrt(kT), u.shape) if kT > 0 else np.zeros(u.shape)
+  P = (P - P.mean(0, keepdims=True)).astype(F32)
+  return u, T, P
 
 
 def bidisperse_2d(N, phi=0.84, d_small=1.0, d_large=1.4, seed=0):

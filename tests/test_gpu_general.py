@@ -1,0 +1,205 @@
+"""periodic_general (fractional coordinates) on the GPU against the oracle.
+
+Bars: neighbour lists bit-exact against the oracle's float32 build (Dense rows
+in candidate order, Sparse entry by entry, error word and capacities); forces
+norm-wise within 1e-5 and per particle within 5e-5 x rms of the float64
+oracle, energies within 1e-6; a diagonal box matches space.periodic on the
+same crystal; 100 NVE steps within 1e-4 (positions, real space) and 1e-3
+(momenta) of the float64 trajectory; the pressure within 1e-5 of the
+oracle's virial.  The crystal is FCC written as a rotated triclinic supercell
+of its primitive cell (box matrix upper triangular).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import jaxmd_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+DEV = 'cuda:0'
+
+
+def _jmd():
+  import jax_md_amd
+  return jax_md_amd
+
+
+def _fmt(jmd, code):
+  return {O.DENSE: jmd.partition.Dense, O.SPARSE: jmd.partition.Sparse,
+          O.ORDERED_SPARSE: jmd.partition.OrderedSparse}[code]
+
+
+def _force_err(F, F64):
+  F = np.asarray(F, np.float64)
+  rel = np.linalg.norm(F - F64) / np.linalg.norm(F64)
+  worst = np.max(np.abs(F - F64)) / np.sqrt(np.mean(F64 ** 2))
+  return rel, worst
+
+
+def _tri_2d(N, seed):
+  rng = np.random.default_rng(seed)
+  T = np.array([[14.0, 3.5], [0.0, 12.5]], F32)
+  u = rng.uniform(0, 1, (N, 2)).astype(F32)
+  return u, T
+
+
+@pytest.mark.parametrize('n_rep,dc', [(14, False), (8, False), (14, True)])
+@pytest.mark.parametrize('fmt', [O.DENSE, O.SPARSE, O.ORDERED_SPARSE])
+def test_general_neighbor_list_bit_exact(fmt, n_rep, dc):
+  print(f'case general list fmt={fmt} n={n_rep} disable_cells={dc}', flush=True)
+  jmd = _jmd()
+  u, T, _ = O.fcc_triclinic(n_rep, jitter=0.1, seed=4)
+  disp, _ = jmd.space.periodic_general(T)
+  nfn = jmd.partition.neighbor_list(disp, T, 2.5, 0.5, disable_cell_list=dc,
+                                    fractional_coordinates=True, format=_fmt(jmd, fmt))
+  spec = O.NeighborListSpec(O.GeneralSpace(T), T, 2.5, 0.5, disable_cell_list=dc, format=fmt,
+                            fractional_coordinates=True)
+  g = nfn.allocate(torch.as_tensor(u, device=DEV))
+  o = O.neighbor_list_build(spec, u)
+  gi = g.idx.cpu().numpy()
+  print(f'  cells {o.use_cell_list} occupancy {o.occupancy} error {o.error}')
+  assert gi.shape == o.idx.shape
+  assert g.max_occupancy == o.max_occupancy
+  assert g.cell_list_capacity == o.cell_list_capacity
+  assert int(g.error.code.item()) == o.error
+  assert not bool(g.did_buffer_overflow)
+  assert np.array_equal(gi, o.idx)
+
+
+@pytest.mark.parametrize('fmt', [O.DENSE, O.SPARSE])
+def test_general_neighbor_list_2d_bit_exact(fmt):
+  print(f'case general list 2-D fmt={fmt}', flush=True)
+  jmd = _jmd()
+  u, T = _tri_2d(900, 7)
+  disp, _ = jmd.space.periodic_general(T)
+  nfn = jmd.partition.neighbor_list(disp, T, 1.1, 0.2, fractional_coordinates=True,
+                                    format=_fmt(jmd, fmt))
+  spec = O.NeighborListSpec(O.GeneralSpace(T), T, 1.1, 0.2, format=fmt,
+                            fractional_coordinates=True)
+  g = nfn.allocate(torch.as_tensor(u, device=DEV))
+  o = O.neighbor_list_build(spec, u)
+  assert o.use_cell_list
+  assert int(g.error.code.item()) == o.error
+  assert np.array_equal(g.idx.cpu().numpy(), o.idx)
+
+
+@pytest.mark.parametrize('fmt', [O.DENSE, O.SPARSE, O.ORDERED_SPARSE])
+def test_general_energy_force_vs_fp64(fmt):
+  print(f'case general LJ force fmt={fmt}', flush=True)
+  jmd = _jmd()
+  u, T, _ = O.fcc_triclinic(14, jitter=0.1, seed=5)
+  disp, _ = jmd.space.periodic_general(T)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, T, fractional_coordinates=True,
+                                                    format=_fmt(jmd, fmt))
+  ut = torch.as_tensor(u, device=DEV)
+  nb = nfn.allocate(ut)
+  E, F = efn.energy_and_force(ut, nb)
+  E64, _, F64 = O.energy_force_brute(O.PairModel('lj'), O.GeneralSpace(T), u.astype(np.float64))
+  rel, worst = _force_err(F.cpu().numpy(), F64)
+  print(f'  force rel={rel:.3e} worst/rms={worst:.3e} E={float(E):.8f} E64={E64:.8f}')
+  assert rel <= 1e-5 and worst <= 5e-5
+  assert abs(float(E) - E64) <= 1e-6 * abs(E64)
+
+
+def test_general_diagonal_box_matches_periodic():
+  """A diagonal box of power-of-two side: x = 16 u is exact in float32, so
+  both paths see the same displacements and must agree to rounding."""
+  print('case general diagonal == periodic', flush=True)
+  jmd = _jmd()
+  rng = np.random.default_rng(8)
+  L = F32(16.0)
+  g = (np.stack(np.meshgrid(*([np.arange(12)] * 3), indexing='ij'), -1).reshape(-1, 3) + 0.5)
+  u = np.mod(g / 12.0 + rng.normal(0, 0.003, g.shape), 1.0).astype(F32)
+  u = np.where(u >= 1, F32(0), u).astype(F32)
+  R = (u * L).astype(F32)
+  dp, _ = jmd.space.periodic(L)
+  nfp, efp = jmd.energy.lennard_jones_neighbor_list(dp, L, format=jmd.partition.Dense)
+  dg, _ = jmd.space.periodic_general(np.diag([L, L, L]).astype(F32))
+  nfg, efg = jmd.energy.lennard_jones_neighbor_list(dg, L, fractional_coordinates=True,
+                                                    format=jmd.partition.Dense)
+  Rt, ut = torch.as_tensor(R, device=DEV), torch.as_tensor(u, device=DEV)
+  Ep, Fp = efp.energy_and_force(Rt, nfp.allocate(Rt))
+  Eg, Fg = efg.energy_and_force(ut, nfg.allocate(ut))
+  rel, worst = _force_err(Fg.cpu().numpy(), Fp.cpu().numpy().astype(np.float64))
+  print(f'  E {float(Eg):.8f} vs {float(Ep):.8f}; force rel {rel:.2e} worst {worst:.2e}')
+  assert abs(float(Eg) - float(Ep)) <= 1e-6 * abs(float(Ep))
+  assert rel <= 1e-6 and worst <= 1e-5
+
+
+def test_general_nve_vs_fp64():
+  print('case general NVE 100 steps', flush=True)
+  jmd = _jmd()
+  u, T, P = O.fcc_triclinic(8, jitter=0.03, kT=0.6, seed=6)
+  disp, shift = jmd.space.periodic_general(T)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, T, fractional_coordinates=True,
+                                                    format=jmd.partition.Dense)
+  ut = torch.as_tensor(u, device=DEV)
+  nb = nfn.allocate(ut)
+  init_fn, apply_fn = jmd.simulate.nve(efn, shift, 0.005)
+  st = init_fn(0, ut, 0.0, momenta=torch.as_tensor(P, device=DEV), neighbor=nb)
+  for _ in range(100):
+    nb = nb.update(st.position)
+    st = apply_fn(st, neighbor=nb)
+  assert not bool(nb.did_buffer_overflow)
+  sp = O.GeneralSpace(T)
+  m = O.PairModel('lj')
+  ff = lambda X: O.energy_force_brute(m, sp, X)[2]
+  o = O.NVEState(u.astype(np.float64), P.astype(np.float64), ff(u.astype(np.float64)))
+  for _ in range(100):
+    o = O.nve_step(o, ff, sp, 0.005)
+  dR = sp.displacement(st.position_hi.cpu().numpy(), o.position)
+  dP = st.momentum.cpu().numpy() - o.momentum
+  print(f'  max |dR|={np.abs(dR).max():.3e} max |dP|={np.abs(dP).max():.3e}')
+  assert np.abs(dR).max() < 1e-4 and np.abs(dP).max() < 1e-3
+  assert np.all((st.position_hi.cpu().numpy() >= 0) & (st.position_hi.cpu().numpy() < 1))
+
+
+@pytest.mark.parametrize('reorder', [False, True])
+def test_general_nve_loop_matches_drop_in(reorder):
+  print(f'case general NVELoop reorder={reorder}', flush=True)
+  jmd = _jmd()
+  u, T, P = O.fcc_triclinic(14, jitter=0.03, kT=0.6, seed=7)
+  disp, shift = jmd.space.periodic_general(T)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, T, fractional_coordinates=True,
+                                                    format=jmd.partition.Dense)
+  ut = torch.as_tensor(u, device=DEV)
+  Pt = torch.as_tensor(P, device=DEV)
+  init_fn, apply_fn = jmd.simulate.nve(efn, shift, 0.005)
+  nb1 = nfn.allocate(ut)
+  s1 = init_fn(0, ut, 0.0, momenta=Pt, neighbor=nb1)
+  for _ in range(40):
+    nb1 = nb1.update(s1.position)
+    s1 = apply_fn(s1, neighbor=nb1)
+  nb2 = nfn.allocate(ut)
+  s2 = init_fn(0, ut, 0.0, momenta=Pt, neighbor=nb2)
+  loop = jmd.simulate.NVELoop(efn, shift, 0.005, reorder=reorder)
+  loop.prime(s2, nb2)
+  loop.run(s2, nb2, 20)
+  loop.run(s2, nb2, 20)
+  assert not bool(nb2.did_buffer_overflow)
+  sp = O.GeneralSpace(T)
+  dR = sp.displacement(s1.position_hi.cpu().numpy(), s2.position_hi.cpu().numpy())
+  dP = (s1.momentum - s2.momentum).abs().max().item()
+  print(f'  max |dR|={np.abs(dR).max():.3e} max |dP|={dP:.3e}')
+  assert np.abs(dR).max() < 1e-5 and dP < 1e-4
+
+
+def test_general_pressure_vs_oracle():
+  print('case general pressure', flush=True)
+  jmd = _jmd()
+  u, T, _ = O.fcc_triclinic(14, jitter=0.1, seed=8)
+  disp, _ = jmd.space.periodic_general(T)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, T, fractional_coordinates=True,
+                                                    format=jmd.partition.Dense)
+  ut = torch.as_tensor(u, device=DEV)
+  nb = nfn.allocate(ut)
+  Pg = float(jmd.quantity.pressure(efn, ut, T, neighbor=nb))
+  sp = O.GeneralSpace(T)
+  i, j = O.all_pairs(sp, u.astype(np.float64), 2.5)
+  W = O.pair_virial(O.PairModel('lj'), sp, u.astype(np.float64), i, j, 0.5)
+  P64 = O.pressure(W, abs(np.linalg.det(T.astype(np.float64))))
+  print(f'  pressure {Pg:.8e} vs {P64:.8e}')
+  assert abs(Pg - P64) <= 1e-5 * abs(P64)

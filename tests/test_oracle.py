@@ -35,6 +35,16 @@ Which check pins which oracle function:
                                    436-461), jammed pressure fixture
                                    (:134-146), central differences of an
                                    affine strain
+  GeneralSpace, transform          diagonal box == periodic on x = T u; an
+                                   FCC crystal has one E/N as a cubic cell and
+                                   as a rotated triclinic primitive supercell
+                                   (the transposed matrix fails); forces are
+                                   real-space central differences; the shift's
+                                   displacement returns dR
+  fractional_cell_size             perpendicular widths from cross products
+                                   (V / |a_j x a_k|) for triclinic boxes,
+                                   scalar and vector closed forms
+  neighbor_list_build (fractional) brute-force sets in the triclinic box
 """
 
 import math
@@ -614,3 +624,95 @@ def test_nvt_second_order():
     drifts.append(worst)
   ratio = drifts[0] / drifts[1]
   assert 3.0 < ratio < 5.5, (drifts, ratio)
+
+
+# ---------------------------------------------------------------------------
+# periodic_general with fractional coordinates
+# ---------------------------------------------------------------------------
+
+
+def test_general_space_diagonal_equals_periodic():
+  rng = np.random.default_rng(11)
+  Ls = np.array([7.5, 9.25, 11.0])
+  u = rng.uniform(0, 1, (300, 3))
+  g = O.GeneralSpace(np.diag(Ls).astype(F32))
+  p = O.Space(Ls)
+  d_g = g.displacement(u[:, None], u[None])
+  d_p = p.displacement((u * Ls)[:, None], (u * Ls)[None])
+  assert np.max(np.abs(d_g - d_p)) < 1e-12
+
+
+def test_general_space_fcc_energy_invariant():
+  # one crystal, two cells: the energy per atom cannot depend on the cell
+  m = O.PairModel('lj')
+  R, L = O.fcc_cubic(4)
+  E1, _, _ = O.energy_force_brute(m, O.Space(np.float64(L)), R.astype(np.float64))
+  u, T, _ = O.fcc_triclinic(6)
+  E2, _, _ = O.energy_force_brute(m, O.GeneralSpace(T), u.astype(np.float64))
+  assert abs(E2 / len(u) - E1 / len(R)) < 1e-8 * abs(E1 / len(R))
+  # a plausible mistake (the transposed box) is caught
+  E3, _, _ = O.energy_force_brute(m, O.GeneralSpace(T.T.copy()), u.astype(np.float64))
+  assert abs(E3 / len(u) - E1 / len(R)) > 1e-2
+
+
+def test_general_space_forces_are_real_space_gradients():
+  m = O.PairModel('lj')
+  u, T, _ = O.fcc_triclinic(6, jitter=0.08, seed=3)
+  u = u.astype(np.float64)
+  g = O.GeneralSpace(T)
+  _, _, F = O.energy_force_brute(m, g, u)
+  Ti = np.linalg.inv(T.astype(np.float64))
+  h = 1e-6
+  for i in (0, 17, 101):
+    for k in range(3):
+      e = np.zeros(3)
+      e[k] = h
+      up, um = u.copy(), u.copy()
+      up[i] = up[i] + Ti @ e      # move particle i by +h along x_k in real space
+      um[i] = um[i] - Ti @ e
+      fd = -(O.energy_force_brute(m, g, up)[0] - O.energy_force_brute(m, g, um)[0]) / (2 * h)
+      assert abs(fd - F[i, k]) < 1e-6 * max(1.0, np.abs(F).max())
+
+
+def test_general_shift_moves_by_dR():
+  rng = np.random.default_rng(5)
+  _, T, _ = O.fcc_triclinic(6)
+  g = O.GeneralSpace(T)
+  u = rng.uniform(0, 1, (200, 3))
+  dR = rng.normal(0, 0.3, (200, 3))
+  v = g.shift(u, dR)
+  assert np.all((v >= 0) & (v < 1))
+  assert np.max(np.abs(g.displacement(v, u) - dR)) < 1e-6
+
+
+def test_fractional_cell_size_from_perpendicular_widths():
+  assert O.fractional_cell_size(F32(10.0), 2.5) == F32(0.25)
+  assert O.fractional_cell_size(np.array([8.0, 10.0, 12.0], F32), 2.0) == F32(0.25)
+  rng = np.random.default_rng(2)
+  for _ in range(20):
+    T = np.triu(rng.uniform(-2, 2, (3, 3))) + np.diag(rng.uniform(6, 14, 3))
+    T = T.astype(F32)
+    c = F32(rng.uniform(1.0, 2.5))
+    a = [T[:, 0].astype(np.float64), T[:, 1].astype(np.float64), T[:, 2].astype(np.float64)]
+    V = abs(np.linalg.det(T.astype(np.float64)))
+    widths = [V / np.linalg.norm(np.cross(a[1], a[2])), V / np.linalg.norm(np.cross(a[2], a[0])),
+              V / np.linalg.norm(np.cross(a[0], a[1]))]
+    n = max(1.0, np.floor(min(widths) / c))
+    got = O.fractional_cell_size(T, c)
+    assert abs(got - 1.0 / n) < 1e-6, (got, 1.0 / n)
+
+
+def test_general_neighbor_list_matches_brute_force():
+  # 14^3 primitive cells: perpendicular width 12.5 > 4 cutoffs, 4 cells a side
+  u, T, _ = O.fcc_triclinic(14, jitter=0.1, seed=9)
+  g = O.GeneralSpace(T)
+  spec = O.NeighborListSpec(g, T, 2.5, 0.5, fractional_coordinates=True)
+  nl = O.neighbor_list_build(spec, u)
+  assert nl.use_cell_list
+  assert nl.error & O.MALFORMED_BOX   # the reference's predicate for a valid box
+  rows = O.sorted_rows(nl)
+  N = len(u)
+  for i in range(0, N, 53):
+    d2 = g.metric_sq(u, u[i][None, :])
+    ref = np.nonzero((d2 < spec.cutoff_sq) & (np.arange(N) != i))[0]
+    assert np.array_equal(rows[i], ref)
