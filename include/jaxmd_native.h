@@ -16,9 +16,17 @@
  * Positions are [n, dim] float32, row-major (the reference's layout), with
  * dim 2 or 3, passed with a row stride ``position_stride`` in floats: dim
  * for a contiguous array, or 4 for 3-D rows padded to 16 bytes (base
- * 16-byte aligned), which the gather-heavy kernels read with one load.  A periodic space is described by ``periodic`` = 1 and
- * ``side`` (host pointer, dim float32 values: the float32 value of the side
- * passed to space.periodic); ``periodic`` = 0 is space.free().
+ * 16-byte aligned), which the gather-heavy kernels read with one load.
+ * A periodic space is described by ``periodic`` = 1 and ``side`` (host
+ * pointer, dim float32 values: the float32 value of the side passed to
+ * space.periodic); ``periodic`` = 0 is space.free(); ``periodic`` = 2 is
+ * space.periodic_general with fractional coordinates (space.py:332-472):
+ * positions are fractional in [0, 1), ``side`` holds the [dim, dim] box
+ * matrix T row-major (invertible), displacements are T wrap(u_a - u_b),
+ * forces and virials are in real space, the integrators' shift is
+ * mod(u + T^-1 dR, 1), and cell grids are on the unit cube.  Entry points
+ * that do not take periodic = 2 (the x64 and adjoint calls)
+ * return JMD_ERR_UNSUPPORTED for it.
  *
  * Each declaration cites the reference interface it replaces
  * (paths relative to the jax-md 0.2.29 source tree).
