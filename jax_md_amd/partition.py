@@ -22,7 +22,7 @@ import dataclasses
 from enum import Enum, IntEnum
 from functools import reduce
 from operator import mul
-from typing import Any, Callable, Dict, Optional
+from typing import NamedTuple, Any, Callable, Dict, Optional
 
 import numpy as onp
 import torch
@@ -565,6 +565,70 @@ def neighbor_list_mask(neighbor: NeighborList, mask_self: bool = False) -> torch
                                              device=neighbor.idx.device).reshape(N, 1)
     mask = mask & self_mask
   return mask
+
+
+class GraphsTuple(NamedTuple):
+  """The fields of jraph.GraphsTuple (jraph is not a dependency here)."""
+  nodes: Any
+  edges: Any
+  receivers: torch.Tensor
+  senders: torch.Tensor
+  globals: Any
+  n_node: torch.Tensor
+  n_edge: torch.Tensor
+
+
+def _tree_map(fn, tree):
+  if tree is None:
+    return None
+  if isinstance(tree, dict):
+    return {k: _tree_map(fn, v) for k, v in tree.items()}
+  if isinstance(tree, (list, tuple)) and not hasattr(tree, '_fields'):
+    return type(tree)(_tree_map(fn, v) for v in tree)
+  return fn(tree)
+
+
+def to_jraph(neighbor: NeighborList, mask: Optional[torch.Tensor] = None, nodes=None,
+             edges=None, globals=None) -> GraphsTuple:
+  """A Sparse-format list as a graph (partition.py:1185-1250): padding is one
+  fictitious graph of one node; with ``mask`` the kept edges move to the
+  front in their order (edge features with them) and the rest become padding
+  edges (receiver and sender N, features zero)."""
+  if not is_sparse(neighbor.format):
+    raise ValueError('Cannot convert a dense neighbor list to jraph format. '
+                     'Please use either NeighborListFormat.Sparse or '
+                     'NeighborListFormat.OrderedSparse.')
+  receivers, senders = neighbor.idx
+  N = len(neighbor.reference_position)
+  _mask = neighbor_list_mask(neighbor)
+
+  def pad(x):
+    return torch.cat((x, torch.zeros((1,) + tuple(x.shape[1:]), dtype=x.dtype,
+                                     device=x.device)), 0)
+
+  nodes = _tree_map(pad, nodes)
+  globals = _tree_map(pad, globals)
+  if mask is not None:
+    _mask = _mask & mask.to(device=_mask.device, dtype=torch.bool)
+    E = receivers.shape[0]
+    index = torch.where(_mask, torch.cumsum(_mask.to(torch.int64), 0) - 1,
+                        torch.full_like(receivers, E, dtype=torch.int64))
+    keep = index < E
+
+    def reorder(x, fill):
+      out = torch.full((E,) + tuple(x.shape[1:]), fill, dtype=x.dtype, device=x.device)
+      out[index[keep]] = x[keep]
+      return out
+
+    receivers = reorder(receivers, N)
+    senders = reorder(senders, N)
+    edges = _tree_map(lambda x: reorder(x, 0), edges)
+  dev = receivers.device
+  n_valid = _mask.sum().to(torch.int32)
+  return GraphsTuple(nodes=nodes, edges=edges, receivers=receivers, senders=senders,
+                     globals=globals,
+                     n_node=torch.tensor([N, 1], dtype=torch.int32, device=dev),
+                     n_edge=torch.stack((n_valid, (~_mask).sum().to(torch.int32))))
 
 
 def to_dense(neighbor: NeighborList) -> torch.Tensor:
