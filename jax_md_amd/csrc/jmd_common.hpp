@@ -34,7 +34,7 @@ struct Space {
   int periodic;
   float L[3];
   float half[3];  // float32(L) * 0.5, exact
-  int general;    // 1: positions are fractional, d = T d_unit
+  int general = 0;  // 1: positions are fractional, d = T d_unit
   float T[3][3];  // box matrix, row-major (general only)
   float Ti[3][3]; // its inverse (float64 inverse rounded once)
 };

@@ -287,6 +287,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   Space sp;
   sp.dim = 2;
   sp.periodic = 1;
+  sp.general = 0;
   for (int d = 0; d < 3; ++d) {
     sp.L[d] = d < 2 ? S.side[d] : 1.0f;
     sp.half[d] = sp.L[d] * 0.5f;
@@ -579,6 +580,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
   Space sp;
   sp.dim = 2;
   sp.periodic = 1;
+  sp.general = 0;
   for (int d = 0; d < 3; ++d) {
     sp.L[d] = d < 2 ? S.side[d] : 1.0f;
     sp.half[d] = sp.L[d] * 0.5f;
