@@ -36,8 +36,31 @@ struct Space {
   float half[3];  // float32(L) * 0.5, exact
   int general = 0;  // 1: positions are fractional, d = T d_unit
   float T[3][3];  // box matrix, row-major (general only)
-  float Ti[3][3]; // its inverse (float64 inverse rounded once)
+  // (kept small: it is passed by value, in registers, to the exact pair
+  // test; the inverse lives with the integrator's parameters)
 };
+
+// T^-1 of a general box (float64 cofactors, rounded once); identity otherwise.
+inline void box_inverse(const Space& s, float Ti[3][3]) {
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) Ti[a][b] = a == b ? 1.0f : 0.0f;
+  if (!s.general) return;
+  double m[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) m[a][b] = (double)s.T[a][b];
+  const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                     m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                     m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  const double inv[3][3] = {
+      {(m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det, (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det,
+       (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det},
+      {(m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det, (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det,
+       (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det},
+      {(m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det, (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det,
+       (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det}};
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) Ti[a][b] = (float)inv[a][b];
+}
 
 // periodic: 0 free, 1 periodic box of sides side[0..dim), 2 periodic_general
 // with fractional coordinates, side = the [dim, dim] box matrix (row-major).
@@ -47,36 +70,15 @@ inline Space make_space(int dim, int periodic, const float* side) {
   s.periodic = periodic != 0;
   s.general = periodic == 2;
   for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) {
-      s.T[a][b] = a == b ? 1.0f : 0.0f;
-      s.Ti[a][b] = a == b ? 1.0f : 0.0f;
-    }
+    for (int b = 0; b < 3; ++b) s.T[a][b] = a == b ? 1.0f : 0.0f;
   for (int d = 0; d < 3; ++d) {
     float L = (!s.general && side && d < dim) ? side[d] : 1.0f;
     s.L[d] = L;
     s.half[d] = L * 0.5f;
   }
-  if (s.general && side) {
-    double m[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  if (s.general && side)
     for (int a = 0; a < dim; ++a)
-      for (int b = 0; b < dim; ++b) {
-        s.T[a][b] = side[a * dim + b];
-        m[a][b] = (double)side[a * dim + b];
-      }
-    // inverse by cofactors (dim <= 3; the 2-D matrix sits in the upper block)
-    const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
-                       m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
-                       m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
-    const double inv[3][3] = {
-        {(m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det, (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det,
-         (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det},
-        {(m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det, (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det,
-         (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det},
-        {(m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det, (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det,
-         (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det}};
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) s.Ti[a][b] = (float)inv[a][b];
-  }
+      for (int b = 0; b < dim; ++b) s.T[a][b] = side[a * dim + b];
   return s;
 }
 
@@ -170,7 +172,42 @@ __device__ __forceinline__ float ref_wrap1(float d, float L, float half) {
   return r - half;
 }
 
-// Reference square distance of disp(a, b) = wrap(a - b) in float32.
+// The orthorhombic part of a Space (28 bytes): small enough to travel in
+// registers to an out-of-line function.
+struct SpaceO {
+  int periodic;
+  float L[3];
+  float half[3];
+};
+
+inline __host__ __device__ SpaceO ortho_of(const Space& s) {
+  SpaceO o;
+  o.periodic = s.periodic;
+  for (int d = 0; d < 3; ++d) {
+    o.L[d] = s.L[d];
+    o.half[d] = s.half[d];
+  }
+  return o;
+}
+
+// Reference square distance of disp(a, b) = wrap(a - b) in float32 (a
+// periodic box or free space).
+template <int DIM>
+__device__ __forceinline__ float ref_metric_sq(const float* a, const float* b,
+                                               const SpaceO& s) {
+#pragma clang fp contract(off)
+  float acc = 0.0f;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    float v = a[d] - b[d];
+    if (s.periodic) v = ref_wrap1(v, s.L[d], s.half[d]);
+    float sq = v * v;
+    acc = (d == 0) ? sq : acc + sq;
+  }
+  return acc;
+}
+
+// The same for any Space, periodic_general included (T wrap(a - b)).
 template <int DIM>
 __device__ __forceinline__ float ref_metric_sq(const float* a, const float* b,
                                                const Space& s) {
@@ -246,6 +283,7 @@ struct KickDrift {
   int* gate_next;       // raised when any particle moved beyond the skin
   Space s;
   const double* p_scale;  // thermostat factor applied to p before the kick, or NULL
+  float Ti[3][3];         // T^-1 of a general box (box_inverse), for the shift
 };
 
 inline void kick_drift_mass(KickDrift* kd) {
@@ -276,7 +314,7 @@ __device__ __forceinline__ void kick_drift_atom(const KickDrift& kd, long long i
     for (int a = 0; a < DIM; ++a) {
       double acc = 0.0;
 #pragma unroll
-      for (int b = 0; b < DIM; ++b) acc += (double)kd.s.Ti[a][b] * dx[b];
+      for (int b = 0; b < DIM; ++b) acc += (double)kd.Ti[a][b] * dx[b];
       w[a] = acc;
     }
 #pragma unroll

@@ -382,6 +382,7 @@ static int kick_drift_impl(const double* position_hi, const float* momentum,
   kd.thr_sq = threshold_sq;
   kd.gate_next = kd.ref ? gate : nullptr;
   kd.s = s;
+  box_inverse(s, kd.Ti);
   kd.p_scale = momentum_scale;
   dim3 grid(ceil_div(nn, 256)), block(256);
 #define JMD_KD(D, S) hipLaunchKernelGGL((k_kick_drift<D, S>), grid, block, 0, st, kd, nn)

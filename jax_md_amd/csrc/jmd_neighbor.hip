@@ -344,9 +344,9 @@ constexpr int kHomeGroup = 512;
 #endif
 constexpr int kHomePass = JMD_NL_HOME_PASS;  // home particles per candidate pass
 
-template <int DIM>
+template <int DIM, class G>
 __device__ __forceinline__ bool pair_test(const float* xi, const float* xj,
-                                          const Space& s, float cutoff_sq,
+                                          const G& s, float cutoff_sq,
                                           float margin, int test) {
   if (test == TEST_IJ) return ref_metric_sq<DIM>(xi, xj, s) < cutoff_sq;
   if (test == TEST_JI) return ref_metric_sq<DIM>(xj, xi, s) < cutoff_sq;
@@ -367,7 +367,18 @@ __device__ __forceinline__ bool pair_test(const float* xi, const float* xj,
 // whole candidate loop.
 template <int DIM, int STRIDE>
 __device__ __noinline__ bool exact_pair_test(const float* __restrict__ pos, int i, int j,
-                                             Space s, float cutoff_sq, int test) {
+                                             SpaceO s, float cutoff_sq, int test) {
+  float xo[3] = {0.0f, 0.0f, 0.0f}, xj[3] = {0.0f, 0.0f, 0.0f};
+  load_pos<DIM, STRIDE>(pos, i, xo);
+  load_pos<DIM, STRIDE>(pos, j, xj);
+  return pair_test<DIM>(xo, xj, s, cutoff_sq, 0.0f, test);
+}
+
+// periodic_general (box matrix): inlined, in the folding build variant only.
+template <int DIM, int STRIDE>
+__device__ __forceinline__ bool exact_pair_test_general(const float* __restrict__ pos, int i,
+                                                        int j, const Space& s, float cutoff_sq,
+                                                        int test) {
   float xo[3] = {0.0f, 0.0f, 0.0f}, xj[3] = {0.0f, 0.0f, 0.0f};
   load_pos<DIM, STRIDE>(pos, i, xo);
   load_pos<DIM, STRIDE>(pos, j, xj);
@@ -416,6 +427,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
 
   const Grid& g = A.g;
   const Space& s = A.s;
+  const SpaceO so = ortho_of(A.s);
   const int n = A.n;
   const long long width = A.width;
   const int tid = threadIdx.x;
@@ -628,7 +640,11 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
                   __builtin_amdgcn_ballot_w64(d2 <= hi_thr) & ~m_fast;
               if (near_any) {
                 if (!fast && d2 <= hi_thr && okj)
-                  pass = exact_pair_test<DIM, STRIDE>(A.pos, i, j, s, A.cutoff_sq, A.test);
+                  pass = (MINIMG && s.general)
+                             ? exact_pair_test_general<DIM, STRIDE>(A.pos, i, j, s, A.cutoff_sq,
+                                                                    A.test)
+                             : exact_pair_test<DIM, STRIDE>(A.pos, i, j, so, A.cutoff_sq,
+                                                            A.test);
                 bal = __builtin_amdgcn_ballot_w64(pass);
               }
               if ((MODE == MODE_DENSE || MODE == MODE_SPARSE_WRITE) && pass) {
@@ -1484,6 +1500,7 @@ int jmd_nve_list_update_kick_drift(
     if (grc) return grc;
   }
   kd.s = make_space(dim, shift_periodic, shift_side);
+  box_inverse(kd.s, kd.Ti);
   Work w;
   carve(workspace, n, g.ncell, cell_capacity, &w);
   hipStream_t st = (hipStream_t)stream;
