@@ -358,6 +358,98 @@ def published_section(jmd, O, n_steps=2000, warmup=200, block=100):
   return out
 
 
+def general_section(jmd, O, n=64, n_cells=40, n_steps=1000, warmup=200, block=100):
+  """periodic_general measurement: LJ NVE (r_c 2.5, skin 0.5, dt 0.005,
+  Dense lists, kT 1) on an FCC crystal at rho 0.8442 stored as fractional
+  coordinates in a triclinic cell (an n^3 supercell of the primitive cell,
+  box matrix upper triangular: 262,144 atoms at n = 64), next to the same
+  crystal in a cubic cell with space.periodic (4 n_cells^3 = 256,000 atoms).
+  Also times the virial kernel behind quantity.pressure on the triclinic
+  system."""
+  import numpy as np
+  import torch
+  dev = 'cuda:0'
+  a = float((4.0 / 0.8442) ** (1.0 / 3.0))
+  out = {}
+
+  def run(name, R, shift, nfn, efn, P):
+    Rt = torch.as_tensor(R, device=dev)
+    init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
+    nb = nfn.allocate(Rt, extra_capacity=8)
+    st = init_fn(0, Rt, 0.0, momenta=torch.as_tensor(P, device=dev), neighbor=nb)
+    loop = jmd.simulate.NVELoop(efn, shift, 0.005)
+    reallocs = 0
+
+    def go(k_total):
+      nonlocal nb, reallocs
+      done = 0
+      loop.prime(st, nb)
+      while done < k_total:
+        snap = (st.position_hi.clone(), st.position.clone(), st.momentum.clone(),
+                st.force.clone())
+        loop.run(st, nb, block)
+        if bool(nb.did_buffer_overflow):
+          for t, v in zip((st.position_hi, st.position, st.momentum, st.force), snap):
+            t.copy_(v)
+          nb = nfn.allocate(st.position, extra_capacity=8)
+          loop.prime(st, nb)
+          reallocs += 1
+          continue
+        done += block
+
+    go(warmup)
+    E0, _ = efn.energy_and_force(st.position, nb)
+    K0 = float(jmd.quantity.kinetic_energy(momentum=st.momentum))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    go(n_steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    E1, _ = efn.energy_and_force(st.position, nb)
+    K1 = float(jmd.quantity.kinetic_energy(momentum=st.momentum))
+    N = R.shape[0]
+    res = {'atoms': N, 'ms_per_step': 1e3 * el / n_steps, 'atom_steps_per_s': N * n_steps / el,
+           'total_energy_drift_rel': abs((float(E1) + K1) - (float(E0) + K0)) /
+                                     abs(float(E0) + K0),
+           'reallocs': reallocs}
+    log(f'general section, {name}: {N} atoms, {1e3 * el / n_steps:.3f} ms/step, '
+        f'{N * n_steps / el:.3e} atom-steps/s')
+    return res, st, nb
+
+  u, T, P = O.fcc_triclinic(n, a=a, jitter=0.03, kT=1.0, seed=21)
+  disp, shift = jmd.space.periodic_general(T)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, T, fractional_coordinates=True,
+                                                    format=jmd.partition.Dense)
+  out['triclinic'], st, nb = run('triclinic', u, shift, nfn, efn, P)
+  # the virial kernel (quantity.pressure / stress) on the triclinic state
+  for _ in range(3):
+    efn.virial(st.position, nb)
+  torch.cuda.synchronize()
+  e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(20):
+    W = efn.virial(st.position, nb)
+  e1.record()
+  torch.cuda.synchronize()
+  out['virial_us_per_call'] = 1e3 * e0.elapsed_time(e1) / 20
+  Rc, Lc = O.fcc_cubic(n_cells, a=a)
+  rng = np.random.default_rng(22)
+  Rc = np.mod(Rc + rng.normal(0, 0.03, Rc.shape), Lc).astype(np.float32)
+  Rc = np.where(Rc >= Lc, np.float32(0), Rc).astype(np.float32)
+  Pc = rng.normal(0.0, 1.0, Rc.shape)
+  Pc = (Pc - Pc.mean(0)).astype(np.float32)
+  dc, sc = jmd.space.periodic(Lc)
+  nfc, efc = jmd.energy.lennard_jones_neighbor_list(dc, Lc, format=jmd.partition.Dense)
+  out['cubic'], _, _ = run('cubic', Rc, sc, nfc, efc, Pc)
+  out['workload'] = (f'LJ NVE, FCC at rho 0.8442 (a = {a:.4f}), jitter 0.03, kT 1, r_c 2.5, '
+                     f'skin 0.5, dt 0.005, Dense lists, NVELoop; triclinic: {n}^3 primitive '
+                     f'cells as fractional coordinates in an upper-triangular box '
+                     f'(space.periodic_general); cubic: {n_cells}^3 conventional cells '
+                     f'(space.periodic); {n_steps} steps after {warmup}')
+  log(f'general section: virial {out["virial_us_per_call"]:.1f} us per call')
+  return out
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument('--gpus', type=int, default=1)
@@ -374,7 +466,8 @@ def main():
   ap.add_argument('--copy-calibration', action='store_true',
                   help='only run a plain 256 MiB device copy (the reference kernel that '
                        'scales the FETCH_SIZE / WRITE_SIZE counters) and exit')
-  ap.add_argument('--section', choices=('batched', 'adjoint', 'ensemble_gradient', 'published'),
+  ap.add_argument('--section', choices=('batched', 'adjoint', 'ensemble_gradient', 'published',
+                                        'general'),
                   help='only run this one extra measurement and print its JSON')
   args = ap.parse_args()
   if args.gpus != 1:
@@ -405,7 +498,8 @@ def main():
     fn = {'batched': lambda: batched_section(jmd, O),
           'adjoint': lambda: adjoint_section(jmd, O, check=not args.no_cpu),
           'ensemble_gradient': lambda: ensemble_gradient_section(jmd, O),
-          'published': lambda: published_section(jmd, O)}[args.section]
+          'published': lambda: published_section(jmd, O),
+          'general': lambda: general_section(jmd, O)}[args.section]
     print(json.dumps({args.section: fn()}), flush=True)
     return
   if args.tpa:
@@ -550,7 +644,8 @@ def main():
     for key, fn in (('batched', lambda: batched_section(jmd, O)),
                     ('adjoint', lambda: adjoint_section(jmd, O, check=not args.no_cpu)),
                     ('ensemble_gradient', lambda: ensemble_gradient_section(jmd, O)),
-                    ('published', lambda: published_section(jmd, O))):
+                    ('published', lambda: published_section(jmd, O)),
+                    ('general', lambda: general_section(jmd, O))):
       try:
         out[key] = fn()
       except Exception as e:  # reported, never silently dropped
