@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <math.h>
 
 #include "../../include/jaxmd_native.h"
 
@@ -35,10 +36,14 @@ struct Space {
   float L[3];
   float half[3];  // float32(L) * 0.5, exact
   int general = 0;  // 1: positions are fractional, d = T d_unit
+  int upper = 1;    // T is upper triangular (the zero products are skipped)
   float T[3][3];  // box matrix, row-major (general only)
+  float fw[3];    // |row d of T^-1|: fractional reach per unit of real distance
+  float smin2;    // smallest squared singular value of T: |T v|^2 >= smin2 |v|^2
   // (kept small: it is passed by value, in registers, to the exact pair
   // test; the inverse lives with the integrator's parameters)
 };
+
 
 // T^-1 of a general box (float64 cofactors, rounded once); identity otherwise.
 inline void box_inverse(const Space& s, float Ti[3][3]) {
@@ -79,6 +84,39 @@ inline Space make_space(int dim, int periodic, const float* side) {
   if (s.general && side)
     for (int a = 0; a < dim; ++a)
       for (int b = 0; b < dim; ++b) s.T[a][b] = side[a * dim + b];
+  s.upper = s.T[1][0] == 0.0f && s.T[2][0] == 0.0f && s.T[2][1] == 0.0f;
+  float Ti[3][3];
+  box_inverse(s, Ti);
+  for (int a = 0; a < 3; ++a)
+    s.fw[a] = sqrtf(Ti[a][0] * Ti[a][0] + Ti[a][1] * Ti[a][1] + Ti[a][2] * Ti[a][2]);
+  // smallest eigenvalue of T^T T over the dim x dim block (cyclic Jacobi),
+  // lowered by 1e-5 relative so that the pruning bound stays conservative
+  {
+    double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < dim; ++a)
+      for (int b = 0; b < dim; ++b)
+        for (int k = 0; k < dim; ++k) M[a][b] += (double)s.T[k][a] * (double)s.T[k][b];
+    for (int sweep = 0; sweep < 30; ++sweep)
+      for (int p = 0; p < dim; ++p)
+        for (int q = p + 1; q < dim; ++q) {
+          if (M[p][q] == 0.0) continue;
+          const double th = 0.5 * atan2(2.0 * M[p][q], M[q][q] - M[p][p]);
+          const double c = cos(th), sn = sin(th);
+          for (int k = 0; k < dim; ++k) {  // columns p, q
+            const double mp = M[k][p], mq = M[k][q];
+            M[k][p] = c * mp - sn * mq;
+            M[k][q] = sn * mp + c * mq;
+          }
+          for (int k = 0; k < dim; ++k) {  // rows p, q
+            const double mp = M[p][k], mq = M[q][k];
+            M[p][k] = c * mp - sn * mq;
+            M[q][k] = sn * mp + c * mq;
+          }
+        }
+    double lo = M[0][0];
+    for (int a = 1; a < dim; ++a) lo = lo < M[a][a] ? lo : M[a][a];
+    s.smin2 = s.general ? (float)(lo * (1.0 - 1e-5)) : 1.0f;
+  }
   return s;
 }
 
@@ -115,12 +153,22 @@ inline int space_check(int dim, int periodic, const float* side, bool allow_gene
 template <int DIM>
 __device__ __forceinline__ void box_apply(const Space& s, float* d) {
   float r[3];
+  if (s.upper) {  // row a has nonzero entries from column a on
 #pragma unroll
-  for (int a = 0; a < DIM; ++a) {
-    float acc = s.T[a][0] * d[0];
+    for (int a = 0; a < DIM; ++a) {
+      float acc = s.T[a][a] * d[a];
 #pragma unroll
-    for (int b = 1; b < DIM; ++b) acc = fmaf(s.T[a][b], d[b], acc);
-    r[a] = acc;
+      for (int b = a + 1; b < DIM; ++b) acc = fmaf(s.T[a][b], d[b], acc);
+      r[a] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      float acc = s.T[a][0] * d[0];
+#pragma unroll
+      for (int b = 1; b < DIM; ++b) acc = fmaf(s.T[a][b], d[b], acc);
+      r[a] = acc;
+    }
   }
 #pragma unroll
   for (int a = 0; a < DIM; ++a) d[a] = r[a];

@@ -413,10 +413,16 @@ struct BuildArgs {
 // difference between this value and the reference's wrapped float32 d^2;
 // only candidates within the margin run the reference-exact test on the
 // original positions (pair_test), so sets and order equal the reference's.
-// MINIMG: some dimension has fewer than 5 cells, where the staged image is
+// GEOM 0: staged images; 1 (MINIMG): some dimension has fewer than 5 cells,
+// where the staged image is
 // not always the nearest one and a minimum-image fold is applied.
-template <int DIM, int STRIDE, int MODE, bool MINIMG>
+template <int DIM, int STRIDE, int MODE, int GEOM>
 __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
+  // GEOM 2: periodic_general with 5 or more cells a side: staged images in
+  // fractional units, T applied to each difference, pruning by T's smallest
+  // singular value
+  constexpr bool MINIMG = GEOM == 1;
+  constexpr bool BOXS = GEOM == 2;
   constexpr int NST = DIM == 3 ? 27 : 9;
   __shared__ int s_cs[NST], s_cc[NST], s_off[NST + 1], s_rot[NST];
   __shared__ float s_shift[NST][3];
@@ -493,7 +499,9 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
     // cell index (x in (-cs, 0) belongs to cell 0) and its rounding: a
     // candidate farther than sqrt(c^2 + margin) from it reaches no home
     // particle by either test and is not staged.
-    const bool prune = !MINIMG && s.periodic && margin < 3.0e38f;  // never for general
+    // (folding variant: never; general boxes: through T's smallest singular
+    // value, |T v| >= sqrt(smin2) |v|)
+    const bool prune = !MINIMG && s.periodic && margin < 3.0e38f;
     float blo[3] = {0.0f, 0.0f, 0.0f}, bhi[3] = {0.0f, 0.0f, 0.0f};
     {
       int h = home;
@@ -549,7 +557,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
                 const float e = fmaxf(0.0f, fmaxf(blo[d] - cv[d], cv[d] - bhi[d]));
                 D2 = fmaf(e, e, D2);
               }
-              keep = D2 <= reach_sq;
+              keep = (BOXS ? D2 * s.smin2 : D2) <= reach_sq;
             }
           }
           const unsigned long long bal = __ballot(keep);
@@ -621,8 +629,9 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
                 d[e] = MINIMG ? -min_image_disp(cj[e], xi[h][e], xi[h][e] - s.L[e], s.L[e],
                                                 s.half[e])
                               : xi[h][e] - cj[e];
-              // periodic_general: the fold is in fractional units, then T
-              if (MINIMG && s.general) box_apply<DIM>(s, d);
+              // periodic_general: the fold (or staged image) is in fractional
+              // units, then T
+              if (BOXS || (MINIMG && s.general)) box_apply<DIM>(s, d);
               const float d2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
                                         : fmaf(d[1], d[1], d[0] * d[0]);
               // Dense rows never drop j > i: only the Sparse modes read lim
@@ -640,7 +649,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
                   __builtin_amdgcn_ballot_w64(d2 <= hi_thr) & ~m_fast;
               if (near_any) {
                 if (!fast && d2 <= hi_thr && okj)
-                  pass = (MINIMG && s.general)
+                  pass = (BOXS || (MINIMG && s.general))
                              ? exact_pair_test_general<DIM, STRIDE>(A.pos, i, j, s, A.cutoff_sq,
                                                                     A.test)
                              : exact_pair_test<DIM, STRIDE>(A.pos, i, j, so, A.cutoff_sq,
@@ -877,10 +886,10 @@ __device__ __forceinline__ void dev_finish(int* __restrict__ misc, int occ,
 
 // The list build; with `finish`, its last workgroup completes the build
 // (error bits, occupancy, gate reset).
-template <int DIM, int STRIDE, int MODE, bool MINIMG>
+template <int DIM, int STRIDE, int MODE, int GEOM>
 __global__ __launch_bounds__(256) void k_nl_cells(BuildArgs A, int finish) {
   if (gate_closed(A.gate)) return;
-  const int dep = dev_nl_cells<DIM, STRIDE, MODE, MINIMG>(A, blockIdx.x, gridDim.x);
+  const int dep = dev_nl_cells<DIM, STRIDE, MODE, GEOM>(A, blockIdx.x, gridDim.x);
   if (!finish) return;
   if (!last_workgroup(&A.w.misc[M_TICKET_NL], dep)) return;
   if (threadIdx.x == 0) {
@@ -1104,24 +1113,24 @@ static int bin_and_order(const float* pos, int n, const Grid& g, const Space& s,
   return JMD_OK;
 }
 
-template <int DIM, int STRIDE, bool MINIMG>
+template <int DIM, int STRIDE, int GEOM>
 static void launch_cells(const BuildArgs& A, int mode, int finish, hipStream_t st) {
   dim3 grid(A.g.ncell < kCellBlocks ? A.g.ncell : kCellBlocks), block(256);
   switch (mode) {
     case MODE_DENSE:
-      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_DENSE, MINIMG>), grid, block, 0, st, A,
+      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_DENSE, GEOM>), grid, block, 0, st, A,
                          finish);
       break;
     case MODE_SPARSE_COUNT:
-      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_SPARSE_COUNT, MINIMG>), grid, block, 0,
+      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_SPARSE_COUNT, GEOM>), grid, block, 0,
                          st, A, finish);
       break;
     case MODE_SPARSE_WRITE:
-      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_SPARSE_WRITE, MINIMG>), grid, block, 0,
+      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_SPARSE_WRITE, GEOM>), grid, block, 0,
                          st, A, finish);
       break;
     default:
-      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_COUNT_MAX, MINIMG>), grid, block, 0, st,
+      hipLaunchKernelGGL((k_nl_cells<DIM, STRIDE, MODE_COUNT_MAX, GEOM>), grid, block, 0, st,
                          A, finish);
       break;
   }
@@ -1131,12 +1140,13 @@ template <int DIM, int STRIDE>
 static int cells_dispatch(const BuildArgs& A, int mode, int finish, hipStream_t st) {
   // The staged image of a stencil cell is the nearest one when every
   // periodic dimension has at least 5 cells (|d| < 3 cell sizes < L - c).
-  // periodic_general always folds (in fractional units) and applies T.
-  bool minimg = A.s.general != 0;
+  // periodic_general (fractional units) applies T to each difference.
+  bool minimg = false;
   if (A.s.periodic)
     for (int d = 0; d < DIM; ++d) minimg = minimg || A.g.cps[d] < 5;
-  if (minimg) launch_cells<DIM, STRIDE, true>(A, mode, finish, st);
-  else launch_cells<DIM, STRIDE, false>(A, mode, finish, st);
+  if (minimg) launch_cells<DIM, STRIDE, 1>(A, mode, finish, st);
+  else if (A.s.general) launch_cells<DIM, STRIDE, 2>(A, mode, finish, st);
+  else launch_cells<DIM, STRIDE, 0>(A, mode, finish, st);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }

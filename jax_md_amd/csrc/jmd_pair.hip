@@ -152,16 +152,29 @@ __global__ __launch_bounds__(256) void k_pair_dense(
       interior = interior && xi[c] >= a.rc && xi[c] <= s.L[c] - a.rc;
     }
   }
-  // (fractional coordinates have no real-space faces to test against)
-  const bool wave_interior = !s.general && __all(interior || !active);
+  if (s.general && active) {
+    // fractional coordinates: a neighbour within rc of particle i differs by
+    // at most rc |row c of T^-1| in u_c, so no fold is needed when u_c stays
+    // that far from both faces
+    interior = true;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      const float w = a.rc * s.fw[c];
+      interior = interior && xi[c] >= w && xi[c] <= 1.0f - w;
+    }
+  }
+  const bool wave_interior = __all(interior || !active);
   if (active && s.general) {
     const int* row = idx + (long long)i * width;
-    if (KIND == JMD_LENNARD_JONES && a.smooth)
-      row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false, 1, true>(pos, n, s, a, row, width,
-                                                                       sub, i, xi, xim, f, &e);
-    else
-      row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false, 0, true>(pos, n, s, a, row, width,
-                                                                       sub, i, xi, xim, f, &e);
+#define JMD_ROWG(INT, SM)                                                                      \
+  row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, INT, SM, true>(pos, n, s, a, row, width, sub, \
+                                                                  i, xi, xim, f, &e)
+    if (KIND == JMD_LENNARD_JONES && a.smooth) {
+      if (wave_interior) JMD_ROWG(true, 1); else JMD_ROWG(false, 1);
+    } else {
+      if (wave_interior) JMD_ROWG(true, 0); else JMD_ROWG(false, 0);
+    }
+#undef JMD_ROWG
   } else if (active) {
     const int* row = idx + (long long)i * width;
     // the smoothing switch is uniform: taken once here, not per pair
