@@ -522,6 +522,37 @@ int jmd_fire_update(float* momentum, const float* force, int64_t n,
 /* Scratch bytes for jmd_fire_update. */
 size_t jmd_fire_workspace_bytes(int64_t n);
 
+/* Multi-image neighbour list of a small periodic cell
+ * (custom_partition.neighbor_list_multi_image's build, custom_partition.py:
+ * 212-626): every pair (i, j) against every lattice shift s in
+ * [-shift_max[d], shift_max[d]] (meshgrid order, first dimension slowest),
+ * d = (x_j + s T^T) - x_i in real space with x = T u when fractional (box =
+ * the [dim, dim] matrix T, host, row-major), kept when d^2 < cutoff_sq and
+ * not (i = j, s = 0).  format JMD_SPARSE / JMD_ORDERED_SPARSE: idx [2,
+ * capacity] (receiver i, sender j) in (s, i, j) order, edge_shift [capacity,
+ * dim] int32; OrderedSparse keeps i < j at s = 0 and canonical shifts (first
+ * nonzero component positive) otherwise.  JMD_DENSE: idx [n, capacity], per
+ * atom the capacity nearest kept (s, j) by d^2 (ties by s n + j),
+ * edge_shift [n, capacity, dim].  Padding: index n, shift 0.  occupancy
+ * (device uint64 [1]) receives the total kept (Sparse) or the largest per-atom
+ * count (Dense); overflow (device uint8 [1]) = occupancy > capacity.  With
+ * capacity 0 or idx NULL only occupancy is produced (the allocation probe).
+ * workspace: jmd_multi_image_workspace_bytes(n, stencil size). */
+size_t jmd_multi_image_workspace_bytes(int64_t n, int32_t n_shift);
+
+/* max_i |x_i - x_ref,i|^2 in real space, unwrapped (x = T u, x_ref = T_ref
+ * u_ref when fractional), into out (device float [1]): the displacement of
+ * neighbor_list_multi_image's rebuild test (custom_partition.py:961-985). */
+int jmd_multi_image_max_displacement_sq(const float* position, const float* reference,
+                                        int64_t n, int32_t dim, int32_t fractional,
+                                        const float* box, const float* reference_box,
+                                        float* out, void* stream);
+int jmd_multi_image_build(const float* position, int64_t n, int32_t dim, int32_t fractional,
+                          const float* box, const int32_t* shift_max, float cutoff_sq,
+                          int32_t format, int32_t* idx, int32_t* edge_shift, int64_t capacity,
+                          unsigned long long* occupancy, uint8_t* overflow, void* workspace,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,6 @@ per-particle computation runs in libjaxmd_native.so (HIP, gfx950).
 """
 
 from . import util, space, partition, smap, energy, quantity, simulate, minimize  # noqa: F401
-from . import adjoint, ensemble  # noqa: F401
+from . import adjoint, ensemble, custom_partition  # noqa: F401
 
 __version__ = '0.1.0'

@@ -107,6 +107,13 @@ _SIGNATURES = {
                                c_void_p]),
     'jmd_scale_momentum': (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
     'jmd_pair_virial_workspace_bytes': (c_size_t, []),
+    'jmd_multi_image_workspace_bytes': (c_size_t, [c_int64, c_int32]),
+    'jmd_multi_image_max_displacement_sq': (c_int32, [c_void_p, c_void_p, c_int64, c_int32,
+                                                      c_int32, POINTER(c_float),
+                                                      POINTER(c_float), c_void_p, c_void_p]),
+    'jmd_multi_image_build': (c_int32, [c_void_p, c_int64, c_int32, c_int32, POINTER(c_float),
+                                        POINTER(c_int32), c_float, c_int32, c_void_p, c_void_p,
+                                        c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     'jmd_pair_virial': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
                                   POINTER(c_float), POINTER(JmdPairPotential), c_void_p, c_int64,
                                   c_int32, c_void_p, c_void_p, c_void_p]),

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libjaxmd_native.so')
 
 SOURCES = ['jmd_neighbor.hip', 'jmd_pair.hip', 'jmd_adjoint.hip', 'jmd_ensemble.hip',
-           'jmd_integrate.hip', 'jmd_layout.hip', 'jmd_x64.hip']
+           'jmd_integrate.hip', 'jmd_layout.hip', 'jmd_x64.hip', 'jmd_multi_image.hip']
 
 FLAGS = [
     '--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',

@@ -1248,6 +1248,88 @@ def fcc_triclinic(n, a=1.5496, jitter=0.0, kT=0.0, seed=0):
   return u, T, P
 
 
+# ---------------------------------------------------------------------------
+# Multi-image lists (jax_md/custom_partition.py:147-626)
+# ---------------------------------------------------------------------------
+
+
+def multi_image_shift_n_max(box, cutoff, pbc=None):
+  """ceil(c / h_d), h_d = 1 / |column d of inv(T)^T| (custom_partition.py:
+  147-209); zero for non-periodic directions."""
+  b = np.asarray(box, F64)
+  inv_t = np.linalg.inv(b).T
+  heights = (1.0 / np.linalg.norm(inv_t, axis=0)).astype(F32)
+  n = np.ceil(F32(cutoff) / heights).astype(np.int32)
+  pbc = np.ones(b.shape[0], bool) if pbc is None else np.asarray(pbc, bool)
+  return np.where(pbc, n, 0).astype(np.int32)
+
+
+def multi_image_shifts(n_max):
+  grids = np.meshgrid(*[np.arange(-n, n + 1) for n in n_max], indexing='ij')
+  return np.stack([g.ravel() for g in grids], axis=-1).astype(np.int32)
+
+
+def multi_image_candidates(position, box, cutoff_sq, shifts, fractional=True):
+  """Kept (s, i, j, d^2) over every shift s and pair (i, j): d = (x_j + s T^T)
+  - x_i in float32 (x = T u when fractional; products rounded, sums left to
+  right), d^2 < cutoff_sq, minus i = j at s = 0."""
+  T = np.asarray(box, F32)
+  u = np.asarray(position, F32)
+  x = transform(T, u) if fractional else u
+  sr = transform(T, shifts.astype(F32))
+  N = len(x)
+  zero = int(np.argmin(np.sum(shifts.astype(np.int64) ** 2, axis=1)))
+  out = []
+  for k in range(len(shifts)):
+    d = (x[None, :, :] + sr[k][None, None, :]) - x[:, None, :]   # [i, j, dim]
+    d2 = square_distance(d)
+    keep = d2 < F32(cutoff_sq)
+    if k == zero:
+      keep &= ~np.eye(N, dtype=bool)
+    ii, jj = np.nonzero(keep)
+    out.append((np.full(len(ii), k), ii, jj, d2[ii, jj]))
+  ks, ii, jj, dd = (np.concatenate(v) for v in zip(*out))
+  return ks, ii, jj, dd, zero
+
+
+def multi_image_list(position, box, cutoff_sq, n_max, fmt, capacity, fractional=True):
+  """The reference's multi-image build (custom_partition.py:376-626): Sparse
+  entries in (s, i, j) order (receiver i, sender j, integer shift);
+  OrderedSparse keeps i < j at s = 0 and canonical shifts (first nonzero
+  component positive); Dense rows the capacity nearest (s, j) per atom,
+  ordered by (d^2, s N + j).  Returns (idx, shifts, occupancy)."""
+  shifts = multi_image_shifts(n_max)
+  ks, ii, jj, dd, zero = multi_image_candidates(position, box, cutoff_sq, shifts, fractional)
+  N = len(position)
+  dim = shifts.shape[1]
+  if fmt == DENSE:
+    idx = np.full((N, capacity), N, np.int32)
+    sh = np.zeros((N, capacity, dim), np.int32)
+    occ = 0
+    for i in range(N):
+      m = ii == i
+      flat = ks[m].astype(np.int64) * N + jj[m]
+      order = np.lexsort((flat, dd[m]))
+      occ = max(occ, int(m.sum()))
+      for r, o in enumerate(order[:capacity]):
+        idx[i, r] = jj[m][o]
+        sh[i, r] = shifts[ks[m][o]]
+    return idx, sh, occ
+  if fmt == ORDERED_SPARSE:
+    canon = np.array([next((v > 0 for v in s if v != 0), True) for s in shifts])
+    keep = np.where(ks == zero, ii < jj, canon[ks])
+    ks, ii, jj = ks[keep], ii[keep], jj[keep]
+  order = np.lexsort((jj, ii, ks))            # (s, i, j), shift slowest
+  ks, ii, jj = ks[order], ii[order], jj[order]
+  occ = len(ks)
+  idx = np.full((2, capacity), N, np.int32)
+  sh = np.zeros((capacity, dim), np.int32)
+  n = min(occ, capacity)
+  idx[0, :n], idx[1, :n] = ii[:n], jj[:n]
+  sh[:n] = shifts[ks[:n]]
+  return idx, sh, occ
+
+
 def bidisperse_2d(N, phi=0.84, d_small=1.0, d_large=1.4, seed=0):
   """50:50 bidisperse soft-sphere disks at packing fraction phi in a periodic
   square; uniform random positions.  Returns (R f32, species int32, L f32)."""

@@ -45,6 +45,11 @@ Which check pins which oracle function:
                                    (V / |a_j x a_k|) for triclinic boxes,
                                    scalar and vector closed forms
   neighbor_list_build (fractional) brute-force sets in the triclinic box
+  multi_image_list                 a stencil two shells wider finds no more
+                                   pairs; in a cell larger than twice the
+                                   cutoff the pairs are the general-space
+                                   neighbour sets; OrderedSparse holds one of
+                                   each (i, j, s) / (j, i, -s)
 """
 
 import math
@@ -716,3 +721,57 @@ def test_general_neighbor_list_matches_brute_force():
     d2 = g.metric_sq(u, u[i][None, :])
     ref = np.nonzero((d2 < spec.cutoff_sq) & (np.arange(N) != i))[0]
     assert np.array_equal(rows[i], ref)
+
+
+# ---------------------------------------------------------------------------
+# multi-image lists
+# ---------------------------------------------------------------------------
+
+
+def _small_cell(seed=0, n=40):
+  rng = np.random.default_rng(seed)
+  T = np.array([[3.1, 0.7, -0.4], [0.0, 2.8, 0.5], [0.0, 0.0, 3.3]], F32)
+  return rng.uniform(0, 1, (n, 3)).astype(F32), T
+
+
+def test_multi_image_stencil_is_complete():
+  u, T = _small_cell(1)
+  c = 2.9
+  n_max = O.multi_image_shift_n_max(T, c)
+  ks, ii, jj, _, _ = O.multi_image_candidates(u, T, F32(c * c), O.multi_image_shifts(n_max))
+  wide = O.multi_image_shifts(n_max + 2)
+  kw, iw, jw, _, _ = O.multi_image_candidates(u, T, F32(c * c), wide)
+  sh = O.multi_image_shifts(n_max)
+  a = {(int(i), int(j), tuple(sh[k])) for k, i, j in zip(ks, ii, jj)}
+  b = {(int(i), int(j), tuple(wide[k])) for k, i, j in zip(kw, iw, jw)}
+  assert a == b and len(a) > 0
+
+
+def test_multi_image_large_cell_equals_neighbor_sets():
+  u, T, _ = O.fcc_triclinic(6, jitter=0.05, seed=2)   # widths 5.4 > 2 x 2.5
+  c = 2.5
+  n_max = O.multi_image_shift_n_max(T, c)
+  idx, sh, occ = O.multi_image_list(u, T, F32(c * c), n_max, O.SPARSE, 10 ** 6)
+  N = len(u)
+  got = {(int(i), int(j)) for i, j in zip(idx[0][:occ], idx[1][:occ])}
+  assert len(got) == occ     # one image per pair
+  g = O.GeneralSpace(T)
+  ref = set()
+  for i in range(N):
+    d2 = g.metric_sq(u.astype(np.float64), u[i].astype(np.float64)[None, :])
+    for j in np.nonzero((d2 < c * c) & (np.arange(N) != i))[0]:
+      ref.add((i, int(j)))
+  assert got == ref
+
+
+def test_multi_image_ordered_holds_each_pair_once():
+  u, T = _small_cell(3)
+  c = 2.9
+  n_max = O.multi_image_shift_n_max(T, c)
+  a, sa, na = O.multi_image_list(u, T, F32(c * c), n_max, O.SPARSE, 10 ** 6)
+  b, sb, nb = O.multi_image_list(u, T, F32(c * c), n_max, O.ORDERED_SPARSE, 10 ** 6)
+  assert na == 2 * nb
+  full = {(int(i), int(j), tuple(s)) for i, j, s in zip(a[0][:na], a[1][:na], sa[:na])}
+  for i, j, s in zip(b[0][:nb], b[1][:nb], sb[:nb]):
+    assert (int(i), int(j), tuple(s)) in full
+    assert (int(j), int(i), tuple(-s)) in full
