@@ -39,6 +39,7 @@ class NeighborListMultiImage:
   max_occupancy: int
   update_fn: Callable[..., 'NeighborListMultiImage']
   did_buffer_overflow: torch.Tensor
+  shift_max: Optional[onp.ndarray] = None  # host copy of the stencil's extent
 
   def update(self, position, **kwargs) -> 'NeighborListMultiImage':
     return self.update_fn(position, self, **kwargs)
@@ -151,13 +152,22 @@ def neighbor_list_multi_image(displacement_or_metric, box, r_cutoff, dr_threshol
   cutoff_sq = float(f32(search_cutoff ** 2))
   use_threshold = float(dr_threshold) > 0
 
+  cache = {}
+
+  def _workspace(lib, N, ns, dev):
+    key = (N, ns, str(dev))
+    if key not in cache:
+      cache.clear()
+      cache[key] = torch.zeros((int(lib.jmd_multi_image_workspace_bytes(N, ns)),),
+                               dtype=torch.uint8, device=dev)
+    return cache[key]
+
   def _build(position, cur_box, n_max, capacity, idx, shifts, overflow):
     lib = nat.load()
     N = position.shape[0]
     dev = position.device
     ns = int(onp.prod(2 * n_max.astype(onp.int64) + 1))
-    ws = torch.zeros((int(lib.jmd_multi_image_workspace_bytes(N, ns)),), dtype=torch.uint8,
-                     device=dev)
+    ws = _workspace(lib, N, ns, dev)
     occ = torch.zeros((1,), dtype=torch.int64, device=dev)
     nat.check(lib.jmd_multi_image_build(
         nat.ptr(position), N, dim, int(bool(fractional_coordinates)),
@@ -183,7 +193,8 @@ def neighbor_list_multi_image(displacement_or_metric, box, r_cutoff, dr_threshol
         idx=idx if use_dense else (idx[0], idx[1]), shifts=shifts,
         reference_position=position.clone(), reference_box=cur_box.copy(), box=cur_box.copy(),
         search_shifts=torch.as_tensor(_search_shifts(n_max), device=dev), format=fmt,
-        max_occupancy=capacity, update_fn=update_fn, did_buffer_overflow=overflow[0] != 0)
+        max_occupancy=capacity, update_fn=update_fn, did_buffer_overflow=overflow[0] != 0,
+        shift_max=n_max.copy())
 
   def allocate_fn(position, extra_capacity: int = 0, box=None, **kw) -> NeighborListMultiImage:
     nat.require_device_tensor(position, 'position', torch.float32)
@@ -201,7 +212,8 @@ def neighbor_list_multi_image(displacement_or_metric, box, r_cutoff, dr_threshol
     position = position.contiguous()
     cur_box = _matrix(kw['box']) if 'box' in kw else neighbors.box
     need = _shift_n_max(cur_box, search_cutoff, pbc)
-    have = neighbors.search_shifts.abs().amax(0).cpu().numpy().astype(onp.int32)
+    have = (neighbors.shift_max if neighbors.shift_max is not None else
+            neighbors.search_shifts.abs().amax(0).cpu().numpy().astype(onp.int32))
     range_overflow = bool(onp.any(need > have))
     if use_threshold and not range_overflow:
       lib = nat.load()
@@ -212,7 +224,7 @@ def neighbor_list_multi_image(displacement_or_metric, box, r_cutoff, dr_threshol
           nat.f3(onp.asarray(neighbors.reference_box, f32).reshape(-1)), nat.ptr(out),
           nat.stream_handle(position.device)), 'jmd_multi_image_max_displacement_sq')
       max_atom = float(onp.sqrt(f32(out.item())))
-      sh = neighbors.search_shifts.cpu().numpy().astype(f32)
+      sh = _search_shifts(have).astype(f32)
       delta = sh @ (cur_box - onp.asarray(neighbors.reference_box, f32)).T
       max_shift = float(onp.sqrt(onp.max(onp.sum(delta * delta, axis=-1))))
       if not (2.0 * max_atom + max_shift >= float(dr_threshold)):
