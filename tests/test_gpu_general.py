@@ -203,3 +203,65 @@ def test_general_pressure_vs_oracle():
   P64 = O.pressure(W, abs(np.linalg.det(T.astype(np.float64))))
   print(f'  pressure {Pg:.8e} vs {P64:.8e}')
   assert abs(Pg - P64) <= 1e-5 * abs(P64)
+
+
+def test_general_fire_vs_oracle():
+  """fire_descent in a triclinic cell: the float16 dt / alpha / n_pos sequence
+  equals the oracle's for 40 steps; the energy after 100 steps within 1e-5."""
+  print('case general FIRE', flush=True)
+  jmd = _jmd()
+  u, T, _ = O.fcc_triclinic(8, jitter=0.06, seed=11)
+  disp, shift = jmd.space.periodic_general(T)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, T, fractional_coordinates=True,
+                                                    format=jmd.partition.Dense)
+  ut = torch.as_tensor(u, device=DEV)
+  nb = nfn.allocate(ut)
+  init_fn, apply_fn = jmd.minimize.fire_descent(efn, shift, dt_start=0.005, dt_max=0.02)
+  st = init_fn(ut, neighbor=nb)
+  sp = O.GeneralSpace(T)
+  m = O.PairModel('lj')
+  ff = lambda X: O.energy_force_brute(m, sp, X)[2]
+  params = O.FireParams(0.005, 0.02)
+  ost = O.fire_init(u.astype(np.float64), ff, params)
+  for k in range(100):
+    nb = nb.update(st.position)
+    st = apply_fn(st, neighbor=nb)
+    ost = O.fire_step(ost, ff, sp, params)
+    if k < 40:
+      assert float(st.dt) == float(ost.dt), k
+      assert float(st.alpha) == float(ost.alpha), k
+      assert int(st.n_pos) == ost.n_pos, k
+  E = float(efn(st.position, nb))
+  E64 = O.energy_force_brute(m, sp, ost.position)[0]
+  E0 = O.energy_force_brute(m, sp, u.astype(np.float64))[0]
+  print(f'  E {E0:.6f} -> {E:.8f} (oracle {E64:.8f})')
+  assert E < E0
+  assert abs(E - E64) <= 1e-5 * abs(E64)
+
+
+def test_general_nvt_vs_fp64():
+  """nvt_nose_hoover in a triclinic cell against the oracle's float64 chain
+  and trajectory (40 steps)."""
+  print('case general NVT', flush=True)
+  jmd = _jmd()
+  u, T, P = O.fcc_triclinic(8, jitter=0.03, kT=0.8, seed=12)
+  disp, shift = jmd.space.periodic_general(T)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, T, fractional_coordinates=True,
+                                                    format=jmd.partition.Dense)
+  ut = torch.as_tensor(u, device=DEV)
+  nb = nfn.allocate(ut)
+  kT, dt = 0.8, 0.005
+  init_fn, apply_fn = jmd.simulate.nvt_nose_hoover(efn, shift, dt, kT)
+  st = init_fn(0, ut, momenta=torch.as_tensor(P, device=DEV), neighbor=nb)
+  sp = O.GeneralSpace(T)
+  m = O.PairModel('lj')
+  force = lambda X: O.energy_force_brute(m, sp, X)[2]
+  o, tau = O.nvt_init(u.astype(np.float64), P.astype(np.float64), force, kT, dt)
+  for _ in range(40):
+    nb = nb.update(st.position)
+    st = apply_fn(st, neighbor=nb)
+    o = O.nvt_step(o, force, sp, dt, kT, tau)
+  dR = sp.displacement(st.position_hi.cpu().numpy(), o.position)
+  dp = np.abs(st.chain.cpu().numpy()[1] - o.p_xi).max() / np.abs(o.p_xi).max()
+  print(f'  max |dR| {np.abs(dR).max():.2e}, chain momenta rel {dp:.2e}')
+  assert np.abs(dR).max() <= 1e-5 and dp <= 1e-4
