@@ -462,6 +462,20 @@ int jmd_kick_kinetic_partials(int64_t n);
 int jmd_kick_kinetic(float* momentum, const float* force, const float* mass, float mass_scalar,
                      int64_t n, int32_t dim, float half_dt, double* ke_partials, void* stream);
 
+/* jmd_kick_kinetic followed by jmd_nhc_step(mode, ...) in one launch, with
+ * the same results bit for bit: the closing kick and kinetic energy of
+ * nvt_nose_hoover's apply_fn and the chain half steps around the step
+ * boundary (simulate.py:492-515,650-669).  ke_partials holds
+ * jmd_kick_kinetic_partials(n) entries; ticket is a device uint32 that is 0
+ * before the first call and is left at 0 by every call.  Calls that share a
+ * ticket must not overlap. */
+int jmd_kick_kinetic_nhc(float* momentum, const float* force, const float* mass,
+                         float mass_scalar, int64_t n, int32_t dim, float half_dt,
+                         double* ke_partials, uint32_t* ticket, int32_t mode, double* chain,
+                         int32_t chain_length, int32_t chain_steps, int32_t sy_steps, float dt,
+                         float kT, float tau, double dof, double* kinetic,
+                         double* momentum_scale, void* stream);
+
 /* p *= momentum_scale[0] (the chain's closing factor at the end of a block). */
 int jmd_scale_momentum(float* momentum, int64_t n, int32_t dim, const double* scale,
                        void* stream);

@@ -105,6 +105,10 @@ _SIGNATURES = {
     'jmd_nhc_step': (c_int32, [c_int32, c_void_p, c_int32, c_int32, c_int32, c_float, c_float,
                                c_float, c_double, c_void_p, c_void_p, c_int32, c_void_p,
                                c_void_p]),
+    'jmd_kick_kinetic_nhc': (c_int32, [c_void_p, c_void_p, c_void_p, c_float, c_int64, c_int32,
+                                       c_float, c_void_p, c_void_p, c_int32, c_void_p, c_int32,
+                                       c_int32, c_int32, c_float, c_float, c_float, c_double,
+                                       c_void_p, c_void_p, c_void_p]),
     'jmd_scale_momentum': (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
     'jmd_pair_virial_workspace_bytes': (c_size_t, []),
     'jmd_multi_image_workspace_bytes': (c_size_t, [c_int64, c_int32]),

@@ -215,26 +215,14 @@ enum { NHC_FINISH = 1, NHC_START = 2, NHC_INIT = 4 };
 // mode NHC_INIT: ke = sum of the partials, chain at rest.  NHC_FINISH: ke =
 // sum of the partials (after the closing kick), half step.  NHC_START:
 // chain masses for kT, half step.  scale = the product of the factors.
-// chain = [xi (L), p_xi (L), Q (L)] in float64.
+// chain = [xi (L), p_xi (L), Q (L)] in float64.  One thread; ksum is the
+// partials' sum (used by NHC_FINISH and NHC_INIT).
 template <int L>
-__global__ __launch_bounds__(256) void k_nhc(int mode, double* __restrict__ chain,
-                                             int chain_steps, int sy_steps, float dt, float kT,
-                                             float tau, double dof, double* __restrict__ ke,
-                                             const double* __restrict__ part, int nparts,
-                                             double* __restrict__ scale) {
-  __shared__ double s[256];
-  double v = 0.0;
-  if (mode & (NHC_FINISH | NHC_INIT))
-    for (int b = threadIdx.x; b < nparts; b += 256) v += part[b];
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x != 0) return;
+__device__ void nhc_chain(int mode, double* __restrict__ chain, int chain_steps, int sy_steps,
+                          float dt, float kT, float tau, double dof, double* __restrict__ ke,
+                          double ksum, double* __restrict__ scale) {
   const double kt = (double)kT;
-  double k = (mode & (NHC_FINISH | NHC_INIT)) ? s[0] : ke[0];
+  double k = (mode & (NHC_FINISH | NHC_INIT)) ? ksum : ke[0];
   double xi[L], p[L], Q[L], iQ[L];
 #pragma unroll
   for (int m = 0; m < L; ++m) {
@@ -261,6 +249,35 @@ __global__ __launch_bounds__(256) void k_nhc(int mode, double* __restrict__ chai
   }
   ke[0] = k;
   scale[0] = total;
+}
+
+// Sum of the partials in a fixed order: thread t adds entries t, t + 256,
+// ..., then a tree over the 256 sums (block of 256 threads).  LOAD reads one
+// partial.
+template <typename LOAD>
+__device__ __forceinline__ double partials_sum(int nparts, LOAD load) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += 256) v += load(b);
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  return s[0];
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_nhc(int mode, double* __restrict__ chain,
+                                             int chain_steps, int sy_steps, float dt, float kT,
+                                             float tau, double dof, double* __restrict__ ke,
+                                             const double* __restrict__ part, int nparts,
+                                             double* __restrict__ scale) {
+  const bool sum = (mode & (NHC_FINISH | NHC_INIT)) != 0;
+  const double ksum = partials_sum(sum ? nparts : 0, [&](int b) { return part[b]; });
+  if (threadIdx.x != 0) return;
+  nhc_chain<L>(mode, chain, chain_steps, sy_steps, dt, kT, tau, dof, ke, ksum, scale);
 }
 
 // p += half_dt F (optional), and per-block float64 partials of sum p^2 / m.
@@ -293,6 +310,68 @@ __global__ __launch_bounds__(256) void k_kick_ke(float* __restrict__ p, const fl
     for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_k[w];
     part[blockIdx.x] = 0.5 * t;
   }
+}
+
+// k_kick_ke followed by k_nhc in one launch: each workgroup exchanges its
+// partial into part[] with a device-scope atomic (performed at the device's
+// coherence point) and takes a completion ticket only after that exchange has
+// returned; the workgroup that takes the last ticket sums the partials with
+// coherent loads in k_nhc's order, advances the chain and resets the ticket.
+// The results equal those of the two launches bit for bit.
+template <int DIM, int L>
+__global__ __launch_bounds__(256) void k_kick_ke_nhc(
+    float* __restrict__ p, const float* __restrict__ F, const float* __restrict__ mass,
+    float mass_scalar, int n, float half_dt, double* __restrict__ part,
+    unsigned* __restrict__ ticket, int mode, double* __restrict__ chain, int chain_steps,
+    int sy_steps, float dt, float kT, float tau, double dof, double* __restrict__ ke,
+    double* __restrict__ scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double k = 0.0;
+  if (i < n) {
+    const double m = mass ? (double)mass[i] : (double)mass_scalar;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const long long q = (long long)i * DIM + d;
+      float v = p[q];
+      if (F) {
+        v += half_dt * F[q];
+        p[q] = v;
+      }
+      k += (double)v * (double)v;
+    }
+    k /= m;
+  }
+  __shared__ double s_k[256 / kWave];
+  __shared__ int s_last;
+  k = wave_reduce_sum(k);
+  if (lane_id() == 0) s_k[threadIdx.x / kWave] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_k[w];
+    unsigned long long* slot = reinterpret_cast<unsigned long long*>(part + blockIdx.x);
+    const unsigned long long old =
+        __hip_atomic_exchange(slot, (unsigned long long)__double_as_longlong(0.5 * t),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the ticket is issued only once the exchange has returned
+    asm volatile("" ::"v"(old) : "memory");
+    const unsigned tk =
+        __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = tk == gridDim.x - 1 ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  const int nparts = (int)gridDim.x;
+  const bool sum = (mode & (NHC_FINISH | NHC_INIT)) != 0;
+  const double ksum = partials_sum(sum ? nparts : 0, [&](int b) {
+    const unsigned long long bits = __hip_atomic_load(
+        reinterpret_cast<const unsigned long long*>(part + b), __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)bits);
+  });
+  if (threadIdx.x != 0) return;
+  __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  nhc_chain<L>(mode, chain, chain_steps, sy_steps, dt, kT, tau, dof, ke, ksum, scale);
 }
 
 __global__ __launch_bounds__(256) void k_scale_momentum(float* __restrict__ p, long long m,
@@ -484,6 +563,45 @@ int jmd_nhc_step(int32_t mode, double* chain, int32_t chain_length, int32_t chai
       return JMD_ERR_INVALID;
   }
 #undef JMD_NHC
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+int jmd_kick_kinetic_nhc(float* momentum, const float* force, const float* mass,
+                         float mass_scalar, int64_t n, int32_t dim, float half_dt,
+                         double* ke_partials, uint32_t* ticket, int32_t mode, double* chain,
+                         int32_t chain_length, int32_t chain_steps, int32_t sy_steps, float dt,
+                         float kT, float tau, double dof, double* kinetic,
+                         double* momentum_scale, void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || !ke_partials || !ticket ||
+      (n > 0 && !momentum) || (!mass && !(mass_scalar > 0.0f)) || !chain || !kinetic ||
+      !momentum_scale || chain_length < 2 || chain_length > kMaxChain || chain_steps < 1 ||
+      (sy_steps != 1 && sy_steps != 3 && sy_steps != 5 && sy_steps != 7) || mode < 1 || mode > 7)
+    return JMD_ERR_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const int nn = (int)n;
+  dim3 grid(jmd_kick_kinetic_partials(n)), block(256);
+#define JMD_KNHC(LEN)                                                                        \
+  case LEN:                                                                                  \
+    if (dim == 3)                                                                            \
+      hipLaunchKernelGGL((k_kick_ke_nhc<3, LEN>), grid, block, 0, st, momentum, force, mass, \
+                         mass_scalar, nn, half_dt, ke_partials, ticket, (int)mode, chain,    \
+                         (int)chain_steps, (int)sy_steps, dt, kT, tau, dof, kinetic,         \
+                         momentum_scale);                                                    \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_kick_ke_nhc<2, LEN>), grid, block, 0, st, momentum, force, mass, \
+                         mass_scalar, nn, half_dt, ke_partials, ticket, (int)mode, chain,    \
+                         (int)chain_steps, (int)sy_steps, dt, kT, tau, dof, kinetic,         \
+                         momentum_scale);                                                    \
+    break;
+  switch (chain_length) {
+    JMD_KNHC(2) JMD_KNHC(3) JMD_KNHC(4) JMD_KNHC(5) JMD_KNHC(6) JMD_KNHC(7) JMD_KNHC(8)
+    JMD_KNHC(9) JMD_KNHC(10) JMD_KNHC(11) JMD_KNHC(12) JMD_KNHC(13) JMD_KNHC(14) JMD_KNHC(15)
+    JMD_KNHC(16)
+    default:
+      return JMD_ERR_INVALID;
+  }
+#undef JMD_KNHC
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }

@@ -579,7 +579,8 @@ class _Thermostat:
       lib = nat.load()
       b = dict(part=torch.zeros((lib.jmd_kick_kinetic_partials(N),), dtype=torch.float64,
                                 device=device),
-               scale=torch.ones((1,), dtype=torch.float64, device=device))
+               scale=torch.ones((1,), dtype=torch.float64, device=device),
+               ticket=torch.zeros((1,), dtype=torch.int32, device=device))
       self._bufs = {key: b}
     return b
 
@@ -592,6 +593,22 @@ class _Thermostat:
         float(self.dt), float(kT), float(self.tau), float(N * dim), nat.ptr(state.kinetic_energy),
         nat.ptr(b['part']), int(b['part'].numel()), nat.ptr(b['scale']),
         st if st is not None else nat.stream_handle(state.momentum.device)), 'jmd_nhc_step')
+    return b['scale']
+
+  def kinetic_step(self, mode, state, F, half, st=None, kT=None):
+    """kinetic() then step(mode, ...) in one launch (same results)."""
+    lib = nat.load()
+    N, dim = state.momentum.shape
+    dev = state.momentum.device
+    b = self.buffers(N, dev)
+    kT = self.kT if kT is None else f32(kT)
+    mass_s, mass_v = canonicalize_mass(state.mass, dev)
+    nat.check(lib.jmd_kick_kinetic_nhc(
+        nat.ptr(state.momentum), nat.ptr(F), nat.ptr(mass_v), mass_s if mass_v is None else 0.0,
+        N, dim, float(half), nat.ptr(b['part']), nat.ptr(b['ticket']), int(mode),
+        nat.ptr(state.chain), self.L, self.chain_steps, self.sy_steps, float(self.dt), float(kT),
+        float(self.tau), float(N * dim), nat.ptr(state.kinetic_energy), nat.ptr(b['scale']),
+        st if st is not None else nat.stream_handle(dev)), 'jmd_kick_kinetic_nhc')
     return b['scale']
 
   def kinetic(self, state, F, half, st=None):
@@ -774,8 +791,9 @@ class NVTLoop(NVELoop):
           'jmd_pair_energy_force')
       if events is not None:
         events.append(_close_timing(e0))
-      th.kinetic(state, state.force, self.half, st=st)
-      th.step(_NHC_FINISH | _NHC_START if k + 1 < n_steps else _NHC_FINISH, kin, N, dim, st=st)
+      # closing kick, kinetic energy and the chain in one launch
+      th.kinetic_step(_NHC_FINISH | _NHC_START if k + 1 < n_steps else _NHC_FINISH, kin,
+                      state.force, self.half, st=st)
     nat.check(lib.jmd_scale_momentum(nat.ptr(state.momentum), N, dim, nat.ptr(b['scale']), st),
               'jmd_scale_momentum')
     return state

@@ -145,3 +145,96 @@ def test_nvt_relaxes_to_bath_temperature():
   T_end = float(np.mean(temps[-10:]))
   print(f'  T start {temps[0]:.3f}, mean of the last 1000 steps {T_end:.3f}, bath {kT}')
   assert abs(T_end - kT) / kT < 0.1
+
+
+@pytest.mark.parametrize('n,mode', [(1000, 3), (70001, 3), (70001, 1), (300, 5)])
+def test_fused_kick_kinetic_chain_equals_two_launches(n, mode):
+  """jmd_kick_kinetic_nhc against jmd_kick_kinetic + jmd_nhc_step, bit for bit,
+  on several consecutive calls sharing one ticket (several hundred workgroups
+  at n = 70001)."""
+  print(f'case fused kick+chain n={n} mode={mode}', flush=True)
+  jmd = _jmd()
+  nat = jmd._native
+  lib = nat.load()
+  rng = np.random.default_rng(n + mode)
+  dim, L = 3, 5
+  P0 = torch.as_tensor(rng.normal(size=(n, dim)).astype(F32), device=DEV)
+  F = torch.as_tensor(rng.normal(size=(n, dim)).astype(F32), device=DEV)
+  mass = torch.as_tensor(rng.uniform(0.5, 2.0, size=n).astype(F32), device=DEV)
+  nparts = int(lib.jmd_kick_kinetic_partials(n))
+  out = []
+  for fused in (False, True):
+    P = P0.clone()
+    chain = torch.zeros((3, L), dtype=torch.float64, device=DEV)
+    chain[1] = torch.linspace(-0.3, 0.4, L, dtype=torch.float64)
+    chain[2] = 1.0
+    ke = torch.zeros((1,), dtype=torch.float64, device=DEV)
+    part = torch.zeros((nparts,), dtype=torch.float64, device=DEV)
+    scale = torch.ones((1,), dtype=torch.float64, device=DEV)
+    ticket = torch.zeros((1,), dtype=torch.int32, device=DEV)
+    st = nat.stream_handle(P.device)
+    scales = []
+    for _ in range(3):
+      args = (float(F32(0.005)), float(F32(1.2)), float(F32(0.5)), float(n * dim))
+      if fused:
+        nat.check(lib.jmd_kick_kinetic_nhc(
+            nat.ptr(P), nat.ptr(F), nat.ptr(mass), 0.0, n, dim, float(F32(0.0025)),
+            nat.ptr(part), nat.ptr(ticket), mode, nat.ptr(chain), L, 2, 3, *args[:3], args[3],
+            nat.ptr(ke), nat.ptr(scale), st), 'jmd_kick_kinetic_nhc')
+      else:
+        nat.check(lib.jmd_kick_kinetic(nat.ptr(P), nat.ptr(F), nat.ptr(mass), 0.0, n, dim,
+                                       float(F32(0.0025)), nat.ptr(part), st), 'kick')
+        nat.check(lib.jmd_nhc_step(mode, nat.ptr(chain), L, 2, 3, *args[:3], args[3],
+                                   nat.ptr(ke), nat.ptr(part), nparts, nat.ptr(scale), st),
+                  'nhc')
+      scales.append(scale.clone())
+    torch.cuda.synchronize()
+    out.append((P.cpu(), chain.cpu(), ke.cpu(), torch.cat(scales).cpu(), int(ticket.item())))
+  a, b = out
+  print(f'  scale {a[3].tolist()}, ke {float(a[2]):.10f}')
+  assert b[4] == 0
+  assert torch.equal(a[0], b[0])
+  for x, y in zip(a[1:4], b[1:4]):
+    assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize('rate,substeps,mode', [(0.3, (2, 3), 3), (200.0, (1, 1), 1),
+                                                (400.0, (1, 1), 1)])
+def test_chain_half_steps_vs_oracle(rate, substeps, mode):
+  """jmd_nhc_step against the oracle's float64 chain (simulate.py:492-515).
+  rate = p_xi / Q sets the exponentials' arguments: about 1e-4 (the Taylor
+  polynomial's range) over two half steps of 2 x 3 substeps, and 0.12 / 0.25
+  (the library exp; the sweeps then also reach arguments beyond 708, where
+  the factor underflows to 0) in one substep.  Bar: 1e-12 relative in the
+  polynomial range, 1e-10 beyond."""
+  print(f'case chain half steps rate={rate} substeps={substeps} mode={mode}', flush=True)
+  jmd = _jmd()
+  nat = jmd._native
+  lib = nat.load()
+  L, n, dim = 5, 100, 3
+  dof = float(n * dim)
+  kT, dt, tau = F32(1.2), F32(0.005), F32(0.5)
+  Q = O.nhc_masses(L, kT, tau, dof)
+  p_xi = np.linspace(-1.0, 1.0, L) * rate * Q
+  xi = 0.1 * np.arange(L, dtype=np.float64)
+  ke = 0.5 * dof * float(kT) * 1.3
+  chain = torch.as_tensor(np.stack([xi, p_xi, Q]), device=DEV)
+  part = torch.as_tensor([ke], dtype=torch.float64, device=DEV)
+  kin = torch.zeros((1,), dtype=torch.float64, device=DEV)
+  scale = torch.zeros((1,), dtype=torch.float64, device=DEV)
+  cs, sy = substeps
+  nat.check(lib.jmd_nhc_step(mode, nat.ptr(chain), L, cs, sy, float(dt), float(kT), float(tau),
+                             dof, nat.ptr(kin), nat.ptr(part), 1, nat.ptr(scale),
+                             nat.stream_handle(chain.device)), 'jmd_nhc_step')
+  torch.cuda.synchronize()
+  k64 = np.float64(kT)
+  x2, p2, ke2, s = O.nhc_half_step(dt, cs, sy, xi, p_xi, Q, ke, dof, k64)
+  if mode & 2:
+    x2, p2, ke2, s2 = O.nhc_half_step(dt, cs, sy, x2, p2, Q, ke2, dof, k64)
+    s *= s2
+  c = chain.cpu().numpy()
+  tol = 1e-12 if rate < 1 else 1e-10
+  errs = (np.abs(c[1] - p2).max() / np.abs(p2).max(), np.abs(c[0] - x2).max() / np.abs(x2).max(),
+          abs(float(kin) - ke2) / ke2, abs(float(scale) - s) / s)
+  print(f'  rel errors p_xi {errs[0]:.2e} xi {errs[1]:.2e} ke {errs[2]:.2e} scale {errs[3]:.2e}')
+  assert all(np.isfinite(errs)) and max(errs) < tol
