@@ -47,9 +47,30 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
 #define JMD_PAIR_UNROLL 4
 #endif
   constexpr int UNR = JMD_PAIR_UNROLL;
-  // One pair term; padding entries (j = n) read particle i and add nothing.
+  // Interior waves of the Lennard-Jones kernel with scalar parameters gather
+  // through a buffer descriptor of exactly n rows: a padding entry (j = n)
+  // then reads a zero position, so d = -x_i and r^2 >= 3 r_c^2 (every
+  // coordinate of an interior particle is at least r_c), where the switched
+  // or truncated pair term is exactly zero -- no per-pair validity mask and
+  // no index select.  The caller keeps n below 2^27 on this path.
+  constexpr bool kNoMask = INTERIOR && KIND == JMD_LENNARD_JONES && !BOX && !GENERAL &&
+                           DIM == 3 && STRIDE == 4;
+  const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(pos), (short)0, (int)(n * 16u), 0x00020000);
+  auto gather = [&](unsigned jraw, float* xj) {
+    if (kNoMask) {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(prs, (int)(jraw * 16u), 0, 0);
+      xj[0] = __uint_as_float(v[0]);
+      xj[1] = __uint_as_float(v[1]);
+      xj[2] = __uint_as_float(v[2]);
+    } else {
+      load_pos<DIM, STRIDE>(pos, jraw < n ? jraw : i, xj);
+    }
+  };
+  // One pair term; padding entries (j = n) read particle i (or, without the
+  // mask, a zero position) and add nothing.
   auto term = [&](unsigned jraw, const float* xj) {
-    const bool ok = jraw < n;
+    const bool ok = kNoMask || jraw < n;
     float d[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int c = 0; c < DIM; ++c)
@@ -99,7 +120,7 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
 #pragma unroll
       for (int u = 0; u < UNR; ++u) {
         xj[u][0] = xj[u][1] = xj[u][2] = 0.0f;
-        load_pos<DIM, STRIDE>(pos, jj[u] < n ? jj[u] : i, xj[u]);
+        gather(jj[u], xj[u]);
       }
       // next block, clamped to the last full one (re-read, unused, at the end)
       const int* q = p + min(b + 1, nfull - 1) * (UNR * TPA);
@@ -123,7 +144,7 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
     const unsigned j = (unsigned)p[t * TPA];
     if (j >= n) return;
     float xj[3] = {0.0f, 0.0f, 0.0f};
-    load_pos<DIM, STRIDE>(pos, j, xj);
+    gather(j, xj);
     term(j, xj);
   }
 }
@@ -163,7 +184,8 @@ __global__ __launch_bounds__(256) void k_pair_dense(
       interior = interior && xi[c] >= w && xi[c] <= 1.0f - w;
     }
   }
-  const bool wave_interior = __all(interior || !active);
+  // (n < 2^27 keeps the interior path's buffer offsets j * 16 in range)
+  const bool wave_interior = __all(interior || !active) && n < (1 << 27);
   if (active && s.general) {
     const int* row = idx + (long long)i * width;
 #define JMD_ROWG(INT, SM)                                                                      \
