@@ -989,37 +989,73 @@ __global__ __launch_bounds__(256) void k_nl_allpairs(
   }
 }
 
+// Inclusive scan over the 64 lanes of a wavefront (64-bit values, moved as
+// two 32-bit halves).
+__device__ __forceinline__ long long wave_inclusive_scan(long long x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int lo = __shfl_up((int)(unsigned)(unsigned long long)x, off, kWave);
+    const int hi = __shfl_up((int)((unsigned long long)x >> 32), off, kWave);
+    const long long y = (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
 // Exclusive scan of row_count[0..n) into row_offset[0..n] in one workgroup
 // (Sparse formats: flattened row-major order, partition.py:1000-1005).
+// Chunks of kRowScanChunk counts are loaded into LDS with coalesced reads; each
+// thread sums kRowScanPer consecutive counts, the 1024 sums are scanned with
+// wavefront scans (two barriers), and the offsets go back to memory from LDS
+// with coalesced writes.  Offsets saturate at INT32_MAX as before.
+constexpr int kRowScanPer = 8;
+constexpr int kRowScanChunk = kRowScanPer * 1024;
 __global__ __launch_bounds__(1024) void k_row_scan(const int* __restrict__ row_count,
                                                    int* __restrict__ row_offset, int n,
                                                    int* __restrict__ misc,
                                                    const int* gate) {
   if (gate_closed(gate)) return;
-  __shared__ long long s_sum[1024];
-  int t = threadIdx.x;
-  int per = (n + 1023) / 1024;
-  int beg = t * per;
-  int end = min(beg + per, n);
-  long long local = 0;
-  for (int c = beg; c < end; ++c) local += row_count[c];
-  s_sum[t] = local;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    long long v = (t >= off) ? s_sum[t - off] : 0;
-    __syncthreads();
-    s_sum[t] += v;
-    __syncthreads();
-  }
-  long long run = (t == 0) ? 0 : s_sum[t - 1];
+  __shared__ int s_v[kRowScanChunk + kRowScanChunk / 32];  // one pad word per 32
+  __shared__ long long s_w[1024 / kWave];
+  const int t = threadIdx.x;
+  const int lane = lane_id(), wave = t / kWave;
   const long long kMax = 0x7fffffffLL;
-  for (int c = beg; c < end; ++c) {
-    row_offset[c] = (int)min(run, kMax);
-    run += row_count[c];
+  auto at = [](int k) { return k + (k >> 5); };  // padded LDS index
+  long long carry = 0;
+  for (int base = 0; base < n; base += kRowScanChunk) {
+    const int m = min(kRowScanChunk, n - base);
+    for (int k = t; k < kRowScanChunk; k += 1024) s_v[at(k)] = k < m ? row_count[base + k] : 0;
+    __syncthreads();
+    long long local = 0;
+#pragma unroll
+    for (int q = 0; q < kRowScanPer; ++q) local += s_v[at(t * kRowScanPer + q)];
+    const long long incl = wave_inclusive_scan(local);
+    if (lane == kWave - 1) s_w[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+      long long w = lane < 1024 / kWave ? s_w[lane] : 0;
+      w = wave_inclusive_scan(w);
+      if (lane < 1024 / kWave) s_w[lane] = w;
+    }
+    __syncthreads();
+    long long run = carry + (wave > 0 ? s_w[wave - 1] : 0) + (incl - local);
+    const long long total = s_w[1024 / kWave - 1];
+#pragma unroll
+    for (int q = 0; q < kRowScanPer; ++q) {
+      const int k = at(t * kRowScanPer + q);
+      const int c = s_v[k];
+      s_v[k] = (int)min(run, kMax);
+      run += c;
+    }
+    __syncthreads();
+    for (int k = t; k < m; k += 1024) row_offset[base + k] = s_v[at(k)];
+    carry += total;
+    __syncthreads();  // s_v and s_w are refilled by the next chunk
   }
-  if (t == 1023) {
-    row_offset[n] = (int)min(s_sum[1023], kMax);
-    misc[M_OCC] = (int)min(s_sum[1023], kMax);
+  if (t == 0) {
+    row_offset[n] = (int)min(carry, kMax);
+    misc[M_OCC] = (int)min(carry, kMax);
   }
 }
 

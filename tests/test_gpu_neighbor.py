@@ -298,3 +298,32 @@ def test_unwrapped_positions_bit_exact_sparse():
   print('case unwrapped sparse', flush=True)
   _, _, g, o = _both(R, L, 2.5, 0.5, O.SPARSE)
   _assert_same(g, o, 'unwrapped sparse')
+
+
+def test_sparse_formats_follow_dense_rows_at_64000():
+  """N = 64,000 (many chunks of the row-offset scan): the Sparse list is the
+  Dense rows flattened in row-major order (receiver j, sender i), and the
+  OrderedSparse list the same with j < i (partition.py:982-1032); padding
+  fills every entry after the total.  The Dense list's rows are checked
+  against the oracle at this scale by the sampled-row test."""
+  R, _, L = O.lj_lattice(40, seed=21)
+  jmd = _jmd()
+  disp, _ = jmd.space.periodic(L)
+  Rt = torch.as_tensor(R, device=DEV)
+  N = R.shape[0]
+  print(f'case sparse vs dense rows N={N}', flush=True)
+  d = jmd.partition.neighbor_list(disp, L, 2.5, 0.5, format=jmd.partition.Dense).allocate(Rt)
+  idx = d.idx.cpu().numpy()
+  valid = idx < N
+  send = np.broadcast_to(np.arange(N, dtype=np.int32)[:, None], idx.shape)
+  for fmt in (jmd.partition.Sparse, jmd.partition.OrderedSparse):
+    keep = valid & (idx < send) if fmt is jmd.partition.OrderedSparse else valid
+    want_r, want_s = idx[keep], send[keep]
+    s = jmd.partition.neighbor_list(disp, L, 2.5, 0.5, format=fmt).allocate(Rt)
+    assert int(s.error.code.item()) == 0
+    got = s.idx.cpu().numpy()
+    k = want_r.size
+    print(f'  {fmt}: {k} pairs, capacity {got.shape[1]}')
+    assert got.shape[1] >= k
+    assert np.array_equal(got[0, :k], want_r) and np.array_equal(got[1, :k], want_s)
+    assert np.all(got[:, k:] == N)
