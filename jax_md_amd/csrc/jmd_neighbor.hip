@@ -889,6 +889,17 @@ __device__ __forceinline__ void dev_finish(int* __restrict__ misc, int occ,
 template <int DIM, int STRIDE, int MODE, int GEOM>
 __global__ __launch_bounds__(256) void k_nl_cells(BuildArgs A, int finish) {
   if (gate_closed(A.gate)) return;
+  if (MODE == MODE_SPARSE_WRITE) {
+    // Sparse padding idx[:, total:width] = n beside the pair writes (which
+    // stay below total, the row scan's sum); every workgroup reads total
+    // before it takes its completion ticket, so before the finish resets it.
+    const long long total = A.w.misc[M_OCC];
+    for (long long k = total + (long long)blockIdx.x * blockDim.x + threadIdx.x; k < A.width;
+         k += (long long)gridDim.x * blockDim.x) {
+      A.idx[k] = A.n;
+      A.idx[A.width + k] = A.n;
+    }
+  }
   const int dep = dev_nl_cells<DIM, STRIDE, MODE, GEOM>(A, blockIdx.x, gridDim.x);
   if (!finish) return;
   if (!last_workgroup(&A.w.misc[M_TICKET_NL], dep)) return;
@@ -1221,11 +1232,7 @@ static int build_cells(const BuildArgs& A, bool sparse, bool count_only, hipStre
     JMD_LAUNCH_CHECK();
     return JMD_OK;
   }
-  int pb = ceil_div(A.width, 256);
-  pb = pb > 4096 ? 4096 : pb + 1;
-  hipLaunchKernelGGL(k_sparse_pad, dim3(pb), dim3(256), 0, st, A.idx, A.width, A.n, w.misc,
-                     A.gate);
-  JMD_LAUNCH_CHECK();
+  // the write launch also pads idx[:, total:width]
   return cells_dispatch<DIM, STRIDE>(A, MODE_SPARSE_WRITE, 1, st);
 }
 
