@@ -12,11 +12,15 @@ replay, with a host overflow check per block; an overflowing block is redone
 from its saved start state after re-allocating the list (the reference's
 protocol, examples/nve_neighbor_list.py:186-201).
 
-The force kernel's duration is measured with HIP events around each force
-launch of one more (eager) 100-step block after the timed region, on the
-stream the kernels run on, with the device kept behind the host.  HBM traffic per
+Before the timed region a HIP graph is recorded for every block length the
+timed loop uses, so the timed blocks are graph replays (the line reports the
+blocks' kinds).  The force kernel's duration, the list update with and
+without a rebuild, and the block-boundary cost are measured with HIP events
+in one more (eager) 100-step block after the timed region, on the stream the
+kernels run on, with the device kept behind the host.  HBM traffic per
 launch comes from the committed counter summary of the same kernel and
-workload (profiles/k_pair_dense_lj_n262144_pmc.json), when present.
+workload (profiles/k_pair_dense_lj_n262144_pmc.json), scaled to this run's
+mean row length.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -38,37 +42,113 @@ def log(*a):
   print(*a, file=sys.stderr, flush=True)
 
 
-def force_kernel_us(jmd, loop, state, nbrs, efn, steps=100):
-  """Average duration of the force + half-kick kernel inside the step loop:
-  one eager block of `steps` steps (the kernels of the timed blocks, launched
-  from the host instead of a graph replay) with HIP events around each force
-  launch, on the stream the kernels run on, less the duration of an empty
-  event pair recorded right after each (what the markers themselves add).  A
-  spin kernel queued first keeps the device behind the host, so no interval
-  holds a host gap."""
+def step_breakdown(jmd, loop, state, nbrs, steps=100):
+  """Per-step kernel times inside the step loop: one eager block of `steps`
+  steps (the kernels of the timed blocks, launched from the host instead of a
+  graph replay) with HIP events around each step's list update (binning +
+  half step, cell sort, list build; gated) and around each force launch, on
+  the stream the kernels run on, less what an empty event pair adds.  A spin
+  kernel queued first keeps the device behind the host, so no interval holds
+  a host gap.  Steps that rebuilt the list are told apart by the list's build
+  counter."""
   import torch
-  ev = []
+  ev = jmd.simulate.StepEvents()
   loop.prime(state, nbrs)
   try:
     torch.cuda._sleep(int(5e8))  # ~0.2 s of device time queued ahead
   except Exception:
     pass
+  b0 = torch.cuda.Event(enable_timing=True)
+  b1 = torch.cuda.Event(enable_timing=True)
+  b0.record()
   loop.run(state, nbrs, steps, ev)
+  b1.record()
   torch.cuda.synchronize()
-  kern = sum(e[0].elapsed_time(e[1]) for e in ev) / len(ev)
-  mark = sum(e[2].elapsed_time(e[3]) for e in ev) / len(ev)
-  return 1e3 * (kern - mark)
+  t = ev.summary()
+  rebuilt = t['rebuilt']
+  n_open = sum(rebuilt)
+  closed_upd = [u for u, r in zip(t['update_us'], rebuilt) if not r]
+  open_upd = [u for u, r in zip(t['update_us'], rebuilt) if r]
+
+  def mean(v):
+    return sum(v) / len(v) if v else None
+
+  upd_closed, upd_open = mean(closed_upd), mean(open_upd)
+  block_us = 1e3 * b0.elapsed_time(b1)
+  in_steps = sum(t['update_us']) + sum(t['force_us'])
+  return {
+      'steps': steps, 'rebuilds': n_open,
+      'force_us': mean(t['force_us']),
+      'update_closed_us': upd_closed,
+      'update_open_us': upd_open,
+      'rebuild_us': (upd_open - upd_closed) if (upd_open is not None and upd_closed) else None,
+      # what the block costs outside the steps' kernels: the cell-order
+      # permutation of state and list at its start and end, event markers
+      'block_boundary_us': block_us - in_steps,
+      'block_us': block_us,
+      'note': 'eager block with HIP events; update = binning with the opening half step '
+              '(velocity Verlet kick + drift + skin test) + cell sort + list build, the last '
+              'two gated on the skin test; rebuild_us = open update less closed update',
+  }
 
 
-def counter_traffic(N):
-  """HBM bytes per force launch from the committed counter summary, or None."""
+def counter_traffic(N, kbar):
+  """HBM bytes per force launch from the committed counter summary, scaled to
+  this run's mean row length: the summary's measured traffic over its own
+  algorithmic bytes, times this run's algorithmic bytes.  None without it."""
   path = os.path.join(ROOT, 'profiles', f'k_pair_dense_lj_n{N}_pmc.json')
   try:
     with open(path) as f:
       d = json.load(f)
-    return float(d['traffic_bytes_per_launch'])
+    t = float(d['traffic_bytes_per_launch'])
+    k_prof = float(d['kbar'])
   except (OSError, KeyError, ValueError):
-    return None
+    return None, None
+  ratio = t / (N * (4.0 * k_prof + 60.0))
+  return ratio * N * (4.0 * kbar + 60.0), {
+      'measured_bytes': t, 'measured_kbar': k_prof, 'ratio_to_algorithmic': ratio,
+      'source': os.path.relpath(path, ROOT),
+      'note': 'FETCH_SIZE + WRITE_SIZE of the force kernel (copy-calibrated), measured at '
+              'measured_kbar; roofline.traffic = ratio_to_algorithmic x this run\'s '
+              'algorithmic bytes'}
+
+
+def c1_drift(jmd, O, n_steps=1000, block=100):
+  """Total-energy drift of the small LJ NVE workload (16^3 lattice, N = 4096,
+  the same density, jitter, kT, cutoff, skin and dt) over 1000 steps,
+  through the same step loop; energies in float64 at every block end."""
+  import numpy as np
+  import torch
+  dev = 'cuda:0'
+  R, P, L = O.lj_lattice(16, seed=0)
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense)
+  Rt = torch.as_tensor(R, device=dev)
+  nb = nfn.allocate(Rt, extra_capacity=16)
+  init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
+  st = init_fn(0, Rt, kT=1.0, momenta=torch.as_tensor(P, device=dev), neighbor=nb)
+  loop = jmd.simulate.NVELoop(efn, shift, 0.005)
+
+  def etot():
+    E, _ = efn.energy_and_force(st.position, nb)
+    return float(E) + float(jmd.quantity.kinetic_energy(momentum=st.momentum))
+
+  e0 = etot()
+  series = [e0]
+  loop.prime(st, nb)
+  done = 0
+  while done < n_steps:
+    loop.run(st, nb, block)
+    if bool(nb.did_buffer_overflow):
+      raise RuntimeError('C1 drift run overflowed its list')
+    done += block
+    series.append(etot())
+  dev_max = max(abs(e - e0) for e in series) / abs(e0)
+  return {'workload': 'LJ NVE N=4096 (16^3 lattice, rho=0.8442, jitter 0.05, kT=1), Dense, '
+                      'r_c=2.5 skin=0.5, dt=0.005, 1000 steps from t=0',
+          'E_total_0': e0, 'E_total_1000': series[-1],
+          'drift_rel': abs(series[-1] - e0) / abs(e0), 'max_dev_rel': dev_max,
+          'every_100_steps': series}
 
 
 def adjoint_section(jmd, O, check=True):
@@ -576,16 +656,27 @@ def main():
   builds_before += nbrs.plan.builds
   nbrs = nfn.allocate(state.position)
   run_steps(args.warmup - w1)
+  # the graphs of every block length the timed loop uses are recorded now,
+  # so the timed blocks are replays (an eager warm-up block has loaded the
+  # kernels; a block after a re-allocation in the timed region still records)
+  lengths = {min(args.block, args.steps)}
+  if args.steps > args.block and args.steps % args.block:
+    lengths.add(args.steps % args.block)
+  if loop.stats['eager'] > 0 and not args.eager:
+    for k in sorted(lengths):
+      loop.capture(state, nbrs, k)
   torch.cuda.synchronize()
   reallocs_warm = reallocs
   E_start, _ = efn.energy_and_force(state.position, nbrs)
   KE_start = jmd.quantity.kinetic_energy(momentum=state.momentum)
   builds0 = builds_before + nbrs.plan.builds
+  stats0 = dict(loop.stats)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
   run_steps(args.steps)
   torch.cuda.synchronize()
   elapsed = time.perf_counter() - t0
+  timed_blocks = {k: loop.stats[k] - stats0[k] for k in loop.stats}
   builds = builds_before + nbrs.plan.builds - builds0
   E_end, F_gpu = efn.energy_and_force(state.position, nbrs)
   KE_end = jmd.quantity.kinetic_energy(momentum=state.momentum)
@@ -594,15 +685,25 @@ def main():
   ms = 1e3 * elapsed / args.steps
   idx = nbrs.idx
   kbar = float((idx < N).sum().item()) / N
-  force_us = force_kernel_us(jmd, loop, state, nbrs, efn)
+  brk = step_breakdown(jmd, loop, state, nbrs)
+  force_us = brk['force_us']
   # algorithmic bytes of one force launch: idx row (4 B per valid entry) plus
   # per-atom state: x read 16 (padded row), F write 12, p read+write 24,
   # row-end padding entry 4, energy/params 4 -> 4 Kbar + 60 B per atom
   force_bytes = N * (4.0 * kbar + 60.0)
   achieved = force_bytes / (force_us * 1e-6) / 1e9
   log(f'{args.steps} steps in {elapsed:.3f}s: {ms * 1e3:.1f} us/step, force kernel '
-      f'{force_us:.1f} us, Kbar={kbar:.2f}, list builds={builds}, reallocs={reallocs}')
-  traffic = counter_traffic(N)
+      f'{force_us:.1f} us, Kbar={kbar:.2f}, list builds={builds}, reallocs={reallocs}, '
+      f'timed blocks {timed_blocks}')
+  log(f'breakdown: {json.dumps({k: v for k, v in brk.items() if k != "note"})}')
+  traffic, traffic_src = counter_traffic(N, kbar)
+  drift = None
+  try:
+    drift = c1_drift(jmd, O)
+    log(f'C1 drift over 1000 steps: {drift["drift_rel"]:.2e} (max {drift["max_dev_rel"]:.2e})')
+  except Exception as e:
+    drift = f'failed: {type(e).__name__}: {e}'
+    log(f'C1 drift run failed: {e}')
 
   # force error against the float64 C oracle at the final float32 positions
   force_err = None
@@ -643,12 +744,20 @@ def main():
       'config': {'workload': f'LJ NVE N={N} ({args.n_side}^3 lattice, rho=0.8442, jitter 0.05, '
                              'kT=1), Dense list r_c=2.5 skin=0.5, dt=0.005, fp32 pair '
                              'arithmetic, fp64 positions/energies',
+                 'timed_steps_from_lattice': f'{args.warmup}..{args.warmup + args.steps}',
                  'list_builds': builds, 'reallocs': reallocs - reallocs_warm,
-                 'max_occupancy': nbrs.max_occupancy, 'kbar': kbar},
+                 'max_occupancy': nbrs.max_occupancy, 'kbar': kbar,
+                 'timed_blocks': timed_blocks,
+                 'graph_replays_only': timed_blocks.get('eager', 0) == 0},
       'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                    'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
                    'kernel': 'k_pair_dense (force + closing half kick)',
-                   'kernel_us': force_us, 'algorithmic_bytes': force_bytes},
+                   'kernel_us': force_us, 'algorithmic_bytes': force_bytes,
+                   'traffic_source': traffic_src},
+      'step_breakdown': brk,
+      'rebuild_us': brk['rebuild_us'],
+      'rebuilds_per_step_timed': builds / args.steps,
+      'c1_nve_drift_1000': drift,
       # the whole step against the survey's per-atom-step bytes 4 K + 132
       # (x64 r+w 48, p r+w 24, F w+r 24, x32 w+r 24, x_ref 12): the roofline
       # the 50% target is stated on
@@ -659,7 +768,10 @@ def main():
       'cpu_baseline': cpu,
       'error_vs_fp64': force_err,
       'energy': {'E_total_start': float(E_start) + float(KE_start),
-                 'E_total_end': float(E_end) + float(KE_end)},
+                 'E_total_end': float(E_end) + float(KE_end),
+                 'note': f'total energy at steps {args.warmup} and {args.warmup + args.steps} '
+                         'from the lattice: early steps carry the melting transient of the '
+                         'jittered lattice (see c1_nve_drift_1000 for a 1000-step drift)'},
   }
   if not args.no_extra:
     for key, fn in (('batched', lambda: batched_section(jmd, O)),

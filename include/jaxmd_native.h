@@ -28,6 +28,15 @@
  * that do not take periodic = 2 (the x64 and adjoint calls)
  * return JMD_ERR_UNSUPPORTED for it.
  *
+ * Positions in a periodic space may lie anywhere (space.periodic(side,
+ * wrapped=False) leaves them unwrapped, and the reference's displacement
+ * folds any difference).  A caller that knows every position lies in
+ * [0, side] per dimension ([0, 1] for periodic = 2) -- positions written by a
+ * wrapping shift -- may OR JMD_POSITIONS_IN_BOX into ``periodic``: the force
+ * kernels then skip the fold for particles far from every face and take a
+ * single fold elsewhere.  Without the bit every difference is folded to the
+ * minimum image whatever the positions.
+ *
  * Each declaration cites the reference interface it replaces
  * (paths relative to the jax-md 0.2.29 source tree).
  */
@@ -68,6 +77,10 @@ enum { JMD_LENNARD_JONES = 0, JMD_SOFT_SPHERE = 1 };
  * scalar; per-particle vector combined as 0.5 (p_i + p_j) (smap.py:836);
  * species matrix p[s_i, s_j] (smap.py:787-797). */
 enum { JMD_PARAM_SCALAR = 0, JMD_PARAM_PER_PARTICLE = 1, JMD_PARAM_SPECIES = 2 };
+
+/* Flag ORed into a ``periodic`` argument: every position lies in the box
+ * (see the note on positions above).  No counterpart in the reference. */
+enum { JMD_POSITIONS_IN_BOX = 0x100 };
 
 typedef struct jmd_param {
   int32_t mode;          /* JMD_PARAM_* */

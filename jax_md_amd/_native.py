@@ -19,6 +19,9 @@ LIB_PATH = os.environ.get('JAX_MD_AMD_LIB', os.path.join(_HERE, 'libjaxmd_native
 
 JMD_OK, JMD_ERR_INVALID, JMD_ERR_LAUNCH, JMD_ERR_UNSUPPORTED = 0, 1, 2, 3
 DENSE, SPARSE, ORDERED_SPARSE = 0, 1, 2
+# ORed into a geometry code: every position lies in the box (positions
+# written by a wrapping shift), which lets the force kernels skip folds
+POSITIONS_IN_BOX = 0x100
 LENNARD_JONES, SOFT_SPHERE = 0, 1
 PARAM_SCALAR, PARAM_PER_PARTICLE, PARAM_SPECIES = 0, 1, 2
 

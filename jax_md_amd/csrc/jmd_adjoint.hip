@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_pair_adjoint(
           const double v = pos64[(long long)j * DIM + c] - xi64[c];
           d[c] = (float)(s.periodic ? wrap64(v, (double)s.L[c]) : v);
         } else {
-          d[c] = s.periodic ? min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c])
+          d[c] = s.periodic ? min_image_any(xj[c], xi[c], xim[c], s.L[c], s.half[c], s.inbox)
                             : xj[c] - xi[c];
         }
         w[c] = vj[c] - vi[c];

@@ -40,6 +40,7 @@ struct Space {
   float T[3][3];  // box matrix, row-major (general only)
   float fw[3];    // |row d of T^-1|: fractional reach per unit of real distance
   float smin2;    // smallest squared singular value of T: |T v|^2 >= smin2 |v|^2
+  int inbox = 0;  // every position lies in [0, L] (JMD_POSITIONS_IN_BOX)
   // (kept small: it is passed by value, in registers, to the exact pair
   // test; the inverse lives with the integrator's parameters)
 };
@@ -68,12 +69,15 @@ inline void box_inverse(const Space& s, float Ti[3][3]) {
 }
 
 // periodic: 0 free, 1 periodic box of sides side[0..dim), 2 periodic_general
-// with fractional coordinates, side = the [dim, dim] box matrix (row-major).
-inline Space make_space(int dim, int periodic, const float* side) {
+// with fractional coordinates, side = the [dim, dim] box matrix (row-major);
+// JMD_POSITIONS_IN_BOX may be ORed in.
+inline Space make_space(int dim, int periodic_arg, const float* side) {
+  const int periodic = periodic_arg & ~JMD_POSITIONS_IN_BOX;
   Space s;
   s.dim = dim;
   s.periodic = periodic != 0;
   s.general = periodic == 2;
+  s.inbox = periodic != 0 && (periodic_arg & JMD_POSITIONS_IN_BOX) != 0;
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b) s.T[a][b] = a == b ? 1.0f : 0.0f;
   for (int d = 0; d < 3; ++d) {
@@ -139,7 +143,8 @@ inline bool box_matrix_ok(int dim, const float* T) {
 // Geometry arguments of an entry point: periodic 0 / 1 (side[dim] when 1), or
 // 2 (periodic_general, side = an invertible [dim, dim] box matrix) where the
 // entry point supports it.
-inline int space_check(int dim, int periodic, const float* side, bool allow_general) {
+inline int space_check(int dim, int periodic_arg, const float* side, bool allow_general) {
+  const int periodic = periodic_arg & ~JMD_POSITIONS_IN_BOX;
   if (periodic < 0 || periodic > 2) return JMD_ERR_INVALID;
   if (periodic && !side) return JMD_ERR_INVALID;
   if (periodic == 2) {
@@ -290,6 +295,18 @@ __device__ __forceinline__ float min_image_disp(float xj, float xi, float xim, f
   float hi = (xj - L) - xi;    // x_j near L, x_i near 0
   float d = raw < -half ? lo : raw;
   return raw >= half ? hi : d;
+}
+
+// The same for positions that may lie anywhere (not known to be in [0, L]):
+// when one fold leaves |d| > L/2 (a difference of 1.5 periods or more), the
+// remainder is folded by rint.  With inbox (wave-uniform) the check is not
+// made: one fold is exact for positions in [0, L].
+__device__ __forceinline__ float min_image_any(float xj, float xi, float xim, float L,
+                                               float half, int inbox) {
+#pragma clang fp contract(off)
+  float d = min_image_disp(xj, xi, xim, L, half);
+  if (!inbox && !(fabsf(d) <= half)) d = d - L * rintf(d / L);
+  return d;
 }
 
 // Minimum image of a float64 difference (the x64 form of the periodic

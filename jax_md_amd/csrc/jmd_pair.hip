@@ -74,8 +74,9 @@ __device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned 
     float d[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int c = 0; c < DIM; ++c)
-      d[c] = (INTERIOR || !s.periodic) ? xj[c] - xi[c]
-                                       : min_image_disp(xj[c], xi[c], xim[c], s.L[c], s.half[c]);
+      d[c] = (INTERIOR || !s.periodic)
+                 ? xj[c] - xi[c]
+                 : min_image_any(xj[c], xi[c], xim[c], s.L[c], s.half[c], s.inbox);
     if (BOX) box_apply<DIM>(s, d);  // periodic_general: fractional fold, then T
     float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
                         : fmaf(d[1], d[1], d[0] * d[0]);
@@ -184,8 +185,10 @@ __global__ __launch_bounds__(256) void k_pair_dense(
       interior = interior && xi[c] >= w && xi[c] <= 1.0f - w;
     }
   }
-  // (n < 2^27 keeps the interior path's buffer offsets j * 16 in range)
-  const bool wave_interior = __all(interior || !active) && n < (1 << 27);
+  // (n < 2^27 keeps the interior path's buffer offsets j * 16 in range;
+  // the fold-free path is exact only when every neighbour lies in the box)
+  const bool wave_interior =
+      s.inbox && __all(interior || !active) && n < (1 << 27);
   if (active && s.general) {
     const int* row = idx + (long long)i * width;
 #define JMD_ROWG(INT, SM)                                                                      \
@@ -302,7 +305,8 @@ __global__ __launch_bounds__(256) void k_pair_sparse(
         load_pos<DIM, STRIDE>(pos, sb, xb);
 #pragma unroll
         for (int c = 0; c < DIM; ++c)
-          d[c] = s.periodic ? min_image_disp(xa[c], xb[c], xb[c] - s.L[c], s.L[c], s.half[c])
+          d[c] = s.periodic ? min_image_any(xa[c], xb[c], xb[c] - s.L[c], s.L[c], s.half[c],
+                                            s.inbox)
                             : xa[c] - xb[c];
         if (s.general) box_apply<DIM>(s, d);
         float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
@@ -502,7 +506,7 @@ __global__ __launch_bounds__(256) void k_pair_virial(const float* __restrict__ p
     load_pos<DIM, STRIDE>(pos, jj, xj);
 #pragma unroll
     for (int c = 0; c < DIM; ++c)
-      d[c] = s.periodic ? min_image_disp(xj[c], xi[c], xi[c] - s.L[c], s.L[c], s.half[c])
+      d[c] = s.periodic ? min_image_any(xj[c], xi[c], xi[c] - s.L[c], s.L[c], s.half[c], s.inbox)
                         : xj[c] - xi[c];
     if (s.general) box_apply<DIM>(s, d);
     const float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))

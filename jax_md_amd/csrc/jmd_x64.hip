@@ -205,6 +205,8 @@ int jmd_pair_energy_force_f64(const double* position, int64_t n, int32_t dim, in
                               const int32_t* idx, int64_t idx_width, int32_t format,
                               double* force, double* energy_total, double* momentum,
                               double half_dt, void* workspace, void* stream) {
+  periodic &= ~JMD_POSITIONS_IN_BOX;  // the float64 fold (rint) takes any position
+  if (periodic < 0 || periodic > 2) return JMD_ERR_INVALID;
   if (periodic == 2) return JMD_ERR_UNSUPPORTED;  // periodic_general: float32 path only
   if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || idx_width < 0 ||
       idx_width > 0x7fffffff || (n > 0 && !position) || (idx_width > 0 && !idx) ||

@@ -76,6 +76,17 @@ def _side(shift: space_mod.Shift, dim):
   return [float(s)] * dim if s.ndim == 0 else [float(v) for v in s]
 
 
+def positions_in_box(shift: space_mod.Shift, energy, dim: int) -> bool:
+  """True when ``shift`` wraps into the energy's own box, so every position it
+  writes lies in that box (the force kernels may then skip folds)."""
+  if not shift.wraps:
+    return False
+  geo = energy.geometry
+  if not geo.periodic:
+    return False
+  return [float(v) for v in _side(shift, dim)] == [float(v) for v in geo.side_values(dim)]
+
+
 def velocity_verlet_native(energy, shift, state: NVEState, dt, neighbor, dt_device=None,
                            **kwargs) -> NVEState:
   """One step: p += dt/2 F; R = shift(R, dt p/m); F = F(R); p += dt/2 F
@@ -101,7 +112,8 @@ def velocity_verlet_native(energy, shift, state: NVEState, dt, neighbor, dt_devi
       nat.position_stride(x_new), nat.ptr(p_new), None, 0.0, None, nat.stream_handle(dev)),
       'jmd_nve_kick_drift')
   F_new = energy.force_and_kick(x_new, neighbor, p_new, half_dt=float(half),
-                                dt_device=dt_device, **kwargs)
+                                dt_device=dt_device, in_box=positions_in_box(shift, energy, dim),
+                                **kwargs)
   return NVEState(x_new, p_new, F_new, state.mass, x_hi_new)
 
 
@@ -206,13 +218,48 @@ def nve(energy_or_force_fn, shift_fn, dt=1e-3, **sim_kwargs):
   return init_fn, step_fn
 
 
-def _close_timing(e0):
-  """(e0, e1, e2, e3): e0..e1 brackets a kernel, e2..e3 (recorded right
-  after, nothing between) measures what an event pair itself adds."""
-  ev = [e0] + [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-  for e in ev[1:]:
+class StepEvents:
+  """Per-step timing of an eager block: HIP events around each step's list
+  update (with the half step when fused) and around its force launch, an
+  empty event pair after each (what a pair of markers itself adds), and the
+  list's build counter after each update (copied on the device, read after
+  the block), so that steps that rebuilt can be told from steps that did not.
+  Events are recorded on the current stream, which is the one the kernels
+  are launched on."""
+
+  def __init__(self):
+    self.update = []   # (start, end, mark_start, mark_end) per step
+    self.force = []
+    self._builds = []  # device int32 [1] per step
+
+  @staticmethod
+  def _ev():
+    e = torch.cuda.Event(enable_timing=True)
     e.record()
-  return tuple(ev)
+    return e
+
+  def start(self):
+    return self._ev()
+
+  def stop(self, kind, e0):
+    e1, m0, m1 = self._ev(), self._ev(), self._ev()
+    getattr(self, kind).append((e0, e1, m0, m1))
+
+  def builds_after_update(self, plan):
+    self._builds.append(plan.workspace.buf[16:20].view(torch.int32).clone())
+
+  @staticmethod
+  def _us(pairs):
+    return [1e3 * (a.elapsed_time(b) - c.elapsed_time(d)) for a, b, c, d in pairs]
+
+  def summary(self):
+    """Durations in us (marker cost removed); call after a synchronize."""
+    upd, frc = self._us(self.update), self._us(self.force)
+    out = {'force_us': frc, 'update_us': upd}
+    if self._builds:
+      b = [int(t.item()) for t in self._builds]
+      out['rebuilt'] = [b[0] > 0 if k == 0 else b[k] > b[k - 1] for k in range(len(b))]
+    return out
 
 
 class NVELoop:
@@ -251,9 +298,10 @@ class NVELoop:
     self._alt = None   # second position buffer and gate word of the fused step
     self._bufs = None
     self._bufs_key = None
-    self._graph = None
-    self._graph_key = None
+    self._graphs = {}  # block key -> (graph, the objects the key's addresses belong to)
     self._keep = None
+    # blocks run eagerly (host-launched), graphs recorded, blocks replayed
+    self.stats = {'eager': 0, 'recorded': 0, 'replayed': 0}
 
   def prime(self, state: NVEState, nbrs) -> None:
     """Sets the gate from the skin criterion of the current positions."""
@@ -278,11 +326,21 @@ class NVELoop:
 
   @staticmethod
   def _key(state, nbrs, n_steps, reorder):
+    """What a recorded block depends on: buffer addresses and shapes, the
+    list's plan (grid, capacities, workspace) and the block length."""
     def p(t):
       return (t.data_ptr(), tuple(t.shape)) if isinstance(t, torch.Tensor) else t
+    plan = nbrs.plan
     return (p(state.position), p(state.position_hi), p(state.momentum), p(state.force),
             p(state.mass), p(nbrs.idx), p(nbrs.reference_position), p(nbrs.error.code),
-            id(nbrs.plan), int(n_steps), bool(reorder))
+            id(plan), plan.workspace.ptr, int(nbrs.cell_list_capacity or 0),
+            int(nbrs.max_occupancy), int(n_steps), bool(reorder))
+
+  def _key_refs(self, state, nbrs):
+    """The objects whose addresses a recorded block holds: kept alive with
+    the graph so that no freed buffer's address can match its key again."""
+    return (state.position, state.position_hi, state.momentum, state.force, state.mass,
+            nbrs.idx, nbrs.reference_position, nbrs.error.code, nbrs.plan)
 
   def run(self, state: NVEState, nbrs, n_steps: int, events=None) -> NVEState:
     """Advances ``state`` and ``nbrs`` in place by ``n_steps``.  With
@@ -290,27 +348,51 @@ class NVELoop:
     cell order, so that a wavefront's neighbour gathers share cache lines;
     user order, and the list in user ids, are restored at the end (Dense rows
     hold the reference's neighbour sets and Sparse lists its pairs, in the
-    cell order of the block; OrderedSparse pairs keep receiver < sender).  ``events`` (a list) collects, per force kernel, a
-    (start, end) event pair around it and an empty pair recorded right after
-    (the markers' own cost); it forces the eager path."""
+    cell order of the block; OrderedSparse pairs keep receiver < sender).
+    ``events`` (a ``StepEvents``) times each step's list update and force
+    launch; it forces the eager path."""
     if state.position_hi is None:
       state.position_hi = state.position.to(torch.float64)
     reorder = self._use_reorder(state, nbrs)
     if events is not None or not self.graph:
       self._enqueue(state, nbrs, n_steps, reorder, events)
+      self.stats['eager'] += 1
       return state
     key = self._key(state, nbrs, n_steps, reorder)
-    if self._graph is not None and self._graph_key == key:
-      self._graph.replay()
+    g = self._graphs.get(key)
+    if g is not None:
+      g[0].replay()
+      self.stats['replayed'] += 1
       return state
-    self._graph, self._graph_key = None, None
     # eager first block: it also loads every kernel before the capture
     self._enqueue(state, nbrs, n_steps, reorder, None)
+    self.stats['eager'] += 1
+    self._record(state, nbrs, n_steps, reorder, key)
+    return state
+
+  def capture(self, state, nbrs, n_steps: int) -> None:
+    """Records the block graph for ``n_steps`` on these buffers without
+    running it (the state is not advanced), so that a later ``run`` of that
+    length is a replay.  Every kernel must have been loaded by an earlier
+    eager call (a ``run`` of any length does that)."""
+    if state.position_hi is None:
+      state.position_hi = state.position.to(torch.float64)
+    if not self.graph:
+      return
+    reorder = self._use_reorder(state, nbrs)
+    key = self._key(state, nbrs, n_steps, reorder)
+    if key not in self._graphs:
+      self._record(state, nbrs, n_steps, reorder, key)
+
+  def _record(self, state, nbrs, n_steps, reorder, key):
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
       self._enqueue(state, nbrs, n_steps, reorder, None)  # recorded, not run
-    self._graph, self._graph_key = g, key
-    return state
+    # the oldest graph goes first (each entry keeps its buffers alive)
+    if len(self._graphs) >= 4:
+      self._graphs.pop(next(iter(self._graphs)))
+    self._graphs[key] = (g, self._key_refs(state, nbrs))
+    self.stats['recorded'] += 1
 
   def _enqueue(self, state, nbrs, n_steps, reorder, events):
     if reorder:
@@ -348,6 +430,8 @@ class NVELoop:
     lib = nat.load()
     plan = nbrs.plan
     spec = plan.spec
+    if spec.format_code != nat.DENSE:
+      raise NotImplementedError('float64 states run over Dense neighbour lists')
     R = state.position
     N, dim = R.shape
     dev = R.device
@@ -359,11 +443,16 @@ class NVELoop:
     dt32 = float(self.dt)
     half = float(self.half)
     for k in range(n_steps):
+      if events is not None:
+        e0 = events.start()
       nat.check(lib.jmd_neighbor_list_build(
           nat.ptr(R), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq),
           int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0),
           spec.format_code, int(spec.mask_self), idx, int(nbrs.max_occupancy), occ, err, ref,
           gate, ws, st), 'jmd_neighbor_list_build')
+      if events is not None:
+        events.stop('update', e0)
+        events.builds_after_update(plan)
       if hook is not None:
         hook(k)
       nat.check(lib.jmd_nve_kick_drift_f64(
@@ -371,14 +460,13 @@ class NVELoop:
           nat.ptr(mass_v), mass_s, N, dim, wraps, side, dt32, half, nat.ptr(R), stride, ref,
           thr, gate, st), 'jmd_nve_kick_drift_f64')
       if events is not None:
-        e0 = torch.cuda.Event(enable_timing=True)
-        e0.record()
+        e0 = events.start()
       nat.check(lib.jmd_pair_energy_force_f64(
           nat.ptr(state.position_hi), N, dim, int(geo.periodic), dsides, nat.ctypes.byref(pot),
-          idx, width, nat.DENSE, nat.ptr(state.force), None, nat.ptr(state.momentum), half,
+          idx, width, spec.format_code, nat.ptr(state.force), None, nat.ptr(state.momentum), half,
           None, st), 'jmd_pair_energy_force_f64')
       if events is not None:
-        events.append(_close_timing(e0))
+        events.stop('force', e0)
     return state
 
   def _alternates(self, R, gate):
@@ -476,6 +564,7 @@ class NVELoop:
     width = nbrs.idx.shape[1]
     geo = self.energy.geometry
     esides = nat.f3(geo.side_values(dim))
+    gcode = self.energy.geometry_code(positions_in_box(self.shift, self.energy, dim))
     if state.x64:
       return self._run_steps64(state, nbrs, n_steps, events, hook, gate, thr, cps, cs, psides,
                                side, wraps, pot, idx, width, err, ref, occ, ws)
@@ -484,6 +573,8 @@ class NVELoop:
       xs, gs = self._alternates(R, plan.gate)
       for k in range(n_steps):
         xin, xout = xs[k % 2], xs[(k + 1) % 2]
+        if events is not None:
+          e0 = events.start()
         nat.check(lib.jmd_nve_list_update_kick_drift(
             nat.ptr(xin), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq), 1,
             cps, cs, int(nbrs.cell_list_capacity or 0), fmt, int(spec.mask_self), idx,
@@ -492,24 +583,30 @@ class NVELoop:
             None, nat.ptr(xout), thr, nat.ptr(gs[(k + 1) % 2]), st),
             'jmd_nve_list_update_kick_drift')
         if events is not None:
-          e0 = torch.cuda.Event(enable_timing=True)
-          e0.record()
+          events.stop('update', e0)
+          events.builds_after_update(plan)
+          e0 = events.start()
         nat.check(lib.jmd_pair_energy_force(
-            nat.ptr(xout), N, dim, stride, int(geo.periodic), esides, nat.ctypes.byref(pot), idx,
+            nat.ptr(xout), N, dim, stride, gcode, esides, nat.ctypes.byref(pot), idx,
             width, fmt, F, None, None, p, float(self.half), None, pws, st),
             'jmd_pair_energy_force')
         if events is not None:
-          events.append(_close_timing(e0))
+          events.stop('force', e0)
       if n_steps % 2:  # the state's buffer and gate word hold step n's values
         R.copy_(xs[1])
         gs[0].copy_(gs[1])
         gs[1].zero_()
       return state
     for k in range(n_steps):
+      if events is not None:
+        e0 = events.start()
       nat.check(lib.jmd_neighbor_list_build(
           x, N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq), int(plan.use_cell_list),
           cps, cs, int(nbrs.cell_list_capacity or 0), fmt, int(spec.mask_self), idx,
           int(nbrs.max_occupancy), occ, err, ref, gate, ws, st), 'jmd_neighbor_list_build')
+      if events is not None:
+        events.stop('update', e0)
+        events.builds_after_update(plan)
       if hook is not None:
         hook(k)
       nat.check(lib.jmd_nve_kick_drift(
@@ -517,13 +614,12 @@ class NVELoop:
           float(self.dt), float(self.half), None, x_hi, x, stride, p, ref, thr, gate, st),
           'jmd_nve_kick_drift')
       if events is not None:
-        e0 = torch.cuda.Event(enable_timing=True)
-        e0.record()
+        e0 = events.start()
       nat.check(lib.jmd_pair_energy_force(
-          x, N, dim, stride, int(geo.periodic), esides, nat.ctypes.byref(pot), idx, width, fmt, F,
+          x, N, dim, stride, gcode, esides, nat.ctypes.byref(pot), idx, width, fmt, F,
           None, None, p, float(self.half), None, pws, st), 'jmd_pair_energy_force')
       if events is not None:
-        events.append(_close_timing(e0))
+        events.stop('force', e0)
     return state
 
 
@@ -681,7 +777,7 @@ def nvt_nose_hoover(energy_or_force_fn, shift_fn, dt, kT, chain_length=5, chain_
         mass_s if mass_v is None else 0.0, N, dim, int(shift.wraps), nat.f3(_side(shift, dim)),
         float(th.dt), float(half), nat.ptr(x_hi_new), nat.ptr(x_new), nat.position_stride(x_new),
         nat.ptr(new.momentum), None, 0.0, None, nat.ptr(scale), st), 'jmd_nvt_kick_drift')
-    F = energy.force(x_new, neighbor, **kwargs)
+    F = energy.force(x_new, neighbor, in_box=positions_in_box(shift, energy, dim), **kwargs)
     new = new.set(position=x_new, force=F, position_hi=x_hi_new)
     th.kinetic(new, F, half)
     scale = th.step(_NHC_FINISH, new, N, dim, kT=_kT)
@@ -731,9 +827,16 @@ class NVTLoop(NVELoop):
     self._nvt_state = state
     return super().run(state, nbrs, n_steps, events)
 
+  def capture(self, state: NVTNoseHooverState, nbrs, n_steps: int) -> None:
+    self._nvt_state = state
+    super().capture(state, nbrs, n_steps)
+
   def _key(self, state, nbrs, n_steps, reorder):
     return super()._key(state, nbrs, n_steps, reorder) + (state.chain.data_ptr(),
                                                           state.kinetic_energy.data_ptr())
+
+  def _key_refs(self, state, nbrs):
+    return super()._key_refs(state, nbrs) + (state.chain, state.kinetic_energy)
 
   def _run_steps(self, state, nbrs, n_steps: int, events=None, hook=None):
     if hook is not None:
@@ -765,17 +868,23 @@ class NVTLoop(NVELoop):
     width = nbrs.idx.shape[1]
     geo = self.energy.geometry
     esides = nat.f3(geo.side_values(dim))
+    gcode = self.energy.geometry_code(positions_in_box(self.shift, self.energy, dim))
     b = th.buffers(N, dev)
     kin = dataclasses.replace(chain_state, momentum=state.momentum, mass=state.mass)
     if n_steps <= 0:
       return state
     th.step(_NHC_START, kin, N, dim, st=st)
     for k in range(n_steps):
+      if events is not None:
+        e0 = events.start()
       nat.check(lib.jmd_neighbor_list_build(
           nat.ptr(R), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq),
           int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0), fmt,
           int(spec.mask_self), idx, int(nbrs.max_occupancy), occ, err, ref, gate, ws, st),
           'jmd_neighbor_list_build')
+      if events is not None:
+        events.stop('update', e0)
+        events.builds_after_update(plan)
       nat.check(lib.jmd_nvt_kick_drift(
           nat.ptr(state.position_hi), nat.ptr(state.momentum), nat.ptr(state.force),
           nat.ptr(mass_v), mass_s if mass_v is None else 0.0, N, dim, int(self.shift.wraps),
@@ -783,14 +892,13 @@ class NVTLoop(NVELoop):
           nat.ptr(state.momentum), ref, thr, gate, nat.ptr(b['scale']), st),
           'jmd_nvt_kick_drift')
       if events is not None:
-        e0 = torch.cuda.Event(enable_timing=True)
-        e0.record()
+        e0 = events.start()
       nat.check(lib.jmd_pair_energy_force(
-          nat.ptr(R), N, dim, stride, int(geo.periodic), esides, nat.ctypes.byref(pot), idx, width,
+          nat.ptr(R), N, dim, stride, gcode, esides, nat.ctypes.byref(pot), idx, width,
           fmt, nat.ptr(state.force), None, None, None, 0.0, None, pws, st),
           'jmd_pair_energy_force')
       if events is not None:
-        events.append(_close_timing(e0))
+        events.stop('force', e0)
       # closing kick, kinetic energy and the chain in one launch
       th.kinetic_step(_NHC_FINISH | _NHC_START if k + 1 < n_steps else _NHC_FINISH, kin,
                       state.force, self.half, st=st)

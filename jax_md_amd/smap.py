@@ -193,8 +193,14 @@ class PairEnergy:
       self._ws[key] = ws
     return ws
 
+  def geometry_code(self, in_box: bool = False) -> int:
+    """The C ABI's ``periodic`` argument; ``in_box`` asserts that every
+    position lies in the box (written by a wrapping shift)."""
+    code = int(self.geometry.periodic)
+    return code | nat.POSITIONS_IN_BOX if (in_box and code) else code
+
   def _run(self, R, neighbor, want_force, want_energy_total, want_energy_atom,
-           dynamic_kwargs, momentum=None, half_dt=0.0, dt_device=None):
+           dynamic_kwargs, momentum=None, half_dt=0.0, dt_device=None, in_box=False):
     if not isinstance(neighbor, partition.NeighborList):
       raise TypeError('energy_fn needs a partition.NeighborList as `neighbor`')
     if isinstance(R, torch.Tensor) and R.dtype == torch.float64:
@@ -216,7 +222,7 @@ class PairEnergy:
     width = idx.shape[1]
     ws = self._workspace(dev, N)
     nat.check(lib.jmd_pair_energy_force(
-        nat.ptr(R), N, dim, stride, int(geo.periodic), nat.f3(side), ctypes.byref(pot),
+        nat.ptr(R), N, dim, stride, self.geometry_code(in_box), nat.f3(side), ctypes.byref(pot),
         nat.ptr(idx),
         width, _FMT[neighbor.format], nat.ptr(F), nat.ptr(e_atom), nat.ptr(e_tot),
         nat.ptr(momentum), float(half_dt), nat.ptr(dt_device), nat.ptr(ws),
@@ -255,9 +261,10 @@ class PairEnergy:
     e_tot, _, F = self._run(R, neighbor, True, True, False, dynamic_kwargs)
     return e_tot.reshape(()), F
 
-  def force(self, R, neighbor=None, **dynamic_kwargs):
-    """-dE/dR from the native kernel (quantity.force routes here)."""
-    _, _, F = self._run(R, neighbor, True, False, False, dynamic_kwargs)
+  def force(self, R, neighbor=None, in_box=False, **dynamic_kwargs):
+    """-dE/dR from the native kernel (quantity.force routes here).
+    ``in_box``: the caller guarantees every position lies in the box."""
+    _, _, F = self._run(R, neighbor, True, False, False, dynamic_kwargs, in_box=in_box)
     return F
 
   def virial(self, R, neighbor=None, **dynamic_kwargs):
@@ -285,10 +292,11 @@ class PairEnergy:
     del keep
     return W
 
-  def force_and_kick(self, R, neighbor, momentum, half_dt=0.0, dt_device=None, **dynamic_kwargs):
+  def force_and_kick(self, R, neighbor, momentum, half_dt=0.0, dt_device=None, in_box=False,
+                     **dynamic_kwargs):
     """Force, plus the closing half kick p += half_dt F fused in the same pass."""
     _, _, F = self._run(R, neighbor, True, False, False, dynamic_kwargs, momentum=momentum,
-                        half_dt=half_dt, dt_device=dt_device)
+                        half_dt=half_dt, dt_device=dt_device, in_box=in_box)
     return F
 
 

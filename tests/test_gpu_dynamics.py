@@ -336,3 +336,139 @@ def test_nve_loop_graph_replays_match_eager(reorder):
       assert torch.equal(a[k], b[k]), (reorder, blk, k)
     assert a[5] == b[5], (reorder, blk)
   assert runs[1][-1][5] > runs[1][0][5]  # rebuilds happened inside the replays
+
+
+def test_fire_bookkeeping_hand_table_on_device():
+  # The float16 dt / alpha / n_pos bookkeeping of jmd_fire_update against the
+  # hand-derived table of the oracle's pin (tests/test_oracle.py), driven
+  # with the scenario's forces and the momenta of its NVE steps.
+  from tests.test_oracle import _FIRE_TABLE, fire_scheduled_force
+  jmd = _jmd()
+  nat = jmd._native
+  lib = nat.load()
+  cast = jmd.util.static_cast(0.1, 0.4, 5, 1.1, 0.5, 0.1, 0.99)
+  params = (nat.ctypes.c_float * 7)(*[float(np.float32(c)) for c in cast])
+  scal = torch.tensor([float(np.float32(cast[0])), float(np.float32(cast[5]))],
+                      dtype=torch.float32, device=DEV)
+  npos = torch.zeros((1,), dtype=torch.int32, device=DEV)
+  ws = torch.empty((lib.jmd_fire_workspace_bytes(1),), dtype=torch.uint8, device=DEV)
+  sums = torch.empty((8,), dtype=torch.float64, device=DEV)
+  P = np.zeros((1, 3), np.float32)
+  F_old = fire_scheduled_force(0)
+  for step, n_pos, dt_bits, alpha_bits in _FIRE_TABLE:
+    dt = np.float32(scal[0].item())
+    F_new = fire_scheduled_force(step)
+    half = np.float32(dt / np.float32(2))
+    P[0, 0] = np.float32(np.float32(P[0, 0] + half * np.float32(F_old)) + half * np.float32(F_new))
+    Pt = torch.as_tensor(P, device=DEV).contiguous()
+    Ft = torch.zeros((1, 3), dtype=torch.float32, device=DEV)
+    Ft[0, 0] = F_new
+    nat.check(lib.jmd_fire_update(nat.ptr(Pt), nat.ptr(Ft), 1, 3, nat.ptr(scal), nat.ptr(npos),
+                                  params, nat.ptr(sums), nat.ptr(ws), nat.stream_handle(DEV)),
+              'jmd_fire_update')
+    got = (int(npos.item()), int(np.float16(scal[0].item()).view(np.uint16)),
+           int(np.float16(scal[1].item()).view(np.uint16)))
+    assert got == (n_pos, dt_bits, alpha_bits), (step, got)
+    P = Pt.cpu().numpy()
+    if step == 31:
+      assert np.all(P == 0.0)
+    F_old = F_new
+
+
+def test_skin_threshold_exact_cases_on_device():
+  # the gate of jmd_neighbor_list_check at (dr/2)^2 exactly and one float32
+  # step beyond it, including across a periodic face (cases of the oracle's
+  # pin in tests/test_oracle.py, decided by hand)
+  jmd = _jmd()
+  L = F32(16.0)
+  disp, _ = jmd.space.periodic(L)
+  nfn = jmd.partition.neighbor_list(disp, L, 2.5, 0.5, format=jmd.partition.Dense)
+  base = np.array([[15.875, 4.0, 4.0], [8.0, 8.0, 8.0], [1.0, 12.0, 6.0]], F32)
+  nb = nfn.allocate(torch.as_tensor(base, device=DEV))
+
+  def rebuilt(pos):
+    nonlocal nb
+    b0 = nb.plan.builds
+    nb = nb.update(torch.as_tensor(pos, device=DEV))
+    hit = nb.plan.builds > b0
+    if hit:  # start the next case from the original reference again
+      nb = nfn.allocate(torch.as_tensor(base, device=DEV))
+    return hit
+
+  cases = []
+  p = base.copy(); p[1, 0] = F32(8.25); cases.append((p, False))
+  p = base.copy(); p[1, 0] = np.nextafter(F32(8.25), F32(9)); cases.append((p, True))
+  p = base.copy(); p[1, 1] = F32(7.75); cases.append((p, False))
+  p = base.copy(); p[1, 1] = np.nextafter(F32(7.75), F32(0)); cases.append((p, True))
+  p = base.copy(); p[0, 0] = F32(0.125); cases.append((p, False))
+  p = base.copy(); p[0, 0] = F32(0.125) + F32(2.0 ** -26); cases.append((p, False))
+  p = base.copy(); p[0, 0] = F32(0.125) + F32(2.0 ** -20); cases.append((p, True))
+  for k, (pos, want) in enumerate(cases):
+    print(f'case skin threshold {k}', flush=True)
+    assert rebuilt(pos) == want, k
+
+
+def test_skin_rule_keeps_every_pair_within_cutoff_on_device():
+  # after gated updates along a random walk, every pair within r_cutoff at the
+  # current positions is in the GPU list (brute force over all pairs)
+  jmd = _jmd()
+  rng = np.random.default_rng(31)
+  L = F32(9.0)
+  disp, _ = jmd.space.periodic(L)
+  nfn = jmd.partition.neighbor_list(disp, L, 2.0, 0.6, format=jmd.partition.Dense)
+  R = rng.uniform(0, L, (400, 3)).astype(F32)
+  nb = nfn.allocate(torch.as_tensor(R, device=DEV), extra_capacity=8)
+  osp = O.Space(L)
+  b0 = nb.plan.builds
+  for step in range(40):
+    R = np.mod(R + rng.normal(0, 0.04, R.shape), L).astype(F32)
+    nb = nb.update(torch.as_tensor(R, device=DEV))
+    assert not bool(nb.did_buffer_overflow)
+    idx = nb.idx.cpu().numpy()
+    for i in range(0, len(R), 7):
+      d2 = osp.metric_sq(R, np.broadcast_to(R[i], R.shape))
+      want = set(np.flatnonzero(d2 < F32(4.0)).tolist()) - {i}
+      have = set(idx[i][idx[i] < len(R)].tolist())
+      assert want <= have, (step, i, sorted(want - have))
+  assert 0 < nb.plan.builds - b0 < 40
+
+
+@pytest.mark.parametrize('wrapped', [True, False])
+def test_nve_unwrapped_positions_vs_fp64(wrapped):
+  # space.periodic(side, wrapped=False) leaves positions unwrapped
+  # (space.py:287-301): forces and trajectory must not depend on it.
+  print(f'case NVE wrapped={wrapped}', flush=True)
+  jmd = _jmd()
+  R, P, L = O.lj_lattice(6, seed=8)
+  disp, shift = jmd.space.periodic(L, wrapped=wrapped)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense)
+  # start with some particles one and two periods away
+  rng = np.random.default_rng(9)
+  k = rng.integers(-2, 3, R.shape).astype(F32)
+  R0 = (R + k * F32(L)).astype(F32)
+  Rt = torch.as_tensor(R0, device=DEV)
+  nb = nfn.allocate(Rt)
+  init_fn, step_fn = jmd.simulate.nve(efn, shift, 0.005)
+  st = init_fn(0, Rt, kT=1.0, momenta=torch.as_tensor(P, device=DEV), neighbor=nb)
+  sp = O.Space(np.float64(L))
+  model = O.PairModel('lj')
+  _, _, F64 = O.energy_force_brute(model, sp, R0.astype(np.float64))
+  Fg = st.force.cpu().numpy().astype(np.float64)
+  assert np.linalg.norm(Fg - F64) / np.linalg.norm(F64) < 1e-5
+  for _ in range(30):
+    nb = nb.update(st.position)
+    st = step_fn(st, neighbor=nb)
+  X = st.position.cpu().numpy().astype(np.float64)
+  if not wrapped:
+    assert np.any(X < 0) or np.any(X > L)  # the shift did not wrap
+  _, _, F64 = O.energy_force_brute(model, sp, X)
+  Fg = st.force.cpu().numpy().astype(np.float64)
+  assert np.linalg.norm(Fg - F64) / np.linalg.norm(F64) < 1e-5
+  # the same 30 steps in float64 from the same start
+  o = O.NVEState(R0.astype(np.float64), P.astype(np.float64),
+                 O.energy_force_brute(model, sp, R0.astype(np.float64))[2])
+  osp = O.Space(np.float64(L)) if wrapped else O.Space(None, False)
+  for _ in range(30):
+    o = O.nve_step(o, lambda X: O.energy_force_brute(model, sp, X)[2], osp, 0.005)
+  dx = sp.displacement(X, o.position)
+  assert np.max(np.abs(dx)) < 1e-4

@@ -23,7 +23,17 @@ Which check pins which oracle function:
   smooth                           S(r_on)=1, S(r_c)=0, C1 continuity
   nve_step                         energy conservation, time reversibility
   fire_step                        static_cast float16 values (F4), monotone
-                                   descent on a quadratic (minimize_test.py:111-150)
+                                   descent on a quadratic (minimize_test.py:111-150),
+                                   and step by step a hand-derived float16
+                                   dt / alpha / n_pos table for a scheduled force
+                                   (growth after n_pos > n_min, the dt_max cap, a
+                                   reset on F.P < 0, F.P == 0 twice)
+  needs_rebuild                    (dr/2)^2 exactly and one float32 step beyond,
+                                   across a periodic face (an offset the float32
+                                   wrap rounds away does not rebuild); every pair
+                                   within r_cutoff stays listed along a random
+                                   walk; an adversarial pair that a dr threshold
+                                   would miss
   force_vjp                        central differences of the force in R,
                                    sigma, epsilon; a dropped term fails
   fire_energy_gradient             central differences of fire_energy_after
@@ -775,3 +785,173 @@ def test_multi_image_ordered_holds_each_pair_once():
   for i, j, s in zip(b[0][:nb], b[1][:nb], sb[:nb]):
     assert (int(i), int(j), tuple(s)) in full
     assert (int(j), int(i), tuple(-s)) in full
+
+
+# ---------------------------------------------------------------------------
+# FIRE bookkeeping, step by step, against a hand-derived table
+# ---------------------------------------------------------------------------
+
+# One particle under a scheduled force, so that the sign of F.P at every step
+# is known without integrating: F = +1 along x except F = -1 at step 31 and
+# F = +1 again from step 32.  With P >= 0 before step 31 the NVE step gives
+# F.P > 0 at steps 1-30; at step 31 P = P_mix + dt/2 - dt/2 > 0 meets F = -1,
+# so F.P < 0 (reset); at step 32 P = 0 - dt/2 + dt/2 = 0 exactly, so F.P = 0
+# (n_pos counts it, dt and alpha hold: minimize.py:209-220 tests > 0 and < 0
+# for dt / alpha and >= 0 for n_pos and the momentum mask); from step 33
+# F.P > 0 again, except at step 41 where F = 0 (F.P = 0 with n_pos > n_min
+# and alpha below alpha_start: nothing but n_pos moves).  With the defaults (dt_start 0.1, dt_max 0.4, n_min 5,
+# f_inc 1.1, f_dec 0.5, alpha_start 0.1, f_alpha 0.99, all float16 after
+# static_cast, Bitzek et al. 2006): dt and alpha hold while n_pos <= 5, then
+# each step dt = min(f16(dt * 1.1), f16(0.4)) and alpha = f16(alpha * 0.99)
+# (products rounded to nearest even in float16: 0x2e66 * 0x3c66 = 0.109924...
+# -> 0x2f09); dt reaches the cap f16(0.4) = 0x3666 at step 20; the reset
+# halves it exactly (0x3266) and restores alpha = f16(0.1) = 0x2e66.
+# Rows: (step, n_pos, dt bits, alpha bits), float16 bit patterns.
+_FIRE_TABLE = [
+    (1, 1, 0x2e66, 0x2e66), (2, 2, 0x2e66, 0x2e66), (3, 3, 0x2e66, 0x2e66),
+    (4, 4, 0x2e66, 0x2e66), (5, 5, 0x2e66, 0x2e66), (6, 6, 0x2f09, 0x2e56),
+    (7, 7, 0x2fbc, 0x2e46), (8, 8, 0x3041, 0x2e36), (9, 9, 0x30ad, 0x2e26),
+    (10, 10, 0x3124, 0x2e17), (11, 11, 0x31a7, 0x2e08), (12, 12, 0x3237, 0x2df9),
+    (13, 13, 0x32d5, 0x2dea), (14, 14, 0x3383, 0x2ddb), (15, 15, 0x3421, 0x2dcc),
+    (16, 16, 0x348a, 0x2dbe), (17, 17, 0x34fe, 0x2db0), (18, 18, 0x357d, 0x2da2),
+    (19, 19, 0x3609, 0x2d94), (20, 20, 0x3666, 0x2d86), (21, 21, 0x3666, 0x2d78),
+    (22, 22, 0x3666, 0x2d6a), (23, 23, 0x3666, 0x2d5c), (24, 24, 0x3666, 0x2d4f),
+    (25, 25, 0x3666, 0x2d42), (26, 26, 0x3666, 0x2d35), (27, 27, 0x3666, 0x2d28),
+    (28, 28, 0x3666, 0x2d1b), (29, 29, 0x3666, 0x2d0e), (30, 30, 0x3666, 0x2d01),
+    (31, 0, 0x3266, 0x2e66), (32, 1, 0x3266, 0x2e66), (33, 2, 0x3266, 0x2e66),
+    (34, 3, 0x3266, 0x2e66), (35, 4, 0x3266, 0x2e66), (36, 5, 0x3266, 0x2e66),
+    (37, 6, 0x3309, 0x2e56), (38, 7, 0x33bc, 0x2e46), (39, 8, 0x3441, 0x2e36),
+    (40, 9, 0x34ad, 0x2e26), (41, 10, 0x34ad, 0x2e26), (42, 11, 0x3524, 0x2e17),
+]
+
+
+def fire_scheduled_force(step):
+  """Force of the table's scenario at a given force evaluation (0 = init)."""
+  return {31: -1.0, 41: 0.0}.get(step, 1.0)
+
+
+def _f16_bits(x):
+  return int(np.asarray(x, np.float16).view(np.uint16))
+
+
+def test_fire_bookkeeping_hand_table():
+  calls = [0]
+
+  def force_fn(R):
+    f = np.zeros_like(R)
+    f[0, 0] = fire_scheduled_force(calls[0])
+    calls[0] += 1
+    return f
+
+  sp = O.Space(None, False)
+  params = O.FireParams()
+  st = O.fire_init(np.zeros((1, 3)), force_fn, params)
+  for step, n_pos, dt_bits, alpha_bits in _FIRE_TABLE:
+    st = O.fire_step(st, force_fn, sp, params)
+    got = (int(st.n_pos), _f16_bits(st.dt), _f16_bits(st.alpha))
+    assert got == (n_pos, dt_bits, alpha_bits), (step, got)
+    if step == 31:
+      assert np.all(st.momentum == 0.0), 'momentum not reset when F.P < 0'
+    if step == 32:
+      # F.P == 0: the mask F.P >= 0 keeps P (which is exactly zero here)
+      assert np.all(st.momentum == 0.0)
+    if step in (30, 33):
+      assert st.momentum[0, 0] > 0.0
+  # structural facts of the table, independent of its bit patterns
+  assert [r[1] for r in _FIRE_TABLE[:5]] == [1, 2, 3, 4, 5]
+  assert _FIRE_TABLE[4][2] == _FIRE_TABLE[0][2] != _FIRE_TABLE[5][2]   # growth from n_pos 6
+  assert _FIRE_TABLE[19][2] == _f16_bits(np.float16(0.4))               # the dt_max cap
+  assert _FIRE_TABLE[30][2] == _f16_bits(np.float16(np.float16(0.4) * np.float16(0.5)))
+
+
+# ---------------------------------------------------------------------------
+# The skin rule any(metric_sq(R, R_ref) > (dr/2)^2) at its threshold
+# ---------------------------------------------------------------------------
+
+
+def test_skin_threshold_exact_cases():
+  # partition.py:901,1141-1154: strict '>' against float32 (dr/2)^2
+  F = np.float32
+  spec = O.NeighborListSpec(O.Space(None, False), F(100.0), 2.5, 0.5)
+  assert spec.threshold_sq == F(0.0625)
+  ref = np.array([[1.0, 2.0, 3.0], [10.0, 10.0, 10.0]], F)
+  at = ref.copy()
+  at[0, 0] = F(1.25)                       # d = 0.25, d^2 = 0.0625 exactly: no rebuild
+  assert not O.needs_rebuild(spec, at, ref)
+  up = ref.copy()
+  up[0, 0] = np.nextafter(F(1.25), F(2))   # d = 0.25 + 2^-23, d^2 = 0.0625 + 2^-24
+  assert O.needs_rebuild(spec, up, ref)
+  down = ref.copy()
+  down[0, 0] = np.nextafter(F(1.25), F(0))
+  assert not O.needs_rebuild(spec, down, ref)
+  neg = ref.copy()
+  neg[1, 2] = F(9.75)                      # d = -0.25 on another particle: no rebuild
+  assert not O.needs_rebuild(spec, neg, ref)
+  neg[1, 2] = np.nextafter(F(9.75), F(0))
+  assert O.needs_rebuild(spec, neg, ref)
+  # periodic, across the face (L = 16): the float32 difference 0.125 - 15.875
+  # = -15.75 wraps to 0.25 exactly
+  pspec = O.NeighborListSpec(O.Space(F(16.0)), F(16.0), 2.5, 0.5)
+  pref = np.array([[15.875, 4.0, 4.0]], F)
+  pos = np.array([[0.125, 4.0, 4.0]], F)
+  assert not O.needs_rebuild(pspec, pos, pref)
+  # one ulp of 0.125 is lost in the float32 difference (ulp(15.75) = 2^-20):
+  # the reference's float32 wrap does not see it
+  pos[0, 0] = F(0.125) + F(2.0 ** -26)
+  assert not O.needs_rebuild(pspec, pos, pref)
+  pos[0, 0] = F(0.125) + F(2.0 ** -20)     # d = 0.25 + 2^-20 after the wrap
+  assert O.needs_rebuild(pspec, pos, pref)
+  # dr = 0: every update rebuilds (partition.py:892,1156-1159)
+  zspec = O.NeighborListSpec(O.Space(None, False), F(100.0), 2.5, 0.0)
+  assert O.needs_rebuild(zspec, ref, ref)
+
+
+def _pairs_within(space, R, rc):
+  d2 = np.stack([space.metric_sq(R, np.broadcast_to(R[j], R.shape)) for j in range(len(R))], 1)
+  i, j = np.nonzero(d2 < np.float32(rc) ** 2)
+  keep = i != j
+  return set(zip(i[keep].tolist(), j[keep].tolist()))
+
+
+def test_skin_rule_keeps_every_pair_within_cutoff():
+  # The guarantee the skin rule exists for: after any sequence of gated
+  # updates, every pair within r_cutoff at the current positions is in the
+  # list (no particle has moved dr/2 from where the list was built).
+  rng = np.random.default_rng(31)
+  L = np.float32(9.0)
+  space = O.Space(L)
+  spec = O.NeighborListSpec(space, L, 2.0, 0.6)
+  R = rng.uniform(0, L, (160, 3)).astype(np.float32)
+  nl = O.neighbor_list_build(spec, R)
+  rebuilds = 0
+  for step in range(40):
+    R = np.mod(R + rng.normal(0, 0.04, R.shape), L).astype(np.float32)
+    new = O.neighbor_list_update(spec, R, nl)
+    rebuilds += new is not nl
+    nl = new
+    have = {(i, int(j)) for i, row in enumerate(nl.idx) for j in row if j < len(R)}
+    missing = _pairs_within(space, R, 2.0) - have
+    assert not missing, (step, sorted(missing)[:5])
+  assert 0 < rebuilds < 40
+
+
+def test_skin_rule_adversarial_pair():
+  # Two particles just beyond the list cutoff c = r_c + dr move towards each
+  # other by just under dr/2 each: no rebuild, and they end up just beyond
+  # r_c.  A threshold of dr instead of dr/2 would let them move dr each
+  # (also no rebuild) and end inside r_c, missing from the list.
+  F = np.float32
+  L = F(20.0)
+  space = O.Space(L)
+  spec = O.NeighborListSpec(space, L, 2.0, 0.5)
+  R = np.array([[5.0, 5.0, 5.0], [F(5.0) + F(2.5) + F(0.001), 5.0, 5.0]], F)
+  nl = O.neighbor_list_build(spec, R)
+  assert np.all(nl.idx >= 2)                # not neighbours: beyond c = 2.5
+  R2 = R.copy()
+  R2[0, 0] += F(0.249)
+  R2[1, 0] -= F(0.249)
+  assert not O.needs_rebuild(spec, R2, nl.reference_position)
+  assert space.metric_sq(R2[:1], R2[1:])[0] > F(4.0)   # still beyond r_c = 2
+  R3 = R.copy()
+  R3[0, 0] += F(0.26)                       # over dr/2: rebuilds
+  assert O.needs_rebuild(spec, R3, nl.reference_position)
