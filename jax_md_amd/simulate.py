@@ -333,7 +333,7 @@ class NVELoop:
     plan = nbrs.plan
     return (p(state.position), p(state.position_hi), p(state.momentum), p(state.force),
             p(state.mass), p(nbrs.idx), p(nbrs.reference_position), p(nbrs.error.code),
-            id(plan), plan.workspace.ptr, int(nbrs.cell_list_capacity or 0),
+            id(plan), plan.workspace.buf.data_ptr(), int(nbrs.cell_list_capacity or 0),
             int(nbrs.max_occupancy), int(n_steps), bool(reorder))
 
   def _key_refs(self, state, nbrs):
