@@ -247,6 +247,27 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
                           float half_dt, const float* dt_device,
                           void* workspace, void* stream);
 
+/* jmd_pair_energy_force without a workspace argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Arguments and results are those of jmd_pair_energy_force; the
+ * float64 energy partials live in a fixed 1 MiB device array of the library
+ * (one per GPU, held until the process ends), so n may not exceed
+ * JMD_PAIR_NOALLOC_MAX_N: a larger n returns JMD_ERR_INVALID before anything
+ * is enqueued.  The call allocates nothing (no hipMalloc, no library
+ * allocation).  All calls on one GPU share that array: calls that overlap
+ * on one GPU (different streams or threads without ordering) race. */
+#define JMD_PAIR_NOALLOC_MAX_N 2000000
+int jmd_pair_energy_force_noalloc(const float* position, int64_t n, int32_t dim,
+                                  int32_t position_stride, int32_t periodic,
+                                  const float* side,
+                                  const jmd_pair_potential* potential,
+                                  const int32_t* idx, int64_t idx_width,
+                                  int32_t format, float* force, float* energy_atom,
+                                  double* energy_total, float* momentum,
+                                  float half_dt, const float* dt_device,
+                                  void* stream);
+
 /* The virial W_ab = dU/d eps_ab of the pair energy with every displacement
  * perturbed as (I + eps) d, i.e. weight * sum over the list's ordered pairs
  * of (U'(r)/r) d_a d_b (weight 1/2 for Dense and Sparse, 1 for

@@ -18,6 +18,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get('JAX_MD_AMD_LIB', os.path.join(_HERE, 'libjaxmd_native.so'))
 
 JMD_OK, JMD_ERR_INVALID, JMD_ERR_LAUNCH, JMD_ERR_UNSUPPORTED = 0, 1, 2, 3
+# largest n jmd_pair_energy_force_noalloc accepts (JMD_PAIR_NOALLOC_MAX_N)
+PAIR_NOALLOC_MAX_N = 2000000
 DENSE, SPARSE, ORDERED_SPARSE = 0, 1, 2
 # ORed into a geometry code: every position lies in the box (positions
 # written by a wrapping shift), which lets the force kernels skip folds
@@ -66,6 +68,11 @@ _SIGNATURES = {
                                         POINTER(c_float), POINTER(JmdPairPotential), c_void_p,
                                         c_int64, c_int32, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_float, c_void_p, c_void_p, c_void_p]),
+    'jmd_pair_energy_force_noalloc': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
+                                                POINTER(c_float), POINTER(JmdPairPotential),
+                                                c_void_p, c_int64, c_int32, c_void_p, c_void_p,
+                                                c_void_p, c_void_p, c_float, c_void_p,
+                                                c_void_p]),
     'jmd_nve_kick_drift': (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int64,
                                      c_int32, c_int32, POINTER(c_float), c_float, c_float,
                                      c_void_p, c_void_p, c_void_p, c_int32, c_void_p,

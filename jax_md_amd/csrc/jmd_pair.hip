@@ -16,6 +16,8 @@
 //   S = (rc^2 - r^2)^2 (rc^2 + 2 r^2 - 3 ro^2) / (rc^2 - ro^2)^3,
 //   S'/r = 12 (rc^2 - r^2)(ro^2 - r^2) / (rc^2 - ro^2)^3   on [r_on, r_c),
 //   F_i += (S V'/r + S'/r V) (R_j - R_i).
+#include <mutex>
+
 #include "jmd_pair_common.hpp"
 
 namespace jmd {
@@ -616,12 +618,15 @@ int jmd_set_threads_per_atom(int32_t tpa) {
   return JMD_OK;
 }
 
-int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
-                          int32_t position_stride, int32_t periodic, const float* side,
-                          const jmd_pair_potential* potential, const int32_t* idx,
-                          int64_t idx_width, int32_t format, float* force, float* energy_atom,
-                          double* energy_total, float* momentum, float half_dt,
-                          const float* dt_device, void* workspace, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int pair_energy_force(const float* position, int64_t n, int32_t dim, int32_t position_stride,
+                      int32_t periodic, const float* side, const jmd_pair_potential* potential,
+                      const int32_t* idx, int64_t idx_width, int32_t format, float* force,
+                      float* energy_atom, double* energy_total, float* momentum, float half_dt,
+                      const float* dt_device, void* workspace, void* stream) {
   if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || idx_width < 0 ||
       (n > 0 && !position) || (idx_width > 0 && !idx) || (periodic && !side))
     return JMD_ERR_INVALID;
@@ -729,6 +734,72 @@ int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
     JMD_LAUNCH_CHECK();
   }
   return JMD_OK;
+}
+
+}  // namespace
+
+// Scratch of jmd_pair_energy_force_noalloc: the float64 block partials of
+// the energy sum, jmd_pair_workspace_bytes(n) bytes for n up to
+// JMD_PAIR_NOALLOC_MAX_N.  Each device holds its own copy for the life of
+// the process; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device.
+constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;
+static_assert((size_t)((JMD_PAIR_NOALLOC_MAX_N * 16LL + 255) / 256 + 64) * 8 + 256 <=
+                  kNoAllocScratchBytes,
+              "scratch too small for JMD_PAIR_NOALLOC_MAX_N");
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int scratch_address(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices)
+    return JMD_ERR_LAUNCH;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return JMD_ERR_LAUNCH;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return JMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jmd_pair_energy_force(const float* position, int64_t n, int32_t dim,
+                          int32_t position_stride, int32_t periodic, const float* side,
+                          const jmd_pair_potential* potential, const int32_t* idx,
+                          int64_t idx_width, int32_t format, float* force, float* energy_atom,
+                          double* energy_total, float* momentum, float half_dt,
+                          const float* dt_device, void* workspace, void* stream) {
+  return pair_energy_force(position, n, dim, position_stride, periodic, side, potential, idx,
+                           idx_width, format, force, energy_atom, energy_total, momentum,
+                           half_dt, dt_device, workspace, stream);
+}
+
+int jmd_pair_energy_force_noalloc(const float* position, int64_t n, int32_t dim,
+                                  int32_t position_stride, int32_t periodic, const float* side,
+                                  const jmd_pair_potential* potential, const int32_t* idx,
+                                  int64_t idx_width, int32_t format, float* force,
+                                  float* energy_atom, double* energy_total, float* momentum,
+                                  float half_dt, const float* dt_device, void* stream) {
+  if (n < 0 || n > JMD_PAIR_NOALLOC_MAX_N) return JMD_ERR_INVALID;
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc) return rc;
+  // every launch writes each block partial it later sums, so nothing in the
+  // scratch needs clearing between calls
+  return pair_energy_force(position, n, dim, position_stride, periodic, side, potential, idx,
+                           idx_width, format, force, energy_atom, energy_total, momentum,
+                           half_dt, dt_device, ws, stream);
 }
 
 size_t jmd_pair_virial_workspace_bytes(void) { return (size_t)kVirBlocks * 6 * sizeof(double) + 256; }
