@@ -562,6 +562,8 @@ def main():
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--tpa', type=int, default=0, help='threads per row of the pair kernel')
   ap.add_argument('--eager', action='store_true', help='no HIP graphs (host-launched steps)')
+  ap.add_argument('--no-reorder', action='store_true',
+                  help='run the blocks in the caller\'s particle order (no cell-order layout)')
   ap.add_argument('--no-extra', action='store_true',
                   help='skip the batched-ensemble and adjoint measurements')
   ap.add_argument('--copy-calibration', action='store_true',
@@ -613,7 +615,8 @@ def main():
   nbrs = nfn.allocate(Rt)
   init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
   state = init_fn(0, Rt.clone(), kT=1.0, momenta=torch.as_tensor(P, device=dev), neighbor=nbrs)
-  loop = jmd.simulate.NVELoop(efn, shift, 0.005, graph=not args.eager)
+  loop = jmd.simulate.NVELoop(efn, shift, 0.005, graph=not args.eager,
+                              reorder=not args.no_reorder)
   log(f'N={N} L={L} max_occupancy={nbrs.max_occupancy} cell_capacity={nbrs.cell_list_capacity}')
 
   def snapshot(s):

@@ -525,6 +525,32 @@ int jmd_nvt_kick_drift(const double* position_hi, const float* momentum, const f
                        float threshold_sq, int32_t* gate, const double* momentum_scale,
                        void* stream);
 
+/* The force pass of simulate.velocity_verlet fused with the start of the
+ * next step (jax_md/simulate.py:227-243, partition.py:1141-1154, the loop of
+ * examples/nve_neighbor_list.py:186-201): over a Dense list,
+ *   F = F(R) (jmd_pair_energy_force, written to force), p += half_dt F;
+ * then, per particle, the skin criterion of R against reference_position
+ * raises *gate_next (the next step's update decision, as
+ * jmd_neighbor_list_check; gate_next may be NULL), and with drift != 0 the
+ * next step's opening half: p += half_dt F, R' = shift(R, dt p / m) as in
+ * jmd_nve_kick_drift (position_hi in place, float32 R' to position_out, a
+ * second buffer of R's layout that must not alias position).  momentum holds
+ * the last momentum written.  The shift's space is shift_periodic /
+ * shift_side; dt_device (may be NULL) overrides dt with half_dt = float32(dt
+ * / 2).  No counterpart in the reference as a single call: with the list
+ * update gated on gate_next it leaves one pass over the particles per step
+ * outside the force launch. */
+int jmd_nve_force_kick_drift(const float* position, int64_t n, int32_t dim,
+                             int32_t position_stride, int32_t periodic, const float* side,
+                             const jmd_pair_potential* potential, const int32_t* idx,
+                             int64_t idx_width, float* force, float* momentum,
+                             float half_dt, const float* dt_device, double* position_hi,
+                             const float* mass, float mass_scalar, int32_t shift_periodic,
+                             const float* shift_side, float dt,
+                             const float* reference_position, float threshold_sq,
+                             int32_t* gate_next, int32_t drift, float* position_out,
+                             void* stream);
+
 /* One step's neighbour-list update fused with the opening half of the next
  * velocity-Verlet step: the loop body
  *   nbrs = nbrs.update(R); state = apply_fn(state, neighbor=nbrs)

@@ -73,6 +73,12 @@ _SIGNATURES = {
                                                 c_void_p, c_int64, c_int32, c_void_p, c_void_p,
                                                 c_void_p, c_void_p, c_float, c_void_p,
                                                 c_void_p]),
+    'jmd_nve_force_kick_drift': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
+                                           POINTER(c_float), POINTER(JmdPairPotential),
+                                           c_void_p, c_int64, c_void_p, c_void_p, c_float,
+                                           c_void_p, c_void_p, c_void_p, c_float, c_int32,
+                                           POINTER(c_float), c_float, c_void_p, c_float,
+                                           c_void_p, c_int32, c_void_p, c_void_p]),
     'jmd_nve_kick_drift': (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int64,
                                      c_int32, c_int32, POINTER(c_float), c_float, c_float,
                                      c_void_p, c_void_p, c_void_p, c_int32, c_void_p,

@@ -16,8 +16,10 @@ CSRC = os.path.join(HERE, 'csrc')
 OBJ = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libjaxmd_native.so')
 
-SOURCES = ['jmd_neighbor.hip', 'jmd_pair.hip', 'jmd_adjoint.hip', 'jmd_ensemble.hip',
-           'jmd_integrate.hip', 'jmd_layout.hip', 'jmd_x64.hip', 'jmd_multi_image.hip']
+SOURCES = ['jmd_neighbor.hip', 'jmd_pair.hip', 'jmd_pair_dense_lj3.hip', 'jmd_pair_dense_ss3.hip',
+           'jmd_pair_dense_u3.hip', 'jmd_pair_dense_2d.hip', 'jmd_adjoint.hip',
+           'jmd_ensemble.hip', 'jmd_integrate.hip', 'jmd_layout.hip', 'jmd_x64.hip',
+           'jmd_multi_image.hip']
 
 FLAGS = [
     '--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17',

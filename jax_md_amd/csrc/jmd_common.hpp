@@ -395,6 +395,44 @@ __device__ __forceinline__ void kick_drift_atom(const KickDrift& kd, long long i
   }
 }
 
+// kick_drift_atom with the momentum and force in registers (the force pass
+// runs the next step's opening half): p <- p + half_dt F, then the same
+// drift in float64, the same roundings; p is left to the caller to store.
+template <int DIM>
+__device__ __forceinline__ void kick_drift_regs(const KickDrift& kd, long long i, float dt,
+                                                float half_dt, float* p, const float* F,
+                                                float* xn) {
+#pragma clang fp contract(off)
+  const double m = kd.mass ? (double)kd.mass[i] : (double)kd.mass_scalar;
+  double dx[3];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    p[d] = p[d] + half_dt * F[d];
+    const double v = (double)dt * (double)p[d];
+    dx[d] = kd.mass_pow2 ? v * kd.inv_mass : v / m;
+  }
+  if (kd.s.general) {  // fractional positions: u += T^-1 dR (space.py:450-470)
+    double w[3];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      double acc = 0.0;
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) acc += (double)kd.Ti[a][b] * dx[b];
+      w[a] = acc;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) dx[d] = w[d];
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const long long k = i * DIM + d;
+    double x = kd.xhi_in[k] + dx[d];
+    if (kd.s.periodic) x = ref_mod64(x, (double)kd.s.L[d]);
+    kd.xhi_out[k] = x;
+    xn[d] = (float)x;
+  }
+}
+
 __device__ __forceinline__ int floor_mod_int(int a, int m) {
   int r = a % m;
   return r < 0 ? r + m : r;

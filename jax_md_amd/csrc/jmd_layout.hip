@@ -64,6 +64,48 @@ __global__ __launch_bounds__(256) void k_remap_dense(const int* __restrict__ in,
   }
 }
 
+// The same over the flat [n * width] array in 16-byte chunks: each lane
+// loads four consecutive entries with one vector load, finds their row once
+// (32-bit division; n * width < 2^31), gathers the four ids' new numbers
+// independently and writes the entries into the destination row(s).  The
+// rows' entries stay contiguous, so consecutive lanes write consecutive
+// addresses except where a chunk crosses a row end.  in must be 16-byte
+// aligned; total % 4 trailing entries are done by the first lanes.
+__global__ __launch_bounds__(256) void k_remap_dense_flat(const int* __restrict__ in,
+                                                          int* __restrict__ out,
+                                                          const int* __restrict__ perm, int n,
+                                                          int width) {
+  const unsigned total = (unsigned)n * (unsigned)width;
+  const unsigned nq = total / 4u;
+  const unsigned step = gridDim.x * blockDim.x;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += step) {
+    const int4 v = reinterpret_cast<const int4*>(in)[q];
+    const unsigned e = q * 4u;
+    unsigned row = e / (unsigned)width;
+    unsigned col = e - row * (unsigned)width;
+    const int jv[4] = {v.x, v.y, v.z, v.w};
+    int pj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pj[u] = jv[u] < n ? perm[jv[u]] : n;
+    long long dst = (long long)perm[row] * width;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      out[dst + col] = pj[u];
+      if (++col == (unsigned)width && u < 3) {
+        col = 0;
+        ++row;
+        dst = (long long)perm[row] * width;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < total % 4u) {
+    const unsigned e = nq * 4u + threadIdx.x;
+    const unsigned row = e / (unsigned)width, col = e - row * (unsigned)width;
+    const int j = in[e];
+    out[(long long)perm[row] * width + col] = j < n ? perm[j] : n;
+  }
+}
+
 // Sparse list [2, width] through perm (entries keep their slots).
 // OrderedSparse keeps receiver < sender (partition.py:998): a pair whose
 // order the renumbering reversed is stored swapped.
@@ -134,9 +176,18 @@ int jmd_remap_neighbor_ids(const int32_t* idx_in, int32_t* idx_out, const int32_
   hipStream_t st = (hipStream_t)stream;
   if (format == JMD_DENSE) {
     if (width > 0x7fffffff) return JMD_ERR_INVALID;
-    long long rb = (n + 3) / 4;  // four rows (wavefronts) per workgroup
-    hipLaunchKernelGGL(k_remap_dense, dim3((unsigned)(rb < 65536 ? (rb < 1 ? 1 : rb) : 65536)),
-                       dim3(256), 0, st, idx_in, idx_out, perm, (int)n, (int)width);
+    if (n * width < 0x7fffffffLL && ((uintptr_t)idx_in & 15) == 0) {
+      long long qb = (n * width / 4 + 255) / 256;
+      if (qb < 1) qb = 1;
+      if (qb > 16384) qb = 16384;
+      hipLaunchKernelGGL(k_remap_dense_flat, dim3((unsigned)qb), dim3(256), 0, st, idx_in,
+                         idx_out, perm, (int)n, (int)width);
+    } else {
+      long long rb = (n + 3) / 4;  // four rows (wavefronts) per workgroup
+      hipLaunchKernelGGL(k_remap_dense,
+                         dim3((unsigned)(rb < 65536 ? (rb < 1 ? 1 : rb) : 65536)), dim3(256), 0,
+                         st, idx_in, idx_out, perm, (int)n, (int)width);
+    }
   } else {
     hipLaunchKernelGGL(k_remap_ids, dim3(grid_for(width)), dim3(256), 0, st, idx_in, idx_out,
                        perm, (int)n, width, format == JMD_ORDERED_SPARSE ? 1 : 0);

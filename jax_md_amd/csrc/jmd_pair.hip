@@ -18,237 +18,11 @@
 //   F_i += (S V'/r + S'/r V) (R_j - R_i).
 #include <mutex>
 
-#include "jmd_pair_common.hpp"
+#include "jmd_pair_dense.hpp"
 
 namespace jmd {
 
-// Sum over one Dense row slice (entries sub, sub+TPA, ...) into f, e.
-// INTERIOR: every particle of the wavefront lies at least the interaction
-// range from each box face, so for any pair within range the plain
-// difference x_j - x_i is already the minimum image and no wrap is needed
-// (pairs whose plain difference is not the minimum image are then beyond
-// range and contribute zero either way).
-template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, bool ENERGY, bool INTERIOR,
-          int SMOOTH, bool BOX = false>
-__device__ __forceinline__ void row_sum(const float* __restrict__ pos, unsigned n,
-                                        const Space& s, const PairArgs& a,
-                                        const int* __restrict__ row, int width, int sub,
-                                        unsigned i, const float* xi, const float* xim,
-                                        float* f, float* e) {
-#pragma clang fp contract(off)
-  float sig = a.sig, eps = a.eps, alp = a.alp;
-  int si = 0;
-  if (GENERAL && a.species) si = a.species[i];
-#ifndef JMD_SPARSE_CHUNK
-#define JMD_SPARSE_CHUNK 4096
-#endif
-#ifndef JMD_SPARSE_TABLE
-#define JMD_SPARSE_TABLE 2048
-#endif
-#ifndef JMD_PAIR_UNROLL
-#define JMD_PAIR_UNROLL 4
-#endif
-  constexpr int UNR = JMD_PAIR_UNROLL;
-  // Interior waves of the Lennard-Jones kernel with scalar parameters gather
-  // through a buffer descriptor of exactly n rows: a padding entry (j = n)
-  // then reads a zero position, so d = -x_i and r^2 >= 3 r_c^2 (every
-  // coordinate of an interior particle is at least r_c), where the switched
-  // or truncated pair term is exactly zero -- no per-pair validity mask and
-  // no index select.  The caller keeps n below 2^27 on this path.
-  constexpr bool kNoMask = INTERIOR && KIND == JMD_LENNARD_JONES && !BOX && !GENERAL &&
-                           DIM == 3 && STRIDE == 4;
-  const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float*>(pos), (short)0, (int)(n * 16u), 0x00020000);
-  auto gather = [&](unsigned jraw, float* xj) {
-    if (kNoMask) {
-      const auto v = __builtin_amdgcn_raw_buffer_load_b128(prs, (int)(jraw * 16u), 0, 0);
-      xj[0] = __uint_as_float(v[0]);
-      xj[1] = __uint_as_float(v[1]);
-      xj[2] = __uint_as_float(v[2]);
-    } else {
-      load_pos<DIM, STRIDE>(pos, jraw < n ? jraw : i, xj);
-    }
-  };
-  // One pair term; padding entries (j = n) read particle i (or, without the
-  // mask, a zero position) and add nothing.
-  auto term = [&](unsigned jraw, const float* xj) {
-    const bool ok = kNoMask || jraw < n;
-    float d[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int c = 0; c < DIM; ++c)
-      d[c] = (INTERIOR || !s.periodic)
-                 ? xj[c] - xi[c]
-                 : min_image_any(xj[c], xi[c], xim[c], s.L[c], s.half[c], s.inbox);
-    if (BOX) box_apply<DIM>(s, d);  // periodic_general: fractional fold, then T
-    float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
-                        : fmaf(d[1], d[1], d[0] * d[0]);
-    if (GENERAL) {
-      unsigned j = ok ? jraw : i;
-      int sj = a.species ? a.species[j] : 0;
-      sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
-      eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
-      if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
-    }
-    float dEr;
-    float uu = pair_eval<KIND, SMOOTH>(r2, sig, eps, alp, a, &dEr);
-    dEr = ok ? dEr : 0.0f;
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) f[c] = fmaf(dEr, d[c], f[c]);
-    if (ENERGY) *e += ok ? uu : 0.0f;
-  };
-  // This lane's entries are sub, sub + TPA, ... < width: nfull blocks of UNR
-  // entries, then fewer than UNR in a tail.  In the blocks every entry lies
-  // in the row, so they are read through one pointer with constant offsets;
-  // the next block's entries are loaded before this block's terms are
-  // computed (software pipeline: one dependent round trip per block, the
-  // position gather).  Rows are left-compacted with padding n, so the row
-  // ends at the first block whose entries are all padding.
-  const int nent = width > sub ? (width - sub + TPA - 1) / TPA : 0;
-  const int nfull = nent / UNR;
-  const int* p = row + sub;
-  if (nfull > 0) {
-    unsigned nxt[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) nxt[u] = (unsigned)p[u * TPA];
-    for (int b = 0; b < nfull; ++b) {
-      unsigned jj[UNR];
-      unsigned jmin = n;
-#pragma unroll
-      for (int u = 0; u < UNR; ++u) {
-        jj[u] = nxt[u];
-        jmin = min(jmin, jj[u]);
-      }
-      if (jmin >= n) return;
-      float xj[UNR][3];
-#pragma unroll
-      for (int u = 0; u < UNR; ++u) {
-        xj[u][0] = xj[u][1] = xj[u][2] = 0.0f;
-        gather(jj[u], xj[u]);
-      }
-      // next block, clamped to the last full one (re-read, unused, at the end)
-      const int* q = p + min(b + 1, nfull - 1) * (UNR * TPA);
-#pragma unroll
-      for (int u = 0; u < UNR; ++u) nxt[u] = (unsigned)q[u * TPA];
-      if (__builtin_amdgcn_ballot_w64(jj[UNR - 1] >= n) == 0ull) {
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) term(jj[u], xj[u]);
-      } else {
-        // some row of the wavefront ends in this block: terms only while a
-        // lane still holds an entry (validity is monotone in u per lane)
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-          if (u > 0 && __builtin_amdgcn_ballot_w64(jj[u] < n) == 0ull) break;
-          term(jj[u], xj[u]);
-        }
-      }
-    }
-  }
-  for (int t = nfull * UNR; t < nent; ++t) {
-    const unsigned j = (unsigned)p[t * TPA];
-    if (j >= n) return;
-    float xj[3] = {0.0f, 0.0f, 0.0f};
-    gather(j, xj);
-    term(j, xj);
-  }
-}
-
-template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, bool ENERGY, bool KICK>
-__global__ __launch_bounds__(256) void k_pair_dense(
-    const float* __restrict__ pos, int n, Space s, PairArgs a, const int* __restrict__ idx,
-    int width, float* __restrict__ force, float* __restrict__ e_atom,
-    double* __restrict__ e_part, float* __restrict__ mom, float half_dt,
-    const float* __restrict__ dt_dev) {
-  if (KICK && dt_dev) half_dt = dt_dev[0] / 2.0f;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int gt = tile * blockDim.x + threadIdx.x;
-  const int i = gt / TPA;
-  const int sub = gt % TPA;
-  const bool active = i < n;
-  float f[3] = {0.0f, 0.0f, 0.0f};
-  float e = 0.0f;
-  float xi[3] = {0, 0, 0}, xim[3] = {0, 0, 0};
-  bool interior = true;
-  if (active) {
-    load_pos<DIM, STRIDE>(pos, i, xi);
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      xim[c] = xi[c] - s.L[c];
-      interior = interior && xi[c] >= a.rc && xi[c] <= s.L[c] - a.rc;
-    }
-  }
-  if (s.general && active) {
-    // fractional coordinates: a neighbour within rc of particle i differs by
-    // at most rc |row c of T^-1| in u_c, so no fold is needed when u_c stays
-    // that far from both faces
-    interior = true;
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      const float w = a.rc * s.fw[c];
-      interior = interior && xi[c] >= w && xi[c] <= 1.0f - w;
-    }
-  }
-  // (n < 2^27 keeps the interior path's buffer offsets j * 16 in range;
-  // the fold-free path is exact only when every neighbour lies in the box)
-  const bool wave_interior =
-      s.inbox && __all(interior || !active) && n < (1 << 27);
-  if (active && s.general) {
-    const int* row = idx + (long long)i * width;
-#define JMD_ROWG(INT, SM)                                                                      \
-  row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, INT, SM, true>(pos, n, s, a, row, width, sub, \
-                                                                  i, xi, xim, f, &e)
-    if (KIND == JMD_LENNARD_JONES && a.smooth) {
-      if (wave_interior) JMD_ROWG(true, 1); else JMD_ROWG(false, 1);
-    } else {
-      if (wave_interior) JMD_ROWG(true, 0); else JMD_ROWG(false, 0);
-    }
-#undef JMD_ROWG
-  } else if (active) {
-    const int* row = idx + (long long)i * width;
-    // the smoothing switch is uniform: taken once here, not per pair
-    if (KIND == JMD_LENNARD_JONES && a.smooth) {
-      if (wave_interior)
-        row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, true, 1>(pos, n, s, a, row, width, sub,
-                                                                  i, xi, xim, f, &e);
-      else
-        row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false, 1>(pos, n, s, a, row, width, sub,
-                                                                   i, xi, xim, f, &e);
-    } else {
-      if (wave_interior)
-        row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, true, 0>(pos, n, s, a, row, width, sub,
-                                                                  i, xi, xim, f, &e);
-      else
-        row_sum<DIM, STRIDE, KIND, TPA, GENERAL, ENERGY, false, 0>(pos, n, s, a, row, width, sub,
-                                                                   i, xi, xim, f, &e);
-    }
-  }
-  double fd[3];
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) fd[c] = group_sum<TPA>((double)f[c]);
-  double ed = 0.0;
-  if (ENERGY) ed = group_sum<TPA>((double)e);
-  if (active && sub == 0) {
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      float fc = (float)fd[c];
-      if (force) force[(long long)i * DIM + c] = fc;
-      if (KICK) mom[(long long)i * DIM + c] += half_dt * fc;
-    }
-    if (ENERGY && e_atom) e_atom[i] = (float)(0.5 * ed);
-  }
-  if (ENERGY) {
-    // block partial of 0.5 * sum_i e_i in float64
-    __shared__ double s_e[256 / kWave];
-    double v = (active && sub == 0) ? 0.5 * ed : 0.0;
-    v = wave_reduce_sum(v);
-    if (lane_id() == 0) s_e[threadIdx.x / kWave] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += s_e[w];
-      e_part[tile] = t;
-    }
-  }
-}
+JMD_DENSE_INSTANCES(JMD_DENSE_EXTERN)
 
 // Sparse / OrderedSparse: one lane per stored pair (smap.py:926-931,975-977).
 // Lists are sender-major (partition.py:990-1008: row i of the candidate
@@ -441,49 +215,6 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
   if (threadIdx.x == 0) out[0] = s[0];
 }
 
-struct DenseLaunch {
-  dim3 grid;
-  hipStream_t st;
-  const float* pos;
-  int n;
-  Space s;
-  PairArgs a;
-  const int* idx;
-  int width;
-  float* force;
-  float* e_atom;
-  double* e_part;
-  float* mom;
-  float half_dt;
-  const float* dt_dev;
-};
-
-template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL>
-static void launch_dense(const DenseLaunch& L, bool energy, bool kick) {
-#define JMD_PD(E, K)                                                                          \
-  hipLaunchKernelGGL((k_pair_dense<DIM, STRIDE, KIND, TPA, GENERAL, E, K>), L.grid, dim3(256), \
-                     0, L.st, L.pos, L.n, L.s, L.a, L.idx, L.width, L.force, L.e_atom,        \
-                     L.e_part, L.mom, L.half_dt, L.dt_dev)
-  if (energy) JMD_PD(true, false);
-  else if (kick) JMD_PD(false, true);
-  else JMD_PD(false, false);
-#undef JMD_PD
-}
-
-template <int DIM, int STRIDE, int KIND>
-static void dispatch_dense(int tpa, bool general, const DenseLaunch& L, bool energy, bool kick) {
-#define JMD_TPA(T)                                                               \
-  (general ? launch_dense<DIM, STRIDE, KIND, T, true>(L, energy, kick)           \
-           : launch_dense<DIM, STRIDE, KIND, T, false>(L, energy, kick))
-  switch (tpa) {
-    case 2: JMD_TPA(2); break;
-    case 8: JMD_TPA(8); break;
-    case 16: JMD_TPA(16); break;
-    default: JMD_TPA(4); break;
-  }
-#undef JMD_TPA
-}
-
 static int g_tpa_override = 0;
 constexpr long long kSparseBlocks = 2048;
 
@@ -662,21 +393,21 @@ int pair_energy_force(const float* position, int64_t n, int32_t dim, int32_t pos
     int nb = ceil_div(threads, 256);
     nb = (nb + 7) & ~7;
     DenseLaunch L{dim3(nb), st, position, nn, s, a, idx, (int)idx_width, force, energy_atom,
-                  energy ? part : nullptr, momentum, half_dt, dt_device};
+                  energy ? part : nullptr, momentum, half_dt, dt_device, KickDrift{}};
     const bool kick = momentum != nullptr;
     if (dim == 3) {
       if (a.kind == JMD_LENNARD_JONES) {
-        if (s4) dispatch_dense<3, 4, JMD_LENNARD_JONES>(tpa, a.general, L, energy, kick);
-        else dispatch_dense<3, 3, JMD_LENNARD_JONES>(tpa, a.general, L, energy, kick);
+        if (s4) dispatch_dense<3, 4, JMD_LENNARD_JONES>(tpa, a.general, L, energy, kick, false);
+        else dispatch_dense<3, 3, JMD_LENNARD_JONES>(tpa, a.general, L, energy, kick, false);
       } else {
-        if (s4) dispatch_dense<3, 4, JMD_SOFT_SPHERE>(tpa, a.general, L, energy, kick);
-        else dispatch_dense<3, 3, JMD_SOFT_SPHERE>(tpa, a.general, L, energy, kick);
+        if (s4) dispatch_dense<3, 4, JMD_SOFT_SPHERE>(tpa, a.general, L, energy, kick, false);
+        else dispatch_dense<3, 3, JMD_SOFT_SPHERE>(tpa, a.general, L, energy, kick, false);
       }
     } else {
       if (a.kind == JMD_LENNARD_JONES)
-        dispatch_dense<2, 2, JMD_LENNARD_JONES>(tpa, a.general, L, energy, kick);
+        dispatch_dense<2, 2, JMD_LENNARD_JONES>(tpa, a.general, L, energy, kick, false);
       else
-        dispatch_dense<2, 2, JMD_SOFT_SPHERE>(tpa, a.general, L, energy, kick);
+        dispatch_dense<2, 2, JMD_SOFT_SPHERE>(tpa, a.general, L, energy, kick, false);
     }
     JMD_LAUNCH_CHECK();
     if (energy_total) {
@@ -800,6 +531,75 @@ int jmd_pair_energy_force_noalloc(const float* position, int64_t n, int32_t dim,
   return pair_energy_force(position, n, dim, position_stride, periodic, side, potential, idx,
                            idx_width, format, force, energy_atom, energy_total, momentum,
                            half_dt, dt_device, ws, stream);
+}
+
+int jmd_nve_force_kick_drift(const float* position, int64_t n, int32_t dim,
+                             int32_t position_stride, int32_t periodic, const float* side,
+                             const jmd_pair_potential* potential, const int32_t* idx,
+                             int64_t idx_width, float* force, float* momentum, float half_dt,
+                             const float* dt_device, double* position_hi, const float* mass,
+                             float mass_scalar, int32_t shift_periodic, const float* shift_side,
+                             float dt, const float* reference_position, float threshold_sq,
+                             int32_t* gate_next, int32_t drift, float* position_out,
+                             void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || idx_width < 0 ||
+      idx_width > 0x7fffffff || (n > 0 && (!position || !force || !momentum)) ||
+      (idx_width > 0 && !idx) || (periodic && !side) || (shift_periodic && !shift_side) ||
+      (gate_next && !reference_position))
+    return JMD_ERR_INVALID;
+  if (drift && (!position_hi || !position_out || position_out == position)) return JMD_ERR_INVALID;
+  if (!mass && !(mass_scalar > 0.0f)) return JMD_ERR_INVALID;
+  if (n == 0) return JMD_OK;
+  if (!stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  if (drift && !stride_ok(dim, position_stride, position_out)) return JMD_ERR_INVALID;
+  PairArgs a;
+  if (!make_pair_args(potential, &a)) return JMD_ERR_INVALID;
+  {
+    int grc = space_check(dim, periodic, side, true);
+    if (!grc) grc = space_check(dim, shift_periodic, shift_side, true);
+    if (grc) return grc;
+  }
+  Space s = make_space(dim, periodic, side);
+  KickDrift kd{};
+  kd.active = drift ? 1 : 0;
+  kd.xhi_in = position_hi;
+  kd.xhi_out = position_hi;
+  kd.mass = mass;
+  kd.mass_scalar = mass_scalar;
+  kick_drift_mass(&kd);
+  kd.dt = dt;
+  kd.half_dt = half_dt;
+  kd.dt_dev = dt_device;
+  kd.x_out = position_out;
+  kd.ref = reference_position;
+  kd.thr_sq = threshold_sq;
+  kd.gate_next = gate_next;
+  kd.s = make_space(dim, shift_periodic, shift_side);
+  box_inverse(kd.s, kd.Ti);
+  hipStream_t st = (hipStream_t)stream;
+  const int nn = (int)n;
+  const int tpa = g_tpa_override ? g_tpa_override : (dim == 3 ? 8 : 2);
+  int nb = ceil_div((long long)nn * tpa, 256);
+  nb = (nb + 7) & ~7;
+  DenseLaunch L{dim3(nb), st, position, nn, s, a, idx, (int)idx_width, force, nullptr,
+                nullptr, momentum, half_dt, dt_device, kd};
+  const bool s4 = dim == 3 && position_stride == 4;
+  if (dim == 3) {
+    if (a.kind == JMD_LENNARD_JONES) {
+      if (s4) dispatch_dense<3, 4, JMD_LENNARD_JONES>(tpa, a.general, L, false, true, true);
+      else dispatch_dense<3, 3, JMD_LENNARD_JONES>(tpa, a.general, L, false, true, true);
+    } else {
+      if (s4) dispatch_dense<3, 4, JMD_SOFT_SPHERE>(tpa, a.general, L, false, true, true);
+      else dispatch_dense<3, 3, JMD_SOFT_SPHERE>(tpa, a.general, L, false, true, true);
+    }
+  } else {
+    if (a.kind == JMD_LENNARD_JONES)
+      dispatch_dense<2, 2, JMD_LENNARD_JONES>(tpa, a.general, L, false, true, true);
+    else
+      dispatch_dense<2, 2, JMD_SOFT_SPHERE>(tpa, a.general, L, false, true, true);
+  }
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
 }
 
 size_t jmd_pair_virial_workspace_bytes(void) { return (size_t)kVirBlocks * 6 * sizeof(double) + 256; }

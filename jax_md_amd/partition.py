@@ -325,6 +325,10 @@ class _Plan:
   periodic: int               # 0 free, 1 periodic box, 2 box matrix (fractional)
   dim: int
   spec: Any = None
+  # bumped by every build this module enqueues (allocate / update), which
+  # rewrite the list's buffers in place without touching their tensors'
+  # version counters
+  epoch: int = 0
 
   @property
   def builds(self) -> int:
@@ -446,6 +450,7 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
 
   def _build(position, nbrs: Optional[NeighborList], plan: _Plan, gate):
     lib = nat.load()
+    plan.epoch += 1
     N, dim = position.shape
     st = nat.stream_handle(position.device)
     nat.check(lib.jmd_neighbor_list_build(

@@ -275,9 +275,13 @@ class NVELoop:
   half kick.  The rebuild decisions are those of the reference's
   ``update`` before ``apply_fn``; overflow is reported in the list's error
   word and checked by the caller once per block, as the reference does.
-  With a cell list, (1) and (2) are one call (``fused``): the drift writes
-  the new positions into a second buffer while the list stages read the
-  current ones, the two buffers and two gate words alternating by step.
+  ``fused`` (default, Dense lists): (2) runs inside the previous step's
+  force launch, from the force and momentum in its registers, writing the
+  new positions into a second buffer while the next list update still reads
+  the current ones (the two buffers and two gate words alternate by step);
+  the block's first half step is one launch before the loop.
+  ``fused='update'`` (Dense, cell list): (2) runs in the binning launch of
+  (1) instead.  Every form gives the same bits.
 
   With ``graph`` (default) a block is one HIP graph launch: the first call for
   a given set of buffers and block length runs eagerly and records the
@@ -287,7 +291,7 @@ class NVELoop:
   """
 
   def __init__(self, energy_or_force_fn, shift_fn, dt, reorder: bool = True,
-               graph: bool = True, fused: bool = True):
+               graph: bool = True, fused: bool = True, reorder_every: int = 1000):
     self.energy = quantity.native_energy_of(energy_or_force_fn)
     self.shift = native_shift(shift_fn)
     self.dt = f32(dt)
@@ -300,8 +304,16 @@ class NVELoop:
     self._bufs_key = None
     self._graphs = {}  # block key -> (graph, the objects the key's addresses belong to)
     self._keep = None
-    # blocks run eagerly (host-launched), graphs recorded, blocks replayed
-    self.stats = {'eager': 0, 'recorded': 0, 'replayed': 0}
+    # Consecutive blocks keep the cell-order working copies: when the state
+    # and list a block starts from are exactly what the previous block left
+    # (same buffers, untouched since), the block starts from those copies
+    # instead of ordering, permuting and relabelling again; the cell order is
+    # recomputed once it is reorder_every steps old.
+    self.reorder_every = int(reorder_every)
+    self._cont = None  # (signature after the last block, steps since the order was taken)
+    # blocks run eagerly (host-launched), graphs recorded, blocks replayed;
+    # continued: blocks that started from the previous block's cell order
+    self.stats = {'eager': 0, 'recorded': 0, 'replayed': 0, 'continued': 0}
 
   def prime(self, state: NVEState, nbrs) -> None:
     """Sets the gate from the skin criterion of the current positions."""
@@ -325,9 +337,37 @@ class NVELoop:
                 and not self.energy.reads_particle_data())
 
   @staticmethod
+  def _signature(state, nbrs):
+    """Buffers and in-place modification counters of the state and list: a
+    write through torch bumps a tensor's version, a list build through
+    ``partition`` bumps the plan's epoch."""
+    def v(t):
+      return (t.data_ptr(), t._version) if isinstance(t, torch.Tensor) else t
+    return (v(state.position), v(state.position_hi), v(state.momentum), v(state.force),
+            v(state.mass), v(nbrs.idx), v(nbrs.reference_position), v(nbrs.error.code),
+            id(nbrs.plan), nbrs.plan.epoch)
+
+  def _continues(self, state, nbrs, reorder) -> bool:
+    """Whether a block may start from the previous block's cell-order copies."""
+    return bool(reorder and self._cont is not None and self._bufs is not None
+                and self._bufs_key == self._buffers_key(state, nbrs)
+                and self._cont[1] < self.reorder_every
+                and self._cont[0] == self._signature(state, nbrs))
+
+  def _after_block(self, state, nbrs, n_steps, reorder, cont):
+    if not reorder:
+      self._cont = None
+      return
+    age = (self._cont[1] if cont else 0) + int(n_steps)
+    self._cont = (self._signature(state, nbrs), age)
+    if cont:
+      self.stats['continued'] += 1
+
+  @staticmethod
   def _key(state, nbrs, n_steps, reorder):
     """What a recorded block depends on: buffer addresses and shapes, the
-    list's plan (grid, capacities, workspace) and the block length."""
+    list's plan (grid, capacities, workspace) and the block length (and,
+    appended by run, whether it starts from the previous block's copies)."""
     def p(t):
       return (t.data_ptr(), tuple(t.shape)) if isinstance(t, torch.Tensor) else t
     plan = nbrs.plan
@@ -354,20 +394,27 @@ class NVELoop:
     if state.position_hi is None:
       state.position_hi = state.position.to(torch.float64)
     reorder = self._use_reorder(state, nbrs)
+    cont = self._continues(state, nbrs, reorder)
     if events is not None or not self.graph:
-      self._enqueue(state, nbrs, n_steps, reorder, events)
+      self._enqueue(state, nbrs, n_steps, reorder, events, cont)
       self.stats['eager'] += 1
+      self._after_block(state, nbrs, n_steps, reorder, cont)
       return state
-    key = self._key(state, nbrs, n_steps, reorder)
+    key = self._key(state, nbrs, n_steps, reorder) + (cont,)
     g = self._graphs.get(key)
     if g is not None:
       g[0].replay()
       self.stats['replayed'] += 1
+      self._after_block(state, nbrs, n_steps, reorder, cont)
       return state
     # eager first block: it also loads every kernel before the capture
-    self._enqueue(state, nbrs, n_steps, reorder, None)
+    self._enqueue(state, nbrs, n_steps, reorder, None, cont)
     self.stats['eager'] += 1
-    self._record(state, nbrs, n_steps, reorder, key)
+    self._after_block(state, nbrs, n_steps, reorder, cont)
+    # the recorded block starts as the next block will (from these copies)
+    cont = self._continues(state, nbrs, reorder)
+    self._record(state, nbrs, n_steps, reorder, self._key(state, nbrs, n_steps, reorder) + (cont,),
+                 cont)
     return state
 
   def capture(self, state, nbrs, n_steps: int) -> None:
@@ -380,25 +427,38 @@ class NVELoop:
     if not self.graph:
       return
     reorder = self._use_reorder(state, nbrs)
-    key = self._key(state, nbrs, n_steps, reorder)
-    if key not in self._graphs:
-      self._record(state, nbrs, n_steps, reorder, key)
+    # a block that continues the last one, and (cell order only) one that
+    # starts afresh: the order is recomputed every reorder_every steps
+    variants = [self._continues(state, nbrs, reorder)]
+    if reorder and variants[0]:
+      variants.append(False)
+    for cont in variants:
+      key = self._key(state, nbrs, n_steps, reorder) + (cont,)
+      if key not in self._graphs:
+        self._record(state, nbrs, n_steps, reorder, key, cont)
 
-  def _record(self, state, nbrs, n_steps, reorder, key):
+  def _record(self, state, nbrs, n_steps, reorder, key, cont=False):
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
-      self._enqueue(state, nbrs, n_steps, reorder, None)  # recorded, not run
+      self._enqueue(state, nbrs, n_steps, reorder, None, cont)  # recorded, not run
     # the oldest graph goes first (each entry keeps its buffers alive)
-    if len(self._graphs) >= 4:
+    if len(self._graphs) >= 6:
       self._graphs.pop(next(iter(self._graphs)))
     self._graphs[key] = (g, self._key_refs(state, nbrs))
     self.stats['recorded'] += 1
 
-  def _enqueue(self, state, nbrs, n_steps, reorder, events):
+  def _enqueue(self, state, nbrs, n_steps, reorder, events, cont=False):
     if reorder:
-      self._run_reordered(state, nbrs, n_steps, events)
+      self._run_reordered(state, nbrs, n_steps, events, cont)
     else:
       self._run_steps(state, nbrs, n_steps, events)
+
+  @staticmethod
+  def _buffers_key(state, nbrs):
+    R = state.position
+    N, dim = R.shape
+    return (N, dim, str(R.device), tuple(nbrs.idx.shape),
+            isinstance(state.mass, torch.Tensor) and state.mass.ndim > 0)
 
   def _buffers(self, state, nbrs):
     """Cell-order working copies of the state and list, kept across blocks
@@ -406,8 +466,7 @@ class NVELoop:
     R = state.position
     N, dim = R.shape
     dev = R.device
-    key = (N, dim, str(dev), tuple(nbrs.idx.shape), isinstance(state.mass, torch.Tensor)
-           and state.mass.ndim > 0)
+    key = self._buffers_key(state, nbrs)
     if self._bufs_key != key:
       b = dict(
           order=torch.empty((N,), dtype=torch.int32, device=dev),
@@ -483,7 +542,10 @@ class NVELoop:
       self._alt = (key, R2, g2)
     return [R, self._alt[1]], [gate, self._alt[2]]
 
-  def _run_reordered(self, state, nbrs, n_steps, events):
+  def _run_reordered(self, state, nbrs, n_steps, events, cont=False):
+    """One block in cell order.  ``cont``: the working copies already hold
+    this state and list in the previous block's cell order (nothing touched
+    them since it ended), so the block starts from them."""
     lib = nat.load()
     plan = nbrs.plan
     R = state.position
@@ -493,9 +555,6 @@ class NVELoop:
     stride = nat.position_stride(R)
     b = self._buffers(state, nbrs)
     order, inv = b['order'], b['inv']
-    nat.check(lib.jmd_cell_order(nat.ptr(R), N, dim, stride, nat.i3(plan.cells_per_side),
-                                 nat.f3(plan.cell_size3), nat.ptr(order), nat.ptr(inv),
-                                 plan.workspace.ptr, st), 'jmd_cell_order')
 
     def permute(src, dst, row_bytes, src_stride, dst_stride, inverse):
       nat.check(lib.jmd_permute_rows(nat.ptr(src), nat.ptr(dst), nat.ptr(order), N, row_bytes,
@@ -503,21 +562,24 @@ class NVELoop:
 
     rb = 4 * dim
     xh, x, p, F, ref, idx = b['xh'], b['x'], b['p'], b['F'], b['ref'], b['idx']
-    permute(state.position_hi, xh, 8 * dim, 8 * dim, 8 * dim, 0)
     xs = nat.position_stride(x)
-    permute(R, x, rb, 4 * stride, 4 * xs, 0)
-    permute(state.momentum, p, rb, rb, rb, 0)
-    permute(state.force, F, rb, rb, rb, 0)
-    mass = state.mass
-    if b['mass'] is not None:
-      b['m32'].copy_(mass.to(device=dev, dtype=torch.float32).reshape(-1))
-      permute(b['m32'], b['mass'], 4, 4, 4, 0)
-      mass = b['mass']
-    permute(nbrs.reference_position, ref, rb, rb, rb, 0)
+    mass = state.mass if b['mass'] is None else b['mass']
     width = nbrs.idx.shape[1]
     fmt = plan.spec.format_code
-    nat.check(lib.jmd_remap_neighbor_ids(nat.ptr(nbrs.idx), nat.ptr(idx), nat.ptr(inv), N,
-                                         fmt, width, st), 'jmd_remap_neighbor_ids')
+    if not cont:
+      nat.check(lib.jmd_cell_order(nat.ptr(R), N, dim, stride, nat.i3(plan.cells_per_side),
+                                   nat.f3(plan.cell_size3), nat.ptr(order), nat.ptr(inv),
+                                   plan.workspace.ptr, st), 'jmd_cell_order')
+      permute(state.position_hi, xh, 8 * dim, 8 * dim, 8 * dim, 0)
+      permute(R, x, rb, 4 * stride, 4 * xs, 0)
+      permute(state.momentum, p, rb, rb, rb, 0)
+      permute(state.force, F, rb, rb, rb, 0)
+      if b['mass'] is not None:
+        b['m32'].copy_(state.mass.to(device=dev, dtype=torch.float32).reshape(-1))
+        permute(b['m32'], b['mass'], 4, 4, 4, 0)
+      permute(nbrs.reference_position, ref, rb, rb, rb, 0)
+      nat.check(lib.jmd_remap_neighbor_ids(nat.ptr(nbrs.idx), nat.ptr(idx), nat.ptr(inv), N,
+                                           fmt, width, st), 'jmd_remap_neighbor_ids')
     nb = dataclasses.replace(nbrs, idx=idx, reference_position=ref)
     sp = NVEState(x, p, F, mass, xh)
     self._run_steps(sp, nb, n_steps, events)
@@ -568,7 +630,45 @@ class NVELoop:
     if state.x64:
       return self._run_steps64(state, nbrs, n_steps, events, hook, gate, thr, cps, cs, psides,
                                side, wraps, pot, idx, width, err, ref, occ, ws)
-    if (self.fused and hook is None and gate is not None and plan.use_cell_list
+    if (self.fused and self.fused != 'update' and hook is None and gate is not None
+        and fmt == nat.DENSE):
+      # The force launch also starts the next step: skin test of the
+      # positions it used (the next update's gate word), opening half kick
+      # and drift into the other position buffer; the block's first opening
+      # half step is one launch before the loop.  Per step: the gated list
+      # update on x_k, then the force at x_{k+1}.
+      xs, gs = self._alternates(R, plan.gate)
+      if n_steps > 0:
+        nat.check(lib.jmd_nve_kick_drift(
+            x_hi, p, F, mv, mass_s if mass_v is None else 0.0, N, dim, wraps, side,
+            float(self.dt), float(self.half), None, x_hi, nat.ptr(xs[1]), stride, p, None, thr,
+            None, st), 'jmd_nve_kick_drift')
+      for k in range(n_steps):
+        xk, xk1 = xs[k % 2], xs[(k + 1) % 2]
+        if events is not None:
+          e0 = events.start()
+        nat.check(lib.jmd_neighbor_list_build(
+            nat.ptr(xk), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq),
+            int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0), fmt,
+            int(spec.mask_self), idx, int(nbrs.max_occupancy), occ, err, ref,
+            nat.ptr(gs[k % 2]), ws, st), 'jmd_neighbor_list_build')
+        if events is not None:
+          events.stop('update', e0)
+          events.builds_after_update(plan)
+          e0 = events.start()
+        nat.check(lib.jmd_nve_force_kick_drift(
+            nat.ptr(xk1), N, dim, stride, gcode, esides, nat.ctypes.byref(pot), idx, width, F, p,
+            float(self.half), None, x_hi, mv, mass_s if mass_v is None else 0.0, wraps, side,
+            float(self.dt), ref, thr, nat.ptr(gs[(k + 1) % 2]), int(k + 1 < n_steps),
+            nat.ptr(xk), st), 'jmd_nve_force_kick_drift')
+        if events is not None:
+          events.stop('force', e0)
+      if n_steps % 2:  # the state's buffer and gate word hold step n's values
+        R.copy_(xs[1])
+        gs[0].copy_(gs[1])
+        gs[1].zero_()
+      return state
+    if (self.fused == 'update' and hook is None and gate is not None and plan.use_cell_list
         and fmt == nat.DENSE):
       xs, gs = self._alternates(R, plan.gate)
       for k in range(n_steps):

@@ -278,17 +278,19 @@ def test_nve_loop_with_species_matches_drop_in():
 
 @pytest.mark.parametrize('reorder', [False, True])
 def test_nve_fused_update_matches_separate_kernels(reorder):
-  """The fused list update + half kick + drift (alternating position buffers
-  and gate words) gives the bit-identical trajectory, lists, reference
-  positions and rebuild counts of the separate build and kick-drift kernels,
-  for odd and even block lengths."""
+  """Both fused step forms -- the force launch starting the next step (skin
+  test, half kick, drift; the default) and the list update carrying the half
+  kick and drift ('update') -- with alternating position buffers and gate
+  words give the bit-identical trajectory, lists, reference positions, gate
+  and rebuild counts of the separate build, kick-drift and force kernels, for
+  odd and even block lengths."""
   print(f'case NVELoop fused reorder={reorder}', flush=True)
   jmd, R, P, L, shift, nfn, efn = _lj(12, seed=7)
   Rt = torch.as_tensor(R, device=DEV)
   Pt = torch.as_tensor(P, device=DEV)
   init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
   runs = []
-  for fused in (False, True):
+  for fused in (False, True, 'update'):
     nb = nfn.allocate(Rt)
     s = init_fn(0, Rt, kT=1.0, momenta=Pt, neighbor=nb)
     loop = jmd.simulate.NVELoop(efn, shift, 0.005, reorder=reorder, graph=False, fused=fused)
@@ -300,11 +302,62 @@ def test_nve_fused_update_matches_separate_kernels(reorder):
                     s.force.clone(), nb.idx.clone(), nb.reference_position.clone(),
                     nb.plan.gate.clone(), nb.plan.builds))
     runs.append(snaps)
-  for blk, (a, b) in enumerate(zip(*runs)):
-    for k in range(7):
-      assert torch.equal(a[k], b[k]), (reorder, blk, k)
-    assert a[7] == b[7], (reorder, blk)
+  for other in runs[1:]:
+    for blk, (a, b) in enumerate(zip(runs[0], other)):
+      for k in range(7):
+        assert torch.equal(a[k], b[k]), (reorder, blk, k)
+      assert a[7] == b[7], (reorder, blk)
   assert runs[1][-1][7] > runs[1][0][7]  # rebuilds happened
+
+
+def test_nve_loop_continued_blocks():
+  """Consecutive blocks start from the previous block's cell-order copies
+  (no ordering, permutation or relabelling at their start): the trajectory
+  follows blocks that all start afresh to float32 round-off over 100 steps,
+  the list holds the sets of a fresh build at its reference positions, and a
+  write to the state through torch or a list update through partition makes
+  the next block start afresh."""
+  print('case NVELoop continued blocks', flush=True)
+  jmd, R, P, L, shift, nfn, efn = _lj(12, seed=5)
+  N = R.shape[0]
+  Rt = torch.as_tensor(R, device=DEV)
+  Pt = torch.as_tensor(P, device=DEV)
+  init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
+  out = []
+  for every in (1000, 0):
+    nb = nfn.allocate(Rt)
+    s = init_fn(0, Rt, kT=1.0, momenta=Pt, neighbor=nb)
+    loop = jmd.simulate.NVELoop(efn, shift, 0.005, reorder_every=every)
+    loop.prime(s, nb)
+    for _ in range(5):
+      loop.run(s, nb, 20)
+    assert not bool(nb.did_buffer_overflow)
+    out.append((s, nb, loop))
+  (sa, nba, la), (sb, nbb, lb) = out
+  print(f'  stats continued={la.stats} fresh={lb.stats}')
+  assert la.stats['continued'] == 4 and lb.stats['continued'] == 0
+  assert nba.plan.builds > 1  # rebuilds happened inside continued blocks
+  sp = O.Space(np.float64(L))
+  dR = sp.displacement(sa.position_hi.cpu().numpy(), sb.position_hi.cpu().numpy())
+  dP = (sa.momentum - sb.momentum).abs().max().item()
+  print(f'  continued vs fresh: max |dR|={np.abs(dR).max():.3e} max |dP|={dP:.3e}')
+  assert np.abs(dR).max() < 1e-4 and dP < 1e-3
+  assert np.array_equal(sa.position.cpu().numpy(), sa.position_hi.cpu().numpy().astype(F32))
+  a = np.sort(np.where(nba.idx.cpu().numpy() < N, nba.idx.cpu().numpy(), N), axis=1)
+  fresh = nfn.allocate(nba.reference_position.clone())
+  b = np.sort(fresh.idx.cpu().numpy(), axis=1)
+  w = min(a.shape[1], b.shape[1])
+  assert np.array_equal(a[:, :w], b[:, :w])
+  # a torch write to the state, then a list update: each makes the next block fresh
+  c0 = la.stats['continued']
+  sa.momentum.mul_(1.0)
+  la.run(sa, nba, 10)
+  assert la.stats['continued'] == c0
+  la.run(sa, nba, 10)
+  assert la.stats['continued'] == c0 + 1
+  nba = nba.update(sa.position)
+  la.run(sa, nba, 10)
+  assert la.stats['continued'] == c0 + 1
 
 
 @pytest.mark.parametrize('reorder', [True, False])
