@@ -6,21 +6,25 @@ jitter N(0, 0.05), momenta N(0, kT=1) with the mean removed; LJ sigma = eps =
 1 with the C1 switch between r_on = 2.0 and r_c = 2.5, Dense neighbour list
 with skin 0.5 rebuilt on the skin criterion, dt = 0.005, float32 pair
 arithmetic, float64 master positions and energy reductions.  One step = skin
-rebuild (gated on the device) + half kick / drift + force pass with the
-closing half kick.  Steps run in blocks of 100, each block one HIP graph
-replay, with a host overflow check per block; an overflowing block is redone
-from its saved start state after re-allocating the list (the reference's
-protocol, examples/nve_neighbor_list.py:186-201).
+rebuild (gated on the device) + force pass with the closing half kick, which
+also runs the next step's skin test, half kick and drift.  Steps run in
+blocks of 100, each block one HIP graph replay, with a host overflow check per
+block; an overflowing block is redone from its saved start state after
+re-allocating the list (the reference's protocol,
+examples/nve_neighbor_list.py:186-201).
 
-Before the timed region a HIP graph is recorded for every block length the
-timed loop uses, so the timed blocks are graph replays (the line reports the
-blocks' kinds).  The force kernel's duration, the list update with and
-without a rebuild, and the block-boundary cost are measured with HIP events
-in one more (eager) 100-step block after the timed region, on the stream the
-kernels run on, with the device kept behind the host.  HBM traffic per
-launch comes from the committed counter summary of the same kernel and
-workload (profiles/k_pair_dense_lj_n262144_pmc.json), scaled to this run's
-mean row length.
+Setup, before the warm-up: the autotuner (jax_md_amd.autotune) times the
+block loop on copies of the initial state for threads per row, cell size
+and row capacity; the line records its choice.  Before the timed region a
+HIP graph is recorded for every block the timed loop uses, so the timed
+blocks are graph replays (the line reports the blocks' kinds).  The force
+kernel's duration, the list update with and without a rebuild, and the
+block-boundary cost are measured with HIP events in one more (eager)
+100-step block after the timed region, on the stream the kernels run on,
+with the device kept behind the host.  HBM traffic per launch comes from
+the committed counter summary of the same kernel and workload
+(profiles/k_pair_dense_step_lj_n262144_pmc.json), scaled to this run's mean
+row length.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -86,17 +90,29 @@ def step_breakdown(jmd, loop, state, nbrs, steps=100):
       # permutation of state and list at its start and end, event markers
       'block_boundary_us': block_us - in_steps,
       'block_us': block_us,
-      'note': 'eager block with HIP events; update = binning with the opening half step '
-              '(velocity Verlet kick + drift + skin test) + cell sort + list build, the last '
-              'two gated on the skin test; rebuild_us = open update less closed update',
+      'note': 'eager block with HIP events; update = binning, cell sort and list build, '
+              'gated on the skin test (the force launch runs the half step and the test); '
+              'rebuild_us = open update less closed update',
   }
+
+
+# The force launch of the block loop (jmd_nve_force_kick_drift): force, the
+# closing half kick and the next step's start.  Algorithmic bytes per atom:
+# the Dense row, 4 B per valid entry plus the padding entry that ends it (4);
+# float32 position read 16 (padded row); force write 12; momentum read+write
+# 24; parameters 4; float64 position read+write 48; next float32 position
+# write 12; reference position read 12 (skin test) -> 4 Kbar + 132 B, the
+# whole step's per-atom bytes apart from the gated list update.
+FORCE_KERNEL = ('k_pair_dense STEP (force + closing half kick + skin test + next half kick '
+                'and drift)')
+FORCE_STATE_BYTES = 132.0
 
 
 def counter_traffic(N, kbar):
   """HBM bytes per force launch from the committed counter summary, scaled to
   this run's mean row length: the summary's measured traffic over its own
   algorithmic bytes, times this run's algorithmic bytes.  None without it."""
-  path = os.path.join(ROOT, 'profiles', f'k_pair_dense_lj_n{N}_pmc.json')
+  path = os.path.join(ROOT, 'profiles', f'k_pair_dense_step_lj_n{N}_pmc.json')
   try:
     with open(path) as f:
       d = json.load(f)
@@ -104,8 +120,8 @@ def counter_traffic(N, kbar):
     k_prof = float(d['kbar'])
   except (OSError, KeyError, ValueError):
     return None, None
-  ratio = t / (N * (4.0 * k_prof + 60.0))
-  return ratio * N * (4.0 * kbar + 60.0), {
+  ratio = t / (N * (4.0 * k_prof + FORCE_STATE_BYTES))
+  return ratio * N * (4.0 * kbar + FORCE_STATE_BYTES), {
       'measured_bytes': t, 'measured_kbar': k_prof, 'ratio_to_algorithmic': ratio,
       'source': os.path.relpath(path, ROOT),
       'note': 'FETCH_SIZE + WRITE_SIZE of the force kernel (copy-calibrated), measured at '
@@ -560,7 +576,10 @@ def main():
   ap.add_argument('--block', type=int, default=100)
   ap.add_argument('--cpu-steps', type=int, default=250)
   ap.add_argument('--no-cpu', action='store_true')
-  ap.add_argument('--tpa', type=int, default=0, help='threads per row of the pair kernel')
+  ap.add_argument('--tpa', type=int, default=0,
+                  help='threads per row of the pair kernel (skips the autotuner)')
+  ap.add_argument('--no-tune', action='store_true',
+                  help='the reference\'s cell size and capacity, the built-in threads per row')
   ap.add_argument('--eager', action='store_true', help='no HIP graphs (host-launched steps)')
   ap.add_argument('--no-reorder', action='store_true',
                   help='run the blocks in the caller\'s particle order (no cell-order layout)')
@@ -605,13 +624,30 @@ def main():
           'general': lambda: general_section(jmd, O)}[args.section]
     print(json.dumps({args.section: fn()}), flush=True)
     return
-  if args.tpa:
-    nat.check(nat.load().jmd_set_threads_per_atom(args.tpa), 'jmd_set_threads_per_atom')
   R, P, L = O.lj_lattice(args.n_side, seed=0)
   N = R.shape[0]
   disp, shift = jmd.space.periodic(L)
-  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense)
+
+  def make_fns(**kw):
+    return jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense, **kw)
+
   Rt = torch.as_tensor(R, device=dev)
+  tuned = None
+  if args.tpa:
+    jmd.autotune.set_threads_per_atom(args.tpa)
+    nfn, efn = make_fns()
+  elif args.no_tune:
+    nfn, efn = make_fns()
+  else:
+    # setup, before the warm-up: threads per row, cell size and capacity
+    # timed on copies of the initial state (the user's state is untouched)
+    t_tune = time.perf_counter()
+    tuned = jmd.autotune.tune_nve_loop(
+        make_fns, Rt, torch.as_tensor(P, device=dev), shift, 0.005,
+        key=jmd.autotune.workload_key(Rt, L, 2.5, 0.5, jmd.partition.Dense, 'lj'))
+    tuned.apply()
+    nfn, efn = make_fns(**tuned.neighbor_kwargs)
+    log(f'autotune ({time.perf_counter() - t_tune:.1f} s): {json.dumps(tuned.as_dict())}')
   nbrs = nfn.allocate(Rt)
   init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
   state = init_fn(0, Rt.clone(), kT=1.0, momenta=torch.as_tensor(P, device=dev), neighbor=nbrs)
@@ -690,10 +726,7 @@ def main():
   kbar = float((idx < N).sum().item()) / N
   brk = step_breakdown(jmd, loop, state, nbrs)
   force_us = brk['force_us']
-  # algorithmic bytes of one force launch: idx row (4 B per valid entry) plus
-  # per-atom state: x read 16 (padded row), F write 12, p read+write 24,
-  # row-end padding entry 4, energy/params 4 -> 4 Kbar + 60 B per atom
-  force_bytes = N * (4.0 * kbar + 60.0)
+  force_bytes = N * (4.0 * kbar + FORCE_STATE_BYTES)
   achieved = force_bytes / (force_us * 1e-6) / 1e9
   log(f'{args.steps} steps in {elapsed:.3f}s: {ms * 1e3:.1f} us/step, force kernel '
       f'{force_us:.1f} us, Kbar={kbar:.2f}, list builds={builds}, reallocs={reallocs}, '
@@ -751,10 +784,13 @@ def main():
                  'list_builds': builds, 'reallocs': reallocs - reallocs_warm,
                  'max_occupancy': nbrs.max_occupancy, 'kbar': kbar,
                  'timed_blocks': timed_blocks,
-                 'graph_replays_only': timed_blocks.get('eager', 0) == 0},
+                 'graph_replays_only': timed_blocks.get('eager', 0) == 0,
+                 'tuned': (tuned.as_dict() if tuned is not None else
+                           {'threads_per_atom': args.tpa or 'built-in',
+                            'cell_size_factor': 1.0, 'capacity_multiplier': 1.25})},
       'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                    'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
-                   'kernel': 'k_pair_dense (force + closing half kick)',
+                   'kernel': FORCE_KERNEL,
                    'kernel_us': force_us, 'algorithmic_bytes': force_bytes,
                    'traffic_source': traffic_src},
       'step_breakdown': brk,

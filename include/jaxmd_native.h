@@ -304,16 +304,25 @@ int jmd_pair_energy_force_f64(const double* position, int64_t n, int32_t dim,
 /* Scratch bytes for jmd_pair_force_vjp. */
 size_t jmd_adjoint_workspace_bytes(int64_t n, int32_t dim);
 
+/* Number of float64 values in the parameter outputs of jmd_pair_force_vjp
+ * and jmd_adjoint_param_gather: [sigma block][epsilon block], each 1 value
+ * for a scalar parameter or n for a per-particle one (diameters: p_ij =
+ * (p_i + p_j) / 2, smap.py:836); -1 for species matrices (not
+ * differentiated). */
+int64_t jmd_adjoint_param_count(const jmd_pair_potential* potential, int64_t n);
+
 /* Vector-Jacobian product of the force F(R) = -dE/dR of
  * jmd_pair_energy_force (Dense lists), the gradient the reference obtains
  * from jax.grad through smap.pair_neighbor_list (quantity.py:58-60,
  * smap.py:856-979).  For a cotangent v [n, dim]:
  *   position_cotangent [n, dim] = (dF/dR)^T v  (dF/dR is symmetric)
- *   param_cotangent    [2] float64 = v^T dF/dsigma, v^T dF/depsilon
- *   energy_param_grad  [2] float64 = dE/dsigma, dE/depsilon
- * (the last two for scalar sigma and epsilon only; NULL skips them;
- * r_onset / r_cutoff are constants, energy.py:319-320).  Pair terms float32,
- * row sums float64, fixed-order reductions. */
+ *   param_cotangent    = v^T dF/dsigma, v^T dF/depsilon
+ *   energy_param_grad  = dE/dsigma, dE/depsilon
+ * the last two float64 in the layout of jmd_adjoint_param_count (scalar
+ * parameters: [2]; a per-particle sigma gives n values dF/dsigma_i first);
+ * NULL skips them; r_onset / r_cutoff are constants (energy.py:319-320).
+ * Pair terms float32, row sums float64, fixed-order reductions; the
+ * per-particle values are row sums of their own (no atomics). */
 int jmd_pair_force_vjp(const float* position, int64_t n, int32_t dim,
                        int32_t position_stride, int32_t periodic, const float* side,
                        const jmd_pair_potential* potential, const int32_t* idx,
@@ -321,8 +330,9 @@ int jmd_pair_force_vjp(const float* position, int64_t n, int32_t dim,
                        float* position_cotangent, double* param_cotangent,
                        double* energy_param_grad, void* workspace, void* stream);
 
-/* Doubles in the per-workgroup parameter accumulator of
- * jmd_nve_adjoint_step (zero-filled by the caller). */
+/* Doubles in the parameter accumulator of jmd_nve_adjoint_step
+ * (zero-filled by the caller): per-workgroup sums for scalar parameters,
+ * per-particle sums for per-particle ones. */
 size_t jmd_adjoint_param_acc_doubles(int64_t n, int32_t dim);
 
 /* One backward step of the velocity-Verlet adjoint (simulate.py:227-243
@@ -338,7 +348,6 @@ int jmd_nve_adjoint_step(const float* position, int64_t n, int32_t dim,
                          double* x_adjoint, float coef_x, float coef_p, double* param_acc,
                          float coef_param, void* stream);
 
-/* out[0..1] = fixed-order sums of the accumulator's two columns. */
 /* jmd_nve_adjoint_step for an x64 trajectory: positions float64 [n, dim]
  * (each displacement formed in float64, rounded once; the product itself in
  * float32 as in jmd_nve_adjoint_step). */
@@ -349,8 +358,16 @@ int jmd_nve_adjoint_step_x64(const double* position, int64_t n, int32_t dim,
                              double* x_adjoint, float coef_x, float coef_p,
                              double* param_acc, float coef_param, void* stream);
 
+/* out[0..1] = fixed-order sums of the accumulator's two columns (scalar
+ * sigma and epsilon). */
 int jmd_adjoint_param_sum(const double* param_acc, int64_t n, int32_t dim, double* out,
                           void* stream);
+
+/* The accumulated parameter cotangents in the layout of
+ * jmd_adjoint_param_count (scalar parameters summed in a fixed order,
+ * per-particle ones copied); param_acc keeps its values. */
+int jmd_adjoint_param_gather(double* param_acc, int64_t n, int32_t dim,
+                             const jmd_pair_potential* potential, double* out, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Ragged ensembles of 2-D systems (one persistent launch)                   */

@@ -106,6 +106,9 @@ _SIGNATURES = {
                                        c_int64, c_void_p, c_void_p, c_void_p, c_float, c_float,
                                        c_void_p, c_float, c_void_p]),
     'jmd_adjoint_param_sum': (c_int32, [c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    'jmd_adjoint_param_count': (c_int64, [POINTER(JmdPairPotential), c_int64]),
+    'jmd_adjoint_param_gather': (c_int32, [c_void_p, c_int64, c_int32,
+                                           POINTER(JmdPairPotential), c_void_p, c_void_p]),
     'jmd_nve_adjoint_step_x64': (c_int32, [c_void_p, c_int64, c_int32, c_int32,
                                            POINTER(c_float), POINTER(JmdPairPotential), c_void_p,
                                            c_int64, c_void_p, c_void_p, c_void_p, c_float,

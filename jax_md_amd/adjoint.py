@@ -16,8 +16,12 @@ force (quantity.py:58-60), of that force for the Hessian, and through
   replays one segment at a time (same kernels, same list decisions, so the
   replay is bit-identical) and runs one fused adjoint kernel per step.
 
-Only Dense lists and scalar sigma / epsilon parameters are differentiated
-with respect to parameters; r_onset / r_cutoff stay fixed as the reference
+Parameters are differentiated when scalar or per-particle (diameters,
+combined as (p_i + p_j) / 2, smap.py:836): a per-particle parameter gets one
+cotangent per particle, each a row sum of its own.  Species matrices are not
+differentiated.  Sparse and OrderedSparse lists are differentiated through
+their Dense rows (``dense_rows``: every neighbour of a particle, in id order),
+which hold the same pairs.  r_onset / r_cutoff stay fixed as the reference
 fixes them at construction (energy.py:319-320).
 """
 
@@ -48,21 +52,63 @@ def _workspace(device, n, dim):
                      device=device)
 
 
+def dense_rows(idx: torch.Tensor, N: int, fmt) -> torch.Tensor:
+  """Dense rows [N, W] (padding N) of a list: Dense as it is; Sparse
+  (receiver, sender) pairs grouped by sender; OrderedSparse pairs in both
+  orders.  Each row lists its neighbours in increasing id order."""
+  if fmt is partition.Dense:
+    return idx
+  recv, send = idx[0].long(), idx[1].long()
+  valid = (recv < N) & (send < N)
+  recv, send = recv[valid], send[valid]
+  if fmt is partition.OrderedSparse:
+    rows, cols = torch.cat([send, recv]), torch.cat([recv, send])
+  else:
+    rows, cols = send, recv
+  order = torch.argsort(rows * (N + 1) + cols)
+  rows, cols = rows[order], cols[order]
+  counts = torch.bincount(rows, minlength=N)
+  W = max(int(counts.max().item()) if rows.numel() else 0, 1)
+  start = torch.cumsum(counts, 0) - counts
+  slot = torch.arange(rows.numel(), device=idx.device) - start[rows]
+  out = torch.full((N, W), N, dtype=torch.int32, device=idx.device)
+  out[rows, slot] = cols.to(torch.int32)
+  return out
+
+
+def _param_count(pot, N) -> int:
+  c = int(nat.load().jmd_adjoint_param_count(nat.ctypes.byref(pot), N))
+  if c < 0:
+    raise NotImplementedError('parameter cotangents are computed for scalar and per-particle '
+                              'sigma / epsilon; species matrices are not differentiated')
+  return c
+
+
+def split_params(vec: torch.Tensor, pot, N):
+  """(sigma part, epsilon part) of a parameter output: a 0-d tensor for a
+  scalar parameter, [N] for a per-particle one."""
+  ns = N if pot.sigma.mode == nat.PARAM_PER_PARTICLE else 1
+  s, e = vec[:ns], vec[ns:]
+  return (s[0] if ns == 1 else s), (e[0] if e.numel() == 1 else e)
+
+
 def force_vjp(energy_fn, R, neighbor, cotangent, params: bool = True, **dynamic_kwargs):
-  """(J v [N, dim] float32, (v.dF/dsigma, v.dF/deps) float64 [2] or None,
-  (dE/dsigma, dE/deps) float64 [2] or None) for the pair energy ``energy_fn``
-  over the Dense list ``neighbor``."""
+  """(J v [N, dim] float32, v.dF/d(sigma, eps), dE/d(sigma, eps)) for the
+  pair energy ``energy_fn`` over ``neighbor`` (any format).  The parameter
+  outputs are float64 vectors [sigma block, epsilon block] (1 value for a
+  scalar parameter, N for a per-particle one; ``split_params`` splits them),
+  or None without ``params``."""
   energy = _energy(energy_fn)
-  if neighbor.format is not partition.Dense:
-    raise NotImplementedError('the force adjoint runs over Dense neighbour lists')
   lib = nat.load()
   N, dim = R.shape
   dev = R.device
+  rows = dense_rows(neighbor.idx, N, neighbor.format)
   v = cotangent.to(device=dev, dtype=torch.float32).contiguous()
   out = torch.empty((N, dim), dtype=torch.float32, device=dev)
-  pc = torch.empty((2,), dtype=torch.float64, device=dev) if params else None
-  eg = torch.empty((2,), dtype=torch.float64, device=dev) if params else None
   pot, keep = energy._potential(R, dynamic_kwargs)
+  count = _param_count(pot, N) if params else 0
+  pc = torch.empty((count,), dtype=torch.float64, device=dev) if params else None
+  eg = torch.empty((count,), dtype=torch.float64, device=dev) if params else None
   geo = energy.geometry
   if int(geo.periodic) == 2:
     raise NotImplementedError('the adjoint kernels take periodic boxes and free space; '
@@ -74,8 +120,8 @@ def force_vjp(energy_fn, R, neighbor, cotangent, params: bool = True, **dynamic_
     nat.require_device_tensor(R, 'R', torch.float64)
   nat.check(lib.jmd_pair_force_vjp(
       nat.ptr(R), N, dim, stride, int(geo.periodic),
-      nat.f3(geo.side_values(dim)), nat.ctypes.byref(pot), nat.ptr(neighbor.idx),
-      neighbor.idx.shape[1], nat.DENSE, nat.ptr(v), nat.ptr(out), nat.ptr(pc), nat.ptr(eg),
+      nat.f3(geo.side_values(dim)), nat.ctypes.byref(pot), nat.ptr(rows),
+      rows.shape[1], nat.DENSE, nat.ptr(v), nat.ptr(out), nat.ptr(pc), nat.ptr(eg),
       nat.ptr(ws), nat.stream_handle(dev)), 'jmd_pair_force_vjp')
   del keep
   return out, pc, eg
@@ -88,10 +134,13 @@ class ForceFunction(torch.autograd.Function):
   @staticmethod
   def forward(ctx, R, sigma, epsilon, energy_fn, neighbor):
     kw = {}
-    if sigma is not None:
-      kw['sigma'] = float(sigma)
-    if epsilon is not None:
-      kw['epsilon'] = float(epsilon)
+    for name, p in (('sigma', sigma), ('epsilon', epsilon)):
+      if p is None:
+        continue
+      if isinstance(p, torch.Tensor) and p.ndim > 0:
+        kw[name] = p.detach().to(device=R.device, dtype=torch.float32).contiguous()
+      else:
+        kw[name] = float(p)
     F = _energy(energy_fn).force(R.detach(), neighbor, **kw)
     ctx.save_for_backward(R.detach())
     ctx.energy_fn, ctx.neighbor, ctx.kw = energy_fn, neighbor, kw
@@ -105,14 +154,17 @@ class ForceFunction(torch.autograd.Function):
     Jv, pc, _ = force_vjp(ctx.energy_fn, R, ctx.neighbor, g, params=want, **ctx.kw)
     gs = ge = None
     if want:
-      gs = pc[0].to(torch.float32) if ctx.has[0] else None
-      ge = pc[1].to(torch.float32) if ctx.has[1] else None
+      pot, keep = _energy(ctx.energy_fn)._potential(R, ctx.kw)
+      ps, pe = split_params(pc, pot, R.shape[0])
+      del keep
+      gs = ps.to(torch.float32) if ctx.has[0] else None
+      ge = pe.to(torch.float32) if ctx.has[1] else None
     return Jv, gs, ge, None, None
 
 
 def differentiable_force(energy_fn):
   """force(R, neighbor, sigma=None, epsilon=None) differentiable in R and in
-  the (0-d tensor) parameters passed."""
+  the parameters passed (0-d tensors, or [N] per-particle tensors)."""
   def force(R, neighbor, sigma=None, epsilon=None):
     return ForceFunction.apply(R, sigma, epsilon, energy_fn, neighbor)
   return force
@@ -123,8 +175,8 @@ class TrajectoryGradient:
   energy: float                 # E(x_T), float64
   position: torch.Tensor        # dE/dx_0 [N, dim] float64
   momentum: torch.Tensor        # dE/dp_0 [N, dim] float32
-  sigma: float                  # dE/dsigma (through the trajectory and directly)
-  epsilon: float
+  sigma: object                 # dE/dsigma (through the trajectory and directly):
+  epsilon: object               # a float, or float64 [N] for a per-particle parameter
   seconds_forward: float        # forward run with checkpointing
   seconds_backward: float       # replay + adjoint
   checkpoint_bytes: int         # device memory held by checkpoints and a segment
@@ -152,9 +204,12 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
                         checkpoint_every: int = 20, mass=1.0,
                         neighbor=None) -> TrajectoryGradient:
   """Gradient of E(x_T) after ``n_steps`` velocity-Verlet steps from (R, P)
-  with respect to R, P, sigma and epsilon.  A float64 R runs the x64 form:
-  float64 trajectory (positions, momenta, forces), adjoint products formed
-  from float64 displacements.
+  with respect to R, P, sigma and epsilon (scalar or per-particle).  A
+  float64 R runs the x64 form: float64 trajectory (positions, momenta,
+  forces), adjoint products formed from float64 displacements.  Sparse and
+  OrderedSparse lists run the trajectory as given (their forces sum with
+  float atomics, so a segment's replay matches the forward run to rounding,
+  not bit for bit) and each step's adjoint over the list's Dense rows.
 
   Backward recursion (one fused kernel per step): with adjoints X of x and PH
   of the half-step momentum, X_T = -F(x_T), PH = (dt/m) X_T, and for
@@ -172,9 +227,9 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   m = float(f32(mass))
   nbrs = neighbor if neighbor is not None else neighbor_fn.allocate(
       R.to(torch.float32) if R.dtype == torch.float64 else R)
-  if nbrs.format is not partition.Dense:
-    raise NotImplementedError('the trajectory adjoint runs over Dense neighbour lists')
   x64 = R.dtype == torch.float64
+  if x64 and nbrs.format is not partition.Dense:
+    raise NotImplementedError('the x64 trajectory adjoint runs over Dense neighbour lists')
   init_fn, _ = simulate.nve(energy, shift_fn, dt)
   state = init_fn(0, R, kT=0.0, mass=mass, momenta=P, neighbor=nbrs)
   loop = simulate.NVELoop(energy, shift_fn, dt, reorder=False, graph=False)
@@ -203,6 +258,7 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   acc = torch.zeros((lib.jmd_adjoint_param_acc_doubles(N, dim),), dtype=torch.float64,
                     device=dev)
   pot, keep = energy._potential(state.position, {})
+  count = _param_count(pot, N)
   geo = energy.geometry
   if int(geo.periodic) == 2:
     raise NotImplementedError('the adjoint kernels take periodic boxes and free space; '
@@ -216,7 +272,6 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   else:
     xs = [simulate.new_positions(N, dim, dev) for _ in range(K)]
   idxs = [torch.empty_like(nbrs.idx) for _ in range(K)]
-  width = nbrs.idx.shape[1]
   ckpt_bytes = sum(sum(t.numel() * t.element_size() for t in c[2].values()) for c in ckpts)
   ckpt_bytes += sum(t.numel() * t.element_size() for t in xs + idxs)
 
@@ -231,24 +286,27 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
     for q in range(k - 1, -1, -1):
       step = start + q
       c = float(dt32) if step >= 1 else float(f32(dt32 / f32(2)))
+      rows = dense_rows(idxs[q], N, nbrs.format)
+      width = rows.shape[1]
       if x64:
         nat.check(lib.jmd_nve_adjoint_step_x64(
             nat.ptr(xs[q]), N, dim, int(geo.periodic), side, nat.ctypes.byref(pot),
-            nat.ptr(idxs[q]), width, nat.ptr(PH), nat.ptr(PH2), nat.ptr(X), c, float(dt32) / m,
+            nat.ptr(rows), width, nat.ptr(PH), nat.ptr(PH2), nat.ptr(X), c, float(dt32) / m,
             nat.ptr(acc), c, st), 'jmd_nve_adjoint_step_x64')
       else:
         nat.check(lib.jmd_nve_adjoint_step(
             nat.ptr(xs[q]), N, dim, nat.position_stride(xs[q]), int(geo.periodic), side,
-            nat.ctypes.byref(pot), nat.ptr(idxs[q]), width, nat.ptr(PH), nat.ptr(PH2),
+            nat.ctypes.byref(pot), nat.ptr(rows), width, nat.ptr(PH), nat.ptr(PH2),
             nat.ptr(X), c, float(dt32) / m, nat.ptr(acc), c, st), 'jmd_nve_adjoint_step')
       if step >= 1:
         PH, PH2 = PH2, PH
-  out = torch.empty((2,), dtype=torch.float64, device=dev)
-  nat.check(lib.jmd_adjoint_param_sum(nat.ptr(acc), N, dim, nat.ptr(out), st),
-            'jmd_adjoint_param_sum')
-  del keep
+  out = torch.empty((count,), dtype=torch.float64, device=dev)
+  nat.check(lib.jmd_adjoint_param_gather(nat.ptr(acc), N, dim, nat.ctypes.byref(pot),
+                                         nat.ptr(out), st), 'jmd_adjoint_param_gather')
   torch.cuda.synchronize(dev)
   t2 = time.perf_counter()
-  tot = (out + direct).cpu().numpy()
-  return TrajectoryGradient(float(E_T), X, PH.clone(), float(tot[0]), float(tot[1]),
+  gs, ge = split_params(out + direct, pot, N)
+  del keep
+  as_out = lambda t: float(t) if t.ndim == 0 else t
+  return TrajectoryGradient(float(E_T), X, PH.clone(), as_out(gs), as_out(ge),
                             t1 - t0, t2 - t1, int(ckpt_bytes))

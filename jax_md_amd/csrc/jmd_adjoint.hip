@@ -29,17 +29,27 @@ namespace jmd {
 
 enum { ADJ_VJP = 0, ADJ_STEP = 1 };
 
+// Per-particle parameters p_i, combined as p_ij = (p_i + p_j) / 2
+// (smap.py:836), differentiate row by row: with a symmetric list and
+// symmetric pair terms,
+//   v^T dF/dp_k = -1/2 sum_j (dg_kj/dp) (d_kj . w_kj),   w = v_j - v_k,
+//   dE/dp_k     =  1/2 sum_j dU_kj/dp,
+// so each row's owner writes particle k's entry (no scatter, deterministic).
+//
 // STRIDE 0: positions are float64 [n, DIM] (an x64 trajectory); each
 // displacement is formed in float64 and rounded once, the rest is float32.
 // One launch over all rows.  ADJ_VJP: hout = J v, part[4 b + k] = block sums
-// of (v^T dF/dsigma, v^T dF/deps, dE/dsigma, dE/deps).  ADJ_STEP:
-// X += c_x (J v) (float64), vout = v + c_p X, part[2 b + k] += c_t (v^T dF/dtheta).
+// of (v^T dF/dsigma, v^T dF/deps, dE/dsigma, dE/deps) for scalar parameters;
+// per-particle ones go to pp_out[k n + i] (same four quantities).  ADJ_STEP:
+// X += c_x (J v) (float64), vout = v + c_p X, part[2 b + k] += c_t (v^T dF/dtheta)
+// for scalar parameters, pp_out[k n + i] += c_t (v^T dF/dp_i) for per-particle
+// ones (k = 0 sigma, 1 epsilon).
 template <int DIM, int STRIDE, int KIND, int TPA, bool GENERAL, int MODE>
 __global__ __launch_bounds__(256) void k_pair_adjoint(
     const float* __restrict__ pos, int n, Space s, PairArgs a, const int* __restrict__ idx,
     int width, const float* __restrict__ vin, float* __restrict__ hout,
     double* __restrict__ part, int want_params, double* __restrict__ xadj,
-    float* __restrict__ vout, float cx, float cp, float ct) {
+    float* __restrict__ vout, float cx, float cp, float ct, double* __restrict__ pp_out) {
 #pragma clang fp contract(off)
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int gt = tile * blockDim.x + threadIdx.x;
@@ -48,6 +58,9 @@ __global__ __launch_bounds__(256) void k_pair_adjoint(
   const bool active = i < n;
   float h[3] = {0.0f, 0.0f, 0.0f};
   float ts = 0.0f, te = 0.0f, us = 0.0f, ue = 0.0f;
+  float ps = 0.0f, pe = 0.0f;  // sum_j (dg/dp) (d . w), per-particle parameters
+  const bool pp_s = GENERAL && a.sig_mode == JMD_PARAM_PER_PARTICLE;
+  const bool pp_e = GENERAL && a.eps_mode == JMD_PARAM_PER_PARTICLE;
   float xi[3] = {0, 0, 0}, xim[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
   double xi64[3] = {0.0, 0.0, 0.0};
   const double* pos64 = reinterpret_cast<const double*>(pos);
@@ -107,6 +120,10 @@ __global__ __launch_bounds__(256) void k_pair_adjoint(
         te = fmaf(D.ge, lam, te);
         us += D.us;
         ue += D.ue;
+        if (GENERAL) {
+          ps = fmaf(D.gs, dw, ps);
+          pe = fmaf(D.ge, dw, pe);
+        }
       }
     }
   }
@@ -120,7 +137,35 @@ __global__ __launch_bounds__(256) void k_pair_adjoint(
     pd[2] = group_sum<TPA>((double)us);
     pd[3] = group_sum<TPA>((double)ue);
   }
+  double pq[2] = {0.0, 0.0};
+  if (GENERAL && want_params) {
+    pq[0] = group_sum<TPA>((double)ps);
+    pq[1] = group_sum<TPA>((double)pe);
+  }
   const bool owner = active && sub == 0;
+  if (GENERAL && want_params && owner) {
+    // per-particle parameters: this row's entries (scalar ones go to part)
+    const bool pp[2] = {pp_s, pp_e};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!pp[k]) continue;
+      const double vjp = -0.5 * pq[k];
+      if (MODE == ADJ_VJP) {
+        pp_out[(long long)k * n + i] = vjp;
+        pp_out[(long long)(2 + k) * n + i] = 0.5 * pd[2 + k];
+      } else {
+        pp_out[(long long)k * n + i] += (double)ct * vjp;
+      }
+    }
+    if (pp_s) {
+      pd[0] = 0.0;
+      pd[2] = 0.0;
+    }
+    if (pp_e) {
+      pd[1] = 0.0;
+      pd[3] = 0.0;
+    }
+  }
   if (owner) {
     if (MODE == ADJ_VJP) {
 #pragma unroll
@@ -179,6 +224,24 @@ __global__ __launch_bounds__(256) void k_sum_columns(const double* __restrict__ 
   }
 }
 
+// The parameter outputs in the layout [sigma block][epsilon block], each 1
+// value (scalar parameter: the column sum `tot[k]`) or n values (per-particle:
+// pp[k n + i]); `koff` selects the columns (0: force VJP / step
+// accumulation, 2: energy gradient).
+__global__ __launch_bounds__(256) void k_param_assemble(const double* __restrict__ tot,
+                                                        const double* __restrict__ pp, int n,
+                                                        int pp_s, int pp_e, int koff,
+                                                        double* __restrict__ out) {
+  const int ns = pp_s ? n : 1, ne = pp_e ? n : 1;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ns + ne;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int k = t < ns ? 0 : 1;
+    const long long q = t < ns ? t : t - ns;
+    const bool per = k == 0 ? pp_s : pp_e;
+    out[t] = per ? pp[(long long)(koff + k) * n + q] : tot[koff + k];
+  }
+}
+
 constexpr int kAdjTPA = 8;  // lanes per row (3-D); 2-D uses 4
 
 static int adj_tpa(int dim) { return dim == 3 ? kAdjTPA : 4; }
@@ -204,6 +267,7 @@ struct AdjLaunch {
   double* xadj;
   float* vout;
   float cx, cp, ct;
+  double* pp_out;
 };
 
 template <int DIM, int STRIDE, int KIND, int TPA, int MODE>
@@ -211,11 +275,11 @@ static void launch_adj(const AdjLaunch& L) {
   if (L.a.general)
     hipLaunchKernelGGL((k_pair_adjoint<DIM, STRIDE, KIND, TPA, true, MODE>), L.grid, dim3(256), 0,
                        L.st, L.pos, L.n, L.s, L.a, L.idx, L.width, L.vin, L.hout, L.part,
-                       L.want_params, L.xadj, L.vout, L.cx, L.cp, L.ct);
+                       L.want_params, L.xadj, L.vout, L.cx, L.cp, L.ct, L.pp_out);
   else
     hipLaunchKernelGGL((k_pair_adjoint<DIM, STRIDE, KIND, TPA, false, MODE>), L.grid, dim3(256),
                        0, L.st, L.pos, L.n, L.s, L.a, L.idx, L.width, L.vin, L.hout, L.part,
-                       L.want_params, L.xadj, L.vout, L.cx, L.cp, L.ct);
+                       L.want_params, L.xadj, L.vout, L.cx, L.cp, L.ct, L.pp_out);
 }
 
 template <int MODE>
@@ -263,7 +327,19 @@ extern "C" {
 
 size_t jmd_adjoint_workspace_bytes(int64_t n, int32_t dim) {
   if (n < 0 || (dim != 2 && dim != 3)) return 0;
-  return (size_t)adj_blocks(n, dim) * 4 * sizeof(double) + 256;
+  // block partials, per-particle outputs [4][n], column sums [4]
+  return ((size_t)adj_blocks(n, dim) * 4 + 4 * (size_t)n + 4) * sizeof(double) + 256;
+}
+
+int64_t jmd_adjoint_param_count(const jmd_pair_potential* potential, int64_t n) {
+  if (!potential || n < 0) return -1;
+  int64_t c = 0;
+  for (const jmd_param* p : {&potential->sigma, &potential->epsilon}) {
+    if (p->mode == JMD_PARAM_SCALAR) c += 1;
+    else if (p->mode == JMD_PARAM_PER_PARTICLE) c += n;
+    else return -1;
+  }
+  return c;
 }
 
 int jmd_pair_force_vjp(const float* position, int64_t n, int32_t dim, int32_t position_stride,
@@ -278,12 +354,17 @@ int jmd_pair_force_vjp(const float* position, int64_t n, int32_t dim, int32_t po
   if (rc) return rc;
   if (format != JMD_DENSE) return JMD_ERR_UNSUPPORTED;
   const bool params = param_cotangent != nullptr || energy_param_grad != nullptr;
-  if (params && a.general) return JMD_ERR_UNSUPPORTED;  // scalar sigma / epsilon only
+  // scalar or per-particle sigma / epsilon (species matrices: unsupported)
+  if (params && jmd_adjoint_param_count(potential, n) < 0) return JMD_ERR_UNSUPPORTED;
   if (n > 0 && !cotangent) return JMD_ERR_INVALID;
   if (params && !workspace) return JMD_ERR_INVALID;
   hipStream_t st = (hipStream_t)stream;
   const int nb = adj_blocks(n, dim);
   double* part = (double*)workspace;
+  double* pp = part + (long long)nb * 4;
+  double* tot = pp + 4 * (long long)n;
+  const int pp_s = a.sig_mode == JMD_PARAM_PER_PARTICLE;
+  const int pp_e = a.eps_mode == JMD_PARAM_PER_PARTICLE;
   if (n == 0) {
     double* outs[2] = {param_cotangent, energy_param_grad};
     for (double* o : outs)
@@ -301,12 +382,20 @@ int jmd_pair_force_vjp(const float* position, int64_t n, int32_t dim, int32_t po
   }
   Space s = make_space(dim, periodic, side);
   AdjLaunch L{dim3(nb), st, position, (int)n, s, a, idx, (int)idx_width, cotangent, hout,
-              part, params ? 1 : 0, nullptr, nullptr, 0.0f, 0.0f, 0.0f};
+              part, params ? 1 : 0, nullptr, nullptr, 0.0f, 0.0f, 0.0f, pp};
   dispatch_adj<ADJ_VJP>(dim, position_stride, L);
   JMD_LAUNCH_CHECK();
   if (params) {
-    hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(256), 0, st, part, nb, 4, param_cotangent,
-                       energy_param_grad);
+    hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(256), 0, st, part, nb, 4, tot, tot + 2);
+    JMD_LAUNCH_CHECK();
+    const long long m = (pp_s ? n : 1) + (pp_e ? n : 1);
+    const unsigned g = (unsigned)((m + 255) / 256 < 1024 ? (m + 255) / 256 : 1024);
+    if (param_cotangent)
+      hipLaunchKernelGGL(k_param_assemble, dim3(g), dim3(256), 0, st, tot, pp, (int)n, pp_s, pp_e,
+                         0, param_cotangent);
+    if (energy_param_grad)
+      hipLaunchKernelGGL(k_param_assemble, dim3(g), dim3(256), 0, st, tot, pp, (int)n, pp_s, pp_e,
+                         2, energy_param_grad);
     JMD_LAUNCH_CHECK();
   }
   return JMD_OK;
@@ -314,7 +403,8 @@ int jmd_pair_force_vjp(const float* position, int64_t n, int32_t dim, int32_t po
 
 size_t jmd_adjoint_param_acc_doubles(int64_t n, int32_t dim) {
   if (n < 0 || (dim != 2 && dim != 3)) return 0;
-  return (size_t)adj_blocks(n, dim) * 2;
+  // block partials (scalar parameters), per-particle accumulators [2][n], sums [2]
+  return (size_t)adj_blocks(n, dim) * 2 + 2 * (size_t)n + 2;
 }
 
 int jmd_nve_adjoint_step(const float* position, int64_t n, int32_t dim, int32_t position_stride,
@@ -329,7 +419,7 @@ int jmd_nve_adjoint_step(const float* position, int64_t n, int32_t dim, int32_t 
   if (rc) return rc;
   if (n == 0) return JMD_OK;
   if (!ph_in || !ph_out || !x_adjoint || ph_in == ph_out) return JMD_ERR_INVALID;
-  if (param_acc && a.general) return JMD_ERR_UNSUPPORTED;
+  if (param_acc && jmd_adjoint_param_count(potential, n) < 0) return JMD_ERR_UNSUPPORTED;
   hipStream_t st = (hipStream_t)stream;
   {
     const int grc = space_check(dim, periodic, side, false);
@@ -338,7 +428,7 @@ int jmd_nve_adjoint_step(const float* position, int64_t n, int32_t dim, int32_t 
   Space s = make_space(dim, periodic, side);
   AdjLaunch L{dim3(adj_blocks(n, dim)), st, position, (int)n, s, a, idx, (int)idx_width, ph_in,
               nullptr, param_acc, param_acc ? 1 : 0, x_adjoint, ph_out, coef_x, coef_p,
-              coef_param};
+              coef_param, param_acc ? param_acc + (long long)adj_blocks(n, dim) * 2 : nullptr};
   dispatch_adj<ADJ_STEP>(dim, position_stride, L);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
@@ -360,6 +450,26 @@ int jmd_adjoint_param_sum(const double* param_acc, int64_t n, int32_t dim, doubl
   if (n < 0 || (dim != 2 && dim != 3) || !param_acc || !out) return JMD_ERR_INVALID;
   hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(256), 0, (hipStream_t)stream, param_acc,
                      adj_blocks(n, dim), 2, out, nullptr);
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+int jmd_adjoint_param_gather(double* param_acc, int64_t n, int32_t dim,
+                             const jmd_pair_potential* potential, double* out, void* stream) {
+  if (n < 0 || (dim != 2 && dim != 3) || !param_acc || !out) return JMD_ERR_INVALID;
+  if (jmd_adjoint_param_count(potential, n) < 0) return JMD_ERR_UNSUPPORTED;
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = adj_blocks(n, dim);
+  double* pp = param_acc + (long long)nb * 2;
+  double* tot = pp + 2 * (long long)n;
+  hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(256), 0, st, param_acc, nb, 2, tot, nullptr);
+  JMD_LAUNCH_CHECK();
+  const int pp_s = potential->sigma.mode == JMD_PARAM_PER_PARTICLE;
+  const int pp_e = potential->epsilon.mode == JMD_PARAM_PER_PARTICLE;
+  const long long m = (pp_s ? n : 1) + (pp_e ? n : 1);
+  const unsigned g = (unsigned)((m + 255) / 256 < 1024 ? (m + 255) / 256 : 1024);
+  hipLaunchKernelGGL(k_param_assemble, dim3(g > 0 ? g : 1), dim3(256), 0, st, tot, pp, (int)n,
+                     pp_s, pp_e, 0, out);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }

@@ -329,6 +329,8 @@ class _Plan:
   # rewrite the list's buffers in place without touching their tensors'
   # version counters
   epoch: int = 0
+  # (unmasked list, mask function) of a list with a custom mask
+  custom_mask: Any = None
 
   @property
   def builds(self) -> int:
@@ -386,20 +388,28 @@ class NeighborListSpec:
 
   def __init__(self, displacement_or_metric, box, r_cutoff, dr_threshold,
                capacity_multiplier, disable_cell_list, mask_self, fmt,
-               fractional_coordinates=False):
+               fractional_coordinates=False, cell_size_factor=1.0):
     self.geometry = space_mod.geometry_of(displacement_or_metric)
     self.fractional = bool(fractional_coordinates)
     rc = f32(onp.asarray(r_cutoff, f32))
     dr = f32(onp.asarray(dr_threshold, f32))
     self.cutoff = f32(rc + dr)
+    # cells at least cell_size_factor x the list cutoff (1: the reference's
+    # grid and candidate order; larger: same neighbour sets, rows in that
+    # grid's candidate order)
+    self.cell_size_factor = float(cell_size_factor)
+    if not self.cell_size_factor >= 1.0:
+      raise ValueError(f'cell_size_factor must be >= 1, got {cell_size_factor}')
+    min_size = self.cutoff if self.cell_size_factor == 1.0 else f32(
+        self.cutoff * f32(self.cell_size_factor))
     if self.fractional:
       # cells on the unit cube, sized from the box (partition.py:1045-1050)
       self.user_box = box
       self.box = onp.asarray(f32(1.0))
-      self.min_cell_size = _fractional_cell_size(box, self.cutoff)
+      self.min_cell_size = _fractional_cell_size(box, min_size)
     else:
       self.box = _box_f32(box)
-      self.min_cell_size = self.cutoff
+      self.min_cell_size = min_size
     self.cutoff_sq = f32(self.cutoff * self.cutoff)
     self.threshold_sq = f32((dr / f32(2)) * (dr / f32(2)))
     self.always_rebuild = bool(dr == 0)
@@ -422,6 +432,7 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
                   mask_self: bool = True, custom_mask_function=None,
                   fractional_coordinates: bool = False,
                   format: NeighborListFormat = NeighborListFormat.Dense,
+                  cell_size_factor: float = 1.0,
                   **static_kwargs) -> NeighborListFns:
   """Neighbour lists for a periodic or free space (partition.py:801-1164).
 
@@ -429,12 +440,23 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
   ``space.periodic_general`` with ``fractional_coordinates=True`` (cells on
   the unit cube, distances through the box matrix) and ``space.free``
   displacements, Dense / Sparse / OrderedSparse, with or without the cell
-  list, ``mask_self``.  ``custom_mask_function`` raises
-  ``NotImplementedError``.
+  list, ``mask_self``.  ``custom_mask_function`` (Dense lists) takes the
+  rows [N, W] (padding N) and returns them with masked entries set to N, as
+  the reference's does with its candidate array (partition.py:1079-1080);
+  the masked rows are compacted in order and sized with the reference's
+  capacity rule.  ``cell_size_factor`` (no counterpart in the
+  reference, where cells are the list cutoff wide, partition.py:146-188)
+  makes cells at least that multiple of the cutoff: the same neighbour sets,
+  rows in that grid's candidate order; ``autotune`` chooses it.
   """
   is_format_valid(format)
   if custom_mask_function is not None:
-    raise NotImplementedError('custom_mask_function is not supported by the native build')
+    if format is not Dense:
+      raise NotImplementedError('custom_mask_function is supported for Dense lists')
+    inner = neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold,
+                          capacity_multiplier, disable_cell_list, mask_self, None,
+                          fractional_coordinates, format, cell_size_factor, **static_kwargs)
+    return _masked_list_fns(inner, custom_mask_function, capacity_multiplier, mask_self)
   general = isinstance(space_mod.geometry_of(displacement_or_metric),
                        space_mod.GeneralDisplacement)
   if fractional_coordinates and not general:
@@ -445,7 +467,7 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
                               'coordinates: pass fractional_coordinates=True')
   spec = NeighborListSpec(displacement_or_metric, box, r_cutoff, dr_threshold,
                           capacity_multiplier, disable_cell_list, mask_self, format,
-                          fractional_coordinates)
+                          fractional_coordinates, cell_size_factor)
   fcode = _FORMAT_CODE[format]
 
   def _build(position, nbrs: Optional[NeighborList], plan: _Plan, gate):
@@ -551,6 +573,63 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
           int(plan.periodic), nat.f3(plan.side), float(spec.threshold_sq),
           nat.ptr(plan.gate), st), 'jmd_neighbor_list_check')
       _build(position, neighbors, plan, nat.ptr(plan.gate))
+    return dataclasses.replace(neighbors)
+
+  return NeighborListFns(allocate_fn, update_fn)
+
+
+def _mask_compact(raw_idx, mask_fn, N, out):
+  """Applies a custom mask to Dense rows and compacts the kept entries to
+  the left of `out` in order (padding N); returns the kept count per row."""
+  m = mask_fn(raw_idx)
+  if not isinstance(m, torch.Tensor) or tuple(m.shape) != tuple(raw_idx.shape):
+    raise ValueError('custom_mask_function must return an array of the shape it was given')
+  m = m.to(device=raw_idx.device, dtype=torch.int32)
+  keep = m < N
+  cnt = keep.sum(1)
+  pos = torch.cumsum(keep.to(torch.int32), 1) - 1
+  out.fill_(N)
+  ok = keep & (pos < out.shape[1])
+  rows = torch.arange(N, device=raw_idx.device)[:, None].expand_as(m)
+  out[rows[ok], pos[ok].long()] = m[ok]
+  return cnt
+
+
+def _masked_list_fns(inner: NeighborListFns, mask_fn, capacity_multiplier, mask_self):
+  """Dense lists with a custom mask: the unmasked list is built and updated
+  natively (gated on the skin criterion as usual); every allocate / update
+  then masks its rows and compacts them into the list the caller sees, whose
+  width follows the reference's capacity rule on the masked occupancy
+  (partition.py:1092-1104).  Both share the error word and reference
+  positions; a masked row longer than the width sets the overflow bit."""
+
+  def finish(raw, out_idx):
+    N = raw.reference_position.shape[0]
+    cnt = _mask_compact(raw.idx, mask_fn, N, out_idx)
+    over = (cnt.max() > out_idx.shape[1]) if N else torch.zeros((), dtype=torch.bool,
+                                                                device=out_idx.device)
+    raw.error.code.bitwise_or_(over.to(torch.uint8) * int(PEC.NEIGHBOR_LIST_OVERFLOW))
+    return cnt
+
+  def allocate_fn(position, extra_capacity: int = 0, **kwargs) -> NeighborList:
+    raw = inner.allocate(position, extra_capacity, **kwargs)
+    N = position.shape[0]
+    tmp = torch.empty_like(raw.idx)
+    cnt = _mask_compact(raw.idx, mask_fn, N, tmp)
+    occupancy = int(cnt.max().item()) if N else 0
+    max_occ = int(f32(occupancy) * f32(capacity_multiplier) + f32(extra_capacity))
+    max_occ = max(min(max_occ, raw.idx.shape[1], N - 1 if mask_self else N), 0)
+    idx = torch.empty((N, max_occ), dtype=torch.int32, device=position.device)
+    nbrs = dataclasses.replace(raw, idx=idx, max_occupancy=max_occ, update_fn=update_fn)
+    nbrs.plan.custom_mask = (raw, mask_fn)
+    finish(raw, idx)
+    return nbrs
+
+  def update_fn(position, neighbors: NeighborList, **kwargs) -> NeighborList:
+    raw, _ = neighbors.plan.custom_mask
+    raw = inner.update(position, raw, **kwargs)
+    neighbors.plan.custom_mask = (raw, mask_fn)
+    finish(raw, neighbors.idx)
     return dataclasses.replace(neighbors)
 
   return NeighborListFns(allocate_fn, update_fn)

@@ -391,6 +391,9 @@ class NVELoop:
     cell order of the block; OrderedSparse pairs keep receiver < sender).
     ``events`` (a ``StepEvents``) times each step's list update and force
     launch; it forces the eager path."""
+    if getattr(nbrs.plan, 'custom_mask', None) is not None:
+      raise NotImplementedError('NVELoop maintains lists without a custom mask; step a '
+                                'masked list with nve and neighbor_fn.update')
     if state.position_hi is None:
       state.position_hi = state.position.to(torch.float64)
     reorder = self._use_reorder(state, nbrs)

@@ -908,6 +908,33 @@ def force_vjp(model, space, R, v, i, j, dsigma=None):
       (0.5 * float(np.sum(us)), 0.5 * float(np.sum(ue)))
 
 
+def force_vjp_per_particle(model, space, R, v, i, j):
+  """FP64 parameter cotangents for per-particle sigma and epsilon (combined
+  as p_ij = (p_i + p_j) / 2, smap.py:836) over the ordered pairs (i, j) of a
+  list holding both orders: with F_i = sum_j g_ij d_ij and E = 1/2 sum U_ij,
+    v^T dF/dp_k = sum_(i,j) (dg_ij/dp) (v_i . d_ij) (delta_ik + delta_jk) / 2,
+    dE/dp_k     = 1/2 sum_(i,j) (dU_ij/dp) (delta_ik + delta_jk) / 2,
+  each pair's term scattered to both of its particles.  Returns
+  (cot_sigma, cot_eps, dE_dsigma, dE_deps), each float64 [N]."""
+  R = np.asarray(R, F64)
+  v = np.asarray(v, F64)
+  N = R.shape[0]
+  d = space.displacement(R[j], R[i]).astype(F64)
+  s = np.sum(d * d, -1)
+  sig = model.pair_params(i, j, 'sigma')
+  eps = model.pair_params(i, j, 'epsilon')
+  alp = model.pair_params(i, j, 'alpha') if model.kind != 'lj' else 2.0
+  _, _, gs, ge, us, ue = pair_d2(model, s, sig, eps, alp)
+  lam = np.sum(v[i] * d, -1)
+  out = []
+  for term in (gs * lam, ge * lam, 0.5 * us, 0.5 * ue):
+    acc = np.zeros(N)
+    np.add.at(acc, i, 0.5 * term)
+    np.add.at(acc, j, 0.5 * term)
+    out.append(acc)
+  return tuple(out)
+
+
 def nve_energy_after(model, space, R, P, dt, n_steps):
   """E(x_T) after n_steps of FP64 velocity Verlet with brute-force forces:
   the function whose finite differences pin the trajectory adjoint."""

@@ -1,9 +1,11 @@
 """Adjoint kernels on the GPU against the FP64 oracle and finite differences.
 
-Bars: the force VJP (J v and the scalar-parameter cotangents) within 1e-5
-norm-wise of the FP64 second-derivative oracle on the same pairs; the
+Bars: the force VJP (J v, the scalar-parameter cotangents and the
+per-particle ones) within 1e-5 norm-wise of the FP64 oracle on the same
+pairs; Sparse-format lists give the Dense list's VJP to 1e-6; the
 checkpointed NVE trajectory gradient within 1e-4 of central finite
-differences (h = 1e-4, FP64 oracle trajectory with brute-force forces).
+differences (h = 1e-4, FP64 oracle trajectory with brute-force forces), also
+with respect to per-particle diameters over an OrderedSparse list.
 """
 
 import dataclasses
@@ -59,7 +61,8 @@ def test_lj_force_vjp_matches_oracle(n_side):
 
 def test_soft_sphere_force_vjp_2d():
   """Soft spheres (alpha = 2) in 2-D: J v against the oracle; the parameter
-  cotangents are refused for per-particle diameters and given for scalars."""
+  cotangents for a scalar sigma, and one per particle for per-particle
+  diameters (against the oracle's pair-scattered form)."""
   jmd = _jmd()
   rng = np.random.default_rng(8)
   L = F32(14.0)
@@ -86,12 +89,111 @@ def test_soft_sphere_force_vjp_2d():
       assert abs(pc[0] - ts) <= 1e-5 * (abs(ts) + scale)
       assert abs(pc[1] - te) <= 1e-5 * (abs(te) + scale)
     else:
-      Jv, _, _ = jmd.adjoint.force_vjp(efn, Rt, nb, torch.as_tensor(v, device=DEV),
-                                       params=False)
-      with pytest.raises(Exception):
-        jmd.adjoint.force_vjp(efn, Rt, nb, torch.as_tensor(v, device=DEV), params=True)
+      Jv, pc, eg = jmd.adjoint.force_vjp(efn, Rt, nb, torch.as_tensor(v, device=DEV))
+      pc, eg = pc.cpu().numpy(), eg.cpu().numpy()
+      assert pc.shape == (301,) and eg.shape == (301,)  # [sigma_i] + [epsilon]
+      cs, ce, es, ee = O.force_vjp_per_particle(model, sp, R, v, i, j)
+      assert np.linalg.norm(pc[:300] - cs) <= 1e-5 * np.linalg.norm(cs)
+      assert np.linalg.norm(eg[:300] - es) <= 1e-5 * np.linalg.norm(es)
+      scale = np.sqrt(np.sum(v.astype(np.float64) ** 2)) * np.sqrt(len(i))
+      assert abs(pc[300] - te) <= 1e-5 * (abs(te) + scale)  # the scalar epsilon
     got = Jv.cpu().numpy().astype(np.float64)
     assert np.linalg.norm(got - want) <= 1e-5 * np.linalg.norm(want)
+
+
+def test_lj_force_vjp_per_particle_parameters():
+  """Per-particle sigma and epsilon (LJ, 3-D): one cotangent per particle for
+  each, against the oracle's pair-scattered FP64 form (1e-5 norm-wise), and
+  through ForceFunction's backward."""
+  print('case LJ per-particle vjp', flush=True)
+  jmd = _jmd()
+  R, _, L = O.lj_lattice(8, seed=14)
+  N = R.shape[0]
+  rng = np.random.default_rng(15)
+  sig = (1.0 + 0.05 * rng.uniform(-1, 1, N)).astype(F32)
+  eps = (1.0 + 0.2 * rng.uniform(-1, 1, N)).astype(F32)
+  disp, _ = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(
+      disp, L, sigma=torch.as_tensor(sig, device=DEV), epsilon=torch.as_tensor(eps, device=DEV),
+      format=jmd.partition.Dense)
+  Rt = torch.as_tensor(R, device=DEV)
+  nb = nfn.allocate(Rt)
+  v = rng.normal(size=R.shape).astype(F32)
+  Jv, pc, eg = jmd.adjoint.force_vjp(efn, Rt, nb, torch.as_tensor(v, device=DEV))
+  i, j = O.pairs_from_dense(nb.idx.cpu().numpy())
+  smax = float(np.max(sig))
+  model = O.PairModel('lj', sigma=sig.astype(np.float64), epsilon=eps.astype(np.float64),
+                      fixed_smoothing=(float(F32(2.0 * smax)), float(F32(2.5 * smax))))
+  sp = O.Space(np.float64(L))
+  want, _, _ = O.force_vjp(model, sp, R, v, i, j)
+  cs, ce, es, ee = O.force_vjp_per_particle(model, sp, R, v, i, j)
+  pc, eg = pc.cpu().numpy(), eg.cpu().numpy()
+  assert pc.shape == (2 * N,)
+  for got, want_k, name in ((pc[:N], cs, 'sigma'), (pc[N:], ce, 'epsilon'),
+                            (eg[:N], es, 'dE/dsigma'), (eg[N:], ee, 'dE/deps')):
+    rel = np.linalg.norm(got - want_k) / np.linalg.norm(want_k)
+    print(f'  {name}: rel err {rel:.2e}')
+    assert rel <= 1e-5
+  assert np.linalg.norm(Jv.cpu().numpy() - want) <= 1e-5 * np.linalg.norm(want)
+  # ForceFunction with per-particle tensors
+  sig_t = torch.as_tensor(sig, device=DEV).requires_grad_(True)
+  force = jmd.adjoint.differentiable_force(efn)
+  F = force(Rt, nb, sig_t, None)
+  (F * torch.as_tensor(v, device=DEV)).sum().backward()
+  assert sig_t.grad.shape == (N,)
+  assert np.linalg.norm(sig_t.grad.cpu().numpy() - cs) <= 1e-5 * np.linalg.norm(cs)
+
+
+@pytest.mark.parametrize('fmt', ['Sparse', 'OrderedSparse'])
+def test_force_vjp_over_sparse_lists(fmt):
+  """The VJP over a Sparse-format list equals the VJP over the Dense list of
+  the same pairs (J v and the scalar parameter cotangents, 1e-6)."""
+  print(f'case vjp {fmt}', flush=True)
+  jmd, R, _, L, _, nfn, efn = _lj_system(8, seed=16)
+  disp, _ = jmd.space.periodic(L)
+  nfs, efs = jmd.energy.lennard_jones_neighbor_list(disp, L, format=getattr(jmd.partition, fmt))
+  Rt = torch.as_tensor(R, device=DEV)
+  v = torch.as_tensor(np.random.default_rng(17).normal(size=R.shape).astype(F32), device=DEV)
+  Jd, pd, ed = jmd.adjoint.force_vjp(efn, Rt, nfn.allocate(Rt), v)
+  Js, ps, es = jmd.adjoint.force_vjp(efs, Rt, nfs.allocate(Rt), v)
+  assert torch.linalg.norm(Js - Jd) <= 1e-6 * torch.linalg.norm(Jd)
+  assert torch.allclose(ps, pd, rtol=1e-6, atol=0) and torch.allclose(es, ed, rtol=1e-6, atol=0)
+
+
+def test_trajectory_gradient_per_particle_diameters():
+  """d E(x_T) / d(diameters) through 30 NVE steps of 2-D bidisperse soft
+  spheres (N = 256, per-particle diameters, an OrderedSparse list) against
+  central differences of the FP64 oracle trajectory along random directions
+  of the diameter vector (h = 1e-4), 1e-4 relative."""
+  print('case trajectory gradient diameters', flush=True)
+  jmd = _jmd()
+  R, species, L = O.bidisperse_2d(256, seed=18)
+  N = R.shape[0]
+  diam = np.where(species == 0, 1.0, 1.4).astype(F32)
+  rng = np.random.default_rng(19)
+  P = (0.3 * rng.normal(size=R.shape)).astype(F32)
+  P -= P.mean(0, keepdims=True)
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.soft_sphere_neighbor_list(
+      disp, L, sigma=torch.as_tensor(diam, device=DEV), format=jmd.partition.OrderedSparse)
+  dt, T = 0.01, 30
+  res = jmd.adjoint.nve_energy_gradient(efn, shift, nfn, torch.as_tensor(R, device=DEV),
+                                        torch.as_tensor(P, device=DEV), dt, T, checkpoint_every=10)
+  g = res.sigma.cpu().numpy()
+  assert g.shape == (N,)
+  sp = O.Space(np.float64(L))
+  R64, P64 = R.astype(np.float64), P.astype(np.float64)
+  h = 1e-4
+  for trial in range(3):
+    u = rng.normal(size=N)
+    u /= np.linalg.norm(u)
+    mp = O.PairModel('soft_sphere', sigma=diam.astype(np.float64) + h * u)
+    mm = O.PairModel('soft_sphere', sigma=diam.astype(np.float64) - h * u)
+    fd = (O.nve_energy_after(mp, sp, R64, P64, dt, T)
+          - O.nve_energy_after(mm, sp, R64, P64, dt, T)) / (2 * h)
+    ad = float(np.sum(g * u))
+    print(f'  direction {trial}: adjoint {ad:.6e} fd {fd:.6e} |grad| {np.linalg.norm(g):.3e}')
+    assert abs(ad - fd) <= 1e-4 * max(abs(fd), np.linalg.norm(g))
 
 
 def test_force_function_autograd():

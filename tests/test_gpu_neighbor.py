@@ -327,3 +327,40 @@ def test_sparse_formats_follow_dense_rows_at_64000():
     assert got.shape[1] >= k
     assert np.array_equal(got[0, :k], want_r) and np.array_equal(got[1, :k], want_s)
     assert np.all(got[:, k:] == N)
+
+
+def test_custom_mask_function_known_answer_and_oracle():
+  """custom_mask_function (partition.py:1079-1080): the reference's known
+  answer (10 coincident particles, |id_i - id_j| > 3 kept: 42 entries,
+  partition_test.py:403-459), and on a periodic lattice the masked Dense rows
+  equal the oracle's entry by entry (order, width, error word) after
+  allocate and after an update that rebuilds."""
+  import jax_md_amd as jmd
+  print('case custom mask', flush=True)
+  disp, _ = jmd.space.free()
+  mask = lambda idx: torch.where((idx - torch.arange(idx.shape[0], device=idx.device)[:, None]).abs()
+                                 > 3, idx, torch.full_like(idx, idx.shape[0]))
+  nfn = jmd.partition.neighbor_list(disp, 1.0, 3.0, 0.0, custom_mask_function=mask)
+  R = torch.zeros((10, 3), dtype=torch.float32, device=DEV)
+  nb = nfn.allocate(R)
+  nb = nb.update(R)
+  assert int((nb.idx != 10).sum()) == 42
+  # periodic lattice, parity mask: keep pairs of different id parity
+  Rl, _, L = O.lj_lattice(8, seed=23)
+  N = Rl.shape[0]
+  dispp, _ = jmd.space.periodic(L)
+  pmask = lambda idx: torch.where((idx % 2) != (torch.arange(N, device=idx.device)[:, None] % 2),
+                                  idx, torch.full_like(idx, N))
+  omask = lambda cand, rows: np.where((cand % 2) != (rows[:, None] % 2), cand, N)
+  nfn = jmd.partition.neighbor_list(dispp, L, 2.5, 0.5, custom_mask_function=pmask)
+  spec = O.NeighborListSpec(O.Space(L), L, 2.5, 0.5, custom_mask_function=omask)
+  nb = nfn.allocate(torch.as_tensor(Rl, device=DEV))
+  onl = O.neighbor_list_build(spec, Rl)
+  assert np.array_equal(nb.idx.cpu().numpy(), onl.idx)
+  assert int(nb.error.code) == int(onl.error)
+  R2 = np.mod(Rl + np.random.default_rng(24).normal(0, 0.4, Rl.shape), np.float32(L)).astype(np.float32)
+  R2 = np.where(R2 >= L, np.float32(0), R2).astype(np.float32)
+  nb = nb.update(torch.as_tensor(R2, device=DEV))
+  onl2 = O.neighbor_list_update(spec, R2, onl)
+  assert np.array_equal(nb.idx.cpu().numpy(), onl2.idx)
+  assert int(nb.error.code) == int(onl2.error)

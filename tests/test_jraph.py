@@ -1,4 +1,6 @@
-"""partition.to_jraph (partition.py:1185-1250) on host tensors (CPU only).
+"""partition.to_jraph (partition.py:1185-1250), neighbor_list_mask
+(:1167-1182) and to_dense (:1253-1271) on host tensors of oracle-built lists
+(CPU only; tests/test_gpu_formats.py repeats them on GPU-built lists).
 
 Pins are properties of the conversion: without a mask the edges are the
 list's entries unchanged; with a mask the kept edges are exactly the valid,
@@ -70,3 +72,55 @@ def test_to_jraph_refuses_dense():
   jmd, nb, _ = _list(O.DENSE)
   with pytest.raises(ValueError):
     jmd.partition.to_jraph(nb)
+
+
+def test_neighbor_list_mask_properties():
+  """neighbor_list_mask (partition.py:1167-1182): the valid entries of each
+  format, and mask_self removing exactly the self pairs of a list built
+  without self-masking."""
+  import jax_md_amd as jmd
+  for fmt in (O.DENSE, O.SPARSE, O.ORDERED_SPARSE):
+    _, nb, N = _list(fmt, seed=4)
+    m = jmd.partition.neighbor_list_mask(nb)
+    ids = nb.idx[0] if fmt != O.DENSE else nb.idx
+    assert torch.equal(m, ids < N)
+  # a Dense list with self entries (mask_self=False in the build)
+  R, _, L = O.lj_lattice(5, seed=5)
+  L32 = np.asarray(L, np.float32)
+  spec = O.NeighborListSpec(O.Space(L32), L32, 2.0, 0.3, format=O.DENSE, mask_self=False)
+  o = O.neighbor_list_build(spec, R)
+  N = R.shape[0]
+  nb = jmd.partition.NeighborList(
+      idx=torch.as_tensor(o.idx), reference_position=torch.as_tensor(R),
+      error=jmd.partition.PartitionError(torch.zeros((), dtype=torch.uint8)),
+      cell_list_capacity=None, max_occupancy=o.max_occupancy, format=jmd.partition.Dense,
+      cell_size=None, cell_list_fn=None, update_fn=None)
+  all_valid = jmd.partition.neighbor_list_mask(nb)
+  no_self = jmd.partition.neighbor_list_mask(nb, mask_self=True)
+  assert int(all_valid.sum()) - int(no_self.sum()) == N
+  rows = torch.arange(N)[:, None].expand_as(nb.idx)
+  assert not torch.any(no_self & (nb.idx == rows))
+
+
+def test_to_dense_holds_each_senders_receivers():
+  """to_dense (partition.py:1253-1271) of a Sparse list: shape [N, max
+  receiver count], and row i holds exactly the receivers of i's pairs (the
+  Dense list's neighbour set of i); OrderedSparse lists are refused."""
+  import jax_md_amd as jmd
+  _, nb, N = _list(O.SPARSE, seed=6)
+  dense = jmd.partition.to_dense(nb)
+  r, s = nb.idx[0].numpy(), nb.idx[1].numpy()
+  valid = r < N
+  counts = np.bincount(r[valid], minlength=N)
+  assert dense.shape == (N, counts.max())
+  d = dense.numpy()
+  for i in range(N):
+    got = np.sort(d[i][d[i] < N])
+    want = np.sort(r[valid & (s == i)])
+    assert np.array_equal(got, want), i
+  _, nbd, _ = _list(O.DENSE, seed=6)
+  for i in range(N):
+    assert np.array_equal(np.sort(d[i][d[i] < N]), np.sort(nbd.idx[i].numpy()[nbd.idx[i].numpy() < N]))
+  _, nbo, _ = _list(O.ORDERED_SPARSE, seed=6)
+  with pytest.raises(ValueError):
+    jmd.partition.to_dense(nbo)

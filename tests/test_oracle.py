@@ -36,6 +36,9 @@ Which check pins which oracle function:
                                    would miss
   force_vjp                        central differences of the force in R,
                                    sigma, epsilon; a dropped term fails
+  force_vjp_per_particle           central differences in each particle's
+                                   sigma and epsilon (force and energy); the
+                                   row-only form fails
   fire_energy_gradient             central differences of fire_energy_after
                                    in r and x_0, also across momentum resets
   nvt_step (Nose-Hoover chain)     invariant conserved for sy = 1, 3, 5, 7 on
@@ -525,6 +528,56 @@ def test_force_vjp_matches_central_differences(kind):
     dF = (Fp - Fm) / (2 * h)
     assert abs(np.sum(v * dF) - t) <= 1e-6 * (abs(t) + np.linalg.norm(dF) * np.linalg.norm(v))
     assert abs((Ep - Em) / (2 * h) - e) <= 1e-6 * (abs(e) + 1.0)
+
+
+@pytest.mark.parametrize('kind', ['lj', 'soft_sphere'])
+def test_force_vjp_per_particle_matches_central_differences(kind):
+  """Per-particle sigma and epsilon (diameters, p_ij = (p_i + p_j) / 2):
+  v^T dF/dp_k and dE/dp_k for every k against central differences of the
+  brute-force force and energy in p_k; swapping the pair's two halves for
+  its first particle only (a dropped (delta_jk) term) fails."""
+  import dataclasses
+  R, L, model, cutoff = _d2_system(kind, 7)
+  N = R.shape[0]
+  rng = np.random.default_rng(9)
+  sig0 = float(model.sigma) * (1.0 + 0.1 * rng.uniform(-1, 1, N))
+  eps0 = float(model.epsilon) * (1.0 + 0.2 * rng.uniform(-1, 1, N))
+  fixed = model.smoothing() if kind == 'lj' else None
+  model = dataclasses.replace(model, sigma=sig0, epsilon=eps0, fixed_smoothing=fixed)
+  sp = O.Space(np.float64(L))
+  v = rng.normal(size=R.shape)
+  reach = cutoff * 1.15 + 0.3
+  i, j = O.all_pairs(sp, R, reach)
+  cs, ce, es, ee = O.force_vjp_per_particle(model, sp, R, v, i, j)
+  h = 1e-6
+
+  def EF(m):
+    ii, jj = O.all_pairs(sp, R, reach)
+    E, _, F = O.pair_energy_force(m, sp, R, ii, jj, 0.5)
+    return E, F
+
+  for name, cot, eg in (('sigma', cs, es), ('epsilon', ce, ee)):
+    base = np.asarray(getattr(model, name), np.float64)
+    for k in rng.choice(N, 6, replace=False):
+      dp = np.zeros(N)
+      dp[k] = h
+      Ep, Fp = EF(dataclasses.replace(model, **{name: base + dp}))
+      Em, Fm = EF(dataclasses.replace(model, **{name: base - dp}))
+      fd_v = np.sum(v * (Fp - Fm)) / (2 * h)
+      fd_e = (Ep - Em) / (2 * h)
+      assert abs(cot[k] - fd_v) <= 1e-5 * (abs(fd_v) + 1e-3 * np.abs(cot).max())
+      assert abs(eg[k] - fd_e) <= 1e-5 * (abs(fd_e) + 1e-3 * np.abs(eg).max())
+  # the scalar cotangent is the sum of the per-particle ones
+  _, (ts, _), _ = O.force_vjp(dataclasses.replace(model, sigma=float(np.mean(sig0))), sp,
+                              R, v, i, j)
+  assert np.isfinite(ts)
+  d = sp.displacement(R[j], R[i])
+  _, _, gs, _, _, _ = O.pair_d2(model, np.sum(d * d, -1), model.pair_params(i, j, 'sigma'),
+                                model.pair_params(i, j, 'epsilon'),
+                                model.pair_params(i, j, 'alpha') if kind != 'lj' else 2.0)
+  wrong = np.zeros(N)
+  np.add.at(wrong, i, gs * np.sum(v[i] * d, -1))
+  assert np.linalg.norm(wrong - cs) > 1e-2 * np.linalg.norm(cs)
 
 
 def test_force_vjp_dropped_term_fails():
