@@ -644,7 +644,8 @@ def main():
     t_tune = time.perf_counter()
     tuned = jmd.autotune.tune_nve_loop(
         make_fns, Rt, torch.as_tensor(P, device=dev), shift, 0.005,
-        key=jmd.autotune.workload_key(Rt, L, 2.5, 0.5, jmd.partition.Dense, 'lj'))
+        key=jmd.autotune.workload_key(Rt, L, 2.5, 0.5, jmd.partition.Dense, 'lj'),
+        cache_path=os.environ.get('JAX_MD_AMD_TUNE_CACHE'))
     tuned.apply()
     nfn, efn = make_fns(**tuned.neighbor_kwargs)
     log(f'autotune ({time.perf_counter() - t_tune:.1f} s): {json.dumps(tuned.as_dict())}')
