@@ -61,6 +61,14 @@ enum {
 /* NeighborListFormat (jax_md/partition.py:641-661). */
 enum { JMD_DENSE = 0, JMD_SPARSE = 1, JMD_ORDERED_SPARSE = 2 };
 
+/* ORed into JMD_DENSE for jmd_neighbor_list_build: Dense rows kept by the
+ * Sparse formats' pair test wrap(R_i - R_j) (partition.py:990-995) instead
+ * of Dense's two-sided test, so that row i read in order (entries j < i for
+ * OrderedSparse) is exactly the Sparse list's run of sender i.  No
+ * counterpart in the reference: the block loops run Sparse-format lists as
+ * such rows. */
+enum { JMD_DENSE_SPARSE_TEST = 0x10 };
+
 /* PartitionErrorCode bits (jax_md/partition.py:494-518). */
 enum {
   JMD_NEIGHBOR_LIST_OVERFLOW = 1,

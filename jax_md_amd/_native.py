@@ -21,6 +21,7 @@ JMD_OK, JMD_ERR_INVALID, JMD_ERR_LAUNCH, JMD_ERR_UNSUPPORTED = 0, 1, 2, 3
 # largest n jmd_pair_energy_force_noalloc accepts (JMD_PAIR_NOALLOC_MAX_N)
 PAIR_NOALLOC_MAX_N = 2000000
 DENSE, SPARSE, ORDERED_SPARSE = 0, 1, 2
+DENSE_SPARSE_TEST = 0x10  # Dense rows with the Sparse formats' pair test
 # ORed into a geometry code: every position lies in the box (positions
 # written by a wrapping shift), which lets the force kernels skip folds
 POSITIONS_IN_BOX = 0x100

@@ -1242,6 +1242,9 @@ static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
                       int* idx, long long width, int* occupancy, uint8_t* error,
                       float* ref_out, int* gate, const Work& w, hipStream_t st,
                       const KickDrift* kd = nullptr) {
+  // JMD_DENSE_SPARSE_TEST: Dense rows kept by the Sparse formats' pair test
+  const bool ij_rows = (format & JMD_DENSE_SPARSE_TEST) != 0;
+  format &= ~JMD_DENSE_SPARSE_TEST;
   const bool sparse = format != JMD_DENSE;
   const int ordered = format == JMD_ORDERED_SPARSE;
   const bool count_only = error == nullptr;
@@ -1268,7 +1271,7 @@ static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
     float c = sqrtf(cutoff_sq);
     float margin = 128.0f * (c + 1.0f) * (Lmax + c + 1.0f) * 1.1920929e-7f;
     A.margin = fmaxf(margin, 1e-4f * cutoff_sq);
-    A.test = sparse ? TEST_IJ : TEST_BOTH;
+    A.test = (sparse || ij_rows) ? TEST_IJ : TEST_BOTH;
     A.mask_self = mask_self;
     A.ordered = ordered;
     A.idx = idx;
@@ -1288,7 +1291,7 @@ static int build_impl(const float* pos, int n, const Space& s, float cutoff_sq,
   }
   // Dense without cells tests wrap(Rj - Ri) (partition.py:965-971);
   // Sparse tests wrap(Ri - Rj) (partition.py:990-995).
-  const int test = sparse ? TEST_IJ : TEST_JI;
+  const int test = (sparse || ij_rows) ? TEST_IJ : TEST_JI;
   dim3 grid(ceil_div((long long)n * kWave, 256)), block(256);
 #define JMD_ALLP(MODE)                                                                      \
   hipLaunchKernelGGL((k_nl_allpairs<DIM, STRIDE, MODE>), grid, block, 0, st, pos, n, s,      \
@@ -1449,14 +1452,16 @@ int jmd_neighbor_list_build(const float* position, int64_t n, int32_t dim,
                             int32_t* idx, int64_t max_occupancy, int32_t* occupancy,
                             uint8_t* error, float* reference_position, int32_t* gate,
                             void* workspace, void* stream) {
+  const int base_format = format == (JMD_DENSE | JMD_DENSE_SPARSE_TEST) ? JMD_DENSE : format;
   if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || max_occupancy < 0 ||
-      (format != JMD_DENSE && format != JMD_SPARSE && format != JMD_ORDERED_SPARSE) ||
+      (base_format != JMD_DENSE && base_format != JMD_SPARSE &&
+       base_format != JMD_ORDERED_SPARSE) ||
       !workspace || (n > 0 && !position) || (periodic && !side))
     return JMD_ERR_INVALID;
   if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
   if (error == nullptr && !occupancy) return JMD_ERR_INVALID;
   if (error != nullptr && max_occupancy > 0 && !idx) return JMD_ERR_INVALID;
-  if (format == JMD_DENSE && (long long)n * max_occupancy > 0x7fffffffffffLL)
+  if (base_format == JMD_DENSE && (long long)n * max_occupancy > 0x7fffffffffffLL)
     return JMD_ERR_INVALID;
   Grid g{};
   if (use_cell_list) {
@@ -1512,14 +1517,16 @@ int jmd_nve_list_update_kick_drift(
                               position_out, position_stride, momentum, reference_position,
                               threshold_sq, gate_next, stream);
   }
+  const int base_format = format == (JMD_DENSE | JMD_DENSE_SPARSE_TEST) ? JMD_DENSE : format;
   if ((dim != 2 && dim != 3) || n > 0x7fffffff || max_occupancy < 0 ||
-      (format != JMD_DENSE && format != JMD_SPARSE && format != JMD_ORDERED_SPARSE) ||
+      (base_format != JMD_DENSE && base_format != JMD_SPARSE &&
+       base_format != JMD_ORDERED_SPARSE) ||
       !workspace || (periodic && !side) || (max_occupancy > 0 && !idx))
     return JMD_ERR_INVALID;
   if (!stride_ok(dim, position_stride, position) ||
       !stride_ok(dim, position_stride, position_out))
     return JMD_ERR_INVALID;
-  if (format == JMD_DENSE && (long long)n * max_occupancy > 0x7fffffffffffLL)
+  if (base_format == JMD_DENSE && (long long)n * max_occupancy > 0x7fffffffffffLL)
     return JMD_ERR_INVALID;
   if (!cells_per_side || !cell_size || cell_capacity < 1) return JMD_ERR_INVALID;
   Grid g{};

@@ -218,6 +218,81 @@ def nve(energy_or_force_fn, shift_fn, dt=1e-3, **sim_kwargs):
   return init_fn, step_fn
 
 
+def _dense_spec(spec):
+  """A copy of a list's static configuration with the Dense format."""
+  import copy
+  from . import partition
+  d = copy.copy(spec)
+  d.format = partition.Dense
+  # rows kept by the Sparse test, read off as the Sparse list after a block
+  d.format_code = nat.DENSE | nat.DENSE_SPARSE_TEST
+  return d
+
+
+def _sparse_pairs(nbrs, N):
+  """(rows, cols) of every ordered pair of a Sparse-format list, in entry
+  order (OrderedSparse pairs in both orders, the stored one first)."""
+  from . import partition
+  recv, send = nbrs.idx[0].long(), nbrs.idx[1].long()
+  valid = (recv < N) & (send < N)
+  recv, send = recv[valid], send[valid]
+  if nbrs.format is partition.OrderedSparse:
+    return torch.cat([send, recv]), torch.cat([recv, send])
+  return send, recv
+
+
+def _sparse_row_counts(nbrs, N):
+  rows, _ = _sparse_pairs(nbrs, N)
+  counts = torch.zeros((N,), dtype=torch.int64, device=rows.device)
+  counts.scatter_add_(0, rows, torch.ones_like(rows))
+  return counts
+
+
+def _sparse_to_dense_rows(nbrs, out):
+  """Dense rows [N, W] of a Sparse-format list's pairs: row i holds the
+  partners of i in entry order (deterministic); a row longer than W sets the
+  list's overflow bit."""
+  from . import partition
+  N, W = out.shape
+  rows, cols = _sparse_pairs(nbrs, N)
+  M = rows.numel()
+  out.fill_(N)
+  if M == 0:
+    return
+  order = torch.argsort(rows * M + torch.arange(M, device=rows.device))
+  rows, cols = rows[order], cols[order]
+  counts = torch.zeros((N,), dtype=torch.int64, device=rows.device)
+  counts.scatter_add_(0, rows, torch.ones_like(rows))
+  start = torch.cumsum(counts, 0) - counts
+  slot = torch.arange(M, device=rows.device) - start[rows]
+  ok = slot < W
+  out[rows[ok], slot[ok]] = cols[ok].to(torch.int32)
+  over = (counts.max() > W).to(torch.uint8) * int(partition.PEC.NEIGHBOR_LIST_OVERFLOW)
+  nbrs.error.code.bitwise_or_(over)
+
+
+def _dense_rows_to_sparse(dense, nbrs):
+  """A Sparse-format list from Dense rows: (receiver j, sender i) for each
+  row i's entries in order (OrderedSparse: j < i), rows in order, padded
+  with N; more pairs than the list's capacity set the overflow bit."""
+  from . import partition
+  d = dense.idx
+  N = d.shape[0]
+  cap = nbrs.idx.shape[1]
+  keep = d < N
+  if nbrs.format is partition.OrderedSparse:
+    keep = keep & (d < torch.arange(N, device=d.device, dtype=d.dtype)[:, None])
+  flat = keep.reshape(-1)
+  pos = torch.cumsum(flat.to(torch.int64), 0) - 1
+  ok = flat & (pos < cap)
+  senders = torch.arange(N, device=d.device, dtype=torch.int32)[:, None].expand_as(d).reshape(-1)
+  nbrs.idx.fill_(N)
+  nbrs.idx[0, pos[ok]] = d.reshape(-1)[ok]
+  nbrs.idx[1, pos[ok]] = senders[ok]
+  over = (flat.sum() > cap).to(torch.uint8) * int(partition.PEC.NEIGHBOR_LIST_OVERFLOW)
+  nbrs.error.code.bitwise_or_(over)
+
+
 class StepEvents:
   """Per-step timing of an eager block: HIP events around each step's list
   update (with the half step when fused) and around its force launch, an
@@ -291,7 +366,8 @@ class NVELoop:
   """
 
   def __init__(self, energy_or_force_fn, shift_fn, dt, reorder: bool = True,
-               graph: bool = True, fused: bool = True, reorder_every: int = 1000):
+               graph: bool = True, fused: bool = True, reorder_every: int = 1000,
+               sparse_as_dense: bool = True):
     self.energy = quantity.native_energy_of(energy_or_force_fn)
     self.shift = native_shift(shift_fn)
     self.dt = f32(dt)
@@ -310,6 +386,10 @@ class NVELoop:
     # instead of ordering, permuting and relabelling again; the cell order is
     # recomputed once it is reorder_every steps old.
     self.reorder_every = int(reorder_every)
+    # Sparse-format lists run as a Dense copy (deterministic row sums, no
+    # float atomics), converted back to the Sparse format after each block
+    self.sparse_as_dense = bool(sparse_as_dense)
+    self._shadow = None  # [key, Dense list, Sparse list signature at the last sync]
     self._cont = None  # (signature after the last block, steps since the order was taken)
     # blocks run eagerly (host-launched), graphs recorded, blocks replayed;
     # continued: blocks that started from the previous block's cell order
@@ -394,6 +474,77 @@ class NVELoop:
     if getattr(nbrs.plan, 'custom_mask', None) is not None:
       raise NotImplementedError('NVELoop maintains lists without a custom mask; step a '
                                 'masked list with nve and neighbor_fn.update')
+    if self._via_dense(nbrs):
+      shadow = self._shadow_of(nbrs)
+      self._run_block(state, shadow, n_steps, events)
+      self._shadow_to_list(state, shadow, nbrs)
+      return state
+    return self._run_block(state, nbrs, n_steps, events)
+
+  # -- Sparse-format lists: the block runs over a Dense copy ------------------
+  def _via_dense(self, nbrs) -> bool:
+    from . import partition
+    spec = nbrs.plan.spec
+    return bool(self.sparse_as_dense and partition.is_sparse(nbrs.format)
+                and nbrs.plan.use_cell_list and not spec.always_rebuild)
+
+  @staticmethod
+  def _list_signature(nbrs):
+    def v(t):
+      return (t.data_ptr(), t._version)
+    return (v(nbrs.idx), v(nbrs.reference_position), v(nbrs.error.code), id(nbrs.plan),
+            nbrs.plan.epoch)
+
+  def _shadow_of(self, nbrs):
+    """The Dense list a Sparse-format list's block runs over: the same pairs
+    in both orders (rows built on the same grid, in the reference's candidate
+    order), sharing the list's reference positions, error word and gate.  It
+    is filled from the Sparse list when that list changed since the last
+    block, and converted back after each block."""
+    from . import partition
+    plan = nbrs.plan
+    key = (id(plan), nbrs.idx.data_ptr(), tuple(nbrs.idx.shape))
+    if self._shadow is None or self._shadow[0] != key:
+      N, dim = nbrs.reference_position.shape
+      dev = nbrs.idx.device
+      rows = _sparse_row_counts(nbrs, N)
+      longest = int(rows.max().item()) if N else 0
+      limit = N - 1 if plan.spec.mask_self else N
+      cand = (3 ** dim) * int(nbrs.cell_list_capacity or 0)
+      width = max(min(max(2 * longest, longest + 32), limit, cand), 1)
+      spec = _dense_spec(plan.spec)
+      ncell = 1
+      for c in plan.cells_per_side:
+        ncell *= int(c)
+      lib = nat.load()
+      ws = partition._Workspace(dev, lib.jmd_neighbor_workspace_bytes(
+          N, dim, ncell, int(nbrs.cell_list_capacity or 0), nat.DENSE))
+      plan2 = partition._Plan(ws, plan.gate, torch.zeros((1,), dtype=torch.int32, device=dev),
+                              plan.use_cell_list, plan.cells_per_side, plan.cell_size3,
+                              plan.side, plan.periodic, plan.dim, spec)
+      shadow = partition.NeighborList(
+          idx=torch.empty((N, width), dtype=torch.int32, device=dev),
+          reference_position=nbrs.reference_position, error=nbrs.error,
+          cell_list_capacity=nbrs.cell_list_capacity, max_occupancy=width,
+          format=partition.Dense, cell_size=nbrs.cell_size, cell_list_fn=None, update_fn=None,
+          plan=plan2)
+      self._shadow = [key, shadow, None]
+    shadow = self._shadow[1]
+    if self._shadow[2] != self._list_signature(nbrs):
+      _sparse_to_dense_rows(nbrs, shadow.idx)
+      shadow.plan.epoch += 1
+    return shadow
+
+  def _shadow_to_list(self, state, shadow, nbrs):
+    """Writes the block's list back in the Sparse format: each row's entries
+    in row order (OrderedSparse: those below the row's id), the reference's
+    sender-major layout, overflow when the pairs exceed the capacity."""
+    _dense_rows_to_sparse(shadow, nbrs)
+    self._shadow[2] = self._list_signature(nbrs)
+    if self._cont is not None:  # the conversion leaves the working copies valid
+      self._cont = (self._signature(state, shadow), self._cont[1])
+
+  def _run_block(self, state: NVEState, nbrs, n_steps: int, events=None) -> NVEState:
     if state.position_hi is None:
       state.position_hi = state.position.to(torch.float64)
     reorder = self._use_reorder(state, nbrs)
@@ -429,6 +580,8 @@ class NVELoop:
       state.position_hi = state.position.to(torch.float64)
     if not self.graph:
       return
+    if self._via_dense(nbrs):
+      nbrs = self._shadow_of(nbrs)
     reorder = self._use_reorder(state, nbrs)
     # a block that continues the last one, and (cell order only) one that
     # starts afresh: the order is recomputed every reorder_every steps
@@ -492,7 +645,7 @@ class NVELoop:
     lib = nat.load()
     plan = nbrs.plan
     spec = plan.spec
-    if spec.format_code != nat.DENSE:
+    if (spec.format_code & 0xF) != nat.DENSE:
       raise NotImplementedError('float64 states run over Dense neighbour lists')
     R = state.position
     N, dim = R.shape
@@ -525,7 +678,7 @@ class NVELoop:
         e0 = events.start()
       nat.check(lib.jmd_pair_energy_force_f64(
           nat.ptr(state.position_hi), N, dim, int(geo.periodic), dsides, nat.ctypes.byref(pot),
-          idx, width, spec.format_code, nat.ptr(state.force), None, nat.ptr(state.momentum), half,
+          idx, width, spec.format_code & 0xF, nat.ptr(state.force), None, nat.ptr(state.momentum), half,
           None, st), 'jmd_pair_energy_force_f64')
       if events is not None:
         events.stop('force', e0)
@@ -568,7 +721,7 @@ class NVELoop:
     xs = nat.position_stride(x)
     mass = state.mass if b['mass'] is None else b['mass']
     width = nbrs.idx.shape[1]
-    fmt = plan.spec.format_code
+    fmt = plan.spec.format_code & 0xF
     if not cont:
       nat.check(lib.jmd_cell_order(nat.ptr(R), N, dim, stride, nat.i3(plan.cells_per_side),
                                    nat.f3(plan.cell_size3), nat.ptr(order), nat.ptr(inv),
@@ -625,7 +778,8 @@ class NVELoop:
     pot, keep = self.energy._potential(R, {})
     self._keep = (pot, keep)  # device parameter arrays stay alive for graph replays
     pws = nat.ptr(self.energy._workspace(dev, N))
-    fmt = spec.format_code
+    bfmt = spec.format_code  # the build's format (Sparse-test flag included)
+    fmt = bfmt & 0xF
     width = nbrs.idx.shape[1]
     geo = self.energy.geometry
     esides = nat.f3(geo.side_values(dim))
@@ -652,7 +806,7 @@ class NVELoop:
           e0 = events.start()
         nat.check(lib.jmd_neighbor_list_build(
             nat.ptr(xk), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq),
-            int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0), fmt,
+            int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0), bfmt,
             int(spec.mask_self), idx, int(nbrs.max_occupancy), occ, err, ref,
             nat.ptr(gs[k % 2]), ws, st), 'jmd_neighbor_list_build')
         if events is not None:
@@ -680,7 +834,7 @@ class NVELoop:
           e0 = events.start()
         nat.check(lib.jmd_nve_list_update_kick_drift(
             nat.ptr(xin), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq), 1,
-            cps, cs, int(nbrs.cell_list_capacity or 0), fmt, int(spec.mask_self), idx,
+            cps, cs, int(nbrs.cell_list_capacity or 0), bfmt, int(spec.mask_self), idx,
             int(nbrs.max_occupancy), occ, err, ref, nat.ptr(gs[k % 2]), ws, x_hi, p, F, mv,
             mass_s if mass_v is None else 0.0, wraps, side, float(self.dt), float(self.half),
             None, nat.ptr(xout), thr, nat.ptr(gs[(k + 1) % 2]), st),
@@ -705,7 +859,7 @@ class NVELoop:
         e0 = events.start()
       nat.check(lib.jmd_neighbor_list_build(
           x, N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq), int(plan.use_cell_list),
-          cps, cs, int(nbrs.cell_list_capacity or 0), fmt, int(spec.mask_self), idx,
+          cps, cs, int(nbrs.cell_list_capacity or 0), bfmt, int(spec.mask_self), idx,
           int(nbrs.max_occupancy), occ, err, ref, gate, ws, st), 'jmd_neighbor_list_build')
       if events is not None:
         events.stop('update', e0)
@@ -967,7 +1121,8 @@ class NVTLoop(NVELoop):
     pot, keep = self.energy._potential(R, {})
     self._keep = (pot, keep)
     pws = nat.ptr(self.energy._workspace(dev, N))
-    fmt = spec.format_code
+    bfmt = spec.format_code  # the build's format (Sparse-test flag included)
+    fmt = bfmt & 0xF
     width = nbrs.idx.shape[1]
     geo = self.energy.geometry
     esides = nat.f3(geo.side_values(dim))
@@ -982,7 +1137,7 @@ class NVTLoop(NVELoop):
         e0 = events.start()
       nat.check(lib.jmd_neighbor_list_build(
           nat.ptr(R), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq),
-          int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0), fmt,
+          int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0), bfmt,
           int(spec.mask_self), idx, int(nbrs.max_occupancy), occ, err, ref, gate, ws, st),
           'jmd_neighbor_list_build')
       if events is not None:

@@ -243,6 +243,59 @@ def test_nve_loop_sparse_reordered(fmt):
   assert sorted(zip(a.tolist(), b.tolist())) == sorted(zip(fa.tolist(), fb.tolist()))
 
 
+@pytest.mark.parametrize('fmt', ['Sparse', 'OrderedSparse'])
+def test_nve_loop_sparse_lists_exact_and_deterministic(fmt):
+  """NVELoop over a Sparse-format list runs it as Dense rows kept by the
+  Sparse pair test: without cell-order execution the list handed back is,
+  entry for entry, a fresh Sparse build at its reference positions (and its
+  capacity-padded tail is padding); two runs from one state give
+  bit-identical states (no float atomics in the loop); the trajectory follows
+  the Dense list's to float32 round-off."""
+  print(f'case NVELoop exact {fmt}', flush=True)
+  jmd = _jmd()
+  R, P, L = O.lj_lattice(12, seed=12)
+  N = R.shape[0]
+  disp, shift = jmd.space.periodic(L)
+  Rt = torch.as_tensor(R, device=DEV)
+  Pt = torch.as_tensor(P, device=DEV)
+  runs = []
+  s0 = None
+  for f in (fmt, fmt, 'Dense'):
+    nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, format=getattr(jmd.partition, f))
+    nb = nfn.allocate(Rt)
+    init_fn, _ = jmd.simulate.nve(efn, shift, 0.005)
+    s = init_fn(0, Rt.clone(), 0.0, momenta=Pt, neighbor=nb)
+    if s0 is None:  # the standalone Sparse-format force sums with float atomics: one start
+      s0 = (s.position.clone(), s.momentum.clone(), s.force.clone(), s.position_hi.clone())
+    for t, v in zip((s.position, s.momentum, s.force, s.position_hi), s0):
+      t.copy_(v)
+    loop = jmd.simulate.NVELoop(efn, shift, 0.005, reorder=False)
+    loop.prime(s, nb)
+    for _ in range(3):
+      loop.run(s, nb, 25)
+    assert not bool(nb.did_buffer_overflow)
+    runs.append((s, nb, nfn))
+  (s1, nb1, nfn1), (s2, nb2, _), (s3, _, _) = runs
+  assert torch.equal(s1.position_hi, s2.position_hi) and torch.equal(s1.momentum, s2.momentum)
+  assert torch.equal(nb1.idx, nb2.idx)
+  dR = (s1.position_hi - s3.position_hi).abs()
+  dR = torch.minimum(dR, float(L) - dR).max().item()
+  print(f'  vs Dense: max |dR|={dR:.3e}')
+  assert dR < 1e-4
+  # a fresh build at the reference positions on the same cell grid and cell
+  # capacity (the slot rotation, and so the entry order, depends on both):
+  # a skin-less list of the same cutoff allocated at the same start, updated
+  nfr = jmd.partition.neighbor_list(disp, L, 3.0, 0.0, format=getattr(jmd.partition, fmt))
+  fresh = nfr.allocate(Rt)
+  assert fresh.cell_list_capacity == nb1.cell_list_capacity
+  fresh = fresh.update(nb1.reference_position.clone())
+  got, want = nb1.idx.cpu().numpy(), fresh.idx.cpu().numpy()
+  total = int((want[0] < N).sum())
+  assert int((got[0] < N).sum()) == total
+  assert np.array_equal(got[:, :total], want[:, :total])
+  assert np.all(got[:, total:] == N)
+
+
 def test_nve_loop_with_species_matches_drop_in():
   """A species-dependent potential through NVELoop (cell-order execution
   requested) follows the per-step path: arrays indexed by particle id keep the
