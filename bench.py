@@ -259,6 +259,44 @@ def adjoint_section(jmd, O, check=True):
   return out
 
 
+def plain_fire_rate(jmd, R, sp, L, sigma, fire, n_steps, block=50):
+  """The plain per-system path for one 2-D soft-sphere system (Dense list,
+  ``nbrs.update`` + ``fire_descent``'s apply_fn per step) with its steps
+  recorded in a HIP graph of ``block`` steps and replayed: seconds for
+  ``n_steps`` steps (a multiple of block), excluding allocation, warm-up and
+  capture."""
+  import torch
+  dev = 'cuda:0'
+  disp, shift = jmd.space.periodic(L)
+  nfn, efn = jmd.energy.soft_sphere_neighbor_list(
+      disp, L, species=torch.as_tensor(sp, device=dev),
+      sigma=torch.as_tensor(sigma, device=dev), format=jmd.partition.Dense)
+  init_fn, apply_fn = jmd.minimize.fire_descent(efn, shift, **fire)
+  Rt = torch.as_tensor(R, device=dev)
+  nb = nfn.allocate(Rt)
+  st = init_fn(Rt, neighbor=nb)
+  for _ in range(3):  # loads the kernels, fills the workspace caches
+    nb = nb.update(st.position)
+    st = apply_fn(st, neighbor=nb)
+  torch.cuda.synchronize()
+  fields = ('position', 'momentum', 'force', 'fire_scalars', 'fire_n_pos', 'position_hi')
+  g = torch.cuda.CUDAGraph()
+  with torch.cuda.graph(g):
+    cur, nbc = st, nb
+    for _ in range(block):
+      nbc = nbc.update(cur.position)
+      cur = apply_fn(cur, neighbor=nbc)
+    for f in fields:
+      getattr(st, f).copy_(getattr(cur, f))
+  g.replay()  # one untimed replay
+  torch.cuda.synchronize()
+  t = time.perf_counter()
+  for _ in range(max(1, n_steps // block)):
+    g.replay()
+  torch.cuda.synchronize()
+  return time.perf_counter() - t
+
+
 def batched_section(jmd, O, n_systems=512, n_steps=500, plain_sample=8, plain_steps=50):
   """Batched workload: 512 bidisperse 2-D soft-sphere systems (N log-uniform in
   [64, 4096], seed 0), 500 FIRE steps (dt_start 0.01, dt_max 0.1) in one
@@ -287,8 +325,15 @@ def batched_section(jmd, O, n_systems=512, n_steps=500, plain_sample=8, plain_st
   total = int(sizes.sum())
   value = total * n_steps / t_ens
   overflow = int((ens.did_buffer_overflow != 0).sum().item())
-  # plain per-system path on a sample spread over the sizes
+  # plain per-system path on a sample spread over the sizes: every system's
+  # steps as graph replays (the fair comparator), and host-launched
   pick = np.argsort(sizes)[np.linspace(0, n_systems - 1, plain_sample).astype(int)]
+  t_graph, graph_atoms = 0.0, 0
+  for k in pick:
+    R, sp, L = systems[k]
+    t_graph += plain_fire_rate(jmd, R, sp, L, sigma, fire, n_steps)
+    graph_atoms += len(R) * n_steps
+  plain_graph = graph_atoms / t_graph
   t_plain, steps_atoms = 0.0, 0
   for k in pick:
     R, sp, L = systems[k]
@@ -318,34 +363,50 @@ def batched_section(jmd, O, n_systems=512, n_steps=500, plain_sample=8, plain_st
     pos = ens.position[ens.system_slice(int(k))].cpu().numpy().astype(np.float64)
     E64 = O.energy_force_brute(model, O.Space(np.float64(L)), pos)[0]
     errs.append(abs(E[k] - E64) / max(abs(E64), 1e-30))
-  log(f'batched: {value:.3e} atom-steps/s in {t_ens * 1e3:.1f} ms, plain {plain:.3e}, '
-      f'x{value / plain:.1f}')
+  log(f'batched: {value:.3e} atom-steps/s in {t_ens * 1e3:.1f} ms, plain graph '
+      f'{plain_graph:.3e} (x{value / plain_graph:.1f}), plain host loop {plain:.3e}')
   return {'workload': f'{n_systems} bidisperse 2-D soft-sphere systems (N log-uniform '
                       f'64..4096, total {total}), {n_steps} FIRE steps, one launch',
           'value': value, 'unit': 'atom-steps/s', 'seconds': t_ens,
+          'plain_graph_value': plain_graph,
+          'plain_graph_sample': f'{plain_sample} systems spread over the sizes, {n_steps} '
+                                f'steps each as replays of a 50-step HIP graph, one system '
+                                f'after another',
+          'speedup_vs_plain_graph': value / plain_graph,
           'plain_per_system_value': plain,
-          'plain_sample': f'{plain_sample} systems spread over the sizes, {plain_steps} steps each',
+          'plain_sample': f'{plain_sample} systems spread over the sizes, {plain_steps} steps '
+                          f'each, host-launched',
           'speedup_vs_plain': value / plain, 'systems_overflowed': overflow,
           'energy_rel_err_vs_fp64_max': float(max(errs)) if errs else None}
 
 
-def ensemble_gradient_section(jmd, O, n_rep=64, n=8192, n_steps=300, every=25):
+def ensemble_gradient_section(jmd, O, n_rep=64, n=8192, n_steps=300, every=25, phi=0.84,
+                              plain_sample=8):
   """Ensemble-gradient workload: 64 replicas of a 2-D bidisperse soft-sphere
   packing (N = 8192 each, phi = 0.84), replica k with diameter ratio
-  r_k = 1.2 + 0.4 k / 63 (sigma_ab = [[1, (1+r)/2], [(1+r)/2, r]]), 300 FIRE
-  steps in one batched launch, then d mean_k E_k(x_T) / d r_k for every k
-  through the checkpointed FIRE adjoint (checkpoints every 25 steps).  Each
-  box is set from r_k at construction and held fixed: the gradient is taken
-  at fixed box."""
+  r_k = 1.2 + 0.4 k / 63 (sigma_ab = [[1, (1+r)/2], [(1+r)/2, r]]) in a
+  square of side L(r_k) = sqrt(N pi (1 + r_k^2) / (8 phi)), 300 FIRE steps in
+  one batched launch, then d mean_k E_k(x_T) / d r_k for every k through the
+  checkpointed FIRE adjoint (checkpoints every 25 steps).  The reported
+  derivative lets the box follow the ratio, as the reference's
+  meta-optimisation notebook does (periodic_general(L(d) I) with positions
+  in real coordinates, x_0 held fixed); the fixed-box derivative is reported
+  beside it.  The plain per-replica path (graph-replayed FIRE steps of one
+  system at a time) is timed on a sample of the replicas."""
   import numpy as np
   import torch
   r = [float(np.float32(1.2 + 0.4 * k / (n_rep - 1))) for k in range(n_rep)]
   systems = [O.bidisperse_2d(n, d_large=r[k], seed=2000 + k) for k in range(n_rep)]
   fire = dict(dt_start=0.01, dt_max=0.1)
   dsig = np.array([[0.0, 0.5], [0.5, 1.0]], np.float32)
+  side = [float(np.sqrt(n * np.pi * (1 + x * x) / (8 * phi))) for x in r]
+  dLdr = np.array([L * x / (1 + x * x) for L, x in zip(side, r)])
+
+  def sig_of(x):
+    return np.array([[1.0, (1 + x) / 2], [(1 + x) / 2, x]], np.float32)
 
   def make():
-    sig = np.stack([np.array([[1.0, (1 + x) / 2], [(1 + x) / 2, x]], np.float32) for x in r])
+    sig = np.stack([sig_of(x) for x in r])
     return jmd.ensemble.SoftSphereFireEnsemble([s[0] for s in systems], [s[2] for s in systems],
                                                [s[1] for s in systems], sigma=sig,
                                                **fire).allocate()
@@ -357,31 +418,59 @@ def ensemble_gradient_section(jmd, O, n_rep=64, n=8192, n_steps=300, every=25):
   ens.run(n_steps)
   torch.cuda.synchronize()
   t_fwd = time.perf_counter() - t
-  ens = make()
-  ens.energy_gradient(5, dsig, checkpoint_every=every)  # warm
+  make().energy_gradient(5, dsig, checkpoint_every=every, dbox=dLdr)  # warm
   ens = make()
   torch.cuda.synchronize()
   t = time.perf_counter()
-  E, th, _ = ens.energy_gradient(n_steps, dsig, checkpoint_every=every)
+  E, th, _ = ens.energy_gradient(n_steps, dsig, checkpoint_every=every, dbox=dLdr)
   torch.cuda.synchronize()
   t_grad = time.perf_counter() - t
   overflow = int((ens.did_buffer_overflow != 0).sum().item())
+  box_adj = ens.box_adjoint.cpu().numpy()
   th = th.cpu().numpy()
+  ens = make()
+  torch.cuda.synchronize()
+  t = time.perf_counter()
+  _, th_fixed, _ = ens.energy_gradient(n_steps, dsig, checkpoint_every=every)
+  torch.cuda.synchronize()
+  t_grad_fixed = time.perf_counter() - t
+  th_fixed = th_fixed.cpu().numpy()
+  # plain per-replica forward on a sample spread over the ratios
+  pick = np.linspace(0, n_rep - 1, plain_sample).astype(int)
+  t_plain = 0.0
+  for k in pick:
+    R, sp, L = systems[k]
+    t_plain += plain_fire_rate(jmd, R, sp, L, sig_of(r[k]), fire, n_steps)
+  plain = plain_sample * n * n_steps / t_plain
+  fwd = n_rep * n * n_steps / t_fwd
   out = {'workload': f'{n_rep} replicas of a 2-D bidisperse soft-sphere packing (N={n} each, '
-                     f'phi 0.84), diameter ratio r_k = 1.2 + 0.4 k/{n_rep - 1}, {n_steps} FIRE '
-                     f'steps batched, d mean E(x_T) / d r_k by the checkpointed adjoint '
-                     f'(checkpoints every {every}); boxes fixed at their initial r_k',
-         'forward_s': t_fwd, 'gradient_s': t_grad, 'ratio_vs_forward': t_grad / t_fwd,
+                     f'phi {phi}), diameter ratio r_k = 1.2 + 0.4 k/{n_rep - 1}, side L(r_k) = '
+                     f'sqrt(N pi (1 + r_k^2) / (8 phi)), {n_steps} FIRE steps batched, '
+                     f'd mean E(x_T) / d r_k by the checkpointed adjoint (checkpoints every '
+                     f'{every})',
+         'derivative': 'dmeanE_dr: the box follows r_k (through the minimum images and the '
+                       'shift folds, x_0 held in real coordinates), as the reference notebook '
+                       'differentiates periodic_general(L(d) I); dmeanE_dr_fixed_box: boxes '
+                       'held constant',
+         'forward_s': t_fwd, 'forward_atom_steps_per_s': fwd,
+         'gradient_s': t_grad, 'ratio_vs_forward': t_grad / t_fwd,
+         'gradient_fixed_box_s': t_grad_fixed,
+         'plain_per_replica_atom_steps_per_s': plain,
+         'plain_sample': f'{plain_sample} replicas spread over r, {n_steps} steps each as '
+                         f'replays of a 50-step HIP graph, one replica after another',
+         'forward_speedup_vs_plain': fwd / plain,
          'mean_energy_T': float(E.mean().item()), 'systems_overflowed': overflow,
-         'dmeanE_dr': [float(x) for x in th]}
+         'dmeanE_dr': [float(x) for x in th],
+         'dmeanE_dr_fixed_box': [float(x) for x in th_fixed],
+         'dmeanE_dside': [float(x) for x in box_adj]}
   # FIRE's dt / alpha / reset decisions make E(x_T) piecewise smooth in r: in
   # FP64, central differences over 300 steps reach the derivative only for
   # h <~ 1e-7, out of float32's reach, so parity is checked on small replicas
   # against the FP64 adjoint (itself pinned by such differences) in the tests
   out['parity'] = ('tests/test_gpu_ensemble.py: FP64 FIRE adjoint on four replicas '
-                   '(N 64..256, 30 steps), 1e-5 relative')
-  log(f'ensemble gradient: forward {t_fwd * 1e3:.1f} ms, gradient {t_grad * 1e3:.1f} ms, '
-      f'ratio {t_grad / t_fwd:.2f}')
+                   '(N 64..256, 30 steps), fixed box and box following r, 1e-5 relative')
+  log(f'ensemble gradient: forward {t_fwd * 1e3:.1f} ms ({fwd:.3e} atom-steps/s, plain '
+      f'per replica {plain:.3e}), gradient {t_grad * 1e3:.1f} ms, ratio {t_grad / t_fwd:.2f}')
   return out
 
 

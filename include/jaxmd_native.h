@@ -448,7 +448,13 @@ int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int3
  * [n][n_systems][5]).  mode: 0 initialises x_adjoint = -scale F,
  * p_adjoint = pending = 0, theta = scale dE/dtheta at (position, idx);
  * 1 runs n_steps backward steps through a tape; 2 applies the pending
- * vector at the tape's first node (call once, after the earliest segment). */
+ * vector at the tape's first node (call once, after the earliest segment).
+ * box_adjoint (optional, [n_systems] float64): the loss's derivative with
+ * respect to each system's box side L, the particles held in real
+ * coordinates (the meta-optimisation notebook's periodic_general(L(d) I,
+ * fractional_coordinates=False) under jax.grad): L moves the minimum images
+ * of the pair terms and the folds of the shift; mode 0 sets it, modes 1 and
+ * 2 add to it.  NULL: the boxes are constants. */
 int jmd_fire_ensemble_adjoint_2d(const jmd_system2d* systems, int32_t n_systems,
                                  int64_t n_total, int32_t mode, int32_t n_steps,
                                  const float* position, const float* force, const int32_t* idx,
@@ -457,7 +463,8 @@ int jmd_fire_ensemble_adjoint_2d(const jmd_system2d* systems, int32_t n_systems,
                                  const float* tape_force, const float* tape_q,
                                  const int32_t* tape_idx, const float* tape_scalars,
                                  double* x_adjoint, double* p_adjoint, float* pending,
-                                 float* scratch, double* theta, float scale, void* stream);
+                                 float* scratch, double* theta, double* box_adjoint,
+                                 float scale, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Integrators                                                               */

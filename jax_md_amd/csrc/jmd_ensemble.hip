@@ -504,16 +504,33 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
 // One product with the force Jacobian per step.  The loss is
 // scale * sum_s E_s(x_T), theta the derivative with respect to each system's
 // parameter (the species matrix's derivative coefficients dsigma/dtheta).
+//
+// Box adjoint (optional): the derivative with respect to the side L of each
+// system's square, positions held in real coordinates.  L enters through the
+// minimum image, d = x_j - x_i - L n (n the pair's integer image vector, so
+// dd/dL = -n), and through the fold of the shift, x_{k+1} = x_k + dt ph -
+// L m_k (m_k = rint((x_k - x_{k+1}) / L), a step being far shorter than L/2):
+//   Lb  = -scale/2 sum_ij g (d.n)                           at x_T,
+//   Lb += -sum_ij V_i.(g n + 2 g' (d.n) d)    at every node (V = Fb + W),
+//   Lb += -X_{k+1}.m_k                        for every step.
 // ---------------------------------------------------------------------------
 
 enum { BACK_INIT = 0, BACK_STEPS = 1, BACK_FINISH = 2 };
 
+// Integer image of a pair along one axis: the fold min_image_disp applied to
+// raw = x_j - x_i (d = raw - n L).
+__device__ __forceinline__ float image_of(float raw, float half) {
+  return raw >= half ? 1.0f : (raw < -half ? -1.0f : 0.0f);
+}
+
 // h_i = (J v)_i over the system's rows at positions x with list idx; returns
-// this thread's share of v^T dF/dtheta (rows it owns).
+// this thread's share of v^T dF/dtheta (rows it owns) and, with box, adds its
+// share of v^T dF/dL to lacc.
 __device__ double ens_hvp_rows(const EnsSys& S, const Space& sp, const PairArgs& a,
                                const float* __restrict__ dsig, const float* __restrict__ x,
                                const int* __restrict__ idx, int width, int pad,
-                               const float* __restrict__ v, double* __restrict__ X) {
+                               const float* __restrict__ v, double* __restrict__ X, bool box,
+                               double& lacc) {
 #pragma clang fp contract(off)
   double tacc = 0.0;
   for (int q = threadIdx.x; q < S.n; q += blockDim.x) {
@@ -524,7 +541,7 @@ __device__ double ens_hvp_rows(const EnsSys& S, const Space& sp, const PairArgs&
     xim[0] = xi[0] - sp.L[0];
     xim[1] = xi[1] - sp.L[1];
     const int si = a.species ? a.species[i] : 0;
-    float h0 = 0.0f, h1 = 0.0f, tp = 0.0f;
+    float h0 = 0.0f, h1 = 0.0f, tp = 0.0f, lp = 0.0f;
     const int* row = idx + (long long)i * width;
     for (int k = 0; k < width; ++k) {
       const int j = row[k];
@@ -552,10 +569,19 @@ __device__ double ens_hvp_rows(const EnsSys& S, const Space& sp, const PairArgs&
         const float lam = fmaf(vi[1], d[1], vi[0] * d[0]);
         tp = fmaf(c * D.gs, lam, tp);
       }
+      if (box) {
+        const float n0 = image_of(xj[0] - xi[0], sp.half[0]);
+        const float n1 = image_of(xj[1] - xi[1], sp.half[1]);
+        const float dn = fmaf(d[1], n1, d[0] * n0);
+        const float vn = fmaf(vi[1], n1, vi[0] * n0);
+        const float vd = fmaf(vi[1], d[1], vi[0] * d[0]);
+        lp = fmaf(D.g, vn, fmaf(2.0f * D.gp * dn, vd, lp));
+      }
     }
     X[2LL * i] += (double)h0;
     X[2LL * i + 1] += (double)h1;
     tacc += (double)tp;
+    lacc -= (double)lp;
   }
   return tacc;
 }
@@ -565,7 +591,8 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
     const float* __restrict__ x, const float* __restrict__ F, const int* __restrict__ idx,
     int width, PairArgs a0, const float* __restrict__ dsig0, EnsTape tape,
     double* __restrict__ X, double* __restrict__ P, float* __restrict__ W,
-    float* __restrict__ V, double* __restrict__ theta, float scale) {
+    float* __restrict__ V, double* __restrict__ theta, double* __restrict__ box_adj,
+    float scale) {
 #pragma clang fp contract(off)
   __shared__ double s_d[3 * (kEnsThreads / kWave)];
   const int sid = blockIdx.x;
@@ -588,15 +615,16 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
   const int pad = ntot;
   const long long n2 = 2LL * ntot;
   const long long b2 = 2LL * S.offset;
+  const bool box = box_adj != nullptr;
 
   if (mode == BACK_INIT) {  // X = -scale F, P = W = 0, theta = scale dE/dtheta
-    double e = 0.0, z1 = 0.0, z2 = 0.0;
+    double e = 0.0, el = 0.0, z2 = 0.0;
     for (int k = t; k < 2 * S.n; k += nt) {
       X[b2 + k] = -(double)scale * (double)F[b2 + k];
       P[b2 + k] = 0.0;
       W[b2 + k] = 0.0f;
     }
-    if (dsig) {
+    if (dsig || box) {
       for (int q = t; q < S.n; q += nt) {
         const int i = S.offset + q;
         float xi[2], xim[2];
@@ -618,22 +646,33 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
           const float eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
           const float alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
           const PairD2 D = pair_eval2<JMD_SOFT_SPHERE>(r2, sig, eps, alp, a);
-          e += 0.5 * (double)(dsig[si * a.ns + sj] * D.us);
+          if (dsig) e += 0.5 * (double)(dsig[si * a.ns + sj] * D.us);
+          if (box) {
+            const float n0 = image_of(xj[0] - xi[0], sp.half[0]);
+            const float n1 = image_of(xj[1] - xi[1], sp.half[1]);
+            el -= 0.5 * (double)(D.g * fmaf(d[1], n1, d[0] * n0));
+          }
         }
       }
     }
-    wg_sum3(e, z1, z2, s_d);
-    if (t == 0) theta[sid] = (double)scale * e;
+    wg_sum3(e, el, z2, s_d);
+    if (t == 0) {
+      theta[sid] = (double)scale * e;
+      if (box) box_adj[sid] = (double)scale * el;
+    }
     return;
   }
   if (mode == BACK_FINISH) {  // pending W applied at node 0 of the tape
-    const double tp = ens_hvp_rows(S, sp, a, dsig, tape.x, tape.idx, width, pad, W, X);
-    double e = tp, z1 = 0.0, z2 = 0.0;
-    wg_sum3(e, z1, z2, s_d);
-    if (t == 0) theta[sid] += e;
+    double el = 0.0, z2 = 0.0;
+    double e = ens_hvp_rows(S, sp, a, dsig, tape.x, tape.idx, width, pad, W, X, box, el);
+    wg_sum3(e, el, z2, s_d);
+    if (t == 0) {
+      theta[sid] += e;
+      if (box) box_adj[sid] += el;
+    }
     return;
   }
-  double tsum = 0.0;
+  double tsum = 0.0, lsum = 0.0;
   for (int k = K - 1; k >= 0; --k) {
     const float* sc = tape.sc + ((long long)k * nsys + sid) * kTapeScalars;
     const float dt = sc[0], alpha = sc[1], keep = sc[2];
@@ -659,20 +698,30 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
     __syncthreads();
     // (2) product with the force Jacobian at node k + 1
     tsum += ens_hvp_rows(S, sp, a, dsig, tape.x + (long long)(k + 1) * n2,
-                         tape.idx + (long long)(k + 1) * ntot * width, width, pad, V, X);
+                         tape.idx + (long long)(k + 1) * ntot * width, width, pad, V, X, box,
+                         lsum);
     __syncthreads();
-    // (3) x_{k+1} = x_k + dt ph, ph = p_k + h F_k
+    // (3) x_{k+1} = x_k + dt ph - L m_k, ph = p_k + h F_k
+    const float* xk = tape.x + (long long)k * n2;
+    const float* xk1 = tape.x + (long long)(k + 1) * n2;
     for (int m = t; m < 2 * S.n; m += nt) {
       const long long g = b2 + m;
+      if (box) {
+        const float fold = rintf((xk[g] - xk1[g]) / sp.L[m & 1]);
+        lsum -= X[g] * (double)fold;
+      }
       const double PH = P[g] + (double)dt * X[g];
       P[g] = PH;
       W[g] = (float)((double)h * PH);
     }
     __syncthreads();
   }
-  double e = tsum, z1 = 0.0, z2 = 0.0;
-  wg_sum3(e, z1, z2, s_d);
-  if (t == 0) theta[sid] += e;
+  double e = tsum, el = lsum, z2 = 0.0;
+  wg_sum3(e, el, z2, s_d);
+  if (t == 0) {
+    theta[sid] += e;
+    if (box) box_adj[sid] += el;
+  }
 }
 
 }  // namespace jmd
@@ -728,7 +777,8 @@ int jmd_fire_ensemble_adjoint_2d(const jmd_system2d* systems, int32_t n_systems,
                                  const float* tape_force, const float* tape_q,
                                  const int32_t* tape_idx, const float* tape_scalars,
                                  double* x_adjoint, double* p_adjoint, float* pending,
-                                 float* scratch, double* theta, float scale, void* stream) {
+                                 float* scratch, double* theta, double* box_adjoint,
+                                 float scale, void* stream) {
   if (n_systems < 0 || n_total < 0 || n_total >= 0x7fffffff || width < 1 || n_steps < 0 ||
       mode < BACK_INIT || mode > BACK_FINISH)
     return JMD_ERR_INVALID;
@@ -747,7 +797,7 @@ int jmd_fire_ensemble_adjoint_2d(const jmd_system2d* systems, int32_t n_systems,
   hipLaunchKernelGGL(k_fire_ensemble_back, dim3(n_systems), dim3(kEnsThreads), 0,
                      (hipStream_t)stream, reinterpret_cast<const EnsSys*>(systems), n_systems,
                      (int)n_total, mode, n_steps, position, force, idx, width, a, dsigma, tape,
-                     x_adjoint, p_adjoint, pending, scratch, theta, scale);
+                     x_adjoint, p_adjoint, pending, scratch, theta, box_adjoint, scale);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }

@@ -146,6 +146,7 @@ class SoftSphereFireEnsemble:
     self._systems = torch.empty((ctypes.sizeof(recs),), dtype=torch.uint8, device=dev)
     self.idx = None
     self.width = 0
+    self.box_adjoint = None
 
   # -- native call -------------------------------------------------------------
   def _launch(self, mode, n_steps=0, tape=None):
@@ -232,7 +233,7 @@ class SoftSphereFireEnsemble:
       getattr(self, k).copy_(snap[k])
 
   def energy_gradient(self, n_steps: int, dsigma, checkpoint_every: int = 25,
-                      scale: Optional[float] = None):
+                      scale: Optional[float] = None, dbox=None):
     """FIRE for n_steps from the allocated state, then the gradient of
     scale * sum_s E_s(x_T) (default scale 1/S: the mean final energy) with
     respect to each system's parameter theta_s, where the species matrix
@@ -241,8 +242,16 @@ class SoftSphereFireEnsemble:
     ``checkpoint_every`` steps; each segment is replayed once with a tape and
     run backward (one force-Jacobian product per step).  Returns
     (E_T float64 [S], dL/dtheta float64 [S], dL/dx_0 float64 [N_total, 2]).
-    The ensemble is left in its final state.  Boxes stay fixed: a theta that
-    should also move a box is not followed here."""
+    The ensemble is left in its final state.
+
+    ``dbox`` (None, a scalar or [S]): d side / d theta_s of each box.  With
+    it the box follows theta_s as in the meta-optimisation notebook
+    (``periodic_general(L(d) * eye(2), fractional_coordinates=False)`` with x_0
+    held in real coordinates): theta's derivative gains
+    dbox_s * d loss / d side_s, taken through the minimum images of every
+    force and of E(x_T) and through the folds of the shift.
+    ``self.box_adjoint`` then holds d loss / d side_s [S].  None: the boxes
+    are constants (the derivative at fixed box)."""
     if self.idx is None:
       raise RuntimeError('allocate() first')
     lib = nat.load()
@@ -254,6 +263,10 @@ class SoftSphereFireEnsemble:
     dsig = torch.as_tensor(onp.ascontiguousarray(ds).reshape(-1), device=dev)
     if dsig.numel() != self._sig.numel():
       raise ValueError('dsigma must be laid out like sigma')
+    box_adj = None
+    if dbox is not None:
+      db = onp.broadcast_to(onp.asarray(dbox, onp.float64), (S,))
+      box_adj = torch.zeros((S,), dtype=torch.float64, device=dev)
     K = max(1, int(checkpoint_every))
     snaps = []
     done = 0
@@ -284,7 +297,7 @@ class SoftSphereFireEnsemble:
           nat.ptr(self.force), nat.ptr(self.idx), self.width, ctypes.byref(self._pot),
           nat.ptr(dsig), nat.ptr(tape['x']), nat.ptr(tape['F']), nat.ptr(tape['q']),
           nat.ptr(tape['idx']), nat.ptr(tape['sc']), nat.ptr(X), nat.ptr(P), nat.ptr(W),
-          nat.ptr(V), nat.ptr(theta), float(scale), nat.stream_handle(dev)),
+          nat.ptr(V), nat.ptr(theta), nat.ptr(box_adj), float(scale), nat.stream_handle(dev)),
           'jmd_fire_ensemble_adjoint_2d')
 
     back(0, 0)
@@ -297,4 +310,7 @@ class SoftSphereFireEnsemble:
     self._restore(final)
     self.builds.copy_(final_builds)
     self.energy.copy_(E_T)
+    self.box_adjoint = box_adj
+    if box_adj is not None:
+      theta = theta + torch.as_tensor(db.copy(), device=dev) * box_adj
     return E_T, theta, X

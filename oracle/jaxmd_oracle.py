@@ -935,6 +935,30 @@ def force_vjp_per_particle(model, space, R, v, i, j):
   return tuple(out)
 
 
+def box_vjp(model, space, R, v, i, j):
+  """FP64 derivatives with respect to the side L of a periodic hypercube whose
+  sides all equal L, positions held in real coordinates (periodic_general(L I,
+  fractional_coordinates=False), space.py:332-472): L enters the pair terms
+  only through the minimum image d = R_j - R_i - L n, n the integer image
+  vector of the pair, so dd/dL = -n and over the ordered pairs (i, j)
+    v^T dF/dL = -sum v_i . (g n + 2 g' (d.n) d),   dE/dL = -1/2 sum g (d.n).
+  Returns (v^T dF/dL, dE/dL)."""
+  R = np.asarray(R, F64)
+  v = np.asarray(v, F64)
+  L = F64(np.asarray(space.side).reshape(-1)[0])
+  raw = R[j] - R[i]
+  d = space.displacement(R[j], R[i]).astype(F64)
+  n = np.rint((raw - d) / L)
+  s = np.sum(d * d, -1)
+  sig = model.pair_params(i, j, 'sigma')
+  eps = model.pair_params(i, j, 'epsilon')
+  alp = model.pair_params(i, j, 'alpha') if model.kind != 'lj' else 2.0
+  g, gp = pair_d2(model, s, sig, eps, alp)[:2]
+  dn = np.sum(d * n, -1)
+  vF = -np.sum(g * np.sum(v[i] * n, -1) + 2 * gp * dn * np.sum(v[i] * d, -1))
+  return float(vF), float(-0.5 * np.sum(g * dn))
+
+
 def nve_energy_after(model, space, R, P, dt, n_steps):
   """E(x_T) after n_steps of FP64 velocity Verlet with brute-force forces:
   the function whose finite differences pin the trajectory adjoint."""
@@ -1021,8 +1045,8 @@ def fire_step(state, force_fn, space, params, tape=None):
   F_dot_P = np.sum(F * P)
   alpha = state.alpha
   if tape is not None:
-    tape.append(dict(x=R, q=P, F=F, dt=F64(F32(state.dt)), alpha=F64(alpha),
-                     keep=1.0 if F_dot_P >= 0 else 0.0, pn=P_norm, fn=F_norm))
+    tape.append(dict(x=R, x_prev=state.position, q=P, F=F, dt=F64(F32(state.dt)),
+                     alpha=F64(alpha), keep=1.0 if F_dot_P >= 0 else 0.0, pn=P_norm, fn=F_norm))
   P = P + F64(alpha) * (F * P_norm / F_norm - P)
   n_pos = state.n_pos + 1 if F_dot_P >= 0 else 0
   dt = state.dt
@@ -1049,10 +1073,12 @@ def fire_energy_after(model, space, R, n_steps, params):
   return energy_force_brute(model, space, st.position)[0]
 
 
-def fire_energy_gradient(model, space, R, n_steps, params, dsigma):
+def fire_energy_gradient(model, space, R, n_steps, params, dsigma, box=False):
   """(E(x_T), dE(x_T)/dtheta, dE(x_T)/dx_0) through n_steps of FP64 FIRE from
   rest, where the species matrix sigma depends on theta with coefficients
-  ``dsigma``.  The reference would take this with jax.grad through the
+  ``dsigma``; with ``box`` a fourth entry, dE(x_T)/dL for the side L of a
+  periodic square or cube (x_0 held fixed in real coordinates, see
+  ``box_vjp``).  The reference would take this with jax.grad through the
   minimiser's loop (minimize.py:184-224); the step-size / alpha decisions and
   the F.P >= 0 reset are piecewise constant and carry no gradient.
 
@@ -1062,7 +1088,10 @@ def fire_energy_gradient(model, space, R, n_steps, params, dsigma):
     Fb = a (|q| / |F|~) G - a s1 |q| F / |F|~^3 + h Q + W
     X += J(x_{k+1}) Fb;  theta += Fb . dF/dtheta (x_{k+1})
     P  = Q + dt X;  W = h P
-  and finally X += J(x_0) W, theta += W . dF/dtheta (x_0)."""
+  and finally X += J(x_0) W, theta += W . dF/dtheta (x_0).  The box adjoint
+  collects dE/dL at x_T, Fb . dF/dL at every node, and -X . m_k for the wrap
+  x_{k+1} = x_k + dt ph - L m_k of the shift (m_k the integer fold of step k,
+  rint((x_k - x_{k+1}) / L) while a step moves less than L / 2)."""
   cutoff = float(np.max(np.asarray(model.sigma, F64)))
 
   def force(X):
@@ -1073,6 +1102,10 @@ def fire_energy_gradient(model, space, R, n_steps, params, dsigma):
     Jv, (ts, _), (es, _) = force_vjp(model, space, X, v, i, j, dsigma)
     return Jv, ts, es
 
+  def lvp(X, v):
+    i, j = all_pairs(space, X, cutoff)
+    return box_vjp(model, space, X, v, i, j)
+
   st = fire_init(R, force, params)
   x0 = st.position
   tape = []
@@ -1080,6 +1113,8 @@ def fire_energy_gradient(model, space, R, n_steps, params, dsigma):
     st = fire_step(st, force, space, params, tape)
   E_T, _, F_T = energy_force_brute(model, space, st.position)
   _, _, theta = jvp(st.position, np.zeros_like(F_T))
+  lbar = lvp(st.position, np.zeros_like(F_T))[1] if box else 0.0
+  L = F64(np.asarray(space.side).reshape(-1)[0]) if box else 1.0
   X = -F_T
   P = np.zeros_like(X)
   W = np.zeros_like(X)
@@ -1094,9 +1129,14 @@ def fire_energy_gradient(model, space, R, n_steps, params, dsigma):
     Jv, ts, _ = jvp(rec['x'], Fb)
     X = X + Jv
     theta += ts
+    if box:
+      m = np.rint((rec['x_prev'] - rec['x']) / L)
+      lbar += lvp(rec['x'], Fb)[0] - float(np.sum(X * m))
     P = Q + dt * X
     W = h * P
   Jv, ts, _ = jvp(x0, W)
+  if box:
+    return E_T, theta + ts, X + Jv, lbar + lvp(x0, W)[0]
   return E_T, theta + ts, X + Jv
 
 

@@ -165,6 +165,59 @@ def test_ensemble_gradient_matches_fp64_fire_adjoint():
     assert abs(px - po) <= 1e-5 * np.linalg.norm(Xo) * np.linalg.norm(v[sl]), (s, px, po)
 
 
+def test_ensemble_gradient_with_box_following_ratio():
+  """The meta-optimisation notebook's derivative: the box side follows the
+  diameter ratio, L(r) = sqrt(N pi (1 + r^2) / (8 phi)), with x_0 held in
+  real coordinates.  Per system, d E_s(x_T) / d side and the total
+  d E_s / d r = (fixed-box part) + dL/dr * d E_s / d side against the FP64
+  oracle's box adjoint (pinned to central differences over the side and over
+  r with both sigma and L moving); 1e-5 relative, as the fixed-box test.  The
+  oracle's runs fold particles across the boundary, so the shift's fold term
+  is exercised."""
+  jmd = _jmd()
+  sizes, ratios = [64, 150, 97, 256], [1.2, 1.3, 1.45, 1.6]
+  syst = [O.bidisperse_2d(n, d_large=r, seed=40 + k) for k, (n, r) in
+          enumerate(zip(sizes, ratios))]
+  sig = np.stack([_sigma_r(r) for r in ratios])
+  T, phi = 30, 0.84
+  dLdr = np.array([np.sqrt(n * np.pi * (1 + r * r) / (8 * phi)) * r / (1 + r * r)
+                   for n, r in zip(sizes, ratios)])
+  print('case ensemble gradient, box following the ratio', flush=True)
+
+  def make():
+    return jmd.ensemble.SoftSphereFireEnsemble(
+        [s[0] for s in syst], [s[2] for s in syst], [s[1] for s in syst], sigma=sig,
+        **FIRE).allocate()
+
+  _, th_fixed, X_fixed = make().energy_gradient(T, DSIGMA, checkpoint_every=7, scale=1.0)
+  ens = make()
+  E, th, X = ens.energy_gradient(T, DSIGMA, checkpoint_every=7, scale=1.0, dbox=dLdr)
+  assert int(ens.did_buffer_overflow.max()) == 0
+  lb = ens.box_adjoint.cpu().numpy()
+  # the box terms leave dE/dx_0 and the fixed-box part as they were
+  assert torch.equal(X, X_fixed)
+  assert np.array_equal(th.cpu().numpy(), th_fixed.cpu().numpy() + dLdr * lb)
+  th = th.cpu().numpy()
+  par = O.FireParams(dt_start=0.01, dt_max=0.1)
+  folds = 0
+  for s, ((R, sp, L), r) in enumerate(zip(syst, ratios)):
+    model = O.PairModel('soft_sphere', sigma=_sigma_r(r).astype(np.float64), species=sp)
+    space = O.Space(np.float64(L))
+    R64 = R.astype(np.float64)
+    _, tho, _, lbo = O.fire_energy_gradient(model, space, R64, T, par, DSIGMA, box=True)
+    force = lambda x: O.energy_force_brute(model, space, x)[2]
+    st, tape = O.fire_init(R64, force, par), []
+    for _ in range(T):
+      st = O.fire_step(st, force, space, par, tape)
+    folds += sum(int(np.abs(np.rint((t['x_prev'] - t['x']) / np.float64(L))).sum())
+                 for t in tape)
+    print(s, lb[s], lbo, th[s], tho + dLdr[s] * lbo, flush=True)
+    assert abs(lb[s] - lbo) <= 1e-5 * abs(lbo), (s, lb[s], lbo)
+    total = tho + dLdr[s] * lbo
+    assert abs(th[s] - total) <= 1e-5 * abs(total), (s, th[s], total)
+  assert folds >= 3
+
+
 def test_ensemble_gradient_mean_energy_scale():
   """scale defaults to 1 / S (the mean final energy)."""
   jmd = _jmd()

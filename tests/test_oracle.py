@@ -645,6 +645,47 @@ def test_fire_adjoint_across_momentum_resets():
   assert abs(th - fd) <= 1e-7 * abs(fd), (th, fd)
 
 
+def test_fire_box_adjoint_matches_central_differences():
+  """dE(x_T)/dL for the side L of the periodic square, x_0 held in real
+  coordinates (the meta-optimisation notebook's periodic_general(L(d) I,
+  fractional_coordinates=False)), against central differences of
+  fire_energy_after over L; then the derivative along the diameter ratio with
+  the box following it, L(r) = sqrt(N pi (1 + r^2) / (8 phi)), against
+  differences of E(r) with both sigma and L moving.  The run folds particles
+  across the boundary 13 times; leaving out the fold term -X.m_k moves dE/dL
+  by 24%, leaving out the pair image term (n = 0) changes dE/dL at fixed x."""
+  r, T, h, n, phi = 1.4, 40, 1e-6, 40, 0.84
+  R, L, model = _fire_system(r, seed=5)
+  L = np.float64(L)
+  par = O.FireParams(dt_start=0.1, dt_max=0.4)
+  ds = np.array([[0.0, 0.5], [0.5, 1.0]])
+  force = lambda X: O.energy_force_brute(model, O.Space(L), X)[2]
+  st, tape = O.fire_init(R, force, par), []
+  for _ in range(T):
+    st = O.fire_step(st, force, O.Space(L), par, tape)
+  assert sum(int(np.abs(np.rint((t['x_prev'] - t['x']) / L)).sum()) for t in tape) >= 10
+  E, th, X, lb = O.fire_energy_gradient(model, O.Space(L), R, T, par, ds, box=True)
+  E3, th3, X3 = O.fire_energy_gradient(model, O.Space(L), R, T, par, ds)
+  assert E == E3 and th == th3 and np.array_equal(X, X3)
+  fd = (O.fire_energy_after(model, O.Space(L + h), R, T, par)
+        - O.fire_energy_after(model, O.Space(L - h), R, T, par)) / (2 * h)
+  assert abs(lb - fd) <= 1e-7 * abs(fd), (lb, fd)
+  # static part: dE/dL at fixed positions
+  i, j = O.all_pairs(O.Space(L), R, r)
+  dEdL = O.box_vjp(model, O.Space(L), R, np.zeros_like(R), i, j)[1]
+  fds = (O.energy_force_brute(model, O.Space(L + h), R)[0]
+         - O.energy_force_brute(model, O.Space(L - h), R)[0]) / (2 * h)
+  assert abs(dEdL - fds) <= 1e-7 * abs(fds) and abs(dEdL) > 0.1
+  # the box following the ratio
+  Lr = lambda x: np.sqrt(n * np.pi * (1 + x * x) / (8 * phi))
+  _, th2, _, lb2 = O.fire_energy_gradient(model, O.Space(Lr(r)), R, T, par, ds, box=True)
+  total = th2 + lb2 * Lr(r) * r / (1 + r * r)
+  Er = lambda x: O.fire_energy_after(_fire_system(x, seed=5)[2], O.Space(Lr(x)), R, T, par)
+  fdt = (Er(r + h) - Er(r - h)) / (2 * h)
+  assert abs(total - fdt) <= 1e-7 * abs(fdt), (total, fdt)
+  assert abs(total - th2) > 0.1 * abs(fdt)  # a different derivative from the fixed box
+
+
 def _jammed_nvt_setup(kT=1e-3, seed=0):
   st = fixtures.jammed_state()
   L = np.diag(st['box']).astype(np.float64)
