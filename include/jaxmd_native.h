@@ -290,6 +290,18 @@ int jmd_pair_virial(const float* position, int64_t n, int32_t dim, int32_t posit
                     const int32_t* idx, int64_t idx_width, int32_t format, double* virial,
                     void* workspace, void* stream);
 
+/* Per-entry pair energies: smap.pair_neighbor_list with reduce_axis=()
+ * (smap.py:940-953), the energy of each list entry divided by the
+ * reference's normalization (2 for Dense and Sparse, 1 for OrderedSparse),
+ * 0 for padding entries.  out: device float32, [n, idx_width] for Dense
+ * (row-major, like idx), [idx_width] for the Sparse formats.  Same
+ * position, geometry, potential and list arguments as jmd_pair_energy_force;
+ * no workspace. */
+int jmd_pair_edge_energies(const float* position, int64_t n, int32_t dim,
+                           int32_t position_stride, int32_t periodic, const float* side,
+                           const jmd_pair_potential* potential, const int32_t* idx,
+                           int64_t idx_width, int32_t format, float* out, void* stream);
+
 /* The x64 form (the reference's JAX_ENABLE_X64 run): float64 positions,
  * forces, momenta and pair arithmetic, parameters float32-valued as
  * maybe_downcast leaves them (util.py:109-112).  Dense lists only

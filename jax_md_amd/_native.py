@@ -141,6 +141,9 @@ _SIGNATURES = {
     'jmd_pair_virial': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
                                   POINTER(c_float), POINTER(JmdPairPotential), c_void_p, c_int64,
                                   c_int32, c_void_p, c_void_p, c_void_p]),
+    'jmd_pair_edge_energies': (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_int32,
+                                         POINTER(c_float), POINTER(JmdPairPotential), c_void_p,
+                                         c_int64, c_int32, c_void_p, c_void_p]),
     'jmd_pair_energy_force_f64': (c_int32, [c_void_p, c_int64, c_int32, c_int32,
                                             POINTER(c_double), POINTER(JmdPairPotential),
                                             c_void_p, c_int64, c_int32, c_void_p, c_void_p,

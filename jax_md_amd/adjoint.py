@@ -52,28 +52,7 @@ def _workspace(device, n, dim):
                      device=device)
 
 
-def dense_rows(idx: torch.Tensor, N: int, fmt) -> torch.Tensor:
-  """Dense rows [N, W] (padding N) of a list: Dense as it is; Sparse
-  (receiver, sender) pairs grouped by sender; OrderedSparse pairs in both
-  orders.  Each row lists its neighbours in increasing id order."""
-  if fmt is partition.Dense:
-    return idx
-  recv, send = idx[0].long(), idx[1].long()
-  valid = (recv < N) & (send < N)
-  recv, send = recv[valid], send[valid]
-  if fmt is partition.OrderedSparse:
-    rows, cols = torch.cat([send, recv]), torch.cat([recv, send])
-  else:
-    rows, cols = send, recv
-  order = torch.argsort(rows * (N + 1) + cols)
-  rows, cols = rows[order], cols[order]
-  counts = torch.bincount(rows, minlength=N)
-  W = max(int(counts.max().item()) if rows.numel() else 0, 1)
-  start = torch.cumsum(counts, 0) - counts
-  slot = torch.arange(rows.numel(), device=idx.device) - start[rows]
-  out = torch.full((N, W), N, dtype=torch.int32, device=idx.device)
-  out[rows, slot] = cols.to(torch.int32)
-  return out
+dense_rows = partition.dense_rows
 
 
 def _param_count(pot, N) -> int:

@@ -328,6 +328,55 @@ __global__ __launch_bounds__(256) void k_virial_sum(const double* __restrict__ p
   }
 }
 
+// Per-entry pair energies (reduce_axis=() of smap.pair_neighbor_list,
+// smap.py:940-953): one thread per list entry, U(r_ij) * scale for a valid
+// entry, 0 for padding.  Dense: entry t = i * width + k; Sparse: entry k of
+// the [2, width] list (receiver idx[k], sender idx[width + k]).
+template <int DIM, int STRIDE, int KIND, bool GENERAL, bool DENSE>
+__global__ __launch_bounds__(256) void k_pair_edges(const float* __restrict__ pos, int n,
+                                                    Space s, PairArgs a,
+                                                    const int* __restrict__ idx,
+                                                    long long width, float scale,
+                                                    float* __restrict__ out) {
+#pragma clang fp contract(off)
+  const long long total = DENSE ? (long long)n * width : width;
+  const long long nth = (long long)gridDim.x * blockDim.x;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += nth) {
+    unsigned ii, jj;
+    if (DENSE) {
+      ii = (unsigned)(t / width);
+      jj = (unsigned)idx[t];
+    } else {
+      ii = (unsigned)idx[t];
+      jj = (unsigned)idx[width + t];
+    }
+    float u = 0.0f;
+    if (ii < (unsigned)n && jj < (unsigned)n) {
+      float xi[3] = {0, 0, 0}, xj[3] = {0, 0, 0}, d[3] = {0, 0, 0};
+      load_pos<DIM, STRIDE>(pos, ii, xi);
+      load_pos<DIM, STRIDE>(pos, jj, xj);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+        d[c] = s.periodic ? min_image_any(xj[c], xi[c], xi[c] - s.L[c], s.L[c], s.half[c],
+                                          s.inbox)
+                          : xj[c] - xi[c];
+      if (s.general) box_apply<DIM>(s, d);
+      const float r2 = DIM == 3 ? fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]))
+                                : fmaf(d[1], d[1], d[0] * d[0]);
+      float sig = a.sig, eps = a.eps, alp = a.alp;
+      if (GENERAL) {
+        const int si = a.species ? a.species[ii] : 0, sj = a.species ? a.species[jj] : 0;
+        sig = pick(a.sig_mode, a.sig, a.sig_v, ii, jj, si, sj, a.ns);
+        eps = pick(a.eps_mode, a.eps, a.eps_v, ii, jj, si, sj, a.ns);
+        if (KIND == JMD_SOFT_SPHERE) alp = pick(a.alp_mode, a.alp, a.alp_v, ii, jj, si, sj, a.ns);
+      }
+      float dEr;
+      u = pair_eval<KIND>(r2, sig, eps, alp, a, &dEr) * scale;
+    }
+    out[t] = u;
+  }
+}
+
 }  // namespace jmd
 
 using namespace jmd;
@@ -598,6 +647,62 @@ int jmd_nve_force_kick_drift(const float* position, int64_t n, int32_t dim,
     else
       dispatch_dense<2, 2, JMD_SOFT_SPHERE>(tpa, a.general, L, false, true, true);
   }
+  JMD_LAUNCH_CHECK();
+  return JMD_OK;
+}
+
+int jmd_pair_edge_energies(const float* position, int64_t n, int32_t dim,
+                           int32_t position_stride, int32_t periodic, const float* side,
+                           const jmd_pair_potential* potential, const int32_t* idx,
+                           int64_t idx_width, int32_t format, float* out, void* stream) {
+  if ((dim != 2 && dim != 3) || n < 0 || n > 0x7fffffff || idx_width < 0 ||
+      (n > 0 && !position) || (idx_width > 0 && (!idx || !out)) || (periodic && !side))
+    return JMD_ERR_INVALID;
+  if (format != JMD_DENSE && format != JMD_SPARSE && format != JMD_ORDERED_SPARSE)
+    return JMD_ERR_INVALID;
+  if (n > 0 && !stride_ok(dim, position_stride, position)) return JMD_ERR_INVALID;
+  PairArgs a;
+  if (!make_pair_args(potential, &a)) return JMD_ERR_INVALID;
+  {
+    const int grc = space_check(dim, periodic, side, true);
+    if (grc) return grc;
+  }
+  const bool dense = format == JMD_DENSE;
+  const long long total = dense ? n * idx_width : idx_width;
+  if (total == 0) return JMD_OK;
+  Space s = make_space(dim, periodic, side);
+  hipStream_t st = (hipStream_t)stream;
+  // the reference divides by its normalization: 2 for lists holding both
+  // orders, 1 for OrderedSparse
+  const float scale = format == JMD_ORDERED_SPARSE ? 1.0f : 0.5f;
+  long long nb = (total + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  const int nn = (int)n;
+#define JMD_EDGE(D, S, K)                                                                     \
+  do {                                                                                       \
+    if (a.general) {                                                                         \
+      if (dense) hipLaunchKernelGGL((k_pair_edges<D, S, K, true, true>), dim3(nb), dim3(256),  \
+                                    0, st, position, nn, s, a, idx, idx_width, scale, out);  \
+      else hipLaunchKernelGGL((k_pair_edges<D, S, K, true, false>), dim3(nb), dim3(256), 0,   \
+                              st, position, nn, s, a, idx, idx_width, scale, out);           \
+    } else {                                                                                 \
+      if (dense) hipLaunchKernelGGL((k_pair_edges<D, S, K, false, true>), dim3(nb), dim3(256), \
+                                    0, st, position, nn, s, a, idx, idx_width, scale, out);  \
+      else hipLaunchKernelGGL((k_pair_edges<D, S, K, false, false>), dim3(nb), dim3(256), 0,  \
+                              st, position, nn, s, a, idx, idx_width, scale, out);           \
+    }                                                                                        \
+  } while (0)
+  if (dim == 3 && position_stride == 4) {
+    if (a.kind == JMD_LENNARD_JONES) JMD_EDGE(3, 4, JMD_LENNARD_JONES);
+    else JMD_EDGE(3, 4, JMD_SOFT_SPHERE);
+  } else if (dim == 3) {
+    if (a.kind == JMD_LENNARD_JONES) JMD_EDGE(3, 3, JMD_LENNARD_JONES);
+    else JMD_EDGE(3, 3, JMD_SOFT_SPHERE);
+  } else {
+    if (a.kind == JMD_LENNARD_JONES) JMD_EDGE(2, 2, JMD_LENNARD_JONES);
+    else JMD_EDGE(2, 2, JMD_SOFT_SPHERE);
+  }
+#undef JMD_EDGE
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }

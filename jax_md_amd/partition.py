@@ -402,6 +402,7 @@ class NeighborListSpec:
       raise ValueError(f'cell_size_factor must be >= 1, got {cell_size_factor}')
     min_size = self.cutoff if self.cell_size_factor == 1.0 else f32(
         self.cutoff * f32(self.cell_size_factor))
+    self._min_size = min_size
     if self.fractional:
       # cells on the unit cube, sized from the box (partition.py:1045-1050)
       self.user_box = box
@@ -418,10 +419,15 @@ class NeighborListSpec:
     self.mask_self = mask_self
     self.format = fmt
 
-  def use_cell_list(self) -> bool:
+  def use_cell_list(self, min_cell_size=None) -> bool:
     if self.disable_cell_list:
       return False
-    return bool(onp.all(self.min_cell_size < (self.box / f32(3.0)).astype(f32)))
+    mcs = self.min_cell_size if min_cell_size is None else min_cell_size
+    return bool(onp.all(mcs < (self.box / f32(3.0)).astype(f32)))
+
+  def fractional_cell_size(self, box) -> f32:
+    """Cell size on the unit cube for a box given at call time."""
+    return _fractional_cell_size(box, self._min_size)
 
 
 _FORMAT_CODE = {Dense: nat.DENSE, Sparse: nat.SPARSE, OrderedSparse: nat.ORDERED_SPARSE}
@@ -470,14 +476,15 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
                           fractional_coordinates, cell_size_factor)
   fcode = _FORMAT_CODE[format]
 
-  def _build(position, nbrs: Optional[NeighborList], plan: _Plan, gate):
+  def _build(position, nbrs: Optional[NeighborList], plan: _Plan, gate, side=None):
     lib = nat.load()
     plan.epoch += 1
     N, dim = position.shape
     st = nat.stream_handle(position.device)
     nat.check(lib.jmd_neighbor_list_build(
         nat.ptr(position), N, dim, _check_position(position), int(plan.periodic),
-        nat.f3(plan.side), float(spec.cutoff_sq), int(plan.use_cell_list), nat.i3(plan.cells_per_side),
+        nat.f3(plan.side if side is None else side), float(spec.cutoff_sq),
+        int(plan.use_cell_list), nat.i3(plan.cells_per_side),
         nat.f3(plan.cell_size3), int(nbrs.cell_list_capacity or 0), fcode,
         int(spec.mask_self), nat.ptr(nbrs.idx), int(nbrs.max_occupancy),
         nat.ptr(plan.occupancy), nat.ptr(nbrs.error.code), nat.ptr(nbrs.reference_position),
@@ -487,10 +494,10 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
   spec.format_code = fcode
 
   def allocate_fn(position, extra_capacity: int = 0, **kwargs) -> NeighborList:
-    if 'box' in kwargs:
-      if spec.fractional:
-        raise NotImplementedError('a changing box (box= keyword) is not supported by the '
-                                  'native build; the box matrix is fixed by the space')
+    # box= at allocation (fractional coordinates): the cell grid on the unit
+    # cube and the distances come from that box (partition.py:1045-1050)
+    box_kw = kwargs.get('box')
+    if box_kw is not None and not spec.fractional:
       raise ValueError('Neighbor list cannot accept a box keyword argument '
                        'if fractional_coordinates is not enabled.')
     stride = _check_position(position)
@@ -499,16 +506,20 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
     dev = position.device
     geo = spec.geometry
     side = geo.side_values(dim) if geo.periodic else [1.0] * dim
-    use_cells = spec.use_cell_list()
+    min_cs = spec.min_cell_size
+    if box_kw is not None:
+      side = [float(v) for v in space_mod._box_as_matrix(box_kw, dim).reshape(-1)]
+      min_cs = spec.fractional_cell_size(box_kw)
+    use_cells = spec.use_cell_list(min_cs)
     cps3, cs3, ncell, cap = [1] * dim, [1.0] * dim, 1, None
     cell_size = None
     if use_cells:
-      _, cell_size, cps, ncell = cell_dimensions(dim, spec.box, spec.min_cell_size)
+      _, cell_size, cps, ncell = cell_dimensions(dim, spec.box, min_cs)
       cps3, cs3 = _per_dim(cps, dim), _per_dim(cell_size, dim)
       _, cs2, cps2, _ = cell_dimensions(dim, spec.box, cell_size)
       fill = _filling_max(position, _per_dim(cps2, dim), _per_dim(cs2, dim))
       cap = int(onp.int32(fill) * capacity_multiplier) + extra_capacity
-      cell_size = spec.min_cell_size
+      cell_size = min_cs
     ws = _Workspace(dev, lib.jmd_neighbor_workspace_bytes(N, dim, ncell if use_cells else 0,
                                                           int(cap or 0), fcode))
     plan = _Plan(ws, torch.zeros((1,), dtype=torch.int32, device=dev),
@@ -545,34 +556,57 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
         cell_list_capacity=cap, max_occupancy=max_occ, format=format, cell_size=cell_size,
         cell_list_fn=None, update_fn=update_fn, plan=plan)
     _build(position, nbrs, plan, None)
-    if spec.fractional and not spec.disable_cell_list and is_box_valid(spec.user_box):
+    if spec.fractional and not spec.disable_cell_list and is_box_valid(
+        spec.user_box if box_kw is None else box_kw):
       # the reference raises this bit when is_box_valid holds (its predicate,
       # partition.py:1049, as written)
       nbrs.error.code.bitwise_or_(int(PEC.MALFORMED_BOX))
     return nbrs
 
   def update_fn(position, neighbors: NeighborList, **kwargs) -> NeighborList:
-    if 'box' in kwargs:
-      if spec.fractional:
-        raise NotImplementedError('a changing box (box= keyword) is not supported by the '
-                                  'native build; the box matrix is fixed by the space')
-      raise ValueError('Neighbor list cannot accept a box keyword argument '
-                       'if fractional_coordinates is not enabled.')
     stride = _check_position(position)
     plan = neighbors.plan
     if plan is None or tuple(position.shape) != tuple(neighbors.reference_position.shape):
       raise ValueError('update needs a list allocated for the same particle count')
     lib = nat.load()
     N, dim = position.shape
+    side = None
+    if 'box' in kwargs:
+      # a new box for this update (partition.py:1123-1139): the cells stay
+      # those of the allocation; CELL_SIZE_TOO_SMALL when the new box needs
+      # larger cells on the unit cube, MALFORMED_BOX with the reference's
+      # predicate; the skin test and the build measure in the new box
+      new_box = kwargs['box']
+      if not spec.disable_cell_list:
+        if not spec.fractional:
+          raise ValueError('Neighbor list cannot accept a box keyword argument '
+                           'if fractional_coordinates is not enabled.')
+        bits = 0
+        if plan.use_cell_list:
+          cur = f32(1.0) / onp.floor(f32(1.0) / f32(neighbors.cell_size)).astype(f32)
+          new = f32(1.0) / onp.floor(
+              f32(1.0) / _fractional_cell_size(new_box, spec.cutoff)).astype(f32)
+          if bool(onp.any(new > cur)):
+            bits |= int(PEC.CELL_SIZE_TOO_SMALL)
+        if is_box_valid(new_box):
+          bits |= int(PEC.MALFORMED_BOX)
+        if bits:
+          neighbors.error.code.bitwise_or_(bits)
+      if spec.fractional:
+        side = [float(v) for v in space_mod._box_as_matrix(new_box, dim).reshape(-1)]
+      elif spec.geometry.periodic:
+        raise ValueError('`space.periodic` does not accept a box argument. Perhaps you '
+                         'meant to use `space.periodic_general`?')
+    side_now = plan.side if side is None else side
     if spec.always_rebuild:
-      _build(position, neighbors, plan, None)
+      _build(position, neighbors, plan, None, side_now)
     else:
       st = nat.stream_handle(position.device)
       nat.check(lib.jmd_neighbor_list_check(
           nat.ptr(position), stride, nat.ptr(neighbors.reference_position), N, dim,
-          int(plan.periodic), nat.f3(plan.side), float(spec.threshold_sq),
+          int(plan.periodic), nat.f3(side_now), float(spec.threshold_sq),
           nat.ptr(plan.gate), st), 'jmd_neighbor_list_check')
-      _build(position, neighbors, plan, nat.ptr(plan.gate))
+      _build(position, neighbors, plan, nat.ptr(plan.gate), side_now)
     return dataclasses.replace(neighbors)
 
   return NeighborListFns(allocate_fn, update_fn)
@@ -713,6 +747,30 @@ def to_jraph(neighbor: NeighborList, mask: Optional[torch.Tensor] = None, nodes=
                      globals=globals,
                      n_node=torch.tensor([N, 1], dtype=torch.int32, device=dev),
                      n_edge=torch.stack((n_valid, (~_mask).sum().to(torch.int32))))
+
+
+def dense_rows(idx: torch.Tensor, N: int, fmt) -> torch.Tensor:
+  """Dense rows [N, W] (padding N) of a list: Dense as it is; Sparse
+  (receiver, sender) pairs grouped by sender; OrderedSparse pairs in both
+  orders.  Each row lists its neighbours in increasing id order."""
+  if fmt is Dense:
+    return idx
+  recv, send = idx[0].long(), idx[1].long()
+  valid = (recv < N) & (send < N)
+  recv, send = recv[valid], send[valid]
+  if fmt is OrderedSparse:
+    rows, cols = torch.cat([send, recv]), torch.cat([recv, send])
+  else:
+    rows, cols = send, recv
+  order = torch.argsort(rows * (N + 1) + cols)
+  rows, cols = rows[order], cols[order]
+  counts = torch.bincount(rows, minlength=N)
+  W = max(int(counts.max().item()) if rows.numel() else 0, 1)
+  start = torch.cumsum(counts, 0) - counts
+  slot = torch.arange(rows.numel(), device=idx.device) - start[rows]
+  out = torch.full((N, W), N, dtype=torch.int32, device=idx.device)
+  out[rows, slot] = cols.to(torch.int32)
+  return out
 
 
 def to_dense(neighbor: NeighborList) -> torch.Tensor:

@@ -87,10 +87,21 @@ def positions_in_box(shift: space_mod.Shift, energy, dim: int) -> bool:
   return [float(v) for v in _side(shift, dim)] == [float(v) for v in geo.side_values(dim)]
 
 
+def refuse_box(kwargs, what: str):
+  """A step that changes the box (``box=`` / ``new_box=`` through the
+  integrator, as NPT drives it) is outside the native path: the drift's
+  box is the shift function's."""
+  if 'box' in kwargs or 'new_box' in kwargs:
+    raise NotImplementedError(f'{what}: a changing box (box= / new_box=) is not supported by '
+                              'the native integrators; pass it to the energy and the '
+                              'neighbour list directly')
+
+
 def velocity_verlet_native(energy, shift, state: NVEState, dt, neighbor, dt_device=None,
                            **kwargs) -> NVEState:
   """One step: p += dt/2 F; R = shift(R, dt p/m); F = F(R); p += dt/2 F
   (simulate.py:227-243), dt cast to float32 (F5)."""
+  refuse_box(kwargs, 'velocity Verlet')
   lib = nat.load()
   R = state.position
   N, dim = R.shape
@@ -1014,6 +1025,7 @@ def nvt_nose_hoover(energy_or_force_fn, shift_fn, dt, kT, chain_length=5, chain_
     return state
 
   def apply_fn(state: NVTNoseHooverState, **kwargs):
+    refuse_box(kwargs, 'nvt_nose_hoover')
     _kT = kwargs.pop('kT', kT)
     neighbor = kwargs.pop('neighbor', None)
     lib = nat.load()

@@ -159,13 +159,12 @@ class PairEnergy:
   def _run64(self, R, neighbor, want_force, want_energy_total, dynamic_kwargs,
              momentum=None, half_dt=0.0):
     """The float64 path (the reference's x64 run): R, the force and the
-    momentum are float64; Dense lists only."""
+    momentum are float64; Sparse-format lists run as the Dense rows of the
+    same pairs (the energy counts each pair once either way)."""
     nat.require_device_tensor(R, 'R', torch.float64)
     if int(self.geometry.periodic) == 2:
       raise NotImplementedError('float64 positions with a periodic_general box are not '
                                 'supported natively')
-    if neighbor.format is not partition.Dense:
-      raise NotImplementedError('float64 positions run over Dense neighbour lists')
     if momentum is not None:
       nat.require_device_tensor(momentum, 'momentum', torch.float64)
     lib = nat.load()
@@ -176,13 +175,30 @@ class PairEnergy:
     F = torch.empty((N, dim), dtype=torch.float64, device=dev) if want_force else None
     e_tot = torch.empty((1,), dtype=torch.float64, device=dev) if want_energy_total else None
     ws = self._workspace64(dev, N) if want_energy_total else None
-    idx = neighbor.idx
+    idx = self._dense_rows_of(neighbor, N)
     nat.check(lib.jmd_pair_energy_force_f64(
-        nat.ptr(R), N, dim, int(geo.periodic), nat.d3(geo.side_values(dim)), ctypes.byref(pot),
+        nat.ptr(R), N, dim, int(geo.periodic), nat.d3(self.call_side(dim, dynamic_kwargs)),
+        ctypes.byref(pot),
         nat.ptr(idx), idx.shape[1], nat.DENSE, nat.ptr(F), nat.ptr(e_tot), nat.ptr(momentum),
         float(half_dt), nat.ptr(ws), nat.stream_handle(dev)), 'jmd_pair_energy_force_f64')
     del keep
     return e_tot, None, F
+
+  def _dense_rows_of(self, neighbor, N):
+    """The Dense rows the float64 kernel reads: a Dense list as it is; a
+    Sparse-format list's pairs as rows (both orders, ids ascending), kept
+    until the list is rebuilt (its build epoch) or rewritten (its version)."""
+    if neighbor.format is partition.Dense:
+      return neighbor.idx
+    idx = neighbor.idx
+    key = (idx.data_ptr(), idx._version, tuple(idx.shape), id(neighbor.plan),
+           neighbor.plan.epoch if neighbor.plan is not None else None, N)
+    hit = self._ws.get('rows64')
+    if hit is not None and hit[0] == key:
+      return hit[1]
+    rows = partition.dense_rows(idx, N, neighbor.format)
+    self._ws['rows64'] = (key, rows)
+    return rows
 
   def _workspace64(self, device, n):
     key = ('f64', str(device), int(n))
@@ -192,6 +208,27 @@ class PairEnergy:
       ws = torch.empty((lib.jmd_pair_workspace_bytes_f64(n),), dtype=torch.uint8, device=device)
       self._ws[key] = ws
     return ws
+
+  def call_side(self, dim: int, dynamic_kwargs) -> list:
+    """The geometry's side values for one call.  ``box=`` / ``new_box=``
+    (``new_box`` wins, as in the reference) replace the box matrix of a
+    ``periodic_general`` space with fractional coordinates for this call
+    (space.py:421-443: displacements are the new box applied to the
+    fractional difference); ``space.periodic`` refuses them as the
+    reference's displacement does, free space ignores them."""
+    geo = self.geometry
+    if 'new_box' in dynamic_kwargs:
+      box = dynamic_kwargs['new_box']
+    elif 'box' in dynamic_kwargs:
+      box = dynamic_kwargs['box']
+    else:
+      return geo.side_values(dim)
+    if int(geo.periodic) == 2:
+      return [float(v) for v in space_mod._box_as_matrix(box, dim).reshape(-1)]
+    if geo.periodic:
+      raise ValueError('`space.periodic` does not accept a box argument. Perhaps you '
+                       'meant to use `space.periodic_general`?')
+    return geo.side_values(dim)
 
   def geometry_code(self, in_box: bool = False) -> int:
     """The C ABI's ``periodic`` argument; ``in_box`` asserts that every
@@ -212,7 +249,7 @@ class PairEnergy:
     lib = nat.load()
     N, dim = R.shape
     geo = self.geometry
-    side = geo.side_values(dim)
+    side = self.call_side(dim, dynamic_kwargs)
     pot, keep = self._potential(R, dynamic_kwargs)
     dev = R.device
     F = torch.empty((N, dim), dtype=torch.float32, device=dev) if want_force else None
@@ -237,11 +274,8 @@ class PairEnergy:
       # geometry comes from the displacement
       dynamic_kwargs = dict(dynamic_kwargs)
       dynamic_kwargs.pop('fractional_coordinates')
-    if 'box' in dynamic_kwargs or 'new_box' in dynamic_kwargs:
-      raise NotImplementedError('a changing box (box= / new_box= keyword) is not supported '
-                                'by the native build')
     if self.reduce_axis is not None and len(self.reduce_axis) == 0:
-      raise NotImplementedError('reduce_axis=() (edge-wise output) is not supported natively')
+      return self.edge_energies(R, neighbor, **dynamic_kwargs)
     per_particle = self.reduce_axis is not None
     if per_particle and tuple(self.reduce_axis) != (1,):
       raise NotImplementedError(f'reduce_axis={self.reduce_axis} is not supported natively')
@@ -267,6 +301,28 @@ class PairEnergy:
     _, _, F = self._run(R, neighbor, True, False, False, dynamic_kwargs, in_box=in_box)
     return F
 
+  def edge_energies(self, R, neighbor=None, **dynamic_kwargs):
+    """The energy of every list entry over the reference's normalization
+    (reduce_axis=(), smap.py:940-953): [N, max_occupancy] for Dense lists,
+    [capacity] for the Sparse formats, float32, 0 at padding entries."""
+    if not isinstance(neighbor, partition.NeighborList):
+      raise TypeError('energy_fn needs a partition.NeighborList as `neighbor`')
+    stride = nat.position_stride(R, 'R')
+    lib = nat.load()
+    N, dim = R.shape
+    pot, keep = self._potential(R, dynamic_kwargs)
+    idx = neighbor.idx
+    dense = not partition.is_sparse(neighbor.format)
+    width = idx.shape[1]
+    out = torch.empty((N, width) if dense else (width,), dtype=torch.float32, device=R.device)
+    nat.check(lib.jmd_pair_edge_energies(
+        nat.ptr(R), N, dim, stride, int(self.geometry.periodic),
+        nat.f3(self.call_side(dim, dynamic_kwargs)), ctypes.byref(pot), nat.ptr(idx), width,
+        _FMT[neighbor.format], nat.ptr(out), nat.stream_handle(R.device)),
+        'jmd_pair_edge_energies')
+    del keep
+    return out
+
   def virial(self, R, neighbor=None, **dynamic_kwargs):
     """dU/d eps [dim, dim] float64 for every displacement perturbed as
     (I + eps) d: weight * sum over the list's pairs of (U'(r)/r) d d^T."""
@@ -286,7 +342,7 @@ class PairEnergy:
     idx = neighbor.idx
     width = idx.shape[1]  # Dense: row width; Sparse: capacity of the [2, capacity] list
     nat.check(lib.jmd_pair_virial(
-        nat.ptr(R), N, dim, stride, int(geo.periodic), nat.f3(geo.side_values(dim)),
+        nat.ptr(R), N, dim, stride, int(geo.periodic), nat.f3(self.call_side(dim, dynamic_kwargs)),
         ctypes.byref(pot), nat.ptr(idx), width, _FMT[neighbor.format], nat.ptr(W), nat.ptr(ws),
         nat.stream_handle(R.device)), 'jmd_pair_virial')
     del keep

@@ -731,6 +731,24 @@ def pair_energy_force(model, space, R, i, j, weight=0.5):
   return float(np.sum(E)), e_atom, F
 
 
+def pair_energies(model, space, R, i, j, weight=0.5):
+  """The energy of each ordered pair (i, j) times ``weight``, FP64: the
+  reduce_axis=() output of smap.pair_neighbor_list (smap.py:940-953), whose
+  ``out / normalization`` is weight = 1/2 for lists holding both orders and
+  1 for OrderedSparse.  Same pair terms as ``pair_energy_force``."""
+  R = np.asarray(R, F64)
+  dR = space.displacement(R[j], R[i]).astype(F64)
+  r2 = np.sum(dR * dR, axis=-1)
+  sig = model.pair_params(i, j, 'sigma')
+  eps = model.pair_params(i, j, 'epsilon')
+  if model.kind == 'lj':
+    r_on, r_c = model.smoothing()
+    E = lj_dE_over_r(r2, sig, eps, r_on, r_c)[1]
+  else:
+    E = soft_sphere_dE_over_r(r2, sig, eps, model.pair_params(i, j, 'alpha'))[1]
+  return weight * E
+
+
 def pair_virial(model, space, R, i, j, weight=0.5):
   """W_ab = weight sum over the ordered pairs (i, j) of (U'(r)/r) d_a d_b,
   d = disp(R_j, R_i): the derivative dU/d eps of the energy with every

@@ -265,3 +265,78 @@ def test_general_nvt_vs_fp64():
   dp = np.abs(st.chain.cpu().numpy()[1] - o.p_xi).max() / np.abs(o.p_xi).max()
   print(f'  max |dR| {np.abs(dR).max():.2e}, chain momenta rel {dp:.2e}')
   assert np.abs(dR).max() <= 1e-5 and dp <= 1e-4
+
+
+def test_general_box_keyword():
+  """box= with fractional coordinates (space.py:421-443,
+  partition.py:1045-1050,1123-1139): the energy and force with box=T2 are
+  those of a space built with T2 (same kernel, bit for bit) and within the
+  force bar of the float64 oracle in T2; an update with box=T2 that rebuilds
+  gives the oracle's T2 list bit for bit (T2 keeps the allocation's cell
+  grid); a box that needs larger cells raises CELL_SIZE_TOO_SMALL; the
+  reference's MALFORMED_BOX predicate (set for an upper-triangular box) is
+  applied to the new box; space.periodic refuses box= as the reference's
+  displacement does; the integrators refuse a changing box."""
+  print('case general box keyword', flush=True)
+  jmd = _jmd()
+  u, T, _ = O.fcc_triclinic(14, jitter=0.1, seed=9)
+  T2 = (T * F32(1.01)).astype(F32)
+  T2[0, 1] += F32(0.05)
+  disp, shift = jmd.space.periodic_general(T)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, T, fractional_coordinates=True,
+                                                    format=jmd.partition.Dense)
+  disp2, _ = jmd.space.periodic_general(T2)
+  _, efn2 = jmd.energy.lennard_jones_neighbor_list(disp2, T2, fractional_coordinates=True,
+                                                   format=jmd.partition.Dense)
+  ut = torch.as_tensor(u, device=DEV)
+  nb = nfn.allocate(ut)
+  E_kw, F_kw = efn.energy_and_force(ut, nb, box=torch.as_tensor(T2))
+  E_2, F_2 = efn2.energy_and_force(ut, nb)
+  assert float(E_kw) == float(E_2) and torch.equal(F_kw, F_2)
+  assert torch.equal(efn(ut, neighbor=nb, new_box=T2), efn2(ut, neighbor=nb))
+  E64, _, F64 = O.energy_force_brute(O.PairModel('lj'), O.GeneralSpace(T2),
+                                     u.astype(np.float64))
+  rel, worst = _force_err(F_kw.cpu().numpy(), F64)
+  print(f'  box=T2: force rel={rel:.3e} worst/rms={worst:.3e}')
+  assert rel <= 1e-5 and worst <= 5e-5
+  assert abs(float(E_kw) - E64) <= 1e-6 * abs(E64)
+  # an update that rebuilds (moves beyond half the skin) measured in T2
+  rng = np.random.default_rng(10)
+  x = u.astype(np.float64) @ T.astype(np.float64).T + rng.normal(0, 0.3, u.shape)
+  u1 = np.mod(x @ np.linalg.inv(T2.astype(np.float64)).T, 1.0).astype(F32)
+  u1 = np.where(u1 >= F32(1), F32(0), u1).astype(F32)
+  nb1 = nb.update(torch.as_tensor(u1, device=DEV), box=T2)
+  spec2 = O.NeighborListSpec(O.GeneralSpace(T2), T2, 2.5, 0.5, format=O.DENSE,
+                             fractional_coordinates=True)
+  spec1 = O.NeighborListSpec(O.GeneralSpace(T), T, 2.5, 0.5, format=O.DENSE,
+                             fractional_coordinates=True)
+  o1 = O.neighbor_list_build(spec1, u)
+  assert o1.use_cell_list and o1.max_occupancy == nb.max_occupancy
+  assert o1.cell_list_capacity == nb.cell_list_capacity
+  # T2 keeps the allocation's grid (4 cells a side on the unit cube)
+  assert (jmd.partition._fractional_cell_size(T2, F32(3.0))
+          == jmd.partition._fractional_cell_size(T, F32(3.0)))
+  o2 = O.neighbor_list_build(spec2, u1, neighbors=o1)
+  print(f'  update box=T2: error {int(nb1.error.code)} oracle {o2.error}')
+  assert not bool(nb1.did_buffer_overflow)
+  assert not int(nb1.error.code) & int(jmd.partition.PEC.CELL_SIZE_TOO_SMALL)
+  assert int(nb1.error.code) & int(jmd.partition.PEC.MALFORMED_BOX)
+  assert np.array_equal(nb1.idx.cpu().numpy(), o2.idx)
+  # a smaller box needs larger cells on the unit cube
+  T3 = (T * F32(0.8)).astype(F32)
+  nb.update(torch.as_tensor(u1, device=DEV), box=T3)
+  assert int(nb.error.code) & int(jmd.partition.PEC.CELL_SIZE_TOO_SMALL)
+  # space.periodic refuses box=; the integrators refuse a changing box
+  L = F32(12.0)
+  dp, sp = jmd.space.periodic(L)
+  nfp, efp = jmd.energy.lennard_jones_neighbor_list(dp, L, format=jmd.partition.Dense)
+  R = torch.as_tensor((u * L).astype(F32), device=DEV)
+  nbp = nfp.allocate(R)
+  with pytest.raises(ValueError):
+    efp(R, neighbor=nbp, box=L)
+  with pytest.raises(ValueError):
+    nbp.update(R, box=L)
+  init_fn, apply_fn = jmd.simulate.nve(efn, shift, 0.005)
+  st = init_fn(0, ut, kT=0.1, momenta=torch.zeros_like(ut), neighbor=nb1)
+  with pytest.raises(NotImplementedError):
+    apply_fn(st, neighbor=nb1, box=T2)

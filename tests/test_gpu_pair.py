@@ -254,3 +254,103 @@ def test_jammed_pressure(fmt):
   P = float(jmd.quantity.pressure(efn, Rt, box, neighbor=nbrs))
   print(f'  P {P:.10f} vs {float(st["pressure"]):.10f}')
   assert abs(P - float(st['pressure'])) <= 2e-5 * abs(float(st['pressure'])) + 2e-5
+
+
+@pytest.mark.parametrize('fmt', ['Dense', 'Sparse', 'OrderedSparse'])
+def test_edge_energies_reduce_axis_empty(fmt):
+  """smap.pair_neighbor_list with reduce_axis=() (smap.py:940-953): one
+  energy per list entry over the reference's normalization (2, or 1 for
+  OrderedSparse), shaped like the list's entries, 0 at padding; each entry
+  within 1e-5 relative of the FP64 pair energy plus 1e-6 of the magnitude of
+  the two Lennard-Jones terms that cancel near r = sigma (float32 rounding
+  of 4 eps (s^12 - s^6)), their sum equal to the energy (1e-5).  Soft
+  spheres with species in 2-D as well."""
+  print(f'case edge energies {fmt}', flush=True)
+  jmd, R, L, Rt, nbrs, efn = _lj_setup(8, fmt=fmt, seed=3)
+  N = R.shape[0]
+  disp, _ = jmd.space.periodic(L)
+  fn = jmd.energy.multiplicative_isotropic_cutoff(jmd.energy.lennard_jones, 2.0, 2.5)
+  edges_fn = jmd.smap.pair_neighbor_list(fn, jmd.space.canonicalize_displacement_or_metric(disp),
+                                         reduce_axis=())
+  out = edges_fn(Rt, nbrs).cpu().numpy().astype(np.float64)
+  idx = nbrs.idx.cpu().numpy()
+  m = O.PairModel('lj')
+  sp = O.Space(np.float64(L))
+  if fmt == 'Dense':
+    assert out.shape == idx.shape
+    rows = np.repeat(np.arange(N), idx.shape[1]).reshape(idx.shape)
+    valid = idx < N
+    i, j = rows[valid], idx[valid]
+    want = O.pair_energies(m, sp, R.astype(np.float64), i, j, 0.5)
+    got = out[valid]
+    assert np.all(out[~valid] == 0)
+  else:
+    assert out.shape == (idx.shape[1],)
+    valid = idx[0] < N
+    w = 1.0 if fmt == 'OrderedSparse' else 0.5
+    want = O.pair_energies(m, sp, R.astype(np.float64), idx[0][valid], idx[1][valid], w)
+    got = out[valid]
+    assert np.all(out[~valid] == 0)
+  if fmt == 'Dense':
+    ii, jj = i, j
+  else:
+    ii, jj = idx[0][valid], idx[1][valid]
+  d = sp.displacement(R[jj].astype(np.float64), R[ii].astype(np.float64))
+  s6 = 1.0 / np.sum(d * d, -1) ** 3
+  terms = 0.5 * 4.0 * (s6 * s6 + s6)
+  err = np.abs(got - want)
+  bar = 1e-5 * np.abs(want) + 1e-6 * terms
+  assert np.all(err <= bar), float(np.max(err / bar))
+  E = float(efn(Rt, neighbor=nbrs))
+  assert abs(out.sum() - E) <= 1e-5 * abs(E)
+  # soft spheres with species, 2-D
+  jmd = _jmd()
+  R2, species, L2 = O.bidisperse_2d(400, seed=11)
+  sig = np.array([[1.0, 1.2], [1.2, 1.4]], F32)
+  d2, _ = jmd.space.periodic(L2)
+  nfn2, efn2 = jmd.energy.soft_sphere_neighbor_list(
+      d2, L2, species=torch.as_tensor(species, device=DEV),
+      sigma=torch.as_tensor(sig, device=DEV), format=getattr(jmd.partition, fmt))
+  R2t = torch.as_tensor(R2, device=DEV)
+  nb2 = nfn2.allocate(R2t)
+  ss = jmd.smap.pair_neighbor_list(jmd.energy.soft_sphere,
+                                   jmd.space.canonicalize_displacement_or_metric(d2),
+                                   species=torch.as_tensor(species, device=DEV),
+                                   sigma=torch.as_tensor(sig, device=DEV), reduce_axis=())
+  out2 = ss(R2t, nb2).cpu().numpy().astype(np.float64)
+  E2 = float(efn2(R2t, neighbor=nb2))
+  assert abs(out2.sum() - E2) <= 1e-5 * abs(E2)
+
+
+@pytest.mark.parametrize('fmt', ['Sparse', 'OrderedSparse'])
+def test_x64_over_sparse_lists(fmt):
+  """The x64 form over Sparse-format lists (run as the Dense rows of the
+  same pairs): energy and force against the FP64 oracle to 1e-12, and a
+  20-step float64 NVE run over the Sparse list against the same run over a
+  Dense list to 1e-12 in positions."""
+  print(f'case x64 {fmt}', flush=True)
+  jmd = _jmd()
+  R, P, L = O.lj_lattice(6, seed=21)
+  disp, shift = jmd.space.periodic(L)
+  nfs, efs = jmd.energy.lennard_jones_neighbor_list(disp, L, format=getattr(jmd.partition, fmt))
+  nfd, efd = jmd.energy.lennard_jones_neighbor_list(disp, L, format=jmd.partition.Dense)
+  R64 = torch.as_tensor(R.astype(np.float64), device=DEV)
+  nbs = nfs.allocate(R64.to(torch.float32))
+  nbd = nfd.allocate(R64.to(torch.float32))
+  E, F = efs.energy_and_force(R64, nbs)
+  E64, _, F64 = O.energy_force_brute(O.PairModel('lj'), O.Space(np.float64(L)),
+                                     R.astype(np.float64))
+  assert F.dtype == torch.float64
+  rel = np.linalg.norm(F.cpu().numpy() - F64) / np.linalg.norm(F64)
+  print(f'  force rel {rel:.2e}, E {float(E)!r} vs {E64!r}')
+  assert rel <= 1e-12 and abs(float(E) - E64) <= 1e-12 * abs(E64)
+  out = []
+  for nfn, efn, nb in ((nfs, efs, nbs), (nfd, efd, nbd)):
+    init_fn, apply_fn = jmd.simulate.nve(efn, shift, 0.005)
+    st = init_fn(0, R64.clone(), kT=1.0, momenta=torch.as_tensor(P.astype(np.float64),
+                                                                 device=DEV), neighbor=nb)
+    for _ in range(20):
+      nb = nb.update(st.position.to(torch.float32))
+      st = apply_fn(st, neighbor=nb)
+    out.append(st.position.cpu().numpy())
+  assert np.max(np.abs(out[0] - out[1])) <= 1e-12 * L
