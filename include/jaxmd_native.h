@@ -417,6 +417,13 @@ typedef struct {
 /* Largest cell count (either grid) a system may have. */
 int jmd_fire_ensemble_max_cells(void);
 
+/* Threads per system workgroup of jmd_fire_ensemble_2d and its adjoint: 256,
+ * 512 or 1024; 0 (the default) chooses 1024 when the systems average 4096
+ * particles or more, else 512.  A process-wide tuning knob with no
+ * counterpart in the reference; results differ only in the summation order
+ * of the workgroup reductions. */
+int jmd_fire_ensemble_set_threads(int32_t threads);
+
 /* Soft-sphere FIRE minimisation of every system of a ragged ensemble, one
  * workgroup per system (systems[order[b]] for workgroup b): the reference's
  * loop `nbrs = nbrs.update(R); state = apply_fn(state, neighbor=nbrs)` of

@@ -165,6 +165,7 @@ _SIGNATURES = {
                                                c_void_p, c_void_p, c_void_p, c_void_p,
                                                c_void_p, c_void_p, c_void_p, c_void_p,
                                                c_void_p, c_void_p, c_float, c_void_p]),
+    'jmd_fire_ensemble_set_threads': (c_int32, [c_int32]),
     'jmd_fire_workspace_bytes': (c_size_t, [c_int64]),
     'jmd_fire_update': (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p,
                                   POINTER(c_float), c_void_p, c_void_p, c_void_p]),

@@ -20,7 +20,15 @@
 
 namespace jmd {
 
-constexpr int kEnsThreads = 512;
+// Threads per system: 1024 when the systems average 4096 particles or more
+// (16 wavefronts hide the gathers' latency of a large system), else 512
+// (small systems, two workgroups per CU).
+constexpr int kEnsThreads = 1024;  // the most a workgroup has
+static int g_ens_threads = 0;      // jmd_fire_ensemble_set_threads (0: by size)
+inline int ens_threads(int n_systems, long long n_total) {
+  if (g_ens_threads) return g_ens_threads;
+  return n_systems > 0 && n_total / n_systems >= 4096 ? 1024 : 512;
+}
 constexpr int kEnsMaxCells = 8192;
 
 struct EnsSys {
@@ -758,7 +766,7 @@ int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int3
   if (taped && (mode != ENS_RUN || !tape_force || !tape_q || !tape_idx || !tape_scalars))
     return JMD_ERR_INVALID;
   EnsTape tape{tape_x, tape_force, tape_q, tape_idx, tape_scalars, n_steps};
-  hipLaunchKernelGGL(k_fire_ensemble, dim3(n_systems), dim3(kEnsThreads), 0,
+  hipLaunchKernelGGL(k_fire_ensemble, dim3(n_systems), dim3(ens_threads(n_systems, n_total)), 0,
                      (hipStream_t)stream, reinterpret_cast<const EnsSys*>(systems), order,
                      n_systems, (int)n_total, mode, n_steps, position_hi, position, momentum,
                      force, reference_position, idx, width, scratch, a, prm, fire_state,
@@ -768,6 +776,12 @@ int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int3
 }
 
 int jmd_fire_ensemble_max_cells(void) { return kEnsMaxCells; }
+
+int jmd_fire_ensemble_set_threads(int32_t threads) {
+  if (threads != 0 && threads != 256 && threads != 512 && threads != 1024) return JMD_ERR_INVALID;
+  g_ens_threads = threads;
+  return JMD_OK;
+}
 
 int jmd_fire_ensemble_adjoint_2d(const jmd_system2d* systems, int32_t n_systems,
                                  int64_t n_total, int32_t mode, int32_t n_steps,
@@ -794,7 +808,8 @@ int jmd_fire_ensemble_adjoint_2d(const jmd_system2d* systems, int32_t n_systems,
   EnsTape tape{const_cast<float*>(tape_x), const_cast<float*>(tape_force),
                const_cast<float*>(tape_q), const_cast<int*>(tape_idx),
                const_cast<float*>(tape_scalars), n_steps};
-  hipLaunchKernelGGL(k_fire_ensemble_back, dim3(n_systems), dim3(kEnsThreads), 0,
+  hipLaunchKernelGGL(k_fire_ensemble_back, dim3(n_systems),
+                     dim3(ens_threads(n_systems, n_total)), 0,
                      (hipStream_t)stream, reinterpret_cast<const EnsSys*>(systems), n_systems,
                      (int)n_total, mode, n_steps, position, force, idx, width, a, dsigma, tape,
                      x_adjoint, p_adjoint, pending, scratch, theta, box_adjoint, scale);

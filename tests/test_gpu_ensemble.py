@@ -233,3 +233,51 @@ def test_ensemble_gradient_mean_energy_scale():
   _, th_sum, X_sum = make().energy_gradient(12, DSIGMA, checkpoint_every=5, scale=1.0)
   assert torch.allclose(th_mean * 2, th_sum, rtol=1e-12, atol=0)
   assert torch.allclose(X_mean * 2, X_sum, rtol=1e-6, atol=1e-12)
+
+
+def test_ensemble_large_systems_thread_counts_agree():
+  """Systems of 4096 particles or more run with 1024 threads per system (the
+  size rule), smaller ones with 512: on two systems of 4200 and 4600
+  particles, allocation (capacities, lists, forces) is bit-identical for 512
+  and 1024 threads; 40 FIRE steps give the same float16 dt / alpha
+  sequence, energies within 1e-7 and positions within 1e-5; the gradient
+  with the box following the ratio agrees to 1e-6.  The threads knob only
+  changes the order of the workgroup sums."""
+  jmd = _jmd()
+  from jax_md_amd import _native as nat
+  lib = nat.load()
+  sizes, ratios = [4200, 4600], [1.3, 1.5]
+  syst = [O.bidisperse_2d(n, d_large=r, seed=80 + k) for k, (n, r) in
+          enumerate(zip(sizes, ratios))]
+  sig = np.stack([_sigma_r(r) for r in ratios])
+  dLdr = np.array([np.sqrt(n * np.pi * (1 + r * r) / (8 * 0.84)) * r / (1 + r * r)
+                   for n, r in zip(sizes, ratios)])
+  print('case ensemble large systems, 512 vs 1024 threads', flush=True)
+
+  def make():
+    return jmd.ensemble.SoftSphereFireEnsemble(
+        [s[0] for s in syst], [s[2] for s in syst], [s[1] for s in syst], sigma=sig,
+        **FIRE).allocate()
+
+  out = {}
+  try:
+    for threads in (512, 1024):
+      nat.check(lib.jmd_fire_ensemble_set_threads(threads), 'set_threads')
+      a = make()
+      alloc = (a.idx.clone(), a.force.clone(), [a.capacities(s) for s in range(2)])
+      a.run(40)
+      fwd = (a.position.clone(), a.energy.clone(), a.fire_state.clone())
+      g = make()
+      _, th, _ = g.energy_gradient(30, DSIGMA, checkpoint_every=10, scale=1.0, dbox=dLdr)
+      out[threads] = (alloc, fwd, th.cpu().numpy(), g.box_adjoint.cpu().numpy())
+  finally:
+    nat.check(lib.jmd_fire_ensemble_set_threads(0), 'set_threads')
+  (al5, fw5, th5, lb5), (al1, fw1, th1, lb1) = out[512], out[1024]
+  assert al5[2] == al1[2]
+  assert torch.equal(al5[0], al1[0]) and torch.equal(al5[1], al1[1])
+  assert torch.equal(fw5[2], fw1[2])
+  assert torch.allclose(fw5[1], fw1[1], rtol=1e-7, atol=0)
+  assert float((fw5[0] - fw1[0]).abs().max()) <= 1e-5
+  print(th5, th1, lb5, lb1, flush=True)
+  assert np.allclose(th5, th1, rtol=1e-6, atol=0)
+  assert np.allclose(lb5, lb1, rtol=1e-6, atol=1e-9)
