@@ -181,7 +181,7 @@ def _restore(snap, state, nbrs):
 
 def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int,
                         checkpoint_every: int = 20, mass=1.0,
-                        neighbor=None) -> TrajectoryGradient:
+                        neighbor=None, graph: bool = False) -> TrajectoryGradient:
   """Gradient of E(x_T) after ``n_steps`` velocity-Verlet steps from (R, P)
   with respect to R, P, sigma and epsilon (scalar or per-particle).  A
   float64 R runs the x64 form: float64 trajectory (positions, momenta,
@@ -195,6 +195,12 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   k = T-1 .. 0:  X += c_k J(x_k) PH;  theta += c_k PH^T dF(x_k)/dtheta;
   PH <- PH + (dt/m) X, where c_k = dt for k >= 1 and dt/2 for k = 0 (the
   product at x_k enters steps k and k-1); dE/dp_0 is the last PH.
+
+  With ``graph`` the forward segments and the recording replays run as HIP
+  graph replays (one graph per segment length and kind, recorded after the
+  first eager segment of that kind); the steps are the same kernels either
+  way.  The graphs live for one call, so their recording pays off only for
+  long runs of short segments (at 200 steps in segments of 20 it does not).
   """
   energy = _energy(energy_fn)
   if not isinstance(mass, (int, float)):
@@ -214,6 +220,25 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   loop = simulate.NVELoop(energy, shift_fn, dt, reorder=False, graph=False)
   loop.prime(state, nbrs)
   K = max(1, int(checkpoint_every))
+  graphs = {}
+  use_graph = bool(graph) and state.position.is_cuda
+
+  def segment(k, hook=None):
+    """k steps from the current state (eager the first time a kind runs,
+    then recorded; later segments of that kind replay the graph)."""
+    key = (k, hook is not None)
+    g = graphs.get(key)
+    if g is not None:
+      g[0].replay()
+      return
+    loop._run_steps(state, nbrs, k, hook=hook)
+    if use_graph:
+      g = torch.cuda.CUDAGraph()
+      with torch.cuda.graph(g):
+        loop._run_steps(state, nbrs, k, hook=hook)  # recorded, not run
+      # the parameter arrays the recorded kernels read stay alive with it
+      graphs[key] = (g, loop._keep)
+
   torch.cuda.synchronize(dev)
   t0 = time.perf_counter()
   ckpts = []
@@ -221,7 +246,7 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   while done < n_steps:
     k = min(K, n_steps - done)
     ckpts.append((done, k, _snapshot(state, nbrs)))
-    loop._run_steps(state, nbrs, k)
+    segment(k)
     done += k
   torch.cuda.synchronize(dev)
   t1 = time.perf_counter()
@@ -257,11 +282,11 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   for start, k, snap in reversed(ckpts):
     _restore(snap, state, nbrs)
 
-    def hook(q, start=start):
+    def hook(q):
       xs[q].copy_(state.position_hi if x64 else state.position)
       idxs[q].copy_(nbrs.idx)
 
-    loop._run_steps(state, nbrs, k, hook=hook)
+    segment(k, hook)
     for q in range(k - 1, -1, -1):
       step = start + q
       c = float(dt32) if step >= 1 else float(f32(dt32 / f32(2)))

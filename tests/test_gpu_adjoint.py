@@ -257,6 +257,27 @@ def test_nve_trajectory_gradient_vs_finite_differences(n_steps, every):
     assert abs(ad - fd) <= 1e-4 * abs(fd)
 
 
+@pytest.mark.parametrize('x64', [False, True])
+def test_trajectory_gradient_graph_replays_equal_eager(x64):
+  """The graph-replayed segments (forward and recording replays) give the
+  eager run's gradient bit for bit, over several segments of each kind and
+  a shorter last one, for float32 and x64 trajectories."""
+  print(f'case trajectory gradient graphs x64={x64}', flush=True)
+  jmd, R, P, L, shift, nfn, efn = _lj_system(6, seed=19)
+  dt = 0.005
+  out = []
+  for graph in (False, True):
+    Rt = torch.as_tensor(R.astype(np.float64) if x64 else R, device=DEV)
+    Pt = torch.as_tensor(P.astype(np.float64) if x64 else P, device=DEV)
+    res = jmd.adjoint.nve_energy_gradient(efn, shift, nfn, Rt, Pt, dt, 47,
+                                          checkpoint_every=10, graph=graph)
+    out.append(res)
+  a, b = out
+  assert a.energy == b.energy
+  assert torch.equal(a.position, b.position) and torch.equal(a.momentum, b.momentum)
+  assert a.sigma == b.sigma and a.epsilon == b.epsilon
+
+
 def test_x64_force_and_trajectory_match_fp64():
   """The float64 path: forces against the FP64 oracle to 1e-12 (norm-wise)
   and a 60-step trajectory against the oracle's float64 velocity Verlet to
