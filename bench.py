@@ -535,6 +535,33 @@ def published_section(jmd, O, n_steps=2000, warmup=200, block=100):
     log(f'published setup, {name}: {1e3 * el / n_steps:.3f} ms/step, '
         f'{N * n_steps / el:.3e} atom-steps/s, x{(N * n_steps / el) / 2.86e7:.1f} the A100 '
         f'figure, T_end {T_end:.3f}')
+  # the drop-in per-step form with the notebook's format: nbrs.update and
+  # apply_fn called from Python each step (the list's Dense rows carry the
+  # force; host-launched, no graph)
+  nfn, efn = jmd.energy.lennard_jones_neighbor_list(disp, L, r_cutoff=3.0, dr_threshold=1.0,
+                                                    format=jmd.partition.OrderedSparse)
+  Rt = torch.as_tensor(R, device=dev)
+  init_fn, apply_fn = jmd.simulate.nvt_nose_hoover(efn, shift, 5e-3, kT=1.2)
+  nb = nfn.allocate(Rt)
+  st = init_fn(0, Rt, momenta=torch.as_tensor(P, device=dev), neighbor=nb)
+  steps_api = 300
+  for _ in range(20):
+    nb = nb.update(st.position)
+    st = apply_fn(st, neighbor=nb)
+  torch.cuda.synchronize()
+  t = time.perf_counter()
+  for _ in range(steps_api):
+    nb = nb.update(st.position)
+    st = apply_fn(st, neighbor=nb)
+  torch.cuda.synchronize()
+  el = time.perf_counter() - t
+  out['OrderedSparse_per_step_api'] = {
+      'ms_per_step': 1e3 * el / steps_api, 'atom_steps_per_s': N * steps_api / el,
+      'vs_published_a100': (N * steps_api / el) / 2.86e7,
+      'overflow': bool(nb.did_buffer_overflow),
+      'note': f'{steps_api} steps of nbrs.update + apply_fn from Python after 20, '
+              'host-launched'}
+  log(f'published setup, OrderedSparse per-step API: {1e3 * el / steps_api:.3f} ms/step')
   out['workload'] = ('64,000 LJ atoms, 40^3 lattice a = 1.3782 (rho 0.382), r_c 3.0, skin 1.0, '
                      f'NVT Nose-Hoover kT 1.2, dt 0.005, {n_steps} steps after {warmup}')
   out['published'] = {'ms_per_step': 2.24, 'atom_steps_per_s': 2.86e7,

@@ -331,6 +331,11 @@ class _Plan:
   epoch: int = 0
   # (unmasked list, mask function) of a list with a custom mask
   custom_mask: Any = None
+  # Sparse formats: the Dense rows of the same pairs, rebuilt by every
+  # allocate / update beside the Sparse list (see _shadow_build), and the
+  # (epoch, idx version) at which it matched the Sparse list
+  shadow: Any = None
+  shadow_mark: Any = None
 
   @property
   def builds(self) -> int:
@@ -493,6 +498,55 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
   spec.build = _build
   spec.format_code = fcode
 
+  def _shadow_build(position, shadow, gate, side):
+    """The Dense rows of the Sparse list's pairs: the same grid, skin decision
+    and reference positions, each candidate kept by the Sparse pair test
+    (wrap(R_i - R_j), both orders stored); a row longer than the shadow's
+    width raises the list's overflow bit."""
+    lib = nat.load()
+    sp = shadow.plan
+    N, dim = position.shape
+    nat.check(lib.jmd_neighbor_list_build(
+        nat.ptr(position), N, dim, _check_position(position), int(sp.periodic), nat.f3(side),
+        float(spec.cutoff_sq), int(sp.use_cell_list), nat.i3(sp.cells_per_side),
+        nat.f3(sp.cell_size3), int(shadow.cell_list_capacity or 0),
+        nat.DENSE | nat.DENSE_SPARSE_TEST, int(spec.mask_self), nat.ptr(shadow.idx),
+        int(shadow.max_occupancy), nat.ptr(sp.occupancy), nat.ptr(shadow.error.code),
+        nat.ptr(shadow.reference_position), gate, sp.workspace.ptr,
+        nat.stream_handle(position.device)), 'jmd_neighbor_list_build(shadow)')
+
+  def _shadow_allocate(position, nbrs, plan, side):
+    N, dim = position.shape
+    dev = position.device
+    recv, send = nbrs.idx[0].long(), nbrs.idx[1].long()
+    valid = (recv < N) & (send < N)
+    rows = torch.bincount(send[valid], minlength=N)
+    if format is OrderedSparse:
+      rows = rows + torch.bincount(recv[valid], minlength=N)
+    longest = int(rows.max().item()) if N else 0
+    limit = N - 1 if mask_self else N
+    cand = (3 ** dim) * int(nbrs.cell_list_capacity or 0) if plan.use_cell_list else N
+    width = max(min(max(2 * longest, longest + 32), limit, cand), 1)
+    ncell = 1
+    for c in plan.cells_per_side:
+      ncell *= int(c)
+    lib = nat.load()
+    ws = _Workspace(dev, lib.jmd_neighbor_workspace_bytes(
+        N, dim, ncell if plan.use_cell_list else 0, int(nbrs.cell_list_capacity or 0),
+        nat.DENSE))
+    plan2 = _Plan(ws, torch.zeros((1,), dtype=torch.int32, device=dev),
+                  torch.zeros((1,), dtype=torch.int32, device=dev), plan.use_cell_list,
+                  plan.cells_per_side, plan.cell_size3, plan.side, plan.periodic, plan.dim,
+                  None)
+    shadow = NeighborList(
+        idx=torch.empty((N, width), dtype=torch.int32, device=dev),
+        reference_position=nbrs.reference_position, error=nbrs.error,
+        cell_list_capacity=nbrs.cell_list_capacity, max_occupancy=width, format=Dense,
+        cell_size=nbrs.cell_size, cell_list_fn=None, update_fn=None, plan=plan2)
+    _shadow_build(position, shadow, None, side)
+    plan.shadow = shadow
+    plan.shadow_mark = (plan.epoch, nbrs.idx._version)
+
   def allocate_fn(position, extra_capacity: int = 0, **kwargs) -> NeighborList:
     # box= at allocation (fractional coordinates): the cell grid on the unit
     # cube and the distances come from that box (partition.py:1045-1050)
@@ -556,6 +610,8 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
         cell_list_capacity=cap, max_occupancy=max_occ, format=format, cell_size=cell_size,
         cell_list_fn=None, update_fn=update_fn, plan=plan)
     _build(position, nbrs, plan, None)
+    if sparse:
+      _shadow_allocate(position, nbrs, plan, side)
     if spec.fractional and not spec.disable_cell_list and is_box_valid(
         spec.user_box if box_kw is None else box_kw):
       # the reference raises this bit when is_box_valid holds (its predicate,
@@ -598,15 +654,25 @@ def neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
         raise ValueError('`space.periodic` does not accept a box argument. Perhaps you '
                          'meant to use `space.periodic_general`?')
     side_now = plan.side if side is None else side
+    shadow = plan.shadow if plan.shadow is not None and _shadow_intact(neighbors) else None
+    plan.shadow_mark = None
     if spec.always_rebuild:
       _build(position, neighbors, plan, None, side_now)
+      if shadow is not None:
+        _shadow_build(position, shadow, None, side_now)
     else:
       st = nat.stream_handle(position.device)
       nat.check(lib.jmd_neighbor_list_check(
           nat.ptr(position), stride, nat.ptr(neighbors.reference_position), N, dim,
           int(plan.periodic), nat.f3(side_now), float(spec.threshold_sq),
           nat.ptr(plan.gate), st), 'jmd_neighbor_list_check')
+      if shadow is not None:
+        shadow.plan.gate.copy_(plan.gate)  # the Sparse build resets its gate word
       _build(position, neighbors, plan, nat.ptr(plan.gate), side_now)
+      if shadow is not None:
+        _shadow_build(position, shadow, nat.ptr(shadow.plan.gate), side_now)
+    if shadow is not None:
+      plan.shadow_mark = (plan.epoch, neighbors.idx._version)
     return dataclasses.replace(neighbors)
 
   return NeighborListFns(allocate_fn, update_fn)
@@ -747,6 +813,22 @@ def to_jraph(neighbor: NeighborList, mask: Optional[torch.Tensor] = None, nodes=
                      globals=globals,
                      n_node=torch.tensor([N, 1], dtype=torch.int32, device=dev),
                      n_edge=torch.stack((n_valid, (~_mask).sum().to(torch.int32))))
+
+
+def _shadow_intact(nbrs) -> bool:
+  """The Sparse list was last written by this module's allocate / update
+  (not by a loop, by hand or by another build)."""
+  plan = nbrs.plan
+  return plan.shadow_mark == (plan.epoch, nbrs.idx._version)
+
+
+def current_shadow(nbrs) -> Optional[NeighborList]:
+  """The Dense rows of a Sparse-format list's pairs when they match the
+  list as it stands, else None."""
+  plan = nbrs.plan
+  if plan is None or plan.shadow is None or not is_sparse(nbrs.format):
+    return None
+  return plan.shadow if _shadow_intact(nbrs) else None
 
 
 def dense_rows(idx: torch.Tensor, N: int, fmt) -> torch.Tensor:

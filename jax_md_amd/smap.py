@@ -190,6 +190,9 @@ class PairEnergy:
     until the list is rebuilt (its build epoch) or rewritten (its version)."""
     if neighbor.format is partition.Dense:
       return neighbor.idx
+    shadow = partition.current_shadow(neighbor)
+    if shadow is not None:
+      return shadow.idx
     idx = neighbor.idx
     key = (idx.data_ptr(), idx._version, tuple(idx.shape), id(neighbor.plan),
            neighbor.plan.epoch if neighbor.plan is not None else None, N)
@@ -256,12 +259,20 @@ class PairEnergy:
     e_tot = torch.empty((1,), dtype=torch.float64, device=dev) if want_energy_total else None
     e_atom = torch.empty((N,), dtype=torch.float32, device=dev) if want_energy_atom else None
     idx = neighbor.idx
+    fmt = _FMT[neighbor.format]
+    # a Sparse-format list from partition's allocate / update carries the
+    # Dense rows of its pairs: the row kernel sums each particle's force in
+    # a fixed order (no float atomics); energies and per-particle energies
+    # count the same pairs
+    shadow = partition.current_shadow(neighbor)
+    if shadow is not None:
+      idx, fmt = shadow.idx, nat.DENSE
     width = idx.shape[1]
     ws = self._workspace(dev, N)
     nat.check(lib.jmd_pair_energy_force(
         nat.ptr(R), N, dim, stride, self.geometry_code(in_box), nat.f3(side), ctypes.byref(pot),
         nat.ptr(idx),
-        width, _FMT[neighbor.format], nat.ptr(F), nat.ptr(e_atom), nat.ptr(e_tot),
+        width, fmt, nat.ptr(F), nat.ptr(e_atom), nat.ptr(e_tot),
         nat.ptr(momentum), float(half_dt), nat.ptr(dt_device), nat.ptr(ws),
         nat.stream_handle(dev)), 'jmd_pair_energy_force')
     del keep

@@ -354,3 +354,40 @@ def test_x64_over_sparse_lists(fmt):
       st = apply_fn(st, neighbor=nb)
     out.append(st.position.cpu().numpy())
   assert np.max(np.abs(out[0] - out[1])) <= 1e-12 * L
+
+
+@pytest.mark.parametrize('fmt', ['Sparse', 'OrderedSparse'])
+def test_sparse_lists_force_through_dense_rows(fmt):
+  """A Sparse-format list from allocate / update carries the Dense rows of
+  its pairs: energy and force run the row kernel, so two evaluations are
+  bit-identical and equal the Dense list's force to 1e-6 (the same pairs,
+  summed in another order); after updates that rebuild, the rows follow the
+  list; a list written by anything else (here by hand) falls back to the
+  pair kernel with the same result to 1e-5."""
+  print(f'case sparse through dense rows {fmt}', flush=True)
+  jmd, R, L, Rt, nbrs, efn = _lj_setup(12, fmt=fmt, seed=4)
+  _, _, _, _, nbd, efd = _lj_setup(12, fmt='Dense', seed=4)
+  assert jmd.partition.current_shadow(nbrs) is not None
+  E1, F1 = efn.energy_and_force(Rt, nbrs)
+  E2, F2 = efn.energy_and_force(Rt, nbrs)
+  assert torch.equal(F1, F2) and float(E1) == float(E2)
+  Ed, Fd = efd.energy_and_force(Rt, nbd)
+  rel = float((F1 - Fd).norm() / Fd.norm())
+  assert rel <= 1e-6 and abs(float(E1) - float(Ed)) <= 1e-6 * abs(float(Ed)), rel
+  # moved beyond half the skin: both lists rebuild
+  rng = np.random.default_rng(5)
+  R2 = torch.as_tensor((R + rng.normal(0, 0.2, R.shape)).astype(F32) % F32(L), device=DEV)
+  nbrs = nbrs.update(R2)
+  nbd = nbd.update(R2)
+  assert jmd.partition.current_shadow(nbrs) is not None
+  _, F3 = efn.energy_and_force(R2, nbrs)
+  _, Fd3 = efd.energy_and_force(R2, nbd)
+  assert float((F3 - Fd3).norm() / Fd3.norm()) <= 1e-6
+  E64, _, F64 = O.energy_force_brute(O.PairModel('lj'), O.Space(np.float64(L)),
+                                     R2.cpu().numpy().astype(np.float64))
+  assert float(np.linalg.norm(F3.cpu().numpy() - F64) / np.linalg.norm(F64)) <= 1e-5
+  # a list written by hand: the pair kernel
+  nbrs.idx.copy_(nbrs.idx.clone())
+  assert jmd.partition.current_shadow(nbrs) is None
+  _, F4 = efn.energy_and_force(R2, nbrs)
+  assert float((F4 - F3).norm() / F3.norm()) <= 1e-5
