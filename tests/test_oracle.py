@@ -41,6 +41,14 @@ Which check pins which oracle function:
                                    row-only form fails
   fire_energy_gradient             central differences of fire_energy_after
                                    in r and x_0, also across momentum resets
+  box_vjp, fire_energy_gradient    central differences in the box side (fixed
+    (box=True)                     positions, and through 40 FIRE steps that
+                                   fold particles across the boundary) and in
+                                   the diameter ratio with the box following
+                                   it; the fold term is material (24%)
+  pair_energies                    closed-form pair energies per entry (switched
+                                   LJ, species soft spheres); the entries sum
+                                   to the brute-force energy
   nvt_step (Nose-Hoover chain)     invariant conserved for sy = 1, 3, 5, 7 on
                                    the jammed fixture (simulate_test.py:
                                    317-350), second order in dt
@@ -643,6 +651,33 @@ def test_fire_adjoint_across_momentum_resets():
   fd = (O.fire_energy_after(_fire_system(r + h)[2], sp, R, T, par)
         - O.fire_energy_after(_fire_system(r - h)[2], sp, R, T, par)) / (2 * h)
   assert abs(th - fd) <= 1e-7 * abs(fd), (th, fd)
+
+
+def test_pair_energies_per_entry():
+  """pair_energies (the reduce_axis=() output, smap.py:940-953): each
+  entry is the closed-form pair energy (switched LJ; soft sphere with
+  species) times the normalization weight, and over a list holding both
+  orders the entries sum to the brute-force total energy."""
+  R, _, L = O.lj_lattice(5, seed=2)
+  sp = O.Space(np.float64(L))
+  R = R.astype(np.float64)
+  m = O.PairModel('lj')
+  i, j = O.all_pairs(sp, R, 2.5)
+  e = O.pair_energies(m, sp, R, i, j, 0.5)
+  r = np.sqrt(np.sum(sp.displacement(R[j], R[i]) ** 2, -1))
+  # (the two forms round differently where the switch nearly cancels, r ~ r_c)
+  assert np.allclose(e, 0.5 * O.lj_energy_smoothed(r, 1.0, 1.0, 2.0, 2.5), rtol=1e-9, atol=1e-15)
+  assert abs(e.sum() - O.energy_force_brute(m, sp, R)[0]) <= 1e-12 * abs(e.sum())
+  Rs, species, Ls = O.bidisperse_2d(60, seed=3)
+  sig = np.array([[1.0, 1.2], [1.2, 1.4]])
+  ms = O.PairModel('soft_sphere', sigma=sig, species=species)
+  sps = O.Space(np.float64(Ls))
+  Rs = Rs.astype(np.float64)
+  i, j = O.all_pairs(sps, Rs, 1.4)
+  es = O.pair_energies(ms, sps, Rs, i, j, 1.0)
+  rs = np.sqrt(np.sum(sps.displacement(Rs[j], Rs[i]) ** 2, -1))
+  want = O.soft_sphere_energy(rs, sig[species[i], species[j]], 1.0, 2.0)
+  assert np.allclose(es, want, rtol=1e-12, atol=0) and np.any(es > 0)
 
 
 def test_fire_box_adjoint_matches_central_differences():
