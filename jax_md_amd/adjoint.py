@@ -282,8 +282,8 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   for start, k, snap in reversed(ckpts):
     _restore(snap, state, nbrs)
 
-    def hook(q):
-      xs[q].copy_(state.position_hi if x64 else state.position)
+    def hook(q, x):
+      xs[q].copy_(state.position_hi if x64 else x)
       idxs[q].copy_(nbrs.idx)
 
     segment(k, hook)

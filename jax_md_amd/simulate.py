@@ -680,7 +680,7 @@ class NVELoop:
         events.stop('update', e0)
         events.builds_after_update(plan)
       if hook is not None:
-        hook(k)
+        hook(k, state.position)
       nat.check(lib.jmd_nve_kick_drift_f64(
           nat.ptr(state.position_hi), nat.ptr(state.momentum), nat.ptr(state.force),
           nat.ptr(mass_v), mass_s, N, dim, wraps, side, dt32, half, nat.ptr(R), stride, ref,
@@ -762,8 +762,9 @@ class NVELoop:
 
   def _run_steps(self, state: NVEState, nbrs, n_steps: int, events=None,
                  hook=None) -> NVEState:
-    """The per-step kernel sequence; ``hook(k)``, when given, runs after step
-    k's (gated) list update, when the state holds x_k and a list valid for it."""
+    """The per-step kernel sequence; ``hook(k, x)``, when given, runs after
+    step k's (gated) list update, when ``x`` (float32, the state's position
+    or the step's position buffer) holds x_k and the list is valid for it."""
     lib = nat.load()
     plan = nbrs.plan
     spec = plan.spec
@@ -798,8 +799,7 @@ class NVELoop:
     if state.x64:
       return self._run_steps64(state, nbrs, n_steps, events, hook, gate, thr, cps, cs, psides,
                                side, wraps, pot, idx, width, err, ref, occ, ws)
-    if (self.fused and self.fused != 'update' and hook is None and gate is not None
-        and fmt == nat.DENSE):
+    if (self.fused and self.fused != 'update' and gate is not None and fmt == nat.DENSE):
       # The force launch also starts the next step: skin test of the
       # positions it used (the next update's gate word), opening half kick
       # and drift into the other position buffer; the block's first opening
@@ -824,6 +824,8 @@ class NVELoop:
           events.stop('update', e0)
           events.builds_after_update(plan)
           e0 = events.start()
+        if hook is not None:
+          hook(k, xk)
         nat.check(lib.jmd_nve_force_kick_drift(
             nat.ptr(xk1), N, dim, stride, gcode, esides, nat.ctypes.byref(pot), idx, width, F, p,
             float(self.half), None, x_hi, mv, mass_s if mass_v is None else 0.0, wraps, side,
@@ -876,7 +878,7 @@ class NVELoop:
         events.stop('update', e0)
         events.builds_after_update(plan)
       if hook is not None:
-        hook(k)
+        hook(k, state.position)
       nat.check(lib.jmd_nve_kick_drift(
           x_hi, p, F, mv, mass_s if mass_v is None else 0.0, N, dim, wraps, side,
           float(self.dt), float(self.half), None, x_hi, x, stride, p, ref, thr, gate, st),
