@@ -276,6 +276,19 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   else:
     xs = [simulate.new_positions(N, dim, dev) for _ in range(K)]
   idxs = [torch.empty_like(nbrs.idx) for _ in range(K)]
+  # the backward loop's call arguments, converted once
+  dense = nbrs.format is partition.Dense
+  x_ptrs = [nat.ptr(t) for t in xs]
+  idx_ptrs = [nat.ptr(t) for t in idxs]
+  dense_width = nbrs.idx.shape[1]
+  x_stride = 0 if x64 else nat.position_stride(xs[0])
+  periodic = int(geo.periodic)
+  pot_ref = nat.ctypes.byref(pot)
+  ph_a, ph_b = nat.ptr(PH), nat.ptr(PH2)
+  flip = 0  # PH is ph_a while flip is 0
+  X_p, acc_p = nat.ptr(X), nat.ptr(acc)
+  c_step, c_first = float(dt32), float(f32(dt32 / f32(2)))
+  dt_m = float(dt32) / m
   ckpt_bytes = sum(sum(t.numel() * t.element_size() for t in c[2].values()) for c in ckpts)
   ckpt_bytes += sum(t.numel() * t.element_size() for t in xs + idxs)
 
@@ -289,21 +302,24 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
     segment(k, hook)
     for q in range(k - 1, -1, -1):
       step = start + q
-      c = float(dt32) if step >= 1 else float(f32(dt32 / f32(2)))
-      rows = dense_rows(idxs[q], N, nbrs.format)
-      width = rows.shape[1]
-      if x64:
-        nat.check(lib.jmd_nve_adjoint_step_x64(
-            nat.ptr(xs[q]), N, dim, int(geo.periodic), side, nat.ctypes.byref(pot),
-            nat.ptr(rows), width, nat.ptr(PH), nat.ptr(PH2), nat.ptr(X), c, float(dt32) / m,
-            nat.ptr(acc), c, st), 'jmd_nve_adjoint_step_x64')
+      c = c_step if step >= 1 else c_first
+      if dense:
+        rows_p, width = idx_ptrs[q], dense_width
       else:
-        nat.check(lib.jmd_nve_adjoint_step(
-            nat.ptr(xs[q]), N, dim, nat.position_stride(xs[q]), int(geo.periodic), side,
-            nat.ctypes.byref(pot), nat.ptr(rows), width, nat.ptr(PH), nat.ptr(PH2),
-            nat.ptr(X), c, float(dt32) / m, nat.ptr(acc), c, st), 'jmd_nve_adjoint_step')
+        rows = dense_rows(idxs[q], N, nbrs.format)
+        rows_p, width = nat.ptr(rows), rows.shape[1]
+      ph, ph2 = (ph_a, ph_b) if flip == 0 else (ph_b, ph_a)
+      if x64:
+        rc = lib.jmd_nve_adjoint_step_x64(x_ptrs[q], N, dim, periodic, side, pot_ref, rows_p,
+                                          width, ph, ph2, X_p, c, dt_m, acc_p, c, st)
+      else:
+        rc = lib.jmd_nve_adjoint_step(x_ptrs[q], N, dim, x_stride, periodic, side, pot_ref,
+                                      rows_p, width, ph, ph2, X_p, c, dt_m, acc_p, c, st)
+      if rc:
+        nat.check(rc, 'jmd_nve_adjoint_step')
       if step >= 1:
         PH, PH2 = PH2, PH
+        flip ^= 1
   out = torch.empty((count,), dtype=torch.float64, device=dev)
   nat.check(lib.jmd_adjoint_param_gather(nat.ptr(acc), N, dim, nat.ctypes.byref(pot),
                                          nat.ptr(out), st), 'jmd_adjoint_param_gather')
