@@ -811,26 +811,35 @@ class NVELoop:
             x_hi, p, F, mv, mass_s if mass_v is None else 0.0, N, dim, wraps, side,
             float(self.dt), float(self.half), None, x_hi, nat.ptr(xs[1]), stride, p, None, thr,
             None, st), 'jmd_nve_kick_drift')
+      # the two calls' arguments for even and odd steps, converted once (the
+      # host issues them faster than a small system's steps run)
+      xp, gp = [nat.ptr(t) for t in xs], [nat.ptr(t) for t in gs]
+      build_args = [(xp[e], N, dim, stride, int(plan.periodic), psides,
+                     float(spec.cutoff_sq), int(plan.use_cell_list), cps, cs,
+                     int(nbrs.cell_list_capacity or 0), bfmt, int(spec.mask_self), idx,
+                     int(nbrs.max_occupancy), occ, err, ref, gp[e], ws, st) for e in (0, 1)]
+      pot_ref = nat.ctypes.byref(pot)
+      ms = mass_s if mass_v is None else 0.0
+      step_args = [[(xp[1 - e], N, dim, stride, gcode, esides, pot_ref, idx, width, F, p,
+                     float(self.half), None, x_hi, mv, ms, wraps, side, float(self.dt), ref, thr,
+                     gp[1 - e], drift, xp[e], st) for drift in (0, 1)] for e in (0, 1)]
+      build, step_fn = lib.jmd_neighbor_list_build, lib.jmd_nve_force_kick_drift
       for k in range(n_steps):
-        xk, xk1 = xs[k % 2], xs[(k + 1) % 2]
+        e = k % 2
         if events is not None:
           e0 = events.start()
-        nat.check(lib.jmd_neighbor_list_build(
-            nat.ptr(xk), N, dim, stride, int(plan.periodic), psides, float(spec.cutoff_sq),
-            int(plan.use_cell_list), cps, cs, int(nbrs.cell_list_capacity or 0), bfmt,
-            int(spec.mask_self), idx, int(nbrs.max_occupancy), occ, err, ref,
-            nat.ptr(gs[k % 2]), ws, st), 'jmd_neighbor_list_build')
+        rc = build(*build_args[e])
+        if rc:
+          nat.check(rc, 'jmd_neighbor_list_build')
         if events is not None:
           events.stop('update', e0)
           events.builds_after_update(plan)
           e0 = events.start()
         if hook is not None:
-          hook(k, xk)
-        nat.check(lib.jmd_nve_force_kick_drift(
-            nat.ptr(xk1), N, dim, stride, gcode, esides, nat.ctypes.byref(pot), idx, width, F, p,
-            float(self.half), None, x_hi, mv, mass_s if mass_v is None else 0.0, wraps, side,
-            float(self.dt), ref, thr, nat.ptr(gs[(k + 1) % 2]), int(k + 1 < n_steps),
-            nat.ptr(xk), st), 'jmd_nve_force_kick_drift')
+          hook(k, xs[e])
+        rc = step_fn(*step_args[e][int(k + 1 < n_steps)])
+        if rc:
+          nat.check(rc, 'jmd_nve_force_kick_drift')
         if events is not None:
           events.stop('force', e0)
       if n_steps % 2:  # the state's buffer and gate word hold step n's values
