@@ -343,6 +343,9 @@ constexpr int kHomeGroup = 512;
 #define JMD_NL_HOME_PASS 4
 #endif
 constexpr int kHomePass = JMD_NL_HOME_PASS;  // home particles per candidate pass
+constexpr int kNlBlock = 256;                 // k_nl_cells workgroup size
+constexpr int kNlWaves = kNlBlock / kWave;
+constexpr int kStageQ = (kStage + kNlBlock - 1) / kNlBlock;  // staging loads per lane
 
 template <int DIM, class G>
 __device__ __forceinline__ bool pair_test(const float* xi, const float* xj,
@@ -429,7 +432,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
   __shared__ float4 s_pos[kStage];
   __shared__ int s_cnt[kHomeGroup];
   __shared__ int s_occ;
-  __shared__ int s_wc[16];  // kept candidates per wavefront of a staging chunk
+  __shared__ int s_wc[kStageQ * kNlWaves];  // kept candidates per load round and wavefront
 
   const Grid& g = A.g;
   const Space& s = A.s;
@@ -452,6 +455,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
 
   for (int home = home0; home < g.ncell; home += home_stride) {
     __syncthreads();  // LDS reuse across home cells
+    int ccell = 0;  // members of stencil cell tid (tid < NST)
     if (tid < NST) {
       int hc[3], h = home;
 #pragma unroll
@@ -473,25 +477,30 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
         s_shift[tid][d] = sh;
       }
       const int cs0 = A.w.cell_start[c];
-      int cnt = A.w.cell_start[c + 1] - cs0;
+      ccell = A.w.cell_start[c + 1] - cs0;
       // k_cell_sort orders at most cap members of a cell (more: overflow)
-      if (g.cap > 0) cnt = min(cnt, g.cap);
+      if (g.cap > 0) ccell = min(ccell, g.cap);
       // the slot rotation of a cell's members (start mod cap, the cell
       // slots of partition.py): zero when the members do not wrap
       int rot = 0;
       if (g.cap > 0) {
         const int w = g.cap - (cs0 % g.cap);
-        rot = cnt > w ? w : 0;
+        rot = ccell > w ? w : 0;
       }
       s_cs[tid] = cs0;
-      s_cc[tid] = cnt;
+      s_cc[tid] = ccell;
       s_rot[tid] = rot;
     }
-    __syncthreads();
-    if (tid == 0) {
-      int acc = 0;
-      for (int t = 0; t < NST; ++t) { s_off[t] = acc; acc += s_cc[t]; }
-      s_off[NST] = acc;
+    if (tid < kWave) {
+      // candidate offsets of the stencil cells: an exclusive scan of the
+      // counts over the first 32 lanes (lane NST, count 0, gets the total)
+      int incl = ccell;
+#pragma unroll
+      for (int o = 1; o < 32; o <<= 1) {
+        const int u = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += u;
+      }
+      if (tid <= NST) s_off[tid] = incl - ccell;
     }
     __syncthreads();
     const int total = s_off[NST];
@@ -526,31 +535,44 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
         __syncthreads();
         // Stage candidates c0..c0+ccount in reference order, keeping those
         // that can reach the home cell (compacted in order), padded to a
-        // multiple of 64 with far-away entries.
-        int kept = 0;  // workgroup-uniform
-        for (int q0 = 0; q0 < ccount; q0 += blockDim.x) {
-          const int q = q0 + tid;
-          bool keep = false;
-          float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // multiple of 64 with far-away entries.  Every lane's loads (kStageQ
+        // per lane at most) are issued before any is used, then one
+        // compaction over (load round, wavefront, lane) keeps the order.
+        float4 v[kStageQ];
+        int vt[kStageQ];
+#pragma unroll
+        for (int r = 0; r < kStageQ; ++r) {
+          const int q = r * kNlBlock + tid;
+          v[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          vt[r] = 0;
           if (q < ccount) {
-            int p = c0 + q;
+            const int p = c0 + q;
             // the stencil cell of candidate p: the last t with s_off[t] <= p
             // (empty cells have s_off[t] == s_off[t + 1] and are passed over)
             int t = 0;
 #pragma unroll
             for (int step = 16; step > 0; step >>= 1)
               if (t + step < NST && s_off[t + step] <= p) t += step;
-            int m = p - s_off[t];
+            const int m = p - s_off[t];
             int k = m;
             const int w = s_rot[t];
             if (w > 0) k = (m < s_cc[t] - w) ? (w + m) : (m - (s_cc[t] - w));
-            int gi = s_cs[t] + k;
-            float4 xp = A.w.sorted_pos[gi];  // w: the id bits
-            v = make_float4(xp.x + s_shift[t][0], xp.y + s_shift[t][1], xp.z + s_shift[t][2],
-                            xp.w);
-            keep = true;
+            v[r] = A.w.sorted_pos[s_cs[t] + k];  // w: the id bits
+            vt[r] = t;
+          }
+        }
+        unsigned keepm = 0;
+#pragma unroll
+        for (int r = 0; r < kStageQ; ++r) {
+          const int q = r * kNlBlock + tid;
+          if (q < ccount) {
+            const int t = vt[r];
+            v[r].x += s_shift[t][0];
+            v[r].y += s_shift[t][1];
+            v[r].z += s_shift[t][2];
+            bool keep = true;
             if (prune) {
-              const float cv[3] = {v.x, v.y, v.z};
+              const float cv[3] = {v[r].x, v[r].y, v[r].z};
               float D2 = 0.0f;
 #pragma unroll
               for (int d = 0; d < DIM; ++d) {
@@ -559,19 +581,28 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
               }
               keep = (BOXS ? D2 * s.smin2 : D2) <= reach_sq;
             }
+            if (keep) keepm |= 1u << r;
           }
-          const unsigned long long bal = __ballot(keep);
-          if (lane == 0) s_wc[wave] = (int)__popcll(bal);
-          __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < kStageQ; ++r) {
+          const unsigned long long bal = __ballot((keepm >> r) & 1u);
+          if (lane == 0) s_wc[r * kNlWaves + wave] = (int)__popcll(bal);
+        }
+        __syncthreads();
+        int kept = 0;  // workgroup-uniform
+#pragma unroll
+        for (int r = 0; r < kStageQ; ++r) {
+          const unsigned long long bal = __ballot((keepm >> r) & 1u);
           int before = 0, all = 0;
-          for (int w = 0; w < nwave; ++w) {
-            const int c = s_wc[w];
+#pragma unroll
+          for (int w = 0; w < kNlWaves; ++w) {
+            const int c = s_wc[r * kNlWaves + w];
             before += w < wave ? c : 0;
             all += c;
           }
-          if (keep) s_pos[kept + before + wave_prefix_count(bal)] = v;
+          if ((keepm >> r) & 1u) s_pos[kept + before + wave_prefix_count(bal)] = v[r];
           kept += all;
-          __syncthreads();  // s_wc is rewritten by the next chunk
         }
         const int cpad = (kept + kWave - 1) & ~(kWave - 1);
         for (int q = kept + tid; q < cpad; q += blockDim.x)
