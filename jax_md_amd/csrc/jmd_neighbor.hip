@@ -338,7 +338,10 @@ enum { TEST_IJ = 0, TEST_JI = 1, TEST_BOTH = 2 };
 #define JMD_NL_STAGE 1536
 #endif
 constexpr int kStage = JMD_NL_STAGE;  // candidates staged in LDS per pass
-constexpr int kHomeGroup = 512;
+#ifndef JMD_NL_HOME_GROUP
+#define JMD_NL_HOME_GROUP 512
+#endif
+constexpr int kHomeGroup = JMD_NL_HOME_GROUP;  // home particles per candidate staging
 #ifndef JMD_NL_HOME_PASS
 #define JMD_NL_HOME_PASS 4
 #endif
@@ -431,6 +434,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
   __shared__ float s_shift[NST][3];
   __shared__ float4 s_pos[kStage];
   __shared__ int s_cnt[kHomeGroup];
+  __shared__ float4 s_home[kHomeGroup];  // the home group's sorted positions (id bits in w)
   __shared__ int s_occ;
   __shared__ int s_wc[kStageQ * kNlWaves];  // kept candidates per load round and wavefront
 
@@ -529,7 +533,10 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
 
     for (int g0 = 0; g0 < hcount; g0 += kHomeGroup) {
       const int gcount = min(kHomeGroup, hcount - g0);
-      for (int a = tid; a < gcount; a += blockDim.x) s_cnt[a] = 0;
+      for (int a = tid; a < gcount; a += blockDim.x) {
+        s_cnt[a] = 0;
+        s_home[a] = A.w.sorted_pos[hstart + g0 + a];  // read by the passes and the epilogue
+      }
       for (int c0 = 0; c0 < total; c0 += kStage) {
         const int ccount = min(kStage, total - c0);
         __syncthreads();
@@ -628,8 +635,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
             cnt[h] = 0;
             base[h] = 0;
             if (a < gcount) {
-              const int hi = hstart + g0 + a;
-              const float4 xp = A.w.sorted_pos[hi];
+              const float4 xp = s_home[a];
               xi[h][0] = xp.x;
               xi[h][1] = xp.y;
               xi[h][2] = xp.z;
@@ -714,8 +720,7 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
       __syncthreads();
       // Row epilogue: padding (Dense), counts, occupancy.
       for (int a = wave; a < gcount; a += nwave) {
-        const int hi = hstart + g0 + a;
-        const int i = A.w.sorted_id[hi];
+        const int i = __float_as_int(s_home[a].w);
         const int cnt = s_cnt[a];
         if (MODE == MODE_DENSE) {
           for (long long k = (long long)cnt + lane; k < width; k += kWave)
