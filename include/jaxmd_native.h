@@ -441,6 +441,9 @@ int jmd_fire_ensemble_set_threads(int32_t threads);
  * fire_n_pos [n_systems]; error [n_systems] sticky overflow bits; builds
  * [n_systems] counts list builds; scratch holds 3 * n_total int32.  The
  * potential's parameters index particles by their global (concatenated) id.
+ * width is the row stride of idx (at least every system's max_occupancy); a
+ * multiple of 4 with a 16-byte aligned idx lets the pair loops read rows 16
+ * bytes at a time.
  * No counterpart entry point in the reference, which runs one system at a
  * time. */
 int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int32_t n_systems,

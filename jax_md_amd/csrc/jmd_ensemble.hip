@@ -30,6 +30,14 @@ inline int ens_threads(int n_systems, long long n_total) {
   return n_systems > 0 && n_total / n_systems >= 4096 ? 1024 : 512;
 }
 constexpr int kEnsMaxCells = 8192;
+#ifndef JMD_ENS_ROW_CHUNK
+#define JMD_ENS_ROW_CHUNK 4
+#endif
+constexpr int kEnsRowChunk = JMD_ENS_ROW_CHUNK;  // row entries gathered together (even)
+static_assert(kEnsRowChunk % 2 == 0, "entry parity is the chunk position's");
+constexpr int kEnsParMax = 64;  // species matrices staged in LDS (n_species^2 at most)
+constexpr int kEnsPerThread = 8;  // particles per thread of a system staged in LDS
+static_assert(kEnsThreads * kEnsPerThread <= (1 << 16), "staged ids are 16-bit");
 
 struct EnsSys {
   int offset, n;
@@ -107,24 +115,106 @@ __device__ __forceinline__ int wg_max(int v, int* s) {
   return r;
 }
 
+// Ascending order of 8 values: Batcher's odd-even merge sort network
+// (19 compare-exchanges, all indices static so the values stay in registers).
+__device__ __forceinline__ void cmp_swap(int& a, int& b) {
+  const int lo = min(a, b), hi = max(a, b);
+  a = lo;
+  b = hi;
+}
+__device__ __forceinline__ void sort8(int* v) {
+  cmp_swap(v[0], v[1]); cmp_swap(v[2], v[3]); cmp_swap(v[4], v[5]); cmp_swap(v[6], v[7]);
+  cmp_swap(v[0], v[2]); cmp_swap(v[1], v[3]); cmp_swap(v[4], v[6]); cmp_swap(v[5], v[7]);
+  cmp_swap(v[1], v[2]); cmp_swap(v[5], v[6]);
+  cmp_swap(v[0], v[4]); cmp_swap(v[1], v[5]); cmp_swap(v[2], v[6]); cmp_swap(v[3], v[7]);
+  cmp_swap(v[2], v[4]); cmp_swap(v[3], v[5]);
+  cmp_swap(v[1], v[2]); cmp_swap(v[3], v[4]); cmp_swap(v[5], v[6]);
+}
+
+// Ascending order of one cell's ids m[0..cnt): up to 8 through the sorting
+// network in registers, more by insertion sort in place.
+template <typename T>
+__device__ __forceinline__ void ens_sort_cell(T* m, int cnt) {
+  if (cnt <= 8) {
+    int v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = u < cnt ? (int)m[u] : 0x7fffffff;
+    sort8(v);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u < cnt) m[u] = (T)v[u];
+    return;
+  }
+  for (int u = 1; u < cnt; ++u) {
+    const T v = m[u];
+    int w = u - 1;
+    while (w >= 0 && m[w] > v) {
+      m[w + 1] = m[w];
+      --w;
+    }
+    m[w + 1] = v;
+  }
+}
+
+// A system's float32 positions (and species ids) as the pair and list loops
+// read them: from LDS when the workgroup staged the system there (l, ls), else
+// from global memory.  base is the system's first global id.
+template <bool STAGED>
+struct EnsView {
+  const float* __restrict__ g;
+  const float* l;
+  const int* __restrict__ gs;
+  const uint8_t* ls;  // staged species ids (nullptr: gs)
+  int base;
+  __device__ __forceinline__ void pos(int i, float* xi) const {
+    if constexpr (STAGED) {
+      xi[0] = l[2 * (i - base)];
+      xi[1] = l[2 * (i - base) + 1];
+    } else {
+      load_pos<2>(g, i, xi);
+    }
+  }
+  __device__ __forceinline__ int species(int i) const {
+    if constexpr (STAGED) {
+      if (ls) return (int)ls[i - base];
+    }
+    return gs ? gs[i] : 0;
+  }
+};
+
 // Bins the system on grid g: LDS counts, exclusive starts, arrival ranks,
 // scatter into sorted[] (system-local), then each cell's ids in ascending
 // order (the stable argsort of partition.py:432).  s_cs holds the counts,
 // then, scanned in place, the starts (cell c spans s_cs[c] .. s_cs[c + 1]).
 // Returns the max filling.
-__device__ int ens_bin(const EnsSys& S, const Grid& g, const float* __restrict__ x,
-                       int* __restrict__ cellp, int* __restrict__ rankp,
-                       int* __restrict__ sorted, int* s_cs, int* s_red) {
+template <bool STAGED>
+__device__ __forceinline__ int ens_bin(const EnsSys& S, const Grid& g, const EnsView<STAGED>& V,
+                                       int* __restrict__ cellp, int* __restrict__ rankp,
+                                       int* __restrict__ sorted, int* s_cs, int* s_red,
+                                       uint16_t* ls) {
   const int nt = blockDim.x, t = threadIdx.x;
   for (int c = t; c < g.ncell; c += nt) s_cs[c] = 0;
   __syncthreads();
-  for (int a = t; a < S.n; a += nt) {
-    float xi[2];
-    load_pos<2>(x, S.offset + a, xi);
-    int ci[2];
-    const int h = cell_of<2>(xi, g, ci);
-    cellp[a] = h;
-    rankp[a] = atomicAdd(&s_cs[h], 1);
+  // staged system (ls: its cell-sorted ids in LDS): counts here, then after
+  // the scan a second pass of LDS atomics on the starts places each id (the
+  // cells are sorted by id afterwards, so arrival order does not matter);
+  // cells are recomputed from the LDS positions, nothing goes to memory
+  if constexpr (STAGED) {
+    for (int a = t; a < S.n; a += nt) {
+      float xi[2];
+      V.pos(S.offset + a, xi);
+      int ci[2];
+      atomicAdd(&s_cs[cell_of<2>(xi, g, ci)], 1);
+    }
+  } else {
+    for (int a = t; a < S.n; a += nt) {
+      float xi[2];
+      V.pos(S.offset + a, xi);
+      int ci[2];
+      const int h = cell_of<2>(xi, g, ci);
+      cellp[a] = h;
+      rankp[a] = atomicAdd(&s_cs[h], 1);
+    }
   }
   __syncthreads();
   // exclusive scan of the counts: per-thread chunks, a wavefront scan of the
@@ -164,21 +254,34 @@ __device__ int ens_bin(const EnsSys& S, const Grid& g, const float* __restrict__
   }
   if (t == 0) s_cs[g.ncell] = s_wave[nw];
   __syncthreads();
-  for (int a = t; a < S.n; a += nt) sorted[s_cs[cellp[a]] + rankp[a]] = a;
-  __syncthreads();
-  // order each cell by id (thread per cell, insertion sort of a few ids)
-  for (int c = t; c < g.ncell; c += nt) {
-    int* m = sorted + s_cs[c];
-    const int cnt = s_cs[c + 1] - s_cs[c];
-    for (int u = 1; u < cnt; ++u) {
-      const int v = m[u];
-      int w = u - 1;
-      while (w >= 0 && m[w] > v) {
-        m[w + 1] = m[w];
-        --w;
-      }
-      m[w + 1] = v;
+  if constexpr (STAGED) {
+    for (int a = t; a < S.n; a += nt) {
+      float xi[2];
+      V.pos(S.offset + a, xi);
+      int ci[2];
+      ls[atomicAdd(&s_cs[cell_of<2>(xi, g, ci)], 1)] = (uint16_t)a;
     }
+    __syncthreads();
+    // s_cs[c] now holds the end of cell c, the start of cell c + 1: shift
+    // back by one, chunks from the top (a chunk reads its own and the next
+    // lower chunk's values before any of them is written)
+    for (int c0 = ((g.ncell - 1) / nt) * nt; c0 >= 0; c0 -= nt) {
+      const int c = c0 + t;
+      const int v = c < g.ncell && c > 0 ? s_cs[c - 1] : 0;
+      __syncthreads();
+      if (c < g.ncell) s_cs[c] = v;
+      __syncthreads();
+    }
+  } else {
+    for (int a = t; a < S.n; a += nt) sorted[s_cs[cellp[a]] + rankp[a]] = a;
+  }
+  __syncthreads();
+  // order each cell by id (thread per cell): up to 8 ids through a sorting
+  // network in registers, more by insertion sort in place
+  if constexpr (STAGED) {
+    for (int c = t; c < g.ncell; c += nt) ens_sort_cell(ls + s_cs[c], s_cs[c + 1] - s_cs[c]);
+  } else {
+    for (int c = t; c < g.ncell; c += nt) ens_sort_cell(sorted + s_cs[c], s_cs[c + 1] - s_cs[c]);
   }
   __syncthreads();
   return wg_max(fill, s_red);
@@ -186,17 +289,21 @@ __device__ int ens_bin(const EnsSys& S, const Grid& g, const float* __restrict__
 
 // Dense rows of the system (global ids, padding pad) in the reference's
 // candidate order; returns the largest row count.
-__device__ int ens_rows(const EnsSys& S, const Grid& g, const Space& sp, float cutoff_sq,
-                        const float* __restrict__ x, const int* __restrict__ cellp,
+template <bool STAGED>
+__device__ __forceinline__ int ens_rows(const EnsSys& S, const Grid& g, const Space& sp,
+                                        float cutoff_sq, const EnsView<STAGED>& V,
+                                        const int* __restrict__ cellp,
                         const int* __restrict__ sorted, const int* s_cs,
-                        int* __restrict__ idx, int width, int pad, int* s_red) {
+                        int* __restrict__ idx, int width, int pad, int* s_red,
+                        const uint16_t* ls) {
   const int nt = blockDim.x, t = threadIdx.x;
   int occ = 0;
   for (int a = t; a < S.n; a += nt) {
     const int i = S.offset + a;
     float xi[2];
-    load_pos<2>(x, i, xi);
-    const int h = cellp[a];
+    V.pos(i, xi);
+    int ci[2];
+    const int h = STAGED ? cell_of<2>(xi, g, ci) : cellp[a];  // the binning's cell
     const int hx = h % g.cps[0], hy = h / g.cps[0];
     int cnt = 0;
     for (int st = 0; st < 9; ++st) {  // iota order, x offset slowest
@@ -205,19 +312,34 @@ __device__ int ens_rows(const EnsSys& S, const Grid& g, const Space& sp, float c
       const int start = s_cs[c];
       const int m = min(s_cs[c + 1] - start, g.cap);
       const int w = g.cap - (start % g.cap);
-      for (int q = 0; q < m; ++q) {
-        int k = q;  // slot order: rank (start + k) mod cap
-        if (m > w) k = (q < m - w) ? (w + q) : (q - (m - w));
-        const int jl = sorted[start + k];
-        if (jl == a) continue;  // mask_self
-        const int j = S.offset + jl;
-        float xj[2];
-        load_pos<2>(x, j, xj);
-        // Dense with cells keeps a candidate when both wrap(Ri - Rj) and
-        // wrap(Rj - Ri) are inside (partition.py:951,971)
-        if (ref_metric_sq<2>(xi, xj, sp) < cutoff_sq && ref_metric_sq<2>(xj, xi, sp) < cutoff_sq) {
-          if (cnt < width) idx[(long long)i * width + cnt] = j;
-          ++cnt;
+      // the cell's members in chunks: ids, then their positions, then the
+      // tests in slot order
+      for (int q0 = 0; q0 < m; q0 += kEnsRowChunk) {
+        int jl[kEnsRowChunk];
+        float xj[kEnsRowChunk][2];
+#pragma unroll
+        for (int c2 = 0; c2 < kEnsRowChunk; ++c2) {
+          const int q = q0 + c2;
+          int k = q;  // slot order: rank (start + k) mod cap
+          if (m > w) k = (q < m - w) ? (w + q) : (q - (m - w));
+          jl[c2] = q < m ? (STAGED ? (int)ls[start + k] : sorted[start + k]) : -1;
+        }
+#pragma unroll
+        for (int c2 = 0; c2 < kEnsRowChunk; ++c2) {
+          xj[c2][0] = xj[c2][1] = 0.0f;
+          if (jl[c2] >= 0) V.pos(S.offset + jl[c2], xj[c2]);
+        }
+#pragma unroll
+        for (int c2 = 0; c2 < kEnsRowChunk; ++c2) {
+          if (jl[c2] < 0 || jl[c2] == a) continue;  // past the cell; mask_self
+          const int j = S.offset + jl[c2];
+          // Dense with cells keeps a candidate when both wrap(Ri - Rj) and
+          // wrap(Rj - Ri) are inside (partition.py:951,971)
+          if (ref_metric_sq<2>(xi, xj[c2], sp) < cutoff_sq &&
+              ref_metric_sq<2>(xj[c2], xi, sp) < cutoff_sq) {
+            if (cnt < width) idx[(long long)i * width + cnt] = j;
+            ++cnt;
+          }
         }
       }
     }
@@ -227,46 +349,118 @@ __device__ int ens_rows(const EnsSys& S, const Grid& g, const Space& sp, float c
   return wg_max(occ, s_red);
 }
 
+// A system's species matrices (sigma, epsilon, alpha in species mode, and
+// the derivative coefficients dsig) copied to LDS, so that the pair loops
+// read them there instead of gathering them per pair.  Returns s_par, or
+// nullptr when the matrices are too large (the loops then read memory).
+// Called by the whole workgroup.
+__device__ __forceinline__ const float* ens_stage_params(const PairArgs& a,
+                                                         const float* __restrict__ dsig,
+                                                         float* s_par) {
+  const int nn = a.ns * a.ns;
+  if (a.ns < 1 || nn > kEnsParMax) return nullptr;
+  for (int q = threadIdx.x; q < nn; q += blockDim.x) {
+    s_par[q] = a.sig_mode == JMD_PARAM_SPECIES ? a.sig_v[q] : 0.0f;
+    s_par[kEnsParMax + q] = a.eps_mode == JMD_PARAM_SPECIES ? a.eps_v[q] : 0.0f;
+    s_par[2 * kEnsParMax + q] = a.alp_mode == JMD_PARAM_SPECIES ? a.alp_v[q] : 0.0f;
+    s_par[3 * kEnsParMax + q] = dsig ? dsig[q] : 0.0f;
+  }
+  __syncthreads();
+  return s_par;
+}
+
+// sigma, epsilon, alpha (which = 0, 1, 2) of pair (i, j), species matrices
+// from LDS when staged.
+__device__ __forceinline__ float ens_pick(const PairArgs& a, int which, const float* par, int i,
+                                          int j, int si, int sj) {
+  const int mode = which == 0 ? a.sig_mode : (which == 1 ? a.eps_mode : a.alp_mode);
+  if (par && mode == JMD_PARAM_SPECIES) return par[which * kEnsParMax + si * a.ns + sj];
+  return which == 0 ? pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns)
+                    : (which == 1 ? pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns)
+                                  : pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns));
+}
+
+// dsig[si, sj], from LDS when staged.
+__device__ __forceinline__ float ens_dsig(const PairArgs& a, const float* __restrict__ dsig,
+                                          const float* par, int si, int sj) {
+  return par ? par[3 * kEnsParMax + si * a.ns + sj] : dsig[si * a.ns + sj];
+}
+
 // Soft-sphere force of row i: two accumulators (entries k even / odd) summed
 // in float64, as the Dense kernel's two lanes per row do; optional energy.
-__device__ __forceinline__ void ens_force_row(int i, const float* __restrict__ x,
+template <bool STAGED>
+__device__ __forceinline__ void ens_force_row(int i, const EnsView<STAGED>& V,
                                               const int* __restrict__ idx, int width, int pad,
-                                              const Space& sp, const PairArgs& a, float* F,
-                                              double* e) {
+                                              const Space& sp, const PairArgs& a,
+                                              const float* par, float* F, double* e) {
 #pragma clang fp contract(off)
   float xi[2], xim[2];
-  load_pos<2>(x, i, xi);
+  V.pos(i, xi);
   xim[0] = xi[0] - sp.L[0];
   xim[1] = xi[1] - sp.L[1];
   float f[2][2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
   float en[2] = {0.0f, 0.0f};
-  const int si = a.species ? a.species[i] : 0;
+  const int si = V.species(i);
   const int* row = idx + (long long)i * width;
-  for (int k = 0; k < width; ++k) {
-    const int j = row[k];
-    if (j == pad) break;
-    float xj[2];
-    load_pos<2>(x, j, xj);
-    float d[2];
-    d[0] = min_image_disp(xj[0], xi[0], xim[0], sp.L[0], sp.half[0]);
-    d[1] = min_image_disp(xj[1], xi[1], xim[1], sp.L[1], sp.half[1]);
-    const float r2 = fmaf(d[1], d[1], d[0] * d[0]);
-    const int sj = a.species ? a.species[j] : 0;
-    const float sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
-    const float eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
-    const float alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
-    float dEr;
-    const float u = pair_eval<JMD_SOFT_SPHERE>(r2, sig, eps, alp, a, &dEr);
-    const int l = k & 1;
-    f[l][0] = fmaf(dEr, d[0], f[l][0]);
-    f[l][1] = fmaf(dEr, d[1], f[l][1]);
-    en[l] += u;
+  // The row in chunks: the chunk's ids, then all its gathers (position,
+  // species), then the terms in entry order; a row ends at its first pad.
+  // rows whose stride is a multiple of 4 entries are read 16 bytes at a time
+  const bool vec = kEnsRowChunk == 4 && (width & 3) == 0 && ((uintptr_t)idx & 15) == 0;
+  for (int k0 = 0; k0 < width; k0 += kEnsRowChunk) {
+    int js[kEnsRowChunk], sjs[kEnsRowChunk];
+    float xj[kEnsRowChunk][2];
+    if (vec) {
+      const int4 q = *reinterpret_cast<const int4*>(row + k0);
+      js[0] = q.x;
+      js[1 % kEnsRowChunk] = q.y;
+      js[2 % kEnsRowChunk] = q.z;
+      js[3 % kEnsRowChunk] = q.w;
+    } else {
+#pragma unroll
+      for (int c = 0; c < kEnsRowChunk; ++c) js[c] = k0 + c < width ? row[k0 + c] : pad;
+    }
+#pragma unroll
+    for (int c = 0; c < kEnsRowChunk; ++c) {
+      xj[c][0] = xj[c][1] = 0.0f;
+      sjs[c] = 0;
+      if (js[c] != pad) {
+        V.pos(js[c], xj[c]);
+        sjs[c] = V.species(js[c]);
+      }
+    }
+    bool end = false;
+#pragma unroll
+    for (int c = 0; c < kEnsRowChunk; ++c) {
+      const int j = js[c];
+      if (j == pad) {
+        end = true;
+        break;
+      }
+      float d[2];
+      d[0] = min_image_disp(xj[c][0], xi[0], xim[0], sp.L[0], sp.half[0]);
+      d[1] = min_image_disp(xj[c][1], xi[1], xim[1], sp.L[1], sp.half[1]);
+      const float r2 = fmaf(d[1], d[1], d[0] * d[0]);
+      const int sj = sjs[c];
+      const float sig = ens_pick(a, 0, par, i, j, si, sj);
+      const float eps = ens_pick(a, 1, par, i, j, si, sj);
+      const float alp = ens_pick(a, 2, par, i, j, si, sj);
+      float dEr;
+      const float u = pair_eval<JMD_SOFT_SPHERE>(r2, sig, eps, alp, a, &dEr);
+      const int l = c & 1;  // k0 is even: the parity of k = k0 + c
+      f[l][0] = fmaf(dEr, d[0], f[l][0]);
+      f[l][1] = fmaf(dEr, d[1], f[l][1]);
+      en[l] += u;
+    }
+    if (end) break;
   }
   F[0] = (float)((double)f[0][0] + (double)f[1][0]);
   F[1] = (float)((double)f[0][1] + (double)f[1][1]);
   if (e) *e += 0.5 * ((double)en[0] + (double)en[1]);
 }
 
+// STAGED: the systems of at most lds_n particles, kept in LDS; else the
+// larger ones, read from global memory.  Each instance skips the other's.
+template <bool STAGED>
 __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     const EnsSys* __restrict__ systems, const int* __restrict__ order, int nsys, int ntot,
     int mode, int n_steps, double* __restrict__ x_hi, float* __restrict__ x,
@@ -274,8 +468,11 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     int* __restrict__ idx, int width, int* __restrict__ scratch, PairArgs a0, FireP prm,
     float* __restrict__ fire_state, int* __restrict__ fire_npos, uint8_t* __restrict__ error,
     int* __restrict__ out_stat, double* __restrict__ energy, int* __restrict__ builds,
-    EnsTape tape) {
+    EnsTape tape, int lds_n) {
   __shared__ int s_cs[kEnsMaxCells + 1];
+  // dynamic LDS: [2 lds_n] float32 positions, [lds_n] uint16 cell-sorted
+  // ids, [lds_n] uint8 species ids
+  extern __shared__ float4 s_dyn[];
   __shared__ int s_red[kEnsThreads / kWave];
   __shared__ double s_d[3 * (kEnsThreads / kWave)];
   __shared__ float s_scal[4];  // dt, alpha, ratio, keep
@@ -283,12 +480,30 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   const int sid = order[blockIdx.x];
   if (sid < 0 || sid >= nsys) return;
   const EnsSys S = systems[sid];
+  if (STAGED != (S.n <= lds_n)) return;  // the other instance's system
   const int t = threadIdx.x, nt = blockDim.x;
   const float cutoff_sq = S.cutoff_sq, thr_sq = S.threshold_sq;
   PairArgs a = a0;  // this system's parameter block
   if (a.sig_mode == JMD_PARAM_SPECIES) a.sig_v += S.param_offset;
   if (a.eps_mode == JMD_PARAM_SPECIES) a.eps_v += S.param_offset;
   if (a.alp_mode == JMD_PARAM_SPECIES) a.alp_v += S.param_offset;
+  __shared__ float s_par[4 * kEnsParMax];
+  const float* par = ens_stage_params(a, nullptr, s_par);
+  // A staged system keeps a copy of its positions (and species ids, when
+  // they fit a byte) and its cell-sorted ids in LDS for the list and pair
+  // loops; the integrator writes both position copies.
+  float* lx = STAGED ? reinterpret_cast<float*>(s_dyn) : nullptr;
+  uint16_t* lsorted = STAGED ? reinterpret_cast<uint16_t*>(lx + 2 * lds_n) : nullptr;
+  uint8_t* lsp = STAGED && a.species && a.ns <= 256
+                     ? reinterpret_cast<uint8_t*>(lsorted + lds_n) : nullptr;
+  if constexpr (STAGED) {
+    for (int k = threadIdx.x; k < 2 * S.n; k += blockDim.x) lx[k] = x[2LL * S.offset + k];
+    if (lsp)
+      for (int q = threadIdx.x; q < S.n; q += blockDim.x)
+        lsp[q] = (uint8_t)a.species[S.offset + q];
+    __syncthreads();
+  }
+  const EnsView<STAGED> V{x, lx, a.species, lsp, S.offset};
   int* cellp = scratch + S.offset;
   int* rankp = scratch + ntot + S.offset;
   int* sorted = scratch + 2LL * ntot + S.offset;
@@ -318,7 +533,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     g2.cell_size[0] = S.fill_cell_size[0];
     g2.cell_size[1] = S.fill_cell_size[1];
     g2.ncell = g2.cps[0] * g2.cps[1];
-    const int fill = ens_bin(S, g2, x, cellp, rankp, sorted, s_cs, s_red);
+    const int fill = ens_bin(S, g2, V, cellp, rankp, sorted, s_cs, s_red, lsorted);
     if (t == 0) out_stat[sid] = fill;
     return;
   }
@@ -327,14 +542,14 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   int nbuilds = 0;
   // list build at the current positions (allocate / update)
   auto rebuild = [&](bool store) {
-    const int fill = ens_bin(S, g, x, cellp, rankp, sorted, s_cs, s_red);
-    const int occ = ens_rows(S, g, sp, cutoff_sq, x, cellp, sorted, s_cs, idx,
-                             store ? width : 0, pad, s_red);
+    const int fill = ens_bin(S, g, V, cellp, rankp, sorted, s_cs, s_red, lsorted);
+    const int occ = ens_rows(S, g, sp, cutoff_sq, V, cellp, sorted, s_cs, idx,
+                             store ? width : 0, pad, s_red, lsorted);
     if (fill > g.cap) err |= JMD_CELL_LIST_OVERFLOW;
     if (occ > S.max_occ) err |= JMD_NEIGHBOR_LIST_OVERFLOW;
     for (int q = t; q < S.n; q += nt) {
       float xi[2];
-      load_pos<2>(x, S.offset + q, xi);
+      V.pos(S.offset + q, xi);
       store_pos<2>(x_ref, S.offset + q, xi);
     }
     ++nbuilds;
@@ -355,7 +570,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     for (int q = t; q < S.n; q += nt) {
       const int i = S.offset + q;
       float f[2];
-      ens_force_row(i, x, idx, width, pad, sp, a, f, &e);
+      ens_force_row(i, V, idx, width, pad, sp, a, par, f, &e);
       F[2LL * i] = f[0];
       F[2LL * i + 1] = f[1];
       p[2LL * i] = 0.0f;
@@ -397,9 +612,10 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   for (int step = 0; step < n_steps; ++step) {
     // neighbour update: rebuild when any particle moved beyond half the skin
     bool moved = false;
+#pragma unroll 4
     for (int q = t; q < S.n; q += nt) {
       float xi[2], ri[2];
-      load_pos<2>(x, S.offset + q, xi);
+      V.pos(S.offset + q, xi);
       load_pos<2>(x_ref, S.offset + q, ri);
       moved = moved || ref_metric_sq<2>(xi, ri, sp) > thr_sq;
     }
@@ -414,6 +630,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     // velocity-Verlet sub-step at the FIRE time step (float16 value)
     const float dt = s_scal[0];
     const float half = dt / 2.0f;
+#pragma unroll 4
     for (int k = t; k < 2 * S.n; k += nt) {
       const long long q = 2LL * S.offset + k;
       const float pk = p[q] + half * F[q];
@@ -422,29 +639,41 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
       p[q] = pk;
       x_hi[q] = xv;
       x[q] = (float)xv;
+      if constexpr (STAGED) lx[k] = (float)xv;
     }
     __syncthreads();
     for (int q = t; q < S.n; q += nt) {
       const int i = S.offset + q;
       float f[2];
-      ens_force_row(i, x, idx, width, pad, sp, a, f, nullptr);
+      ens_force_row(i, V, idx, width, pad, sp, a, par, f, nullptr);
       F[2LL * i] = f[0];
       F[2LL * i + 1] = f[1];
     }
     __syncthreads();
     double ff = 0.0, pp = 0.0, fp = 0.0;
-    for (int k = t; k < 2 * S.n; k += nt) {
-      const long long q = 2LL * S.offset + k;
-      const float pk = p[q] + half * F[q];
-      p[q] = pk;
-      const double fv = F[q], pv = pk;
-      ff += fv * fv;
-      pp += pv * pv;
-      fp += fv * pv;
-      if (tape.x) {
+    if (tape.x) {
+      for (int k = t; k < 2 * S.n; k += nt) {
+        const long long q = 2LL * S.offset + k;
+        const float pk = p[q] + half * F[q];
+        p[q] = pk;
+        const double fv = F[q], pv = pk;
+        ff += fv * fv;
+        pp += pv * pv;
+        fp += fv * pv;
         tape.q[step * n2 + q] = pk;
         tape.x[(step + 1) * n2 + q] = x[q];
         tape.F[(step + 1) * n2 + q] = F[q];
+      }
+    } else {
+#pragma unroll 4
+      for (int k = t; k < 2 * S.n; k += nt) {
+        const long long q = 2LL * S.offset + k;
+        const float pk = p[q] + half * F[q];
+        p[q] = pk;
+        const double fv = F[q], pv = pk;
+        ff += fv * fv;
+        pp += pv * pv;
+        fp += fv * pv;
       }
     }
     wg_sum3(ff, pp, fp, s_d);
@@ -471,6 +700,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
       s_scal[0] = dtn;
       s_scal[1] = al;
     }
+#pragma unroll 4
     for (int k = t; k < 2 * S.n; k += nt) {
       const long long q = 2LL * S.offset + k;
       double pv = p[q];
@@ -482,7 +712,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   double e = 0.0, z1 = 0.0, z2 = 0.0;
   for (int q = t; q < S.n; q += nt) {
     float f[2];
-    ens_force_row(S.offset + q, x, idx, width, pad, sp, a, f, &e);
+    ens_force_row(S.offset + q, V, idx, width, pad, sp, a, par, f, &e);
   }
   wg_sum3(e, z1, z2, s_d);
   if (t == 0) {
@@ -534,57 +764,91 @@ __device__ __forceinline__ float image_of(float raw, float half) {
 // h_i = (J v)_i over the system's rows at positions x with list idx; returns
 // this thread's share of v^T dF/dtheta (rows it owns) and, with box, adds its
 // share of v^T dF/dL to lacc.
-__device__ double ens_hvp_rows(const EnsSys& S, const Space& sp, const PairArgs& a,
-                               const float* __restrict__ dsig, const float* __restrict__ x,
-                               const int* __restrict__ idx, int width, int pad,
-                               const float* __restrict__ v, double* __restrict__ X, bool box,
-                               double& lacc) {
+template <bool STAGED>
+__device__ __forceinline__ double ens_hvp_rows(const EnsSys& S, const Space& sp,
+                                               const PairArgs& a, const float* par,
+                                               const float* __restrict__ dsig,
+                                               const EnsView<STAGED>& XV,
+                                               const int* __restrict__ idx, int width, int pad,
+                                               const EnsView<STAGED>& VV,
+                                               double* __restrict__ X, bool box,
+                                               double& lacc) {
 #pragma clang fp contract(off)
   double tacc = 0.0;
+  // rows whose stride is a multiple of 4 entries are read 16 bytes at a time
+  const bool vec = kEnsRowChunk == 4 && (width & 3) == 0 && ((uintptr_t)idx & 15) == 0;
   for (int q = threadIdx.x; q < S.n; q += blockDim.x) {
     const int i = S.offset + q;
     float xi[2], xim[2], vi[2];
-    load_pos<2>(x, i, xi);
-    load_pos<2>(v, i, vi);
+    XV.pos(i, xi);
+    VV.pos(i, vi);
     xim[0] = xi[0] - sp.L[0];
     xim[1] = xi[1] - sp.L[1];
-    const int si = a.species ? a.species[i] : 0;
+    const int si = XV.species(i);
     float h0 = 0.0f, h1 = 0.0f, tp = 0.0f, lp = 0.0f;
     const int* row = idx + (long long)i * width;
-    for (int k = 0; k < width; ++k) {
-      const int j = row[k];
-      if (j == pad) break;
-      float xj[2], vj[2];
-      load_pos<2>(x, j, xj);
-      load_pos<2>(v, j, vj);
-      float d[2], w[2];
-      d[0] = min_image_disp(xj[0], xi[0], xim[0], sp.L[0], sp.half[0]);
-      d[1] = min_image_disp(xj[1], xi[1], xim[1], sp.L[1], sp.half[1]);
-      w[0] = vj[0] - vi[0];
-      w[1] = vj[1] - vi[1];
-      const float r2 = fmaf(d[1], d[1], d[0] * d[0]);
-      const int sj = a.species ? a.species[j] : 0;
-      const float sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
-      const float eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
-      const float alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
-      const PairD2 D = pair_eval2<JMD_SOFT_SPHERE>(r2, sig, eps, alp, a);
-      const float dw = fmaf(d[1], w[1], d[0] * w[0]);
-      const float tt = 2.0f * D.gp * dw;
-      h0 = fmaf(D.g, w[0], fmaf(tt, d[0], h0));
-      h1 = fmaf(D.g, w[1], fmaf(tt, d[1], h1));
-      if (dsig) {
-        const float c = dsig[si * a.ns + sj];
-        const float lam = fmaf(vi[1], d[1], vi[0] * d[0]);
-        tp = fmaf(c * D.gs, lam, tp);
+    // the row in chunks: ids, then their gathers, then the terms in order
+    for (int k0 = 0; k0 < width; k0 += kEnsRowChunk) {
+      int js[kEnsRowChunk], sjs[kEnsRowChunk];
+      float xj[kEnsRowChunk][2], vj[kEnsRowChunk][2];
+      if (vec) {
+        const int4 q4 = *reinterpret_cast<const int4*>(row + k0);
+        js[0] = q4.x;
+        js[1 % kEnsRowChunk] = q4.y;
+        js[2 % kEnsRowChunk] = q4.z;
+        js[3 % kEnsRowChunk] = q4.w;
+      } else {
+#pragma unroll
+        for (int c = 0; c < kEnsRowChunk; ++c) js[c] = k0 + c < width ? row[k0 + c] : pad;
       }
-      if (box) {
-        const float n0 = image_of(xj[0] - xi[0], sp.half[0]);
-        const float n1 = image_of(xj[1] - xi[1], sp.half[1]);
-        const float dn = fmaf(d[1], n1, d[0] * n0);
-        const float vn = fmaf(vi[1], n1, vi[0] * n0);
-        const float vd = fmaf(vi[1], d[1], vi[0] * d[0]);
-        lp = fmaf(D.g, vn, fmaf(2.0f * D.gp * dn, vd, lp));
+#pragma unroll
+      for (int c = 0; c < kEnsRowChunk; ++c) {
+        xj[c][0] = xj[c][1] = vj[c][0] = vj[c][1] = 0.0f;
+        sjs[c] = 0;
+        if (js[c] != pad) {
+          XV.pos(js[c], xj[c]);
+          VV.pos(js[c], vj[c]);
+          sjs[c] = XV.species(js[c]);
+        }
       }
+      bool end = false;
+#pragma unroll
+      for (int c = 0; c < kEnsRowChunk; ++c) {
+        const int j = js[c];
+        if (j == pad) {
+          end = true;
+          break;
+        }
+        float d[2], w[2];
+        d[0] = min_image_disp(xj[c][0], xi[0], xim[0], sp.L[0], sp.half[0]);
+        d[1] = min_image_disp(xj[c][1], xi[1], xim[1], sp.L[1], sp.half[1]);
+        w[0] = vj[c][0] - vi[0];
+        w[1] = vj[c][1] - vi[1];
+        const float r2 = fmaf(d[1], d[1], d[0] * d[0]);
+        const int sj = sjs[c];
+        const float sig = ens_pick(a, 0, par, i, j, si, sj);
+        const float eps = ens_pick(a, 1, par, i, j, si, sj);
+        const float alp = ens_pick(a, 2, par, i, j, si, sj);
+        const PairD2 D = pair_eval2<JMD_SOFT_SPHERE>(r2, sig, eps, alp, a);
+        const float dw = fmaf(d[1], w[1], d[0] * w[0]);
+        const float tt = 2.0f * D.gp * dw;
+        h0 = fmaf(D.g, w[0], fmaf(tt, d[0], h0));
+        h1 = fmaf(D.g, w[1], fmaf(tt, d[1], h1));
+        if (dsig) {
+          const float cc = ens_dsig(a, dsig, par, si, sj);
+          const float lam = fmaf(vi[1], d[1], vi[0] * d[0]);
+          tp = fmaf(cc * D.gs, lam, tp);
+        }
+        if (box) {
+          const float n0 = image_of(xj[c][0] - xi[0], sp.half[0]);
+          const float n1 = image_of(xj[c][1] - xi[1], sp.half[1]);
+          const float dn = fmaf(d[1], n1, d[0] * n0);
+          const float vn = fmaf(vi[1], n1, vi[0] * n0);
+          const float vd = fmaf(vi[1], d[1], vi[0] * d[0]);
+          lp = fmaf(D.g, vn, fmaf(2.0f * D.gp * dn, vd, lp));
+        }
+      }
+      if (end) break;
     }
     X[2LL * i] += (double)h0;
     X[2LL * i + 1] += (double)h1;
@@ -594,24 +858,40 @@ __device__ double ens_hvp_rows(const EnsSys& S, const Space& sp, const PairArgs&
   return tacc;
 }
 
+// STAGED: systems of at most lds_n particles, with the positions of the
+// node and the vector of each Jacobian product copied to LDS; else the
+// larger ones, read from memory.  Each instance skips the other's systems.
+template <bool STAGED>
 __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
     const EnsSys* __restrict__ systems, int nsys, int ntot, int mode, int K,
     const float* __restrict__ x, const float* __restrict__ F, const int* __restrict__ idx,
     int width, PairArgs a0, const float* __restrict__ dsig0, EnsTape tape,
     double* __restrict__ X, double* __restrict__ P, float* __restrict__ W,
     float* __restrict__ V, double* __restrict__ theta, double* __restrict__ box_adj,
-    float scale) {
+    float scale, int lds_n) {
 #pragma clang fp contract(off)
+  // dynamic LDS: [2 lds_n] positions, [2 lds_n] product vector, [lds_n]
+  // uint8 species ids
+  extern __shared__ float4 s_dynb[];
   __shared__ double s_d[3 * (kEnsThreads / kWave)];
   const int sid = blockIdx.x;
   if (sid >= nsys) return;
   const EnsSys S = systems[sid];
+  if (STAGED != (S.n <= lds_n)) return;  // the other instance's system
   const int t = threadIdx.x, nt = blockDim.x;
   PairArgs a = a0;
   if (a.sig_mode == JMD_PARAM_SPECIES) a.sig_v += S.param_offset;
   if (a.eps_mode == JMD_PARAM_SPECIES) a.eps_v += S.param_offset;
   if (a.alp_mode == JMD_PARAM_SPECIES) a.alp_v += S.param_offset;
   const float* dsig = dsig0 ? dsig0 + S.param_offset : nullptr;
+  __shared__ float s_par[4 * kEnsParMax];
+  const float* par = ens_stage_params(a, dsig, s_par);
+  float* lx = STAGED ? reinterpret_cast<float*>(s_dynb) : nullptr;
+  float* lv = STAGED ? lx + 2 * lds_n : nullptr;
+  uint8_t* lsp = STAGED && a.species && a.ns <= 256
+                     ? reinterpret_cast<uint8_t*>(lv + 2 * lds_n) : nullptr;
+  if (lsp)
+    for (int q = t; q < S.n; q += nt) lsp[q] = (uint8_t)a.species[S.offset + q];
   Space sp;
   sp.dim = 2;
   sp.periodic = 1;
@@ -650,11 +930,11 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
           d[1] = min_image_disp(xj[1], xi[1], xim[1], sp.L[1], sp.half[1]);
           const float r2 = fmaf(d[1], d[1], d[0] * d[0]);
           const int sj = a.species ? a.species[j] : 0;
-          const float sig = pick(a.sig_mode, a.sig, a.sig_v, i, j, si, sj, a.ns);
-          const float eps = pick(a.eps_mode, a.eps, a.eps_v, i, j, si, sj, a.ns);
-          const float alp = pick(a.alp_mode, a.alp, a.alp_v, i, j, si, sj, a.ns);
+          const float sig = ens_pick(a, 0, par, i, j, si, sj);
+          const float eps = ens_pick(a, 1, par, i, j, si, sj);
+          const float alp = ens_pick(a, 2, par, i, j, si, sj);
           const PairD2 D = pair_eval2<JMD_SOFT_SPHERE>(r2, sig, eps, alp, a);
-          if (dsig) e += 0.5 * (double)(dsig[si * a.ns + sj] * D.us);
+          if (dsig) e += 0.5 * (double)(ens_dsig(a, dsig, par, si, sj) * D.us);
           if (box) {
             const float n0 = image_of(xj[0] - xi[0], sp.half[0]);
             const float n1 = image_of(xj[1] - xi[1], sp.half[1]);
@@ -672,7 +952,16 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
   }
   if (mode == BACK_FINISH) {  // pending W applied at node 0 of the tape
     double el = 0.0, z2 = 0.0;
-    double e = ens_hvp_rows(S, sp, a, dsig, tape.x, tape.idx, width, pad, W, X, box, el);
+    if constexpr (STAGED) {
+      for (int m = t; m < 2 * S.n; m += nt) {
+        lx[m] = tape.x[b2 + m];
+        lv[m] = W[b2 + m];
+      }
+    }
+    __syncthreads();
+    const EnsView<STAGED> XV{tape.x, lx, a.species, lsp, S.offset};
+    const EnsView<STAGED> VV{W, lv, nullptr, nullptr, S.offset};
+    double e = ens_hvp_rows(S, sp, a, par, dsig, XV, tape.idx, width, pad, VV, X, box, el);
     wg_sum3(e, el, z2, s_d);
     if (t == 0) {
       theta[sid] += e;
@@ -695,18 +984,27 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
     const double ratio = spp / sff;
     const double cq = spp > 0.0 ? (double)alpha * s1 / (spp * sff) : 0.0;
     const double cf = (double)alpha * s1 * spp / (sff * sff * sff);
+    const float* xn = tape.x + (long long)(k + 1) * n2;
     for (int m = t; m < 2 * S.n; m += nt) {
       const long long g = b2 + m;
       const double G = (double)keep * P[g];
       const double Q = (1.0 - (double)alpha) * G + cq * (double)qk[g];
       const double Fb = (double)alpha * ratio * G - cf * (double)Fn[g] + (double)h * Q;
       P[g] = Q;  // PH until step (3)
-      V[g] = (float)(Fb + (double)W[g]);
+      const float vg = (float)(Fb + (double)W[g]);
+      if constexpr (STAGED) {
+        lv[m] = vg;
+        lx[m] = xn[g];
+      } else {
+        V[g] = vg;
+      }
     }
     __syncthreads();
     // (2) product with the force Jacobian at node k + 1
-    tsum += ens_hvp_rows(S, sp, a, dsig, tape.x + (long long)(k + 1) * n2,
-                         tape.idx + (long long)(k + 1) * ntot * width, width, pad, V, X, box,
+    const EnsView<STAGED> XV{xn, lx, a.species, lsp, S.offset};
+    const EnsView<STAGED> VV{V, lv, nullptr, nullptr, S.offset};
+    tsum += ens_hvp_rows(S, sp, a, par, dsig, XV,
+                         tape.idx + (long long)(k + 1) * ntot * width, width, pad, VV, X, box,
                          lsum);
     __syncthreads();
     // (3) x_{k+1} = x_k + dt ph - L m_k, ph = p_k + h F_k
@@ -715,7 +1013,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
     for (int m = t; m < 2 * S.n; m += nt) {
       const long long g = b2 + m;
       if (box) {
-        const float fold = rintf((xk[g] - xk1[g]) / sp.L[m & 1]);
+        const float fold = rintf((xk[g] - xk1[g]) / ((m & 1) ? sp.L[1] : sp.L[0]));
         lsum -= X[g] * (double)fold;
       }
       const double PH = P[g] + (double)dt * X[g];
@@ -737,6 +1035,27 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
 using namespace jmd;
 
 extern "C" {
+
+// Allows k_fire_ensemble more than the default dynamic LDS (once per size).
+static bool ens_lds_ready(size_t bytes) {
+  static size_t done = 0;
+  if (bytes <= done) return true;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fire_ensemble<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+    return false;
+  done = bytes;
+  return true;
+}
+
+static bool ens_lds_ready_back(size_t bytes) {
+  static size_t done = 0;
+  if (bytes <= done) return true;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fire_ensemble_back<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+    return false;
+  done = bytes;
+  return true;
+}
 
 int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int32_t n_systems,
                          int64_t n_total, int32_t mode, int32_t n_steps, double* position_hi,
@@ -766,11 +1085,24 @@ int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int3
   if (taped && (mode != ENS_RUN || !tape_force || !tape_q || !tape_idx || !tape_scalars))
     return JMD_ERR_INVALID;
   EnsTape tape{tape_x, tape_force, tape_q, tape_idx, tape_scalars, n_steps};
-  hipLaunchKernelGGL(k_fire_ensemble, dim3(n_systems), dim3(ens_threads(n_systems, n_total)), 0,
+  // LDS copy of each system's positions, cell-sorted ids and species ids:
+  // systems of up to 8 particles per thread (8 Ki per 1024-thread workgroup,
+  // one per CU; 4 Ki per 512, two per CU); larger systems use global memory
+  const int threads = ens_threads(n_systems, n_total);
+  const int lds_n = threads * kEnsPerThread;
+  const size_t lds_bytes = (size_t)lds_n * (2 * sizeof(float) + sizeof(uint16_t) + 1);
+  if (!ens_lds_ready(lds_bytes)) return JMD_ERR_LAUNCH;
+  hipLaunchKernelGGL(k_fire_ensemble<true>, dim3(n_systems), dim3(threads), lds_bytes,
                      (hipStream_t)stream, reinterpret_cast<const EnsSys*>(systems), order,
                      n_systems, (int)n_total, mode, n_steps, position_hi, position, momentum,
                      force, reference_position, idx, width, scratch, a, prm, fire_state,
-                     fire_n_pos, error, out_stat, energy, builds, tape);
+                     fire_n_pos, error, out_stat, energy, builds, tape, lds_n);
+  JMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fire_ensemble<false>, dim3(n_systems), dim3(threads), 0,
+                     (hipStream_t)stream, reinterpret_cast<const EnsSys*>(systems), order,
+                     n_systems, (int)n_total, mode, n_steps, position_hi, position, momentum,
+                     force, reference_position, idx, width, scratch, a, prm, fire_state,
+                     fire_n_pos, error, out_stat, energy, builds, tape, lds_n);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }
@@ -808,11 +1140,21 @@ int jmd_fire_ensemble_adjoint_2d(const jmd_system2d* systems, int32_t n_systems,
   EnsTape tape{const_cast<float*>(tape_x), const_cast<float*>(tape_force),
                const_cast<float*>(tape_q), const_cast<int*>(tape_idx),
                const_cast<float*>(tape_scalars), n_steps};
-  hipLaunchKernelGGL(k_fire_ensemble_back, dim3(n_systems),
-                     dim3(ens_threads(n_systems, n_total)), 0,
+  // LDS copy of each system's node positions, product vector and species
+  // ids (systems of up to 8 particles per thread; larger ones use memory)
+  const int threads = ens_threads(n_systems, n_total);
+  const int lds_n = threads * kEnsPerThread;
+  const size_t lds_bytes = (size_t)lds_n * (4 * sizeof(float) + 1);
+  if (!ens_lds_ready_back(lds_bytes)) return JMD_ERR_LAUNCH;
+  hipLaunchKernelGGL(k_fire_ensemble_back<true>, dim3(n_systems), dim3(threads), lds_bytes,
                      (hipStream_t)stream, reinterpret_cast<const EnsSys*>(systems), n_systems,
                      (int)n_total, mode, n_steps, position, force, idx, width, a, dsigma, tape,
-                     x_adjoint, p_adjoint, pending, scratch, theta, box_adjoint, scale);
+                     x_adjoint, p_adjoint, pending, scratch, theta, box_adjoint, scale, lds_n);
+  JMD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_fire_ensemble_back<false>, dim3(n_systems), dim3(threads), 0,
+                     (hipStream_t)stream, reinterpret_cast<const EnsSys*>(systems), n_systems,
+                     (int)n_total, mode, n_steps, position, force, idx, width, a, dsigma, tape,
+                     x_adjoint, p_adjoint, pending, scratch, theta, box_adjoint, scale, lds_n);
   JMD_LAUNCH_CHECK();
   return JMD_OK;
 }

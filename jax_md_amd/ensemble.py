@@ -189,8 +189,8 @@ class SoftSphereFireEnsemble:
       self._recs[s].max_occupancy = max(m, 0)
       width = max(width, self._recs[s].max_occupancy)
     self._upload()
-    self.width = width
-    self.idx = torch.empty((self.n_total, width), dtype=torch.int32, device=self.device)
+    self.width = -(-width // 4) * 4  # row stride: whole 16-byte groups
+    self.idx = torch.empty((self.n_total, self.width), dtype=torch.int32, device=self.device)
     self.error.zero_()
     self._launch(_INIT)
     return self
