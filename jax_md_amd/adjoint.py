@@ -168,6 +168,20 @@ def _snapshot(state, nbrs):
               gate=nbrs.plan.gate.clone())
 
 
+_REPLAY_STREAMS = {}
+
+
+def _replay_stream(dev):
+  """The device's stream for segment replays (default priority, one per
+  device for the process)."""
+  key = torch.device(dev).index
+  if key is None:
+    key = torch.cuda.current_device()
+  if key not in _REPLAY_STREAMS:
+    _REPLAY_STREAMS[key] = torch.cuda.Stream(device=key)
+  return _REPLAY_STREAMS[key]
+
+
 def _restore(snap, state, nbrs):
   state.position_hi.copy_(snap['x_hi'])
   state.position.copy_(snap['x'])
@@ -271,17 +285,26 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   st = nat.stream_handle(dev)
   if n_steps == 0:
     PH.zero_()  # dE/dp_0 = 0 when x_T = x_0
-  if x64:
-    xs = [torch.empty((N, dim), dtype=torch.float64, device=dev) for _ in range(K)]
-  else:
-    xs = [simulate.new_positions(N, dim, dev) for _ in range(K)]
-  idxs = [torch.empty_like(nbrs.idx) for _ in range(K)]
+  # Segment replays and the adjoint steps run on two streams: the replay of
+  # the segment before (in time) the one being differentiated fills the
+  # other of two recording sets meanwhile (it depends only on its
+  # checkpoint).  Graph replays keep one stream.
+  overlap = state.position.is_cuda and not use_graph and len(ckpts) > 1
+  n_sets = 2 if overlap else 1
+
+  def new_set():
+    if x64:
+      xs_ = [torch.empty((N, dim), dtype=torch.float64, device=dev) for _ in range(K)]
+    else:
+      xs_ = [simulate.new_positions(N, dim, dev) for _ in range(K)]
+    return xs_, [torch.empty_like(nbrs.idx) for _ in range(K)]
+
+  sets = [new_set() for _ in range(n_sets)]
   # the backward loop's call arguments, converted once
   dense = nbrs.format is partition.Dense
-  x_ptrs = [nat.ptr(t) for t in xs]
-  idx_ptrs = [nat.ptr(t) for t in idxs]
+  set_ptrs = [([nat.ptr(t) for t in xs_], [nat.ptr(t) for t in idxs_]) for xs_, idxs_ in sets]
   dense_width = nbrs.idx.shape[1]
-  x_stride = 0 if x64 else nat.position_stride(xs[0])
+  x_stride = 0 if x64 else nat.position_stride(sets[0][0][0])
   periodic = int(geo.periodic)
   pot_ref = nat.ctypes.byref(pot)
   ph_a, ph_b = nat.ptr(PH), nat.ptr(PH2)
@@ -290,16 +313,41 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   c_step, c_first = float(dt32), float(f32(dt32 / f32(2)))
   dt_m = float(dt32) / m
   ckpt_bytes = sum(sum(t.numel() * t.element_size() for t in c[2].values()) for c in ckpts)
-  ckpt_bytes += sum(t.numel() * t.element_size() for t in xs + idxs)
+  ckpt_bytes += sum(t.numel() * t.element_size() for xs_, idxs_ in sets for t in xs_ + idxs_)
 
-  for start, k, snap in reversed(ckpts):
-    _restore(snap, state, nbrs)
+  back_stream = torch.cuda.current_stream(dev) if state.position.is_cuda else None
+  replay_stream = _replay_stream(dev) if overlap else None
+  if overlap:
+    replay_stream.wait_stream(back_stream)  # the state's last readers above
+  back_done = []
+
+  def replay(i, snap, k):
+    """Replays segment i (in backward order) into set i % n_sets."""
+    xs_, idxs_ = sets[i % n_sets]
 
     def hook(q, x):
-      xs[q].copy_(state.position_hi if x64 else x)
-      idxs[q].copy_(nbrs.idx)
+      xs_[q].copy_(state.position_hi if x64 else x)
+      idxs_[q].copy_(nbrs.idx)
 
-    segment(k, hook)
+    if not overlap:
+      _restore(snap, state, nbrs)
+      segment(k, hook)
+      return None
+    with torch.cuda.stream(replay_stream):
+      if i >= n_sets:  # the set's previous segment is differentiated
+        replay_stream.wait_event(back_done[i - n_sets])
+      _restore(snap, state, nbrs)
+      segment(k, hook)
+      ev = torch.cuda.Event()
+      ev.record(replay_stream)
+    return ev
+
+  for i, (start, k, snap) in enumerate(reversed(ckpts)):
+    ready = replay(i, snap, k)
+    if ready is not None:
+      back_stream.wait_event(ready)
+    x_ptrs, idx_ptrs = set_ptrs[i % n_sets]
+    idxs = sets[i % n_sets][1]
     for q in range(k - 1, -1, -1):
       step = start + q
       c = c_step if step >= 1 else c_first
@@ -320,6 +368,12 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
       if step >= 1:
         PH, PH2 = PH2, PH
         flip ^= 1
+    if overlap:
+      ev = torch.cuda.Event()
+      ev.record(back_stream)
+      back_done.append(ev)
+  if overlap:
+    back_stream.wait_stream(replay_stream)
   out = torch.empty((count,), dtype=torch.float64, device=dev)
   nat.check(lib.jmd_adjoint_param_gather(nat.ptr(acc), N, dim, nat.ctypes.byref(pot),
                                          nat.ptr(out), st), 'jmd_adjoint_param_gather')
