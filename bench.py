@@ -420,7 +420,9 @@ def ensemble_gradient_section(jmd, O, n_rep=64, n=8192, n_steps=300, every=25, p
   ens.run(n_steps)
   torch.cuda.synchronize()
   t_fwd = time.perf_counter() - t
-  make().energy_gradient(5, dsig, checkpoint_every=every, dbox=dLdr)  # warm
+  # warm: two segments, so both tapes and the replay stream exist before the
+  # timed call (allocations the caching allocator keeps)
+  make().energy_gradient(2 * every, dsig, checkpoint_every=every, dbox=dLdr)
   ens = make()
   torch.cuda.synchronize()
   t = time.perf_counter()

@@ -42,6 +42,20 @@ class _System2D(ctypes.Structure):
               ('param_offset', ctypes.c_int32), ('reserved', ctypes.c_int32)]
 
 
+_REPLAY_STREAMS = {}
+
+
+def _replay_stream(dev):
+  """The device's stream for taped segment replays (default priority, one
+  per device for the process)."""
+  key = torch.device(dev).index
+  if key is None:
+    key = torch.cuda.current_device()
+  if key not in _REPLAY_STREAMS:
+    _REPLAY_STREAMS[key] = torch.cuda.Stream(device=key)
+  return _REPLAY_STREAMS[key]
+
+
 class SoftSphereFireEnsemble:
   """FIRE minimisation of many periodic 2-D soft-sphere systems.
 
@@ -284,12 +298,22 @@ class SoftSphereFireEnsemble:
     W = torch.zeros(n2, dtype=torch.float32, device=dev)
     V = torch.zeros(n2, dtype=torch.float32, device=dev)
     theta = torch.zeros((S,), dtype=torch.float64, device=dev)
-    tape = dict(x=torch.empty((K + 1,) + n2, dtype=torch.float32, device=dev),
-                F=torch.empty((K + 1,) + n2, dtype=torch.float32, device=dev),
-                q=torch.empty((K,) + n2, dtype=torch.float32, device=dev),
-                idx=torch.empty((K + 1, self.n_total, self.width), dtype=torch.int32,
-                                device=dev),
-                sc=torch.empty((K, S, 5), dtype=torch.float32, device=dev))
+
+    def new_tape():
+      return dict(x=torch.empty((K + 1,) + n2, dtype=torch.float32, device=dev),
+                  F=torch.empty((K + 1,) + n2, dtype=torch.float32, device=dev),
+                  q=torch.empty((K,) + n2, dtype=torch.float32, device=dev),
+                  idx=torch.empty((K + 1, self.n_total, self.width), dtype=torch.int32,
+                                  device=dev),
+                  sc=torch.empty((K, S, 5), dtype=torch.float32, device=dev))
+
+    # Segment replays (the taped forward runs) go to a second stream, so the
+    # replay of the segment before runs beside the backward kernel of the
+    # segment after (each launch fills one CU per system); two tapes
+    # alternate, events order their reuse.
+    overlap = len(snaps) > 1
+    tapes = [new_tape() for _ in range(2 if overlap else 1)]
+    tape = tapes[0]
 
     def back(mode, k):
       nat.check(lib.jmd_fire_ensemble_adjoint_2d(
@@ -301,10 +325,26 @@ class SoftSphereFireEnsemble:
           'jmd_fire_ensemble_adjoint_2d')
 
     back(0, 0)
-    for k, snap in reversed(snaps):
-      self._restore(snap)
-      self._launch(_RUN, k, tape)
+    back_stream = torch.cuda.current_stream(dev)
+    replay_stream = _replay_stream(dev) if overlap else back_stream
+    if overlap:
+      replay_stream.wait_stream(back_stream)  # the state's last readers above
+    back_done = []
+    for i, (k, snap) in enumerate(reversed(snaps)):
+      tape = tapes[i % len(tapes)]
+      with torch.cuda.stream(replay_stream):
+        if overlap and i >= len(tapes):  # this tape's previous segment is done
+          replay_stream.wait_event(back_done[i - len(tapes)])
+        self._restore(snap)
+        self._launch(_RUN, k, tape)
+        ready = torch.cuda.Event()
+        ready.record(replay_stream)
+      back_stream.wait_event(ready)
       back(1, k)
+      done_ev = torch.cuda.Event()
+      done_ev.record(back_stream)
+      back_done.append(done_ev)
+    back_stream.wait_stream(replay_stream)  # the state, restored below
     if snaps:
       back(2, 0)
     self._restore(final)
