@@ -30,6 +30,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 """
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -40,6 +41,18 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'atom-steps/s LJ NVE N=262k, plain vs batched/adjoint; error vs FP64 oracle'
 HBM_PEAK_GBS = 8000.0
+
+
+def without_collector(fn):
+  """fn() with Python's cyclic garbage collector paused (after one pass):
+  a collection over the garbage of earlier sections landing inside a
+  host-driven timed loop would dominate it (timeit does the same)."""
+  gc.collect()
+  gc.disable()
+  try:
+    return fn()
+  finally:
+    gc.enable()
 
 
 def log(*a):
@@ -672,13 +685,15 @@ def general_section(jmd, O, n=64, n_cells=40, n_steps=1000, warmup=200, block=10
   for _ in range(3):
     mnb = mnb.update(um)
   torch.cuda.synchronize()
+  gc.collect()  # the section's own garbage (the collector is paused)
   t = time.perf_counter()
   for _ in range(20):
     mnb = mnb.update(um)
   torch.cuda.synchronize()
+  elapsed = time.perf_counter() - t
   out['multi_image'] = {'atoms': 500, 'edges': mnb.n_edges,
                         'shift_stencil': int(mnb.search_shifts.shape[0]),
-                        'rebuild_us': 1e6 * (time.perf_counter() - t) / 20,
+                        'rebuild_us': 1e6 * elapsed / 20,
                         'workload': '500 atoms, triclinic cell of widths ~5.4-6.6, cutoff 3.5, '
                                     'Sparse, dr_threshold 0 (a rebuild per update, host '
                                     'overhead included)'}
@@ -742,7 +757,7 @@ def main():
           'ensemble_gradient': lambda: ensemble_gradient_section(jmd, O),
           'published': lambda: published_section(jmd, O),
           'general': lambda: general_section(jmd, O)}[args.section]
-    print(json.dumps({args.section: fn()}), flush=True)
+    print(json.dumps({args.section: without_collector(fn)}), flush=True)
     return
   R, P, L = O.lj_lattice(args.n_side, seed=0)
   N = R.shape[0]
@@ -832,10 +847,14 @@ def main():
   builds0 = builds_before + nbrs.plan.builds
   stats0 = dict(loop.stats)
   torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  run_steps(args.steps)
-  torch.cuda.synchronize()
-  elapsed = time.perf_counter() - t0
+
+  def timed_run():
+    t0 = time.perf_counter()
+    run_steps(args.steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+  elapsed = without_collector(timed_run)
   timed_blocks = {k: loop.stats[k] - stats0[k] for k in loop.stats}
   builds = builds_before + nbrs.plan.builds - builds0
   E_end, F_gpu = efn.energy_and_force(state.position, nbrs)
@@ -940,7 +959,7 @@ def main():
                     ('published', lambda: published_section(jmd, O)),
                     ('general', lambda: general_section(jmd, O))):
       try:
-        out[key] = fn()
+        out[key] = without_collector(fn)
       except Exception as e:  # reported, never silently dropped
         out[key] = f'failed: {type(e).__name__}: {e}'
         log(f'{key} section failed: {e}')

@@ -495,16 +495,30 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
       s_cc[tid] = ccell;
       s_rot[tid] = rot;
     }
-    if (tid < kWave) {
-      // candidate offsets of the stencil cells: an exclusive scan of the
-      // counts over the first 32 lanes (lane NST, count 0, gets the total)
-      int incl = ccell;
+    if constexpr (GEOM == 0) {
+      if (tid < kWave) {
+        // candidate offsets of the stencil cells: an exclusive scan of the
+        // counts over the first 32 lanes (lane NST, count 0, gets the total)
+        int incl = ccell;
 #pragma unroll
-      for (int o = 1; o < 32; o <<= 1) {
-        const int u = __shfl_up(incl, o, kWave);
-        if (lane >= o) incl += u;
+        for (int o = 1; o < 32; o <<= 1) {
+          const int u = __shfl_up(incl, o, kWave);
+          if (lane >= o) incl += u;
+        }
+        if (tid <= NST) s_off[tid] = incl - ccell;
       }
-      if (tid <= NST) s_off[tid] = incl - ccell;
+    } else {
+      // (the folding and general-box builds have fewer registers to spare:
+      // one lane sums the counts)
+      __syncthreads();
+      if (tid == 0) {
+        int acc = 0;
+        for (int t = 0; t < NST; ++t) {
+          s_off[t] = acc;
+          acc += s_cc[t];
+        }
+        s_off[NST] = acc;
+      }
     }
     __syncthreads();
     const int total = s_off[NST];
@@ -545,71 +559,82 @@ __device__ int dev_nl_cells(const BuildArgs& A, int home0, int home_stride) {
         // multiple of 64 with far-away entries.  Every lane's loads (kStageQ
         // per lane at most) are issued before any is used, then one
         // compaction over (load round, wavefront, lane) keeps the order.
-        float4 v[kStageQ];
-        int vt[kStageQ];
-#pragma unroll
-        for (int r = 0; r < kStageQ; ++r) {
-          const int q = r * kNlBlock + tid;
-          v[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-          vt[r] = 0;
-          if (q < ccount) {
-            const int p = c0 + q;
-            // the stencil cell of candidate p: the last t with s_off[t] <= p
-            // (empty cells have s_off[t] == s_off[t + 1] and are passed over)
-            int t = 0;
-#pragma unroll
-            for (int step = 16; step > 0; step >>= 1)
-              if (t + step < NST && s_off[t + step] <= p) t += step;
-            const int m = p - s_off[t];
-            int k = m;
-            const int w = s_rot[t];
-            if (w > 0) k = (m < s_cc[t] - w) ? (w + m) : (m - (s_cc[t] - w));
-            v[r] = A.w.sorted_pos[s_cs[t] + k];  // w: the id bits
-            vt[r] = t;
-          }
-        }
-        unsigned keepm = 0;
-#pragma unroll
-        for (int r = 0; r < kStageQ; ++r) {
-          const int q = r * kNlBlock + tid;
-          if (q < ccount) {
-            const int t = vt[r];
-            v[r].x += s_shift[t][0];
-            v[r].y += s_shift[t][1];
-            v[r].z += s_shift[t][2];
-            bool keep = true;
-            if (prune) {
-              const float cv[3] = {v[r].x, v[r].y, v[r].z};
-              float D2 = 0.0f;
-#pragma unroll
-              for (int d = 0; d < DIM; ++d) {
-                const float e = fmaxf(0.0f, fmaxf(blo[d] - cv[d], cv[d] - bhi[d]));
-                D2 = fmaf(e, e, D2);
-              }
-              keep = (BOXS ? D2 * s.smin2 : D2) <= reach_sq;
-            }
-            if (keep) keepm |= 1u << r;
-          }
-        }
-#pragma unroll
-        for (int r = 0; r < kStageQ; ++r) {
-          const unsigned long long bal = __ballot((keepm >> r) & 1u);
-          if (lane == 0) s_wc[r * kNlWaves + wave] = (int)__popcll(bal);
-        }
-        __syncthreads();
+        // Stage candidates c0..c0+ccount in reference order, keeping those
+        // that can reach the home cell (compacted in order), padded to a
+        // multiple of 64 with far-away entries.  A lane's loads of G rounds
+        // (G = all of them for the staged-image build, one for the folding
+        // and general-box builds, whose registers are tighter) are issued
+        // before any is used, then one compaction over (round, wavefront,
+        // lane) keeps the order.
+        constexpr int G = GEOM == 0 ? kStageQ : 1;
         int kept = 0;  // workgroup-uniform
+        for (int r0 = 0; r0 * kNlBlock < ccount; r0 += G) {
+          float4 v[G];
+          int vt[G];
 #pragma unroll
-        for (int r = 0; r < kStageQ; ++r) {
-          const unsigned long long bal = __ballot((keepm >> r) & 1u);
-          int before = 0, all = 0;
+          for (int u = 0; u < G; ++u) {
+            const int q = (r0 + u) * kNlBlock + tid;
+            v[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            vt[u] = 0;
+            if (q < ccount) {
+              const int p = c0 + q;
+              // the stencil cell of candidate p: the last t with s_off[t] <= p
+              // (empty cells have s_off[t] == s_off[t + 1] and are passed over)
+              int t = 0;
 #pragma unroll
-          for (int w = 0; w < kNlWaves; ++w) {
-            const int c = s_wc[r * kNlWaves + w];
-            before += w < wave ? c : 0;
-            all += c;
+              for (int step = 16; step > 0; step >>= 1)
+                if (t + step < NST && s_off[t + step] <= p) t += step;
+              const int m = p - s_off[t];
+              int k = m;
+              const int w = s_rot[t];
+              if (w > 0) k = (m < s_cc[t] - w) ? (w + m) : (m - (s_cc[t] - w));
+              v[u] = A.w.sorted_pos[s_cs[t] + k];  // w: the id bits
+              vt[u] = t;
+            }
           }
-          if ((keepm >> r) & 1u) s_pos[kept + before + wave_prefix_count(bal)] = v[r];
-          kept += all;
+          unsigned keepm = 0;
+#pragma unroll
+          for (int u = 0; u < G; ++u) {
+            const int q = (r0 + u) * kNlBlock + tid;
+            if (q < ccount) {
+              const int t = vt[u];
+              v[u].x += s_shift[t][0];
+              v[u].y += s_shift[t][1];
+              v[u].z += s_shift[t][2];
+              bool keep = true;
+              if (prune) {
+                const float cv[3] = {v[u].x, v[u].y, v[u].z};
+                float D2 = 0.0f;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                  const float e = fmaxf(0.0f, fmaxf(blo[d] - cv[d], cv[d] - bhi[d]));
+                  D2 = fmaf(e, e, D2);
+                }
+                keep = (BOXS ? D2 * s.smin2 : D2) <= reach_sq;
+              }
+              if (keep) keepm |= 1u << u;
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < G; ++u) {
+            const unsigned long long bal = __ballot((keepm >> u) & 1u);
+            if (lane == 0) s_wc[u * kNlWaves + wave] = (int)__popcll(bal);
+          }
+          __syncthreads();
+#pragma unroll
+          for (int u = 0; u < G; ++u) {
+            const unsigned long long bal = __ballot((keepm >> u) & 1u);
+            int before = 0, all = 0;
+#pragma unroll
+            for (int w = 0; w < kNlWaves; ++w) {
+              const int c = s_wc[u * kNlWaves + w];
+              before += w < wave ? c : 0;
+              all += c;
+            }
+            if ((keepm >> u) & 1u) s_pos[kept + before + wave_prefix_count(bal)] = v[u];
+            kept += all;
+          }
+          if (r0 + G < kStageQ) __syncthreads();  // s_wc is rewritten by the next group
         }
         const int cpad = (kept + kWave - 1) & ~(kWave - 1);
         for (int q = kept + tid; q < cpad; q += blockDim.x)
