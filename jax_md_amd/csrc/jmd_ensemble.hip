@@ -297,18 +297,36 @@ __device__ __forceinline__ int ens_rows(const EnsSys& S, const Grid& g, const Sp
                         int* __restrict__ idx, int width, int pad, int* s_red,
                         const uint16_t* ls) {
   const int nt = blockDim.x, t = threadIdx.x;
+  // The candidate test: the minimum-image |x_j - x_i|^2 of positions in the
+  // box decides every candidate farther than `margin` from the cutoff (the
+  // bound of the main cell build on the few float32 roundings by which it
+  // can differ from the reference's wrapped float32 d^2); the rest, and
+  // positions outside [0, L], take the reference-exact two-way test.
+  const float rc = sqrtf(cutoff_sq), Lmax = fmaxf(sp.L[0], sp.L[1]);
+  const float margin =
+      fmaxf(128.0f * (rc + 1.0f) * (Lmax + rc + 1.0f) * 1.1920929e-7f, 1e-4f * cutoff_sq);
+  const float lo_thr = cutoff_sq - margin, hi_thr = cutoff_sq + margin;
+  // rows whose stride is a multiple of 4 entries are written 16 bytes at a
+  // time: a row's entries collect in a register quad, flushed when full
+  const bool vec = width > 0 && (width & 3) == 0 && ((uintptr_t)idx & 15) == 0;
   int occ = 0;
   for (int a = t; a < S.n; a += nt) {
     const int i = S.offset + a;
     float xi[2];
     V.pos(i, xi);
+    const float xim[2] = {xi[0] - sp.L[0], xi[1] - sp.L[1]};
+    const bool in_i = xi[0] >= 0.0f && xi[0] <= sp.L[0] && xi[1] >= 0.0f && xi[1] <= sp.L[1];
     int ci[2];
     const int h = STAGED ? cell_of<2>(xi, g, ci) : cellp[a];  // the binning's cell
     const int hx = h % g.cps[0], hy = h / g.cps[0];
+    // the stencil's columns and rows, wrapped by comparison (offsets of one)
+    const int cx[3] = {hx == 0 ? g.cps[0] - 1 : hx - 1, hx, hx + 1 == g.cps[0] ? 0 : hx + 1};
+    const int cy[3] = {hy == 0 ? g.cps[1] - 1 : hy - 1, hy, hy + 1 == g.cps[1] ? 0 : hy + 1};
     int cnt = 0;
+    int4 quad = make_int4(pad, pad, pad, pad);
+    int* row = idx + (long long)i * width;
     for (int st = 0; st < 9; ++st) {  // iota order, x offset slowest
-      const int ox = st / 3 - 1, oy = st % 3 - 1;
-      const int c = floor_mod_int(hx + ox, g.cps[0]) + g.cps[0] * floor_mod_int(hy + oy, g.cps[1]);
+      const int c = cx[st / 3] + g.cps[0] * cy[st % 3];
       const int start = s_cs[c];
       const int m = min(s_cs[c + 1] - start, g.cap);
       const int w = g.cap - (start % g.cap);
@@ -333,17 +351,49 @@ __device__ __forceinline__ int ens_rows(const EnsSys& S, const Grid& g, const Sp
         for (int c2 = 0; c2 < kEnsRowChunk; ++c2) {
           if (jl[c2] < 0 || jl[c2] == a) continue;  // past the cell; mask_self
           const int j = S.offset + jl[c2];
+          const float e0 = min_image_disp(xj[c2][0], xi[0], xim[0], sp.L[0], sp.half[0]);
+          const float e1 = min_image_disp(xj[c2][1], xi[1], xim[1], sp.L[1], sp.half[1]);
+          const float d2 = fmaf(e1, e1, e0 * e0);
+          const bool in_j = xj[c2][0] >= 0.0f && xj[c2][0] <= sp.L[0] && xj[c2][1] >= 0.0f &&
+                            xj[c2][1] <= sp.L[1];
+          bool keep = in_i && in_j && d2 < lo_thr;
           // Dense with cells keeps a candidate when both wrap(Ri - Rj) and
           // wrap(Rj - Ri) are inside (partition.py:951,971)
-          if (ref_metric_sq<2>(xi, xj[c2], sp) < cutoff_sq &&
-              ref_metric_sq<2>(xj[c2], xi, sp) < cutoff_sq) {
-            if (cnt < width) idx[(long long)i * width + cnt] = j;
+          if (!keep && (!(in_i && in_j) || d2 <= hi_thr))
+            keep = ref_metric_sq<2>(xi, xj[c2], sp) < cutoff_sq &&
+                   ref_metric_sq<2>(xj[c2], xi, sp) < cutoff_sq;
+          if (keep) {
+            if (cnt < width) {
+              if (vec) {
+                const int slot = cnt & 3;
+                quad.x = slot == 0 ? j : quad.x;
+                quad.y = slot == 1 ? j : quad.y;
+                quad.z = slot == 2 ? j : quad.z;
+                quad.w = slot == 3 ? j : quad.w;
+                if (slot == 3) {
+                  *reinterpret_cast<int4*>(row + (cnt & ~3)) = quad;
+                  quad = make_int4(pad, pad, pad, pad);
+                }
+              } else {
+                row[cnt] = j;
+              }
+            }
             ++cnt;
           }
         }
       }
     }
-    for (int k = cnt; k < width; ++k) idx[(long long)i * width + k] = pad;
+    if (vec) {
+      // the partial quad (pads after the last entry), then whole pad quads
+      int k = cnt < width ? (cnt & ~3) : width;
+      if (k < width && (cnt & 3) != 0) {
+        *reinterpret_cast<int4*>(row + k) = quad;
+        k += 4;
+      }
+      for (; k < width; k += 4) *reinterpret_cast<int4*>(row + k) = make_int4(pad, pad, pad, pad);
+    } else {
+      for (int k = cnt; k < width; ++k) row[k] = pad;
+    }
     occ = max(occ, cnt);
   }
   return wg_max(occ, s_red);
@@ -1036,25 +1086,28 @@ using namespace jmd;
 
 extern "C" {
 
-// Allows k_fire_ensemble more than the default dynamic LDS (once per size).
-static bool ens_lds_ready(size_t bytes) {
-  static size_t done = 0;
-  if (bytes <= done) return true;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fire_ensemble<true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+// Allows a staged kernel more than the default dynamic LDS, once per device
+// and size (the largest size granted so far is kept per device).
+static bool ens_lds_allow(const void* kernel, size_t* granted, size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
+  if (bytes <= granted[dev]) return true;
+  if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+      hipSuccess)
     return false;
-  done = bytes;
+  granted[dev] = bytes;
   return true;
 }
 
+static bool ens_lds_ready(size_t bytes) {
+  static size_t granted[64] = {};
+  return ens_lds_allow(reinterpret_cast<const void*>(&k_fire_ensemble<true>), granted, bytes);
+}
+
 static bool ens_lds_ready_back(size_t bytes) {
-  static size_t done = 0;
-  if (bytes <= done) return true;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fire_ensemble_back<true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
-    return false;
-  done = bytes;
-  return true;
+  static size_t granted[64] = {};
+  return ens_lds_allow(reinterpret_cast<const void*>(&k_fire_ensemble_back<true>), granted,
+                       bytes);
 }
 
 int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int32_t n_systems,

@@ -242,7 +242,11 @@ def test_ensemble_large_systems_thread_counts_agree():
   and 1024 threads; 40 FIRE steps give the same float16 dt / alpha
   sequence, energies within 1e-7 and positions within 1e-5; the gradient
   with the box following the ratio agrees to 1e-6.  The threads knob only
-  changes the order of the workgroup sums."""
+  changes the order of the workgroup sums.  At 512 threads both systems
+  exceed the LDS staging limit (8 particles per thread) and run the
+  global-memory kernel instances; at 1024 they are staged in LDS, so this
+  also compares the two instances (and runs the gradient's overlapped
+  segment replays, three segments)."""
   jmd = _jmd()
   from jax_md_amd import _native as nat
   lib = nat.load()
