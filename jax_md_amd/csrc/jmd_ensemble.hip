@@ -659,15 +659,28 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
     for (long long q = t; q < rows; q += nt) dst[q] = src[q];
   };
   if (tape.x) record_node(0);
+  // The half-step pass also makes the next step's skin test (on the
+  // positions it writes, against the reference positions, which only a
+  // rebuild changes, before the half-step) and applies the previous step's
+  // FIRE mixing to each momentum it reads (the mixing needs the step's
+  // reductions, so it waits for the next pass over the momenta); the
+  // per-element arithmetic is that of the separate passes.
+  // the pending mixing's ratio, keep and alpha (workgroup-uniform: in LDS,
+  // out of the registers the force loop needs)
+  __shared__ double s_mix[3];
+  bool moved_next = false;
+  bool mix_pending = false;
   for (int step = 0; step < n_steps; ++step) {
     // neighbour update: rebuild when any particle moved beyond half the skin
-    bool moved = false;
+    bool moved = moved_next;
+    if (step == 0) {
 #pragma unroll 4
-    for (int q = t; q < S.n; q += nt) {
-      float xi[2], ri[2];
-      V.pos(S.offset + q, xi);
-      load_pos<2>(x_ref, S.offset + q, ri);
-      moved = moved || ref_metric_sq<2>(xi, ri, sp) > thr_sq;
+      for (int q = t; q < S.n; q += nt) {
+        float xi[2], ri[2];
+        V.pos(S.offset + q, xi);
+        load_pos<2>(x_ref, S.offset + q, ri);
+        moved = moved || ref_metric_sq<2>(xi, ri, sp) > thr_sq;
+      }
     }
     if (__syncthreads_or(moved)) rebuild(true);
     __syncthreads();
@@ -677,19 +690,38 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
       const int* src = idx + (long long)S.offset * width;
       for (long long q = t; q < rows; q += nt) dst[q] = src[q];
     }
-    // velocity-Verlet sub-step at the FIRE time step (float16 value)
+    // velocity-Verlet sub-step at the FIRE time step (float16 value), one
+    // particle per iteration
     const float dt = s_scal[0];
     const float half = dt / 2.0f;
-#pragma unroll 4
-    for (int k = t; k < 2 * S.n; k += nt) {
-      const long long q = 2LL * S.offset + k;
-      const float pk = p[q] + half * F[q];
-      double xv = x_hi[q] + (double)dt * (double)pk;
-      xv = ref_mod64(xv, (k & 1) ? L1 : L0);
-      p[q] = pk;
-      x_hi[q] = xv;
-      x[q] = (float)xv;
-      if constexpr (STAGED) lx[k] = (float)xv;
+    const double mix_ratio = s_mix[0], mix_keep = s_mix[1];
+    const float mix_alpha = (float)s_mix[2];
+    moved_next = false;
+    for (int q = t; q < S.n; q += nt) {
+      const long long g0 = 2LL * (S.offset + q);
+      float xn[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const long long gq = g0 + c;
+        float p0 = p[gq];
+        const float fq = F[gq];
+        if (mix_pending) {  // the previous step's FIRE mixing
+          double pv = p0;
+          pv = pv + (double)mix_alpha * ((double)fq * mix_ratio - pv);
+          p0 = (float)(pv * mix_keep);
+        }
+        const float pk = p0 + half * fq;
+        double xv = x_hi[gq] + (double)dt * (double)pk;
+        xv = ref_mod64(xv, c ? L1 : L0);
+        p[gq] = pk;
+        x_hi[gq] = xv;
+        xn[c] = (float)xv;
+        x[gq] = xn[c];
+        if constexpr (STAGED) lx[2 * q + c] = xn[c];
+      }
+      float ri[2];
+      load_pos<2>(x_ref, S.offset + q, ri);
+      moved_next = moved_next || ref_metric_sq<2>(xn, ri, sp) > thr_sq;
     }
     __syncthreads();
     for (int q = t; q < S.n; q += nt) {
@@ -749,13 +781,22 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
       if (fp < 0.0) al = prm.alpha_start;
       s_scal[0] = dtn;
       s_scal[1] = al;
+      s_mix[0] = ratio;
+      s_mix[1] = keep;
+      s_mix[2] = (double)alpha_mix;
     }
+    mix_pending = true;  // applied by the next half-step, or below
+    __syncthreads();
+  }
+  if (mix_pending) {  // the last step's FIRE mixing
+    const double mix_ratio = s_mix[0], mix_keep = s_mix[1];
+    const float mix_alpha = (float)s_mix[2];
 #pragma unroll 4
     for (int k = t; k < 2 * S.n; k += nt) {
       const long long q = 2LL * S.offset + k;
       double pv = p[q];
-      pv = pv + (double)alpha_mix * ((double)F[q] * ratio - pv);
-      p[q] = (float)(pv * keep);
+      pv = pv + (double)mix_alpha * ((double)F[q] * mix_ratio - pv);
+      p[q] = (float)(pv * mix_keep);
     }
     __syncthreads();
   }
