@@ -465,9 +465,11 @@ int jmd_fire_ensemble_2d(const jmd_system2d* systems, const int32_t* order, int3
  * sigma).  FIRE's float16 dt / alpha / reset decisions are piecewise
  * constant and carry no derivative.  Mode 3 of jmd_fire_ensemble_2d with
  * tape_* set records a segment (per node k = 0..n_steps: float32 positions,
- * forces [n+1][n_total][2], the lists [n+1][n_total][width]; per step:
- * momenta before the mixing [n][n_total][2] and five scalars per system
- * [n][n_systems][5]).  mode: 0 initialises x_adjoint = -scale F,
+ * forces [n+1][n_total][2]; list slots [n+1][n_total][width], slot 0 the
+ * first node's list and slot k + 1 a system's list when it rebuilt at step
+ * k; per step: momenta before the mixing [n][n_total][2] and six scalars
+ * per system [n][n_systems][6], the sixth the slot of the list of the
+ * step's second node).  mode: 0 initialises x_adjoint = -scale F,
  * p_adjoint = pending = 0, theta = scale dE/dtheta at (position, idx);
  * 1 runs n_steps backward steps through a tape; 2 applies the pending
  * vector at the tape's first node (call once, after the earliest segment).

@@ -57,10 +57,12 @@ struct EnsSys {
 //   x[k], F[k]: float32 positions and forces at node k ([K+1][ntot][2])
 //   q[k]: momentum of step k after the second half kick, before the FIRE
 //         mixing ([K][ntot][2])
-//   idx[k]: the Dense list the force at node k was computed with
-//         ([K+1][ntot][width])
+//   idx[k]: a Dense list slot ([K+1][ntot][width]); slot 0 holds node 0's
+//         list, slot k + 1 the list a system rebuilt at step k (a system's
+//         other slots are not written)
 //   sc[k][s]: step k's scalars of system s: dt, alpha, keep, sqrt(pp),
-//         sqrt(ff + 1e-6)  ([K][nsys][kTapeScalars])
+//         sqrt(ff + 1e-6), and the slot of the list the force at node k + 1
+//         was computed with  ([K][nsys][kTapeScalars])
 struct EnsTape {
   float* x;
   float* F;
@@ -69,7 +71,7 @@ struct EnsTape {
   float* sc;
   int K;
 };
-constexpr int kTapeScalars = 5;
+constexpr int kTapeScalars = 6;
 
 struct FireP {
   float dt_start, dt_max;
@@ -668,6 +670,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
   // the pending mixing's ratio, keep and alpha (workgroup-uniform: in LDS,
   // out of the registers the force loop needs)
   __shared__ double s_mix[3];
+  int list_slot = 0;  // tape slot of the current list (record_node(0) fills 0)
   bool moved_next = false;
   bool mix_pending = false;
   for (int step = 0; step < n_steps; ++step) {
@@ -682,9 +685,11 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
         moved = moved || ref_metric_sq<2>(xi, ri, sp) > thr_sq;
       }
     }
-    if (__syncthreads_or(moved)) rebuild(true);
+    const bool rebuilt = __syncthreads_or(moved);
+    if (rebuilt) rebuild(true);
     __syncthreads();
-    if (tape.x) {  // the list the force at node step + 1 will use
+    if (tape.x && rebuilt) {  // the list the force at node step + 1 will use
+      list_slot = step + 1;
       const long long rows = (long long)S.n * width;
       int* dst = tape.idx + (long long)(step + 1) * ntot * width + (long long)S.offset * width;
       const int* src = idx + (long long)S.offset * width;
@@ -770,6 +775,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble(
       sc[2] = (float)keep;
       sc[3] = (float)sqrt(pp);
       sc[4] = (float)sqrt(ff + 1e-6);
+      sc[5] = (float)list_slot;
     }
     n_pos = fp >= 0.0 ? n_pos + 1 : 0;
     __syncthreads();
@@ -1095,7 +1101,7 @@ __global__ __launch_bounds__(kEnsThreads) void k_fire_ensemble_back(
     const EnsView<STAGED> XV{xn, lx, a.species, lsp, S.offset};
     const EnsView<STAGED> VV{V, lv, nullptr, nullptr, S.offset};
     tsum += ens_hvp_rows(S, sp, a, par, dsig, XV,
-                         tape.idx + (long long)(k + 1) * ntot * width, width, pad, VV, X, box,
+                         tape.idx + (long long)sc[5] * ntot * width, width, pad, VV, X, box,
                          lsum);
     __syncthreads();
     // (3) x_{k+1} = x_k + dt ph - L m_k, ph = p_k + h F_k

@@ -305,7 +305,7 @@ class SoftSphereFireEnsemble:
                   q=torch.empty((K,) + n2, dtype=torch.float32, device=dev),
                   idx=torch.empty((K + 1, self.n_total, self.width), dtype=torch.int32,
                                   device=dev),
-                  sc=torch.empty((K, S, 5), dtype=torch.float32, device=dev))
+                  sc=torch.empty((K, S, 6), dtype=torch.float32, device=dev))
 
     # Segment replays (the taped forward runs) go to a second stream, so the
     # replay of the segment before runs beside the backward kernel of the
