@@ -212,6 +212,21 @@ def stream_handle(device=None):
   return c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+_REPLAY_STREAMS = {}
+
+
+def replay_stream(device):
+  """The device's second stream (default priority, one per device for the
+  process), on which the gradients replay checkpointed segments beside the
+  backward work of the current stream."""
+  key = torch.device(device).index
+  if key is None:
+    key = torch.cuda.current_device()
+  if key not in _REPLAY_STREAMS:
+    _REPLAY_STREAMS[key] = torch.cuda.Stream(device=key)
+  return _REPLAY_STREAMS[key]
+
+
 def f3(vals):
   """float32 array of at least 3 entries (a box matrix passes 4 or 9)."""
   vals = list(vals)

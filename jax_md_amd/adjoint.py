@@ -168,20 +168,6 @@ def _snapshot(state, nbrs):
               gate=nbrs.plan.gate.clone())
 
 
-_REPLAY_STREAMS = {}
-
-
-def _replay_stream(dev):
-  """The device's stream for segment replays (default priority, one per
-  device for the process)."""
-  key = torch.device(dev).index
-  if key is None:
-    key = torch.cuda.current_device()
-  if key not in _REPLAY_STREAMS:
-    _REPLAY_STREAMS[key] = torch.cuda.Stream(device=key)
-  return _REPLAY_STREAMS[key]
-
-
 def _restore(snap, state, nbrs):
   state.position_hi.copy_(snap['x_hi'])
   state.position.copy_(snap['x'])
@@ -316,7 +302,7 @@ def nve_energy_gradient(energy_fn, shift_fn, neighbor_fn, R, P, dt, n_steps: int
   ckpt_bytes += sum(t.numel() * t.element_size() for xs_, idxs_ in sets for t in xs_ + idxs_)
 
   back_stream = torch.cuda.current_stream(dev) if state.position.is_cuda else None
-  replay_stream = _replay_stream(dev) if overlap else None
+  replay_stream = nat.replay_stream(dev) if overlap else None
   if overlap:
     replay_stream.wait_stream(back_stream)  # the state's last readers above
   back_done = []

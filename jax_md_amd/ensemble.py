@@ -42,20 +42,6 @@ class _System2D(ctypes.Structure):
               ('param_offset', ctypes.c_int32), ('reserved', ctypes.c_int32)]
 
 
-_REPLAY_STREAMS = {}
-
-
-def _replay_stream(dev):
-  """The device's stream for taped segment replays (default priority, one
-  per device for the process)."""
-  key = torch.device(dev).index
-  if key is None:
-    key = torch.cuda.current_device()
-  if key not in _REPLAY_STREAMS:
-    _REPLAY_STREAMS[key] = torch.cuda.Stream(device=key)
-  return _REPLAY_STREAMS[key]
-
-
 class SoftSphereFireEnsemble:
   """FIRE minimisation of many periodic 2-D soft-sphere systems.
 
@@ -326,7 +312,7 @@ class SoftSphereFireEnsemble:
 
     back(0, 0)
     back_stream = torch.cuda.current_stream(dev)
-    replay_stream = _replay_stream(dev) if overlap else back_stream
+    replay_stream = nat.replay_stream(dev) if overlap else back_stream
     if overlap:
       replay_stream.wait_stream(back_stream)  # the state's last readers above
     back_done = []
